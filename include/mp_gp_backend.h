@@ -1,0 +1,279 @@
+/*
+ * mp_gp_backend.h — C ABI of the MI355X (gfx950) global-placement numeric backend.
+ *
+ * Plain C types only: pointers, sizes, doubles.  Every entry point returns an
+ * mp_gp_status; the message of the last failure on the calling thread is
+ * returned by mp_gp_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the MyPlacement tree):
+ *   class CudaPlacementBackend            src/placement/CudaPlacementBackend.hpp:25-39
+ *   tryCreateCudaPlacementBackend         src/placement/CudaPlacementBackend.hpp:44-48
+ *   cudaPlacementBackendCompiled          src/placement/CudaPlacementBackend.hpp:44
+ *   CudaPlacementEvaluation               src/placement/CudaPlacementBackend.hpp:15-23
+ *   DensityMetrics                        include/myplacement/metrics/Metrics.hpp:38-47
+ *   HostLayout / buildHostLayout          src/placement/cuda/CudaPlacementBackend.cu:764-839
+ *   GlobalPlacementOptions (adaptive)     include/myplacement/placement/GlobalPlacer.hpp:26-80
+ *   GlobalPlacementIteration / Result     include/myplacement/placement/GlobalPlacer.hpp:82-129
+ *   runAdaptiveGlobalPlacement            src/placement/AdaptiveGlobalPlacer.cpp:652-953
+ */
+#ifndef MP_GP_BACKEND_H
+#define MP_GP_BACKEND_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  They map one-to-one onto the reference's error behaviour:
+ *   MP_GP_INVALID_ARGUMENT  <-> std::invalid_argument  (.cu:923-928, AdaptiveGlobalPlacer.cpp:186-187)
+ *   MP_GP_RUNTIME_ERROR     <-> std::runtime_error     (.cu:30-43, .cu:945-948)
+ *   MP_GP_UNAVAILABLE       <-> factory returns nullptr with `reason` set (.cu:1156-1216)           */
+typedef enum mp_gp_status {
+    MP_GP_OK = 0,
+    MP_GP_INVALID_ARGUMENT = 1,
+    MP_GP_RUNTIME_ERROR = 2,
+    MP_GP_UNAVAILABLE = 3
+} mp_gp_status;
+
+/* Particle layers of the density scatter (.cu:811-822): standard cell, movable macro, filler. */
+enum { MP_LAYER_STANDARD = 0, MP_LAYER_MACRO = 1, MP_LAYER_FILLER = 2 };
+
+/* Arithmetic used for the DCT bin-grid products. */
+enum { MP_DCT_FP32 = 0, MP_DCT_BF16X3 = 1 };
+
+/* ---- netlist description (the PlacementDatabase fields the GP path reads) ---------------
+ * Mirrors include/myplacement/model/PlacementDatabase.hpp:25-71.  Pins are listed in net
+ * order: the pins of net n are [net_pin_start[n], net_pin_start[n+1]), which is the pin-id
+ * order BookshelfParser produces (src/io/BookshelfParser.cpp:244-271).                     */
+typedef struct mp_design {
+    int64_t module_count;
+    const double* width;          /* [M] */
+    const double* height;         /* [M] */
+    const double* center;         /* [2M] x,y: Module::center */
+    const int32_t* orientation;   /* [M] N,S,E,W,FN,FS,FE,FW,Unknown = 0..8 */
+    const uint8_t* is_terminal;   /* [M] */
+    const uint8_t* is_fixed;      /* [M] as parsed (terminals are forced fixed on refresh) */
+    int64_t net_count;
+    const double* net_weight;     /* [N] */
+    const int64_t* net_pin_start; /* [N+1] */
+    int64_t pin_count;
+    const int32_t* pin_module;    /* [I] */
+    const double* pin_offset;     /* [2I] raw (un-oriented) offsets from the module centre */
+    int64_t row_count;
+    const double* rows;           /* [5R] bottom, height, x_start, site_spacing, site_count */
+} mp_design;
+
+/* ---- flattened backend input: HostLayout + static density layers ------------------------
+ * Mirrors HostLayout (.cu:764-775) and the static upload (.cu:897-917).                    */
+typedef struct mp_gp_static {
+    int64_t particle_count;              /* movable modules (movableModules() order), then fillers */
+    const double* particle_width;        /* [P] */
+    const double* particle_height;       /* [P] */
+    const double* particle_charge_area;  /* [P] densityChargeArea (GlobalPlacementSupport.cpp:105-111) */
+    const int32_t* particle_layer;       /* [P] MP_LAYER_* */
+    int64_t net_count;
+    const int32_t* net_offsets;          /* [N+1] */
+    const double* net_weights;           /* [N] */
+    int64_t pin_count;
+    const int32_t* pin_particles;        /* [I] particle index, or -1 for a fixed pin */
+    const double* pin_offsets_or_static; /* [2I] oriented offset, or absolute position if fixed */
+    int32_t bins_x, bins_y;
+    double region_ll_x, region_ll_y, region_ur_x, region_ur_y;
+    double target_density;
+    const double* fixed_area;            /* [G] row-major, row = y */
+    const double* dark_area;             /* [G] */
+    int32_t device;                      /* HIP device ordinal */
+    int32_t dct_precision;               /* MP_DCT_FP32 or MP_DCT_BF16X3 */
+    uint64_t maximum_device_bytes;       /* allocation budget (GlobalPlacer.hpp:79) */
+} mp_gp_static;
+
+/* DensityMetrics, include/myplacement/metrics/Metrics.hpp:38-47 */
+typedef struct mp_gp_density_metrics {
+    double total_overflow_area;
+    double normalized_overflow;
+    double maximum_density;
+    double average_density;
+    double total_charge_area;
+    double normalization_area;
+    double placeable_area;
+    double dark_area;
+} mp_gp_density_metrics;
+
+/* Scalars of CudaPlacementEvaluation, src/placement/CudaPlacementBackend.hpp:15-20 */
+typedef struct mp_gp_scalars {
+    double hpwl;
+    double smooth_wirelength;
+    double electrostatic_energy;
+    mp_gp_density_metrics optimizer_density; /* with fillers */
+    mp_gp_density_metrics design_density;    /* without fillers */
+} mp_gp_scalars;
+
+/* Adaptive-controller options, GlobalPlacer.hpp:26-80 (RUDY fields omitted: the RUDY term is
+ * disabled by default and never runs on the device in the reference, AdaptiveGlobalPlacer.cpp:284). */
+typedef struct mp_gp_options {
+    int32_t iterations;                    /* 280 */
+    int32_t bins_x, bins_y;                /* 64, 64 */
+    double target_density;                 /* 0.85 */
+    double maximum_bin_overflow;           /* 0.10 */
+    double initial_smoothing;              /* 0.75 */
+    double final_smoothing;                /* 0.12 */
+    double maximum_movement_in_bins;       /* 0.22 */
+    int32_t maximum_backtracks;            /* 6 */
+    double backtracking_ratio;             /* 0.5 */
+    double armijo_coefficient;             /* 1e-4 */
+    double maximum_momentum;               /* 0.92 */
+    double initial_movement_in_bins;       /* 0.12 */
+    double objective_increase_for_density; /* 0.002 */
+    double hpwl_growth_restart_threshold;  /* 0.02 */
+    double density_stall_threshold;        /* 0.002 */
+    double density_penalty_increase;       /* 1.05 */
+    double density_penalty_decrease;       /* 0.88 */
+    double minimum_density_penalty;        /* 1e-4 */
+    double maximum_density_penalty;        /* 1e6 */
+    double smoothing_overflow_exponent;    /* 1.0 */
+    int32_t feasible_refinement_iterations;/* 12 */
+    uint64_t maximum_fillers;              /* 50000 */
+    uint64_t seed;                         /* 2026 */
+    int32_t device;                        /* HIP device ordinal (0) */
+    int32_t dct_precision;                 /* MP_DCT_* */
+    uint64_t maximum_device_bytes;         /* budget for device allocations */
+} mp_gp_options;
+
+/* GlobalPlacementIteration, GlobalPlacer.hpp:82-105 (same 22 columns as global_history.csv) */
+typedef struct mp_gp_history_row {
+    int32_t iteration;
+    double hpwl, overflow, design_overflow, smooth_wirelength, density_energy, objective;
+    double penalty, smoothing, step_size, maximum_displacement, gradient_norm, curvature;
+    double rudy_energy, rudy_proxy_overflow, rudy_maximum_utilization, rudy_weight;
+    int32_t backtracks;
+    int32_t momentum_restarted, accepted, best_checkpoint, rudy_active;
+} mp_gp_history_row;
+
+/* GlobalPlacementResult, GlobalPlacer.hpp:107-129, plus evaluation counters. */
+typedef struct mp_gp_result {
+    double hpwl_before, hpwl_after, overflow_before, overflow_after;
+    int32_t completed_iterations, accepted_iterations, momentum_restarts, rejected_candidates;
+    int32_t best_checkpoint_iteration, restored_best_checkpoint;
+    double best_checkpoint_hpwl, best_checkpoint_overflow;
+    double elapsed_seconds;                 /* set-up + loop, as the reference reports (:656,951) */
+    double loop_seconds;                    /* outer loop only */
+    int64_t gradient_evaluations, value_evaluations;
+    int64_t particle_count, filler_count;
+    int32_t device_used;
+    uint64_t reserved_device_bytes;
+} mp_gp_result;
+
+typedef struct mp_gp_backend mp_gp_backend;   /* one CudaPlacementBackend instance */
+typedef struct mp_gp_session mp_gp_session;   /* device-resident adaptive GP run     */
+
+/* ---- backend boundary --------------------------------------------------------------------- */
+
+/* cudaPlacementBackendCompiled(), CudaPlacementBackend.hpp:44: 1 when the HIP kernels are built in. */
+int mp_gp_compiled(void);
+
+/* Thread-local text of the last failure (empty when none). */
+const char* mp_gp_last_error(void);
+
+/* tryCreateCudaPlacementBackend, CudaPlacementBackend.hpp:45-48 / .cu:1156-1216.
+ * Uploads the static layout once.  MP_GP_UNAVAILABLE means a recoverable reason (no device,
+ * budget exceeded, ...): the text is in mp_gp_last_error() and *out is NULL. */
+mp_gp_status mp_gp_create(const mp_gp_static* layout, mp_gp_backend** out);
+
+/* CudaPlacementBackend::evaluate, CudaPlacementBackend.hpp:32-33 / .cu:921-950.
+ * xy: [2P] particle centres (FP64, particle order).  wire_grad / density_grad: [2P] outputs,
+ * written only when want_grad != 0 (they may be NULL otherwise).  Blocking. */
+mp_gp_status mp_gp_evaluate(mp_gp_backend* backend, const double* xy, double smoothing, int want_grad,
+                            mp_gp_scalars* out, double* wire_grad, double* density_grad);
+
+/* CudaPlacementBackend::device(), ::reservedBytes(), CudaPlacementBackend.hpp:34-35 */
+int mp_gp_device(const mp_gp_backend* backend);
+uint64_t mp_gp_reserved_bytes(const mp_gp_backend* backend);
+
+/* ~CudaPlacementBackend() */
+void mp_gp_destroy(mp_gp_backend* backend);
+
+/* Copies of the device grids of the last evaluate (FP64 [G] each, NULL skips one): the centred
+ * density source, potential (when computed), and the two field components. Test hook for the
+ * field parity checks; reference has the same buffers in CudaNeumannField (.cu:625-762). */
+mp_gp_status mp_gp_debug_grids(mp_gp_backend* backend, double* source, double* potential,
+                               double* field_x, double* field_y);
+
+/* ---- host layout services (GlobalPlacementSession set-up) --------------------------------- */
+
+/* Default option values of GlobalPlacementOptions (GlobalPlacer.hpp:26-80). */
+void mp_gp_default_options(mp_gp_options* options);
+
+/* A prepared design: refreshDerivedData (PlacementDatabase.cpp:162-200), static density
+ * (GlobalPlacementSupport.cpp:21-30), fillers (:52-103), HostLayout (.cu:784-839) and the
+ * preconditioner base (AdaptiveGlobalPlacer.cpp:358-375). */
+typedef struct mp_gp_prepared mp_gp_prepared;
+
+mp_gp_status mp_gp_prepare(const mp_design* design, const mp_gp_options* options, mp_gp_prepared** out);
+void mp_gp_prepared_destroy(mp_gp_prepared* prepared);
+
+/* Sizes of a prepared design. */
+typedef struct mp_gp_prepared_info {
+    int64_t movable_count, filler_count, particle_count, net_count, pin_count, bin_count;
+    double core_ll_x, core_ll_y, core_ur_x, core_ur_y;
+    double nominal_row_height;
+    double filler_width, filler_height;
+} mp_gp_prepared_info;
+mp_gp_status mp_gp_prepared_query(const mp_gp_prepared* prepared, mp_gp_prepared_info* info);
+
+/* The flattened layout (pointers stay valid while `prepared` lives). */
+mp_gp_status mp_gp_prepared_layout(const mp_gp_prepared* prepared, mp_gp_static* layout);
+
+/* Movable module ids (movableModules() order) [movable_count]; filler centres [2F];
+ * preconditioner base [P]. */
+mp_gp_status mp_gp_prepared_arrays(const mp_gp_prepared* prepared, int64_t* movable_ids,
+                                   double* filler_centers, double* preconditioner_base);
+
+/* InitialPlacer randomPlacement, src/placement/InitialPlacer.cpp:102-120: mt19937_64(seed),
+ * uniform centre per movable module, then clampModuleToCore.  centers: [2M] in/out. */
+mp_gp_status mp_gp_random_placement(const mp_design* design, uint64_t seed, double* centers);
+
+/* ---- device-resident adaptive run (runAdaptiveGlobalPlacement with --compute-backend hip) ---- */
+
+/* Creates the backend and the device-resident optimizer state from a prepared design.
+ * centers: [2M] module centres (movable ones are read). */
+mp_gp_status mp_gp_session_create(const mp_gp_prepared* prepared, const mp_gp_options* options,
+                                  const double* centers, mp_gp_session** out);
+
+/* Runs up to `iterations` more outer iterations of the adaptive loop (AdaptiveGlobalPlacer.cpp:735-928);
+ * *completed receives how many ran; *stopped is 1 when the loop finished (budget or early stop). */
+mp_gp_status mp_gp_session_step(mp_gp_session* session, int32_t iterations, int32_t* completed,
+                                int32_t* stopped);
+
+/* Restores the best checkpoint (AdaptiveGlobalPlacer.cpp:930-940) and writes module centres [2M]
+ * (movable ones updated) and the result record. */
+mp_gp_status mp_gp_session_finish(mp_gp_session* session, double* centers, mp_gp_result* result);
+
+/* History rows so far; returns the row count when rows == NULL. */
+int64_t mp_gp_session_history(const mp_gp_session* session, mp_gp_history_row* rows, int64_t capacity);
+
+/* Current particle positions [2P] (host copy). */
+mp_gp_status mp_gp_session_positions(mp_gp_session* session, double* xy);
+
+/* Per-kernel timing of the session's stream with HIP events (enable, then read).
+ * names: comma-separated kernel classes; avg_us/launches per class. */
+mp_gp_status mp_gp_session_enable_kernel_timing(mp_gp_session* session, int enable);
+int32_t mp_gp_session_kernel_timing(mp_gp_session* session, char* names, size_t names_len, double* total_us,
+                                    int64_t* launches, int32_t capacity);
+
+/* Access to the backend owned by a session (for evaluate-level checks). */
+mp_gp_backend* mp_gp_session_backend(mp_gp_session* session);
+
+void mp_gp_session_destroy(mp_gp_session* session);
+
+/* Whole GlobalPlacer::run (GlobalPlacer.cpp:134-146) for the adaptive optimizer on the HIP
+ * backend: prepare, create, loop, restore.  centers: [2M] in/out. history may be NULL. */
+mp_gp_status mp_gp_global_place(const mp_design* design, const mp_gp_options* options, double* centers,
+                                 mp_gp_result* result, mp_gp_history_row* history, int64_t history_capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MP_GP_BACKEND_H */

@@ -1,0 +1,257 @@
+"""ctypes bindings of the CPU checker libraries.
+
+TEST INFRASTRUCTURE.  Imported only by tests/, __graft_entry__.smoke() and the CPU-baseline leg
+of bench.py.  liboracle.so is the FP64 restatement (gp_oracle.cpp); _ref/libmpref.so is the
+reference's own density/filler/parser code (ref_shim.cpp), present only where it was built.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from myplacement_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libmpref.so")
+
+_dp = C.POINTER(C.c_double)
+_oracle = None
+_ref = None
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(status: int, lib) -> None:
+    if status != 0:
+        msg = lib.orc_last_error().decode() if hasattr(lib, "orc_last_error") else lib.ref_last_error().decode()
+        if status == abi.MP_GP_INVALID_ARGUMENT:
+            raise ValueError(msg)
+        raise OracleError(msg)
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            raise OracleError(f"{ORACLE_SO} is not built (run __graft_entry__.build())")
+        lib = C.CDLL(ORACLE_SO)
+        lib.orc_last_error.restype = C.c_char_p
+        lib.orc_session_create.argtypes = [C.POINTER(abi.MpDesign), C.POINTER(abi.MpOptions), C.POINTER(C.c_void_p)]
+        lib.orc_session_destroy.argtypes = [C.c_void_p]
+        lib.orc_session_info.argtypes = [C.c_void_p, C.POINTER(abi.MpPreparedInfo)]
+        lib.orc_session_layout.argtypes = [C.c_void_p, C.POINTER(abi.MpStatic)]
+        lib.orc_session_static_grids.argtypes = [C.c_void_p, _dp, _dp]
+        lib.orc_session_arrays.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_session_evaluate.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int, C.POINTER(abi.MpScalars),
+                                             C.c_void_p, C.c_void_p]
+        lib.orc_session_grids.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_global_place.argtypes = [C.POINTER(abi.MpDesign), C.POINTER(abi.MpOptions), _dp,
+                                         C.POINTER(abi.MpResult), C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]
+        lib.orc_random_placement.argtypes = [C.POINTER(abi.MpDesign), C.c_uint64, _dp]
+        lib.orc_design_metrics.argtypes = [C.POINTER(abi.MpDesign), C.c_int, C.c_int, C.c_double, _dp,
+                                           C.POINTER(abi.MpDensityMetrics)]
+        lib.orc_transform_1d.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        lib.orc_transform_2d.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        lib.orc_field_solve.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+        lib.orc_sample_field.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_int64, _dp, _dp]
+        lib.orc_density_map.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_int64, _dp, C.c_int64, _dp,
+                                        C.POINTER(C.c_int32), C.POINTER(abi.MpDensityMetrics),
+                                        C.POINTER(abi.MpDensityMetrics), C.c_void_p]
+        lib.orc_set_method.argtypes = [C.c_int]
+        _oracle = lib
+    return _oracle
+
+
+def reference():
+    """The reference's own code (None when oracle/_ref was not built here)."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_SO):
+            return None
+        lib = C.CDLL(REF_SO)
+        lib.ref_last_error.restype = C.c_char_p
+        lib.ref_density_map.argtypes = oracle().orc_density_map.argtypes
+        lib.ref_fillers.argtypes = [C.POINTER(abi.MpDesign), C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_uint64,
+                                    C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p, C.c_int64]
+        lib.ref_deposit.argtypes = [C.POINTER(abi.MpDesign), C.c_int, C.c_int, C.c_double, C.c_int64, _dp,
+                                    C.c_void_p, C.POINTER(abi.MpDensityMetrics), C.POINTER(abi.MpDensityMetrics)]
+        lib.ref_design_metrics.argtypes = [C.POINTER(abi.MpDesign), C.c_int, C.c_int, C.c_double, _dp,
+                                           C.POINTER(abi.MpDensityMetrics)]
+        lib.ref_derived.argtypes = [C.POINTER(abi.MpDesign), C.c_void_p, C.POINTER(C.c_int64), C.c_void_p, _dp, _dp]
+        lib.ref_parse_aux.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        lib.ref_db_destroy.argtypes = [C.c_void_p]
+        lib.ref_db_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        lib.ref_db_export.argtypes = [C.c_void_p] + [C.c_void_p] * 12
+        _ref = lib
+    return _ref
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+# ---------------------------------------------------------------------------------------------
+# transforms / field
+
+def transform_1d(kind: int, x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    _check(oracle().orc_transform_1d(kind, x.shape[0], abi.dptr(x), abi.dptr(out)), oracle())
+    return out
+
+
+def transform_2d(kind_y: int, kind_x: int, grid: np.ndarray) -> np.ndarray:
+    g = np.ascontiguousarray(grid, dtype=np.float64)
+    out = np.empty_like(g)
+    _check(oracle().orc_transform_2d(g.shape[1], g.shape[0], kind_y, kind_x, abi.dptr(g), abi.dptr(out)), oracle())
+    return out
+
+
+def set_method(method: int) -> None:
+    """0 auto (FFT for powers of two), 1 direct O(N^2), 2 FFT."""
+    oracle().orc_set_method(method)
+
+
+def field_solve(source: np.ndarray, region) -> tuple:
+    s = np.ascontiguousarray(source, dtype=np.float64)
+    rows, cols = s.shape
+    reg = np.asarray(region, dtype=np.float64)
+    pot, fx, fy = np.empty_like(s), np.empty_like(s), np.empty_like(s)
+    _check(oracle().orc_field_solve(cols, rows, abi.dptr(reg), abi.dptr(s), abi.dptr(pot), abi.dptr(fx),
+                                    abi.dptr(fy)), oracle())
+    return pot, fx, fy
+
+
+def sample_field(fx: np.ndarray, fy: np.ndarray, region, xy: np.ndarray) -> np.ndarray:
+    fx = np.ascontiguousarray(fx, dtype=np.float64)
+    fy = np.ascontiguousarray(fy, dtype=np.float64)
+    xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+    reg = np.asarray(region, dtype=np.float64)
+    out = np.empty_like(xy)
+    rows, cols = fx.shape
+    _check(oracle().orc_sample_field(cols, rows, abi.dptr(reg), abi.dptr(fx), abi.dptr(fy), xy.shape[0],
+                                     abi.dptr(xy), abi.dptr(out)), oracle())
+    return out
+
+
+def density_map(region, cols, rows, t, regions, rects, layers, lib=None):
+    """DensityMap driver: returns (metrics_with, metrics_without, bins[G,5])."""
+    lib = lib or oracle()
+    reg = np.asarray(region, dtype=np.float64)
+    regions = np.ascontiguousarray(np.asarray(regions, dtype=np.float64).reshape(-1, 4))
+    rects = np.ascontiguousarray(np.asarray(rects, dtype=np.float64).reshape(-1, 4))
+    layers = np.ascontiguousarray(np.asarray(layers, dtype=np.int32))
+    w, wo = abi.MpDensityMetrics(), abi.MpDensityMetrics()
+    bins = np.zeros((cols * rows, 5))
+    fn = lib.orc_density_map if hasattr(lib, "orc_density_map") else lib.ref_density_map
+    _check(fn(abi.dptr(reg), cols, rows, t, regions.shape[0], abi.dptr(regions), rects.shape[0], abi.dptr(rects),
+              layers.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(w), C.byref(wo), _p(bins)), lib)
+    return w.as_dict(), wo.as_dict(), bins
+
+
+# ---------------------------------------------------------------------------------------------
+# design-level
+
+def random_placement(design, seed: int = 2026) -> np.ndarray:
+    c = design.center.copy()
+    d = design.to_c()
+    _check(oracle().orc_random_placement(C.byref(d), seed, abi.dptr(c)), oracle())
+    return c
+
+
+def design_metrics(design, center, cols, rows, t) -> tuple:
+    d = design.to_c(center)
+    hpwl = C.c_double()
+    m = abi.MpDensityMetrics()
+    _check(oracle().orc_design_metrics(C.byref(d), cols, rows, t, C.byref(hpwl), C.byref(m)), oracle())
+    return hpwl.value, m.as_dict()
+
+
+class OracleSession:
+    """GlobalPlacementSession on the CPU: static density, fillers, layout and evaluate."""
+
+    def __init__(self, design, options: abi.MpOptions, center=None):
+        self._design = design
+        self._c = design.to_c(center)
+        self._opts = options
+        h = C.c_void_p()
+        _check(oracle().orc_session_create(C.byref(self._c), C.byref(options), C.byref(h)), oracle())
+        self.h = h
+        info = abi.MpPreparedInfo()
+        _check(oracle().orc_session_info(self.h, C.byref(info)), oracle())
+        self.info = info
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            oracle().orc_session_destroy(self.h)
+            self.h = None
+
+    @property
+    def particle_count(self) -> int:
+        return int(self.info.particle_count)
+
+    def layout(self) -> dict:
+        L = abi.MpStatic()
+        _check(oracle().orc_session_layout(self.h, C.byref(L)), oracle())
+        P, N, I = L.particle_count, L.net_count, L.pin_count
+        G = L.bins_x * L.bins_y
+
+        def arr(ptr, n, dt):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True) if n else np.zeros(0, dt)
+
+        fixed, dark = np.empty(G), np.empty(G)
+        _check(oracle().orc_session_static_grids(self.h, abi.dptr(fixed), abi.dptr(dark)), oracle())
+        return dict(
+            particle_width=arr(L.particle_width, P, np.float64), particle_height=arr(L.particle_height, P, np.float64),
+            particle_charge_area=arr(L.particle_charge_area, P, np.float64),
+            particle_layer=arr(L.particle_layer, P, np.int32), net_offsets=arr(L.net_offsets, N + 1, np.int32),
+            net_weights=arr(L.net_weights, N, np.float64), pin_particles=arr(L.pin_particles, I, np.int32),
+            pin_offsets_or_static=arr(L.pin_offsets_or_static, 2 * I, np.float64).reshape(-1, 2),
+            bins_x=L.bins_x, bins_y=L.bins_y,
+            region=(L.region_ll_x, L.region_ll_y, L.region_ur_x, L.region_ur_y), target_density=L.target_density,
+            fixed_area=fixed, dark_area=dark,
+        )
+
+    def arrays(self) -> dict:
+        M, F, P = self.info.movable_count, self.info.filler_count, self.info.particle_count
+        mov = np.empty(M, dtype=np.int64)
+        fc = np.empty((F, 2))
+        pre = np.empty(P)
+        pos = np.empty((P, 2))
+        _check(oracle().orc_session_arrays(self.h, _p(mov), _p(fc), _p(pre), _p(pos)), oracle())
+        return dict(movable_ids=mov, filler_centers=fc, preconditioner=pre, positions=pos)
+
+    def evaluate(self, xy: np.ndarray, smoothing: float, want_grad: bool = True) -> dict:
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        s = abi.MpScalars()
+        wg = np.zeros_like(xy) if want_grad else None
+        dg = np.zeros_like(xy) if want_grad else None
+        _check(oracle().orc_session_evaluate(self.h, abi.dptr(xy), smoothing, int(want_grad), C.byref(s), _p(wg),
+                                             _p(dg)), oracle())
+        return dict(hpwl=s.hpwl, smooth_wirelength=s.smooth_wirelength, electrostatic_energy=s.electrostatic_energy,
+                    optimizer_density=s.optimizer_density.as_dict(), design_density=s.design_density.as_dict(),
+                    wire_gradient=wg, density_gradient=dg)
+
+    def grids(self) -> dict:
+        G = int(self.info.bin_count)
+        src, pot, fx, fy = (np.empty(G) for _ in range(4))
+        _check(oracle().orc_session_grids(self.h, _p(src), _p(pot), _p(fx), _p(fy)), oracle())
+        return dict(source=src, potential=pot, field_x=fx, field_y=fy)
+
+
+def global_place(design, options: abi.MpOptions, center=None, history_capacity: int = 100000):
+    c = np.ascontiguousarray(design.center if center is None else center, dtype=np.float64).copy()
+    d = design.to_c(c)
+    res = abi.MpResult()
+    hist = (abi.MpHistoryRow * history_capacity)()
+    n = C.c_int64()
+    _check(oracle().orc_global_place(C.byref(d), C.byref(options), abi.dptr(c), C.byref(res), hist,
+                                     history_capacity, C.byref(n)), oracle())
+    rows = [hist[i].as_dict() for i in range(min(n.value, history_capacity))]
+    return c, res.as_dict(), rows
