@@ -1,0 +1,153 @@
+"""GP iterations/s of the device-resident adaptive global placement on one MI355X.
+
+A step is one outer iteration of the adaptive loop (AdaptiveGlobalPlacer.cpp:735-928): a
+gradient evaluation, an optional Nesterov look-ahead gradient evaluation, and the backtracking
+value evaluations, with the controller's vector passes on the device.
+
+Workload (default): the single large design at real shape -- 210,000 movable cells, 12 fixed
+macros (15 % of the core), 220,000 nets (+0.2 % at fanout 50-500, ~0.9 M pins), 1,200 rows x
+1,200 sites, 512 x 512 bins, target density 0.8, seeded synthetic netlist, random initial
+placement, early stop disabled.  Inputs are resident in HBM before the timed region starts.
+
+Prints one JSON line.  `roofline` reports the dominant kernel class (by time inside the timed
+region) against MI355X's peak; `cpu_baseline` is the FP64 CPU restatement of the reference path
+(oracle/) timed on one host core on a bounded number of iterations of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name: str):
+    from myplacement_amd import abi, design
+    from myplacement_amd import backend as hb
+    if name == "large":
+        d = design.large_design(seed=2026)
+        o = abi.default_options(target_density=0.8, bins_x=512, bins_y=512)
+        desc = {"workload": "synthetic 210k cells, 12 macros (15%), 220k nets (~0.9M pins), 1200x1200 sites, "
+                            "512x512 bins, t=0.8, random init", "cells": 210000, "bins": 512, "target_density": 0.8}
+    elif name == "plain":
+        d = design.plain_design(seed=2026)
+        o = abi.default_options(target_density=0.9, bins_x=64, bins_y=64)
+        desc = {"workload": "synthetic 2k cells, 2.3k nets, 40x400 sites, 64x64 bins, t=0.9, random init",
+                "cells": 2000, "bins": 64, "target_density": 0.9}
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    d.center = hb.random_placement(d, 2026)
+    o.iterations = 1_000_000
+    o.feasible_refinement_iterations = 2 ** 31 - 1
+    return d, o, desc
+
+
+def cpu_baseline(name: str, iterations: int) -> dict:
+    """The oracle's single-threaded FP64 GP loop on the same design (checker, not product)."""
+    from oracle import bindings as ob
+    d, o, _ = make_workload(name)
+    o.iterations = iterations
+    _, r, _ = ob.global_place(d, o)
+    return {"value": iterations / r["loop_seconds"], "unit": "GP iterations/s", "cores": 1, "kind": "port",
+            "sample": f"{iterations} adaptive iterations of the same workload, FP64 oracle (own FFT DCT), 1 thread",
+            "loop_seconds": r["loop_seconds"]}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="large")
+    ap.add_argument("--cpu-iterations", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU split is a design note, "
+              "not a measured path", file=sys.stderr)
+        return 2
+    from myplacement_amd import backend as hb
+    d, o, desc = make_workload(args.workload)
+    t0 = time.perf_counter()
+    prep = hb.PreparedDesign(d, o)
+    sess = hb.PlacementSession(prep, o)
+    setup_s = time.perf_counter() - t0
+    log(f"setup {setup_s:.2f}s P={prep.info.particle_count} pins={prep.info.pin_count}")
+    sess.step(args.warmup)
+    log(f"warmup {args.warmup} done")
+    h0 = len(sess.history())
+    t1 = time.perf_counter()
+    done = sess.step(args.steps)
+    sess.positions()  # device work of the last step is complete
+    elapsed = time.perf_counter() - t1
+    log(f"timed {done} steps in {elapsed:.3f}s")
+    hist = sess.history()[h0:]
+    _, res_mid = None, None
+
+    # kernel classes timed with HIP events on the session stream (separate region, same workload)
+    sess.enable_kernel_timing(True)
+    sess.step(args.timing_steps)
+    timing = sess.kernel_timing()
+    sess.enable_kernel_timing(False)
+    _, res = sess.finish()
+    dominant = max((k for k in timing if timing[k]["launches"] and k != "controller"),
+                   key=lambda k: timing[k]["total_us"])
+    tk = timing[dominant]
+    work = tk.get("work", 0.0)
+    avg_us = tk["total_us"] / tk["launches"]
+    if dominant == "k_gemm":
+        achieved = work / (tk["total_us"] * 1e-6) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    else:
+        achieved = work / (tk["total_us"] * 1e-6) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = dominant
+    roof["avg_launch_us"] = avg_us
+
+    line = {
+        "metric": "GP iters/s @210k cells 512^2 bins",
+        "value": done / elapsed,
+        "unit": "GP iterations/s",
+        "n_gpus": 1,
+        "steps": done,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / max(done, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 positions/reductions, f32 pin terms and DCT products",
+        "data": "synthetic",
+        "config": dict(desc, evaluations_per_step=(
+            sum(1 for _ in hist) and (res["gradient_evaluations"] + res["value_evaluations"]) / max(1, res["completed_iterations"]))),
+        "roofline": roof,
+        "kernel_timing_us": {k: round(v["total_us"] / max(1, v["launches"]), 3) for k, v in timing.items()},
+        "setup_seconds": setup_s,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cb = cpu_baseline(args.workload, args.cpu_iterations)
+        line["cpu_baseline"] = cb
+    print(json.dumps(line), flush=True)
+    sess.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

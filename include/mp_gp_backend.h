@@ -256,11 +256,12 @@ int64_t mp_gp_session_history(const mp_gp_session* session, mp_gp_history_row* r
 /* Current particle positions [2P] (host copy). */
 mp_gp_status mp_gp_session_positions(mp_gp_session* session, double* xy);
 
-/* Per-kernel timing of the session's stream with HIP events (enable, then read).
- * names: comma-separated kernel classes; avg_us/launches per class. */
+/* Per-kernel timing of the session's stream with HIP events (enable, then read).  Returns the class
+ * count; names: comma-separated kernel classes; per class the summed event time, the launch count
+ * and the summed algorithmic work (bytes, or flops for the DCT products) of the timed launches. */
 mp_gp_status mp_gp_session_enable_kernel_timing(mp_gp_session* session, int enable);
 int32_t mp_gp_session_kernel_timing(mp_gp_session* session, char* names, size_t names_len, double* total_us,
-                                    int64_t* launches, int32_t capacity);
+                                    int64_t* launches, double* work, int32_t capacity);
 
 /* Access to the backend owned by a session (for evaluate-level checks). */
 mp_gp_backend* mp_gp_session_backend(mp_gp_session* session);

@@ -1,0 +1,86 @@
+"""Builds the in-tree shared libraries.
+
+  myplacement_amd/libmp_gp_hip.so   HIP kernels for gfx950 + host C++ + the C ABI (hipcc)
+  oracle/liboracle.so               CPU checker (g++), test infrastructure
+  oracle/_ref/libmpref.so           reference sources + shim (only where the reference tree exists)
+
+Run as `python -m myplacement_amd.build` or through __graft_entry__.build().
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libmp_gp_hip.so")
+SOURCES = ["gp_kernels.hip", "gp_backend.cpp", "gp_host.cpp", "gp_session.cpp"]
+ARCH = os.environ.get("MP_GP_ARCH", "gfx950")
+REFERENCE = "/root/reference"
+
+
+def hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def hip_flags() -> list:
+    return [
+        f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-Wall", "-Wno-unused-result", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}",
+    ]
+
+
+def build_hip(verbose: bool = False) -> str:
+    objs = []
+    obj_dir = os.path.join(ROOT, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    procs = []
+    for src in SOURCES:
+        obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc()] + hip_flags() + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        procs.append((src, cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(obj)
+    failed = []
+    for src, cmd, p in procs:
+        out, _ = p.communicate()
+        out = "\n".join(l for l in out.splitlines() if "packed-fp32-ops" not in l and l.strip())
+        if p.returncode != 0:
+            failed.append(f"{src}:\n{out}")
+        elif verbose and out:
+            print(out)
+    if failed:
+        raise RuntimeError("HIP build failed:\n" + "\n".join(failed))
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("HIP link failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> None:
+    targets = ["oracle"]
+    if os.path.isdir(REFERENCE):
+        targets.append("ref")
+    r = subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")] + targets, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+
+
+def build(verbose: bool = False) -> None:
+    build_hip(verbose)
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print("built", LIB)

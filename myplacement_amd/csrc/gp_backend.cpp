@@ -1,0 +1,583 @@
+// gp_backend.cpp — Backend construction, evaluation orchestration and the backend half of the
+// C ABI (mp_gp_create / mp_gp_evaluate / mp_gp_destroy ...).
+//
+// Reference: CudaPlacementBackendImpl and tryCreateCudaPlacementBackend,
+// src/placement/cuda/CudaPlacementBackend.cu:841-1216.  Differences by design:
+//   * one stream, no blocking scalar copy inside the evaluation (the reference does 14);
+//   * the gradient scatter is atomic-free: per-pin terms, then a particle -> pin CSR sum;
+//   * the DCT stages are dense bin-grid products on MFMA; the electrostatic energy is taken in
+//     the cosine basis (sum_lk c_l c_k / 4 * chat * phihat), which equals the reference's grid sum
+//     (.cu:569-573 / AdaptiveGlobalPlacer.cpp:341-354) by the orthogonality of the DCT-II basis,
+//     so a value-only evaluation needs no inverse transform;
+//   * the density mean is not subtracted before the forward transform: it only moves the DC
+//     coefficient, which the solve drops (ElectrostaticField.cpp:228).
+
+#include "gp_backend.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <sstream>
+
+namespace mpgp {
+
+namespace {
+constexpr double kPiD = 3.141592653589793238462643383279502884;
+constexpr std::size_t kMaximumBinCount = 1U << 20U;  // core/ResourceLimits.hpp:10
+constexpr std::size_t kDeviceReserveBytes = 4ULL << 30;
+
+int roundUp32(int n) { return (n + 31) / 32 * 32; }
+
+// Dense transform factors (FFTW r2r definitions, unnormalised), FP64 then rounded to FP32,
+// zero in the padding.  `rows` x `cols` row-major.
+std::vector<float> basis(int rows, int cols, int n, int kind, double wave_scale) {
+    // kind 0: F[i][k] = 2 cos(pi k (2i+1) / 2n)        (REDFT10, x-side right factor, i = row)
+    // kind 1: F[l][j] = 2 cos(pi l (2j+1) / 2n)        (REDFT10, y-side left factor)
+    // kind 2: F[j][l] = c_l cos(pi l (2j+1) / 2n)      (REDFT01, left factor)
+    // kind 3: F[j][l] = 2 sin(pi l (2j+1) / 2n) * w_l  (RODFT01 with the input shift folded in)
+    std::vector<float> f(static_cast<std::size_t>(rows) * cols, 0.f);
+    auto ang = [n](long long a, long long b) {  // cos(pi * a * b / 2n) with exact integer reduction
+        const long long m = (a * b) % (4LL * n);
+        return kPiD * static_cast<double>(m) / (2.0 * n);
+    };
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) {
+            double v = 0.0;
+            if (kind == 0) v = 2.0 * std::cos(ang(c, 2 * r + 1));
+            else if (kind == 1) v = 2.0 * std::cos(ang(r, 2 * c + 1));
+            else if (kind == 2) v = (c == 0 ? 1.0 : 2.0) * std::cos(ang(c, 2 * r + 1));
+            else v = c == 0 ? 0.0 : 2.0 * std::sin(ang(c, 2 * r + 1)) * (kPiD * c / wave_scale);
+            f[static_cast<std::size_t>(r) * cols + c] = static_cast<float>(v);
+        }
+    return f;
+}
+}  // namespace
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void DeviceBudget::reserve(std::size_t bytes, const char* what) {
+    if (bytes > limit_ - reserved_) {
+        std::ostringstream m;
+        m << "HIP allocation budget exceeded while reserving " << what << " (requested " << bytes << " bytes, reserved "
+          << reserved_ << " bytes, limit " << limit_ << " bytes).";
+        throw std::runtime_error(m.str());
+    }
+    reserved_ += bytes;
+}
+
+Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
+    : device_(L.device), budget_(allocation_limit) {
+    if (L.particle_count < 0 || L.net_count < 0 || L.pin_count < 0)
+        throw InvalidArgument("negative counts in the placement layout");
+    if (L.particle_count > std::numeric_limits<int>::max() || L.pin_count > std::numeric_limits<int>::max() ||
+        L.net_count > std::numeric_limits<int>::max())
+        throw InvalidArgument("HIP backend cannot index more than INT_MAX particles, pins or nets");
+    if (L.bins_x < 2 || L.bins_y < 2) throw InvalidArgument("the density grid needs at least 2 x 2 bins");
+    if (static_cast<std::size_t>(L.bins_x) > kMaximumBinCount / static_cast<std::size_t>(L.bins_y))
+        throw InvalidArgument("Density grid exceeds the safe one-million-bin workspace limit.");
+    const double W = L.region_ur_x - L.region_ll_x, H = L.region_ur_y - L.region_ll_y;
+    if (!(W > 0.0 && H > 0.0 && W * H > 1e-9)) throw InvalidArgument("invalid placement region");
+    if (!(L.target_density > 0.0)) throw InvalidArgument("target density must be positive");
+    if (L.dct_precision != MP_DCT_FP32 && L.dct_precision != MP_DCT_BF16X3)
+        throw InvalidArgument("unknown DCT precision");
+    if (L.dct_precision == MP_DCT_BF16X3) throw std::runtime_error("the split-BF16 DCT products are not built yet");
+    P_ = static_cast<int>(L.particle_count);
+    N_ = static_cast<int>(L.net_count);
+    I_ = static_cast<int>(L.pin_count);
+    nx_ = L.bins_x;
+    ny_ = L.bins_y;
+    nxp_ = roundUp32(nx_);
+    nyp_ = roundUp32(ny_);
+    t_ = L.target_density;
+    dct_precision_ = L.dct_precision;
+    region_ = {L.region_ll_x, L.region_ll_y, L.region_ur_x, L.region_ur_y, W / nx_, H / ny_, nx_, ny_};
+    const std::size_t G = static_cast<std::size_t>(nx_) * ny_;
+
+    // ---- host-side derived arrays ----
+    if (N_ > 0 && (L.net_offsets == nullptr || L.net_offsets[0] != 0 || L.net_offsets[N_] != I_))
+        throw InvalidArgument("net offsets do not cover the pin array");
+    std::vector<int> small, large;
+    for (int n = 0; n < N_; ++n) {
+        const int deg = L.net_offsets[n + 1] - L.net_offsets[n];
+        if (deg < 0) throw InvalidArgument("net offsets are not monotone");
+        if (deg < 2) continue;
+        (deg <= kSmallDegree ? small : large).push_back(n);
+    }
+    n_small_ = static_cast<int>(small.size());
+    n_large_ = static_cast<int>(large.size());
+    // particle -> pin CSR over movable pins of nets with >= 2 pins, pins in net order
+    std::vector<int> count(static_cast<std::size_t>(P_) + 1, 0);
+    for (int n = 0; n < N_; ++n) {
+        if (L.net_offsets[n + 1] - L.net_offsets[n] < 2) continue;
+        for (int p = L.net_offsets[n]; p < L.net_offsets[n + 1]; ++p) {
+            const int part = L.pin_particles[p];
+            if (part >= P_) throw InvalidArgument("pin refers to an unknown particle");
+            if (part >= 0) ++count[part + 1];
+        }
+    }
+    std::partial_sum(count.begin(), count.end(), count.begin());
+    std::vector<int> ppin_idx(static_cast<std::size_t>(count[P_]));
+    {
+        std::vector<int> fill(count.begin(), count.end() - 1);
+        for (int n = 0; n < N_; ++n) {
+            if (L.net_offsets[n + 1] - L.net_offsets[n] < 2) continue;
+            for (int p = L.net_offsets[n]; p < L.net_offsets[n + 1]; ++p)
+                if (L.pin_particles[p] >= 0) ppin_idx[fill[L.pin_particles[p]]++] = p;
+        }
+    }
+    std::vector<double2> sizes(P_);
+    for (int i = 0; i < P_; ++i) sizes[i] = make_double2(L.particle_width[i], L.particle_height[i]);
+    std::vector<double> static_charge(G);
+    for (std::size_t i = 0; i < G; ++i) static_charge[i] = t_ * (L.fixed_area[i] + L.dark_area[i]);
+    // dark / placeable totals in bin order, as DensityMap::metrics sums them (Metrics.cpp:111-113)
+    for (int row = 0; row < ny_; ++row)
+        for (int col = 0; col < nx_; ++col) {
+            const double left = region_.llx + static_cast<double>(col) * region_.bw;
+            const double bottom = region_.lly + static_cast<double>(row) * region_.bh;
+            const double area = std::max(0.0, (left + region_.bw) - left) * std::max(0.0, (bottom + region_.bh) - bottom);
+            const double dark = L.dark_area[static_cast<std::size_t>(row) * nx_ + col];
+            dark_total_ += dark;
+            placeable_total_ += area - dark;
+        }
+
+    // ---- device ----
+    setDevice();
+    hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "create HIP stream");
+    pos_.allocate(P_, budget_, "particle positions");
+    size_.allocate(P_, budget_, "particle sizes");
+    charge_.allocate(P_, budget_, "particle charge areas");
+    layer_.allocate(P_, budget_, "particle layers");
+    net_off_.allocate(static_cast<std::size_t>(N_) + 1, budget_, "net offsets");
+    net_w_.allocate(N_, budget_, "net weights");
+    small_nets_.allocate(n_small_, budget_, "small-net list");
+    large_nets_.allocate(n_large_, budget_, "large-net list");
+    pin_part_.allocate(I_, budget_, "pin-particle map");
+    pin_off_.allocate(I_, budget_, "pin offsets");
+    pin_grad_.allocate(I_, budget_, "per-pin wirelength gradient");
+    ppin_off_.allocate(static_cast<std::size_t>(P_) + 1, budget_, "particle-pin offsets");
+    ppin_idx_.allocate(ppin_idx.size(), budget_, "particle-pin index");
+    movable_.allocate(G, budget_, "movable density layer");
+    filler_.allocate(G, budget_, "filler density layer");
+    static_charge_.allocate(G, budget_, "static density charge");
+    const std::size_t Gp = static_cast<std::size_t>(nxp_) * nyp_;
+    rho_.allocate(Gp, budget_, "density deviation grid");
+    tmp_.allocate(Gp, budget_, "DCT intermediate");
+    phi_.allocate(Gp, budget_, "potential coefficients");
+    t12_.allocate(2 * Gp, budget_, "inverse DCT intermediate");
+    ex_.allocate(Gp, budget_, "horizontal field");
+    ey_.allocate(Gp, budget_, "vertical field");
+    pot_.allocate(Gp, budget_, "potential grid");
+    bxf_.allocate(static_cast<std::size_t>(nxp_) * nxp_, budget_, "forward x factor");
+    ayf_.allocate(static_cast<std::size_t>(nyp_) * nyp_, budget_, "forward y factor");
+    bxi_.allocate(static_cast<std::size_t>(nxp_) * 2 * nxp_, budget_, "inverse x factors");
+    ayi_.allocate(static_cast<std::size_t>(nyp_) * nyp_, budget_, "inverse y cosine factor");
+    syi_.allocate(static_cast<std::size_t>(nyp_) * nyp_, budget_, "inverse y sine factor");
+    const int max_blocks = std::max({wireBlocks(n_small_, n_large_), metricsBlocks(static_cast<int>(G)),
+                                     particleBlocks(P_), (nxp_ / 32) * (nyp_ / 32), 1});
+    for (int k = 0; k < kCounterCount; ++k) partials_[k].allocate(static_cast<std::size_t>(max_blocks) * 10, budget_, "reduction partials");
+    counters_.allocate(kCounterCount, budget_, "reduction counters");
+    scal_.allocate(kScalarCount, budget_, "scalar block");
+    wire_out_.allocate(P_, budget_, "wire gradient");
+    density_out_.allocate(P_, budget_, "density gradient");
+    hipCheck(hipHostMalloc(reinterpret_cast<void**>(&host_scal_), sizeof(HostScalars)), "allocate pinned scalars");
+
+    std::vector<int> layers(L.particle_layer, L.particle_layer + P_);
+    for (int v : layers)
+        if (v < 0 || v > 2) throw InvalidArgument("unknown particle layer");
+    size_.upload(sizes.data(), P_, stream_, "upload particle sizes");
+    charge_.upload(L.particle_charge_area, P_, stream_, "upload particle charges");
+    layer_.upload(layers.data(), P_, stream_, "upload particle layers");
+    if (N_ > 0) {
+        net_off_.upload(L.net_offsets, static_cast<std::size_t>(N_) + 1, stream_, "upload net offsets");
+        net_w_.upload(L.net_weights, N_, stream_, "upload net weights");
+    }
+    small_nets_.upload(small.data(), small.size(), stream_, "upload small nets");
+    large_nets_.upload(large.data(), large.size(), stream_, "upload large nets");
+    pin_part_.upload(L.pin_particles, I_, stream_, "upload pin-particle map");
+    pin_off_.upload(reinterpret_cast<const double2*>(L.pin_offsets_or_static), I_, stream_, "upload pin offsets");
+    ppin_off_.upload(count.data(), count.size(), stream_, "upload particle-pin offsets");
+    ppin_idx_.upload(ppin_idx.data(), ppin_idx.size(), stream_, "upload particle-pin index");
+    static_charge_.upload(static_charge.data(), G, stream_, "upload static density");
+    const auto fx = basis(nxp_, nxp_, nx_, 0, W);
+    const auto fy = basis(nyp_, nyp_, ny_, 1, H);
+    const auto ix = basis(nxp_, nxp_, nx_, 2, W);
+    const auto sx = basis(nxp_, nxp_, nx_, 3, W);
+    const auto iy = basis(nyp_, nyp_, ny_, 2, H);
+    const auto sy = basis(nyp_, nyp_, ny_, 3, H);
+    // inverse x factors side by side: [k][i] = ix[i][k] (DCT-III), [k][nxp + i] = sx[i][k] (DST-III)
+    std::vector<float> bx2(static_cast<std::size_t>(nxp_) * 2 * nxp_, 0.f);
+    for (int k = 0; k < nxp_; ++k)
+        for (int i = 0; i < nxp_; ++i) {
+            bx2[static_cast<std::size_t>(k) * 2 * nxp_ + i] = ix[static_cast<std::size_t>(i) * nxp_ + k];
+            bx2[static_cast<std::size_t>(k) * 2 * nxp_ + nxp_ + i] = sx[static_cast<std::size_t>(i) * nxp_ + k];
+        }
+    bxf_.upload(fx.data(), fx.size(), stream_, "upload forward x factor");
+    ayf_.upload(fy.data(), fy.size(), stream_, "upload forward y factor");
+    bxi_.upload(bx2.data(), bx2.size(), stream_, "upload inverse x factors");
+    ayi_.upload(iy.data(), iy.size(), stream_, "upload inverse y factor");
+    syi_.upload(sy.data(), sy.size(), stream_, "upload inverse y sine factor");
+    hipCheck(hipMemsetAsync(movable_.get(), 0, G * sizeof(double), stream_), "clear density layer");
+    hipCheck(hipMemsetAsync(filler_.get(), 0, G * sizeof(double), stream_), "clear density layer");
+    hipCheck(hipMemsetAsync(rho_.get(), 0, Gp * sizeof(float), stream_), "clear density grid");
+    hipCheck(hipMemsetAsync(counters_.get(), 0, kCounterCount * sizeof(unsigned), stream_), "clear counters");
+    hipCheck(hipMemsetAsync(scal_.get(), 0, kScalarCount * sizeof(double), stream_), "clear scalars");
+    hipCheck(hipStreamSynchronize(stream_), "finish backend upload");
+}
+
+Backend::~Backend() {
+    if (stream_) {
+        (void)hipSetDevice(device_);
+        (void)hipStreamSynchronize(stream_);
+        for (int c = 0; c < kTimingClasses; ++c) {
+            for (auto e : ev_begin_[c]) (void)hipEventDestroy(e);
+            for (auto e : ev_end_[c]) (void)hipEventDestroy(e);
+        }
+        (void)hipStreamDestroy(stream_);
+    }
+    if (host_scal_) (void)hipHostFree(host_scal_);
+}
+
+void Backend::setDevice() const { hipCheck(hipSetDevice(device_), "select HIP device"); }
+
+void Backend::timedBegin(int cls) {
+    if (!timing_on_) return;
+    hipEvent_t e;
+    hipCheck(hipEventCreate(&e), "create timing event");
+    hipCheck(hipEventRecord(e, stream_), "record timing event");
+    ev_begin_[cls].push_back(e);
+}
+
+void Backend::timedEnd(int cls, double work) {
+    if (!timing_on_) return;
+    pending_work_[cls].push_back(work);
+    hipEvent_t e;
+    hipCheck(hipEventCreate(&e), "create timing event");
+    hipCheck(hipEventRecord(e, stream_), "record timing event");
+    ev_end_[cls].push_back(e);
+}
+
+void Backend::enableTiming(bool on) {
+    timing_on_ = on;
+    if (t_total_.empty()) {
+        t_total_.assign(kTimingClasses, 0.0);
+        t_work_.assign(kTimingClasses, 0.0);
+        t_count_.assign(kTimingClasses, 0);
+    }
+}
+
+std::vector<std::string> Backend::timingNames() const {
+    return {"k_wire", "k_scatter", "k_metrics", "k_gemm", "k_gather", "controller"};
+}
+
+void Backend::timing(std::vector<double>& total_us, std::vector<long long>& launches, std::vector<double>& work) {
+    if (t_total_.empty()) {
+        t_total_.assign(kTimingClasses, 0.0);
+        t_work_.assign(kTimingClasses, 0.0);
+        t_count_.assign(kTimingClasses, 0);
+    }
+    hipCheck(hipStreamSynchronize(stream_), "synchronize for timing");
+    for (int c = 0; c < kTimingClasses; ++c) {
+        for (std::size_t i = 0; i < ev_begin_[c].size() && i < ev_end_[c].size(); ++i) {
+            float ms = 0.f;
+            hipCheck(hipEventElapsedTime(&ms, ev_begin_[c][i], ev_end_[c][i]), "read timing event");
+            t_total_[c] += 1000.0 * ms;
+            t_count_[c] += 1;
+            if (i < pending_work_[c].size()) t_work_[c] += pending_work_[c][i];
+        }
+        pending_work_[c].clear();
+        for (auto e : ev_begin_[c]) (void)hipEventDestroy(e);
+        for (auto e : ev_end_[c]) (void)hipEventDestroy(e);
+        ev_begin_[c].clear();
+        ev_end_[c].clear();
+    }
+    total_us = t_total_;
+    launches = t_count_;
+    work = t_work_;
+}
+
+void Backend::enqueueEvaluate(const double2* pos, double gamma, bool want_grad, const SessionOut* session) {
+    hipStream_t s = stream_;
+    double* scal = scal_.get();
+    WireArgs wa{pos, net_off_.get(), net_w_.get(), pin_part_.get(), pin_off_.get(), small_nets_.get(), n_small_,
+                large_nets_.get(), n_large_, gamma, want_grad ? 1 : 0, pin_grad_.get(), partials_[kCtrWire].get(),
+                counters_.get() + kCtrWire, scal};
+    timedBegin(kTWire);
+    if (n_small_ + n_large_ > 0) launchWire(wa, s);
+    else launchZero(scal + kHpwl, 2, s);
+    // algorithmic bytes: 12 per pin, 8 per net, 16 (8 value-only) per particle
+    timedEnd(kTWire, 12.0 * I_ + 8.0 * N_ + (want_grad ? 16.0 : 8.0) * P_);
+
+    ScatterArgs sa{pos, size_.get(), layer_.get(), P_, region_, t_, movable_.get(), filler_.get()};
+    timedBegin(kTScatter);
+    launchScatter(sa, s);
+    timedEnd(kTScatter, 20.0 * P_ + 12.0 * static_cast<double>(nx_) * ny_);
+    MetricsArgs ma{movable_.get(), filler_.get(), static_charge_.get(), region_, t_, dark_total_, placeable_total_,
+                   rho_.get(), nxp_, partials_[kCtrMetrics].get(), counters_.get() + kCtrMetrics, scal};
+    timedBegin(kTMetrics);
+    launchMetrics(ma, s);
+    timedEnd(kTMetrics, 24.0 * static_cast<double>(nx_) * ny_);
+
+    const double W = region_.urx - region_.llx, H = region_.ury - region_.lly;
+    SpectralArgs sp{nx_, ny_, W, H, 1.0 / (4.0 * nx_ * static_cast<double>(ny_)), 0.5 * region_.bw * region_.bh,
+                    partials_[kCtrEnergy].get(), counters_.get() + kCtrEnergy, scal};
+    auto gemm = [&](const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epi) {
+        timedBegin(kTDct);
+        launchGemm(g, batch, M, N, K, lda, ldb, ldc, epi, sp, dct_precision_, s);
+        timedEnd(kTDct, 2.0 * M * static_cast<double>(N) * K * batch);
+    };
+    // forward: T = rho * Bx ; chat = Ay * T -> phihat (+ energy)
+    GemmBatch f1{{rho_.get()}, {bxf_.get()}, {tmp_.get()}};
+    gemm(f1, 1, nyp_, nxp_, nxp_, nxp_, nxp_, nxp_, kEpiStore);
+    GemmBatch f2{{ayf_.get()}, {tmp_.get()}, {phi_.get()}};
+    gemm(f2, 1, nyp_, nxp_, nyp_, nyp_, nxp_, nxp_, kEpiSpectral);
+    if (want_grad) {
+        // [T1 | T2] = phihat * [Cx_inv | Sx_inv] ; Ex = Ay_inv * T2 ; Ey = Sy_inv * T1
+        GemmBatch i1{{phi_.get()}, {bxi_.get()}, {t12_.get()}};
+        gemm(i1, 1, nyp_, 2 * nxp_, nxp_, nxp_, 2 * nxp_, 2 * nxp_, kEpiStore);
+        GemmBatch i2{{ayi_.get(), syi_.get()}, {t12_.get() + nxp_, t12_.get()}, {ex_.get(), ey_.get()}};
+        gemm(i2, 2, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiStore);
+    }
+    have_fields_ = want_grad;
+
+    if (!want_grad) return;
+    GatherArgs ga{};
+    ga.pos = pos;
+    ga.charge = charge_.get();
+    ga.ex = ex_.get();
+    ga.ey = ey_.get();
+    ga.ldf = nxp_;
+    ga.r = region_;
+    ga.ppin_off = ppin_off_.get();
+    ga.ppin_idx = ppin_idx_.get();
+    ga.pin_grad = pin_grad_.get();
+    ga.n = P_;
+    if (session) {
+        ga.precond = session->precond;
+        ga.penalty = session->penalty;
+        ga.inv_strength = session->inv_strength;
+        ga.raw_out = session->raw;
+        ga.pre_out = session->pre;
+    } else {
+        ga.wire_out = wire_out_.get();
+        ga.density_out = density_out_.get();
+    }
+    ga.partials = partials_[kCtrGather].get();
+    ga.counter = counters_.get() + kCtrGather;
+    ga.scal = scal;
+    timedBegin(kTGather);
+    launchGather(ga, s);
+    timedEnd(kTGather, 20.0 * P_ + 8.0 * static_cast<double>(nx_) * ny_);
+}
+
+const HostScalars& Backend::fetchScalars() {
+    hipCheck(hipMemcpyAsync(host_scal_, scal_.get(), sizeof(HostScalars), hipMemcpyDeviceToHost, stream_),
+             "read scalar block");
+    hipCheck(hipStreamSynchronize(stream_), "synchronize placement evaluation");
+    return *host_scal_;
+}
+
+void Backend::toScalars(const HostScalars& h, mp_gp_scalars* out) const {
+    out->hpwl = h.v[kHpwl];
+    out->smooth_wirelength = h.v[kSmooth];
+    out->electrostatic_energy = h.v[kEnergy];
+    auto fill = [&](int base, mp_gp_density_metrics& m) {
+        m.total_overflow_area = h.v[base + 0];
+        m.normalized_overflow = h.v[base + 1];
+        m.maximum_density = h.v[base + 2];
+        m.average_density = h.v[base + 3];
+        m.total_charge_area = h.v[base + 4];
+        m.normalization_area = h.v[base + 5];
+        m.placeable_area = h.v[base + 6];
+        m.dark_area = h.v[base + 7];
+    };
+    fill(kOptDensity, out->optimizer_density);
+    fill(kDesignDensity, out->design_density);
+}
+
+void Backend::evaluateHost(const double* xy, double gamma, bool want_grad, mp_gp_scalars* out, double* wire_grad,
+                           double* density_grad) {
+    if (!std::isfinite(gamma) || gamma <= 0.0)
+        throw InvalidArgument("HIP global placement requires positive finite smoothing.");
+    if (P_ > 0 && xy == nullptr) throw InvalidArgument("HIP placement positions do not match the particle count.");
+    if (want_grad && P_ > 0 && (wire_grad == nullptr || density_grad == nullptr))
+        throw InvalidArgument("gradient outputs are required when want_grad is set");
+    setDevice();
+    pos_.upload(reinterpret_cast<const double2*>(xy), P_, stream_, "upload particle positions");
+    enqueueEvaluate(pos_.get(), gamma, want_grad, nullptr);
+    if (want_grad && P_ > 0) {
+        hipCheck(hipMemcpyAsync(wire_grad, wire_out_.get(), P_ * sizeof(double2), hipMemcpyDeviceToHost, stream_),
+                 "download wire gradients");
+        hipCheck(hipMemcpyAsync(density_grad, density_out_.get(), P_ * sizeof(double2), hipMemcpyDeviceToHost, stream_),
+                 "download density gradients");
+    }
+    const HostScalars& h = fetchScalars();
+    toScalars(h, out);
+    if (!std::isfinite(out->smooth_wirelength) || !std::isfinite(out->hpwl) || !std::isfinite(out->electrostatic_energy))
+        throw std::runtime_error("HIP global-placement evaluation produced a non-finite metric.");
+}
+
+void Backend::debugGrids(double* source, double* potential, double* field_x, double* field_y) {
+    setDevice();
+    if (!have_fields_) throw std::runtime_error("no gradient evaluation has produced field grids yet");
+    const std::size_t Gp = static_cast<std::size_t>(nxp_) * nyp_;
+    if (potential) {
+        SpectralArgs sp{};
+        GemmBatch g{{ayi_.get()}, {t12_.get()}, {pot_.get()}};
+        launchGemm(g, 1, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiStore, sp, dct_precision_, stream_);
+    }
+    std::vector<float> buf(Gp);
+    auto pull = [&](const DeviceArray<float>& a, double* dst) {
+        if (!dst) return;
+        hipCheck(hipMemcpyAsync(buf.data(), a.get(), Gp * sizeof(float), hipMemcpyDeviceToHost, stream_), "read grid");
+        hipCheck(hipStreamSynchronize(stream_), "read grid");
+        for (int r = 0; r < ny_; ++r)
+            for (int c = 0; c < nx_; ++c)
+                dst[static_cast<std::size_t>(r) * nx_ + c] = buf[static_cast<std::size_t>(r) * nxp_ + c];
+    };
+    pull(rho_, source);
+    if (source) {  // centre it as ElectrostaticField::solve does
+        double mean = 0.0;
+        for (std::size_t i = 0; i < static_cast<std::size_t>(nx_) * ny_; ++i) mean += source[i];
+        mean /= static_cast<double>(nx_) * ny_;
+        for (std::size_t i = 0; i < static_cast<std::size_t>(nx_) * ny_; ++i) source[i] -= mean;
+    }
+    pull(pot_, potential);
+    pull(ex_, field_x);
+    pull(ey_, field_y);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Shared C-ABI plumbing.
+thread_local std::string g_last_error;
+
+mp_gp_status translate(const std::exception_ptr& e) {
+    try {
+        std::rethrow_exception(e);
+    } catch (const std::invalid_argument& x) {
+        g_last_error = x.what();
+        return MP_GP_INVALID_ARGUMENT;
+    } catch (const std::out_of_range& x) {
+        g_last_error = x.what();
+        return MP_GP_INVALID_ARGUMENT;
+    } catch (const std::exception& x) {
+        g_last_error = x.what();
+        return MP_GP_RUNTIME_ERROR;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return MP_GP_RUNTIME_ERROR;
+    }
+}
+
+// Device discovery and allocation limit (.cu:1156-1207), HIP device policy: any present device.
+std::size_t allocationLimit(int device, std::uint64_t maximum_bytes, std::string& reason) {
+    int count = 0;
+    const hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        reason = std::string("HIP device discovery failed: ") + (e == hipSuccess ? "no device" : hipGetErrorString(e));
+        return 0;
+    }
+    if (device < 0 || device >= count) {
+        reason = "Requested HIP device does not exist on this host.";
+        return 0;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        reason = "Unable to select the requested HIP device.";
+        return 0;
+    }
+    std::size_t free_bytes = 0, total_bytes = 0;
+    if (hipMemGetInfo(&free_bytes, &total_bytes) != hipSuccess) {
+        reason = "Unable to query free HIP memory.";
+        return 0;
+    }
+    if (free_bytes <= kDeviceReserveBytes) {
+        reason = "Requested HIP device does not have the required 4 GiB memory reserve.";
+        return 0;
+    }
+    const std::size_t limit = std::min<std::size_t>(maximum_bytes, free_bytes - kDeviceReserveBytes);
+    if (limit == 0) reason = "HIP memory limit leaves no allocatable device memory.";
+    return limit;
+}
+
+}  // namespace mpgp
+
+using namespace mpgp;
+
+extern "C" {
+
+int mp_gp_compiled(void) { return 1; }
+
+const char* mp_gp_last_error(void) { return g_last_error.c_str(); }
+
+mp_gp_status mp_gp_create(const mp_gp_static* layout, mp_gp_backend** out) {
+    g_last_error.clear();
+    if (out == nullptr || layout == nullptr) {
+        g_last_error = "null layout or output handle";
+        return MP_GP_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    if (layout->maximum_device_bytes == 0) {
+        g_last_error = "HIP global placement requires a non-zero memory budget.";
+        return MP_GP_INVALID_ARGUMENT;
+    }
+    std::string reason;
+    const std::size_t limit = allocationLimit(layout->device, layout->maximum_device_bytes, reason);
+    if (limit == 0) {
+        g_last_error = reason;
+        return MP_GP_UNAVAILABLE;
+    }
+    try {
+        auto h = std::make_unique<mp_gp_backend>();
+        h->owned = std::make_unique<Backend>(*layout, limit);
+        h->impl = h->owned.get();
+        *out = h.release();
+        return MP_GP_OK;
+    } catch (const std::exception& e) {
+        // the factory turns every construction failure into "unavailable + reason" (.cu:1209-1215)
+        g_last_error = e.what();
+        return MP_GP_UNAVAILABLE;
+    }
+}
+
+mp_gp_status mp_gp_evaluate(mp_gp_backend* b, const double* xy, double smoothing, int want_grad, mp_gp_scalars* out,
+                            double* wire_grad, double* density_grad) {
+    g_last_error.clear();
+    if (b == nullptr || out == nullptr) {
+        g_last_error = "null backend or output";
+        return MP_GP_INVALID_ARGUMENT;
+    }
+    try {
+        b->impl->evaluateHost(xy, smoothing, want_grad != 0, out, wire_grad, density_grad);
+        return MP_GP_OK;
+    } catch (...) {
+        return translate(std::current_exception());
+    }
+}
+
+int mp_gp_device(const mp_gp_backend* b) { return b ? b->impl->device() : -1; }
+
+uint64_t mp_gp_reserved_bytes(const mp_gp_backend* b) { return b ? b->impl->reservedBytes() : 0; }
+
+void mp_gp_destroy(mp_gp_backend* b) {
+    if (b && b->owned) delete b;  // session views are released with their session
+}
+
+mp_gp_status mp_gp_debug_grids(mp_gp_backend* b, double* source, double* potential, double* field_x, double* field_y) {
+    g_last_error.clear();
+    if (b == nullptr) {
+        g_last_error = "null backend";
+        return MP_GP_INVALID_ARGUMENT;
+    }
+    try {
+        b->impl->debugGrids(source, potential, field_x, field_y);
+        return MP_GP_OK;
+    } catch (...) {
+        return translate(std::current_exception());
+    }
+}
+
+}  // extern "C"

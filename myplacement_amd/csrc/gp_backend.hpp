@@ -1,0 +1,168 @@
+// gp_backend.hpp — the device-resident numeric backend behind the C ABI.
+//
+// One Backend owns every device buffer of one placement problem (the role of
+// CudaPlacementBackendImpl, src/placement/cuda/CudaPlacementBackend.cu:841-1148).  Static data is
+// uploaded once; an evaluation enqueues its kernels on the backend's stream and the host reads one
+// 256-byte scalar block back.
+#pragma once
+
+#include "mp_gp_backend.h"
+#include "gp_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <stdexcept>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace mpgp {
+
+class InvalidArgument : public std::invalid_argument {
+public:
+    using std::invalid_argument::invalid_argument;
+};
+
+void hipCheck(hipError_t e, const char* what);
+
+// Allocation accounting against a byte limit (DeviceMemoryBudget, .cu:60-79).
+class DeviceBudget {
+public:
+    explicit DeviceBudget(std::size_t limit) : limit_(limit) {}
+    void reserve(std::size_t bytes, const char* what);
+    std::size_t reserved() const { return reserved_; }
+
+private:
+    std::size_t limit_;
+    std::size_t reserved_ = 0;
+};
+
+template <typename T>
+class DeviceArray {
+public:
+    DeviceArray() = default;
+    ~DeviceArray() {
+        if (p_) (void)hipFree(p_);
+    }
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    void allocate(std::size_t n, DeviceBudget& budget, const char* what) {
+        if (n == 0) n = 1;  // keep a valid pointer for empty sets
+        budget.reserve(n * sizeof(T), what);
+        hipCheck(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)), what);
+        n_ = n;
+    }
+    void upload(const T* host, std::size_t n, hipStream_t s, const char* what) {
+        if (n > n_) throw std::logic_error("device upload exceeds buffer");
+        if (n) hipCheck(hipMemcpyAsync(p_, host, n * sizeof(T), hipMemcpyHostToDevice, s), what);
+    }
+    T* get() const { return p_; }
+    std::size_t size() const { return n_; }
+    void swapWith(DeviceArray& o) {
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+    }
+
+private:
+    T* p_ = nullptr;
+    std::size_t n_ = 0;
+};
+
+struct HostScalars {
+    double v[kScalarCount];
+};
+
+class Backend {
+public:
+    Backend(const mp_gp_static& layout, std::size_t allocation_limit);
+    ~Backend();
+    Backend(const Backend&) = delete;
+    Backend& operator=(const Backend&) = delete;
+
+    // Enqueues one evaluation at device positions `pos`.  With `session` set, the gather writes the
+    // combined raw / preconditioned gradient instead of the two separate gradients.
+    struct SessionOut {
+        const double* precond;
+        double penalty, inv_strength;
+        double2* raw;
+        double2* pre;
+    };
+    void enqueueEvaluate(const double2* pos, double gamma, bool want_grad, const SessionOut* session);
+    // Copies the scalar block to host (pinned) and waits for the stream.
+    const HostScalars& fetchScalars();
+    void toScalars(const HostScalars& h, mp_gp_scalars* out) const;
+
+    // Reference-shaped blocking evaluate (CudaPlacementBackend::evaluate).
+    void evaluateHost(const double* xy, double gamma, bool want_grad, mp_gp_scalars* out, double* wire_grad,
+                      double* density_grad);
+    void debugGrids(double* source, double* potential, double* field_x, double* field_y);
+
+    int device() const { return device_; }
+    std::size_t reservedBytes() const { return budget_.reserved(); }
+    hipStream_t stream() const { return stream_; }
+    int particles() const { return P_; }
+    const Region& region() const { return region_; }
+    DeviceBudget& budget() { return budget_; }
+    const double2* sizes() const { return size_.get(); }
+    double* scalarBlock() const { return scal_.get(); }
+    double* partials(int which) const { return partials_[which].get(); }
+    unsigned* counter(int which) const { return counters_.get() + which; }
+
+    // Optional per-kernel timing with HIP events on the backend's stream.
+    void enableTiming(bool on);
+    std::vector<std::string> timingNames() const;
+    void timing(std::vector<double>& total_us, std::vector<long long>& launches, std::vector<double>& work);
+    void timedBegin(int cls);
+    void timedEnd(int cls, double work = 0.0);
+
+private:
+    void setDevice() const;
+
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    int P_ = 0, N_ = 0, I_ = 0;
+    int nx_ = 0, ny_ = 0, nxp_ = 0, nyp_ = 0;
+    Region region_{};
+    double t_ = 0.0;
+    double dark_total_ = 0.0, placeable_total_ = 0.0;
+    int dct_precision_ = 0;
+    bool have_fields_ = false;
+    DeviceBudget budget_;
+
+    DeviceArray<double2> pos_, size_;
+    DeviceArray<double> charge_;
+    DeviceArray<int> layer_;
+    DeviceArray<int> net_off_, small_nets_, large_nets_, pin_part_, ppin_off_, ppin_idx_;
+    int n_small_ = 0, n_large_ = 0;
+    DeviceArray<double> net_w_;
+    DeviceArray<double2> pin_off_;
+    DeviceArray<float2> pin_grad_;
+    DeviceArray<double> movable_, filler_, static_charge_;
+    DeviceArray<float> rho_, tmp_, phi_, t12_, ex_, ey_, pot_;
+    DeviceArray<float> bxf_, ayf_, bxi_, ayi_, syi_;
+    DeviceArray<double> partials_[kCounterCount];
+    DeviceArray<unsigned> counters_;
+    DeviceArray<double> scal_;
+    DeviceArray<double2> wire_out_, density_out_;
+    HostScalars* host_scal_ = nullptr;
+
+    // timing
+    static constexpr int kTimingClasses = 6;
+    bool timing_on_ = false;
+    std::vector<hipEvent_t> ev_begin_[kTimingClasses], ev_end_[kTimingClasses];
+    std::vector<double> t_total_, t_work_;
+    std::vector<double> pending_work_[kTimingClasses];
+    std::vector<long long> t_count_;
+};
+
+enum TimingClass : int { kTWire = 0, kTScatter = 1, kTMetrics = 2, kTDct = 3, kTGather = 4, kTController = 5 };
+
+}  // namespace mpgp
+
+// C handle of a backend: owns it when made by mp_gp_create, views it when taken from a session.
+struct mp_gp_backend {
+    std::unique_ptr<mpgp::Backend> owned;
+    mpgp::Backend* impl = nullptr;
+};

@@ -1,0 +1,712 @@
+// gp_session.cpp — the adaptive Nesterov / backtracking controller of runAdaptiveGlobalPlacement
+// (src/placement/AdaptiveGlobalPlacer.cpp:413-953) driving the HIP backend with every O(P) vector
+// resident on the device.  The scalar control law runs on the host in FP64 exactly as the reference
+// writes it; each decision reads one small scalar block back from the device.
+//
+// Device vectors (double2[P]): X current positions, B look-ahead base, Y proposal, V velocity,
+// PX / PG previous base positions and preconditioned gradient, Rc/Gc and Rl/Gl raw and
+// preconditioned gradients of the current and look-ahead evaluations, two checkpoint copies.
+//
+// One difference in sequencing, not in result: a proposal and its value-only evaluation are
+// enqueued together and read back with one synchronisation; when the predicted reduction is
+// below 1e-9 (AdaptiveGlobalPlacer.cpp:823-826) that evaluation is discarded unread.
+
+#include "gp_backend.hpp"
+#include "gp_host.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <memory>
+#include <utility>
+#include <vector>
+
+namespace mpgp {
+
+extern thread_local std::string g_last_error;
+mp_gp_status translate(const std::exception_ptr& e);
+std::size_t allocationLimit(int device, std::uint64_t maximum_bytes, std::string& reason);
+
+namespace {
+
+constexpr double kEps = 1e-9;
+double clampd(double v, double lo, double hi) { return std::max(lo, std::min(v, hi)); }
+
+struct EvalRec {
+    mp_gp_scalars s{};
+    double objective = 0.0;
+    double pre_norm = 0.0;
+    double wire_l1 = 0.0, density_l1 = 0.0;
+};
+
+struct Checkpoint {
+    int iteration = 0;
+    double hpwl = std::numeric_limits<double>::infinity();
+    double overflow = std::numeric_limits<double>::infinity();
+    double score = std::numeric_limits<double>::infinity();
+    bool found = false;
+};
+
+void validateOptions(const mp_gp_options& o) {  // AdaptiveGlobalPlacer.cpp:413-454 (non-RUDY part)
+    if (o.iterations <= 0 || o.bins_x < 2 || o.bins_y < 2 || o.target_density <= 0.0 || o.target_density > 1.0 ||
+        o.maximum_bin_overflow < 0.0 || o.initial_smoothing <= 0.0 || o.final_smoothing <= 0.0 ||
+        o.maximum_movement_in_bins <= 0.0 || o.initial_movement_in_bins <= 0.0 || o.maximum_backtracks < 0 ||
+        o.backtracking_ratio <= 0.0 || o.backtracking_ratio >= 1.0 || o.armijo_coefficient <= 0.0 ||
+        o.armijo_coefficient >= 1.0 || o.maximum_momentum < 0.0 || o.maximum_momentum >= 1.0 ||
+        o.density_penalty_increase <= 1.0 || o.density_penalty_decrease <= 0.0 || o.density_penalty_decrease >= 1.0 ||
+        o.minimum_density_penalty <= 0.0 || o.maximum_density_penalty < o.minimum_density_penalty ||
+        o.smoothing_overflow_exponent <= 0.0 || o.feasible_refinement_iterations <= 0 ||
+        !std::isfinite(o.target_density) || !std::isfinite(o.maximum_bin_overflow) ||
+        !std::isfinite(o.initial_smoothing) || !std::isfinite(o.final_smoothing) ||
+        !std::isfinite(o.maximum_density_penalty))
+        throw std::invalid_argument("Adaptive global placement options are outside their valid range.");
+    if (static_cast<std::size_t>(o.bins_x) > (std::size_t{1} << 20) / static_cast<std::size_t>(o.bins_y))
+        throw std::invalid_argument("Density grid exceeds the safe one-million-bin workspace limit.");
+}
+
+}  // namespace
+
+class Session {
+public:
+    Session(std::shared_ptr<const Prepared> prep, const mp_gp_options& o, const double* centers)
+        : prep_(std::move(prep)), o_(o) {
+        validateOptions(o_);
+        const Prepared& p = *prep_;
+        std::string reason;
+        const std::size_t limit = allocationLimit(o_.device, o_.maximum_device_bytes, reason);
+        if (limit == 0) throw std::runtime_error("HIP global-placement backend is unavailable: " + reason);
+        const mp_gp_static L = p.layout(o_.device, o_.dct_precision, o_.maximum_device_bytes);
+        be_ = std::make_unique<Backend>(L, limit);
+        P_ = be_->particles();
+        const std::size_t Mv = p.net.movable.size();
+        std::vector<double2> x(P_);
+        for (std::size_t i = 0; i < Mv; ++i) {
+            const int64_t m = p.net.movable[i];
+            x[i] = centers ? make_double2(centers[2 * m], centers[2 * m + 1]) : make_double2(p.net.cx[m], p.net.cy[m]);
+        }
+        for (std::size_t i = 0; i < p.filler_x.size(); ++i) x[Mv + i] = make_double2(p.filler_x[i], p.filler_y[i]);
+        DeviceBudget& b = be_->budget();
+        for (DeviceArray<double2>* a : {&X_, &B_, &Y_, &V_, &PX_, &PG_, &Rc_, &Gc_, &Rl_, &Gl_, &CKf_, &CKb_})
+            a->allocate(P_, b, "controller vector");
+        precond_.allocate(P_, b, "preconditioner");
+        hipStream_t s = be_->stream();
+        X_.upload(x.data(), P_, s, "upload positions");
+        precond_.upload(p.precond.data(), P_, s, "upload preconditioner");
+        launchZero(reinterpret_cast<double*>(V_.get()), 2 * static_cast<std::size_t>(P_), s);
+        min_bin_ = std::min(p.bw, p.bh);
+        max_bin_ = std::max(p.bw, p.bh);
+        clamp_ = ClampArgs{be_->sizes(), be_->region()};
+        const auto t0 = std::chrono::steady_clock::now();
+        initialize();
+        setup_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    Backend& backend() { return *be_; }
+    bool stopped() const { return stopped_; }
+    const std::vector<mp_gp_history_row>& history() const { return history_; }
+
+    int step(int iterations) {
+        const auto t0 = std::chrono::steady_clock::now();
+        int done = 0;
+        while (done < iterations && !stopped_) {
+            iterate();
+            ++done;
+        }
+        loop_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return done;
+    }
+
+    void positions(double* xy) {
+        hipCheck(hipSetDevice(o_.device), "select HIP device");
+        hipCheck(hipMemcpyAsync(xy, X_.get(), P_ * sizeof(double2), hipMemcpyDeviceToHost, be_->stream()),
+                 "download positions");
+        hipCheck(hipStreamSynchronize(be_->stream()), "download positions");
+    }
+
+    void finish(double* centers, mp_gp_result* r) {
+        hipCheck(hipSetDevice(o_.device), "select HIP device");
+        const Prepared& p = *prep_;
+        const Checkpoint& chosen = feasible_.found ? feasible_ : fallback_;
+        std::vector<double2> pos(P_);
+        const DeviceArray<double2>& src = feasible_.found ? CKf_ : (fallback_.found ? CKb_ : X_);
+        hipCheck(hipMemcpyAsync(pos.data(), src.get(), P_ * sizeof(double2), hipMemcpyDeviceToHost, be_->stream()),
+                 "download checkpoint");
+        hipCheck(hipStreamSynchronize(be_->stream()), "download checkpoint");
+        std::vector<double> cx = p.net.cx, cy = p.net.cy;
+        for (std::size_t i = 0; i < p.net.movable.size(); ++i) {
+            const int64_t m = p.net.movable[i];
+            cx[m] = pos[i].x;
+            cy[m] = pos[i].y;
+        }
+        if (centers)
+            for (std::size_t m = 0; m < p.net.M; ++m) {
+                centers[2 * m] = cx[m];
+                centers[2 * m + 1] = cy[m];
+            }
+        result_.restored_best_checkpoint = chosen.found ? 1 : 0;
+        if (chosen.found) {
+            result_.best_checkpoint_iteration = chosen.iteration;
+            result_.best_checkpoint_hpwl = chosen.hpwl;
+            result_.best_checkpoint_overflow = chosen.overflow;
+        }
+        result_.hpwl_after = p.net.hpwl(cx.data(), cy.data());
+        result_.overflow_after = p.net.designOverflow(cx.data(), cy.data(), p.nx, p.ny, p.t);
+        result_.loop_seconds = loop_seconds_;
+        result_.elapsed_seconds = setup_seconds_ + loop_seconds_;
+        result_.particle_count = P_;
+        result_.filler_count = static_cast<int64_t>(p.filler_x.size());
+        result_.device_used = be_->device();
+        result_.reserved_device_bytes = be_->reservedBytes();
+        if (r) *r = result_;
+    }
+
+    mp_gp_result& result() { return result_; }
+
+private:
+    // ---- evaluation helpers -----------------------------------------------------------------
+    double strength() const { return std::sqrt(1.0 + std::max(0.0, penalty_ / std::max(reference_penalty_, kEps))); }
+
+    void enqueueGrad(const double2* pos, double2* raw, double2* pre) {
+        Backend::SessionOut so{precond_.get(), penalty_, 1.0 / strength(), raw, pre};
+        be_->enqueueEvaluate(pos, smoothing_, true, &so);
+        ++result_.gradient_evaluations;
+    }
+    void enqueueValue(const double2* pos) {
+        be_->enqueueEvaluate(pos, smoothing_, false, nullptr);
+        ++result_.value_evaluations;
+    }
+    EvalRec record(const HostScalars& h, bool grad) {
+        EvalRec e;
+        be_->toScalars(h, &e.s);
+        e.objective = e.s.smooth_wirelength + penalty_ * e.s.electrostatic_energy;
+        if (!std::isfinite(e.objective) || !std::isfinite(e.s.hpwl))
+            throw std::runtime_error("HIP adaptive placement objective produced a non-finite value.");
+        if (grad) {
+            e.pre_norm = std::sqrt(h.v[kPreNorm2]);
+            e.wire_l1 = h.v[kWireL1];
+            e.density_l1 = h.v[kDensityL1];
+        }
+        return e;
+    }
+    EvalRec gradEval(const double2* pos, double2* raw, double2* pre) {
+        enqueueGrad(pos, raw, pre);
+        return record(be_->fetchScalars(), true);
+    }
+    double* scal() { return be_->scalarBlock(); }
+    void copy(DeviceArray<double2>& dst, const double2* src) {
+        if (dst.get() == src) return;
+        hipCheck(hipMemcpyAsync(dst.get(), src, P_ * sizeof(double2), hipMemcpyDeviceToDevice, be_->stream()),
+                 "copy controller vector");
+    }
+
+    // ---- control law (AdaptiveGlobalPlacer.cpp:456-596) ------------------------------------------
+    double smoothingFromOverflow(double overflow) const {
+        const double ratio = clampd(overflow / std::max(initial_overflow_, kEps), 0.0, 1.0);
+        const double annealing = std::pow(ratio, o_.smoothing_overflow_exponent);
+        return (o_.final_smoothing + (o_.initial_smoothing - o_.final_smoothing) * annealing) * max_bin_;
+    }
+    bool accepts(const EvalRec& base, const EvalRec& cand, double predicted) const {
+        const double tol = std::max(1e-8, std::abs(base.objective) * 1e-10);
+        const double armijo_target = base.objective - o_.armijo_coefficient * std::max(0.0, predicted);
+        const bool armijo = cand.objective <= armijo_target + tol;
+        const double bo = base.s.optimizer_density.normalized_overflow, co = cand.s.optimizer_density.normalized_overflow;
+        const bool stable = cand.objective <= base.objective + tol && co <= bo * 1.005;
+        const double gain = std::max(1e-4, o_.density_stall_threshold * 0.25);
+        const bool trade = co <= bo - gain && cand.objective <= base.objective * (1.0 + o_.objective_increase_for_density);
+        return armijo || stable || trade;
+    }
+    double updatePenalty(double penalty, const EvalRec& before, const EvalRec& after) const {
+        const double growth = (after.s.hpwl - before.s.hpwl) / std::max(before.s.hpwl, 1.0);
+        const double ao = after.s.optimizer_density.normalized_overflow;
+        const double improvement = before.s.optimizer_density.normalized_overflow - ao;
+        const bool critical = ao > std::max(2.0 * o_.maximum_bin_overflow, 0.02);
+        if (critical && improvement < o_.density_stall_threshold) penalty *= o_.density_penalty_increase;
+        else if (!critical && growth > o_.hpwl_growth_restart_threshold && improvement < o_.density_stall_threshold)
+            penalty *= o_.density_penalty_decrease;
+        else if (ao > o_.maximum_bin_overflow && improvement < o_.density_stall_threshold)
+            penalty *= o_.density_penalty_increase;
+        else if (after.s.design_density.normalized_overflow <= o_.maximum_bin_overflow)
+            penalty *= o_.density_penalty_decrease;
+        const double lo = std::max(o_.minimum_density_penalty, reference_penalty_ * 0.02);
+        const double hi = std::max(o_.maximum_density_penalty, lo * 10.0);
+        return clampd(penalty, lo, hi);
+    }
+    void considerCheckpoint(const EvalRec& e, int iteration, bool& is_new) {
+        is_new = false;
+        const double ovf = e.s.design_density.normalized_overflow;
+        const double score = e.s.hpwl;
+        if (ovf <= o_.maximum_bin_overflow &&
+            (!feasible_.found || score < feasible_.score - 1e-10 ||
+             (std::abs(score - feasible_.score) <= 1e-10 && e.s.hpwl < feasible_.hpwl))) {
+            copy(CKf_, X_.get());
+            feasible_ = {iteration, e.s.hpwl, ovf, score, true};
+            is_new = true;
+        }
+        if (!feasible_.found && (!fallback_.found || ovf < fallback_.overflow - 1e-12 ||
+                                 (std::abs(ovf - fallback_.overflow) <= 1e-12 && e.s.hpwl < fallback_.hpwl))) {
+            copy(CKb_, X_.get());
+            fallback_ = {iteration, e.s.hpwl, ovf, score, true};
+            is_new = true;
+        }
+    }
+    mp_gp_history_row row(int it, const EvalRec& e, double penalty, double step, double disp, double gnorm,
+                          double curvature, int backtracks, bool restarted, bool accepted, bool best) const {
+        mp_gp_history_row r{};
+        r.iteration = it;
+        r.hpwl = e.s.hpwl;
+        r.overflow = e.s.optimizer_density.normalized_overflow;
+        r.design_overflow = e.s.design_density.normalized_overflow;
+        r.smooth_wirelength = e.s.smooth_wirelength;
+        r.density_energy = e.s.electrostatic_energy;
+        r.objective = e.objective;
+        r.penalty = penalty;
+        r.smoothing = smoothing_;
+        r.step_size = step;
+        r.maximum_displacement = disp;
+        r.gradient_norm = gnorm;
+        r.curvature = curvature;
+        r.backtracks = backtracks;
+        r.momentum_restarted = restarted;
+        r.accepted = accepted;
+        r.best_checkpoint = best;
+        return r;
+    }
+
+    // :672-733
+    void initialize() {
+        smoothing_ = o_.initial_smoothing * max_bin_;
+        penalty_ = 1.0;
+        reference_penalty_ = 1.0;
+        EvalRec init = gradEval(X_.get(), Rc_.get(), Gc_.get());
+        initial_overflow_ = std::max(init.s.optimizer_density.normalized_overflow, kEps);
+        smoothing_ = smoothingFromOverflow(init.s.optimizer_density.normalized_overflow);
+        const double raw_penalty = init.density_l1 > kEps ? init.wire_l1 / init.density_l1 : 1.0;
+        penalty_ = clampd(raw_penalty, o_.minimum_density_penalty, o_.maximum_density_penalty);
+        reference_penalty_ = penalty_;
+        init = gradEval(X_.get(), Rc_.get(), Gc_.get());
+        last_overflow_ = init.s.optimizer_density.normalized_overflow;
+        const double rms = init.pre_norm / std::sqrt(static_cast<double>(std::max(1, P_)));
+        last_step_ = o_.initial_movement_in_bins * min_bin_ / std::max(rms, kEps);
+        bool dummy = false;
+        considerCheckpoint(init, 0, dummy);
+        feasible_refinement_ = init.s.design_density.normalized_overflow <= o_.maximum_bin_overflow ? 1 : 0;
+        current_ = init;
+    }
+
+    // one pass of the loop body :735-928
+    void iterate() {
+        hipStream_t s = be_->stream();
+        const int it = result_.completed_iterations;
+        if (it > 0) {
+            smoothing_ = smoothingFromOverflow(last_overflow_);
+            current_ = gradEval(X_.get(), Rc_.get(), Gc_.get());
+        }
+        const EvalRec& cur = current_;
+        bool restarted = worse_streak_ >= 2;
+        bool momentum_used = false;
+        const double2* base = X_.get();
+        const double2* base_raw = Rc_.get();
+        const double2* base_pre = Gc_.get();
+        const EvalRec* base_eval = &cur;
+        EvalRec look;
+        if (!restarted && momentum_age_ > 0) {
+            const double beta = std::min(o_.maximum_momentum, static_cast<double>(momentum_age_) / (momentum_age_ + 3));
+            be_->timedBegin(kTController);
+            launchDot(V_.get(), Gc_.get(), -1.0, kAlignment, P_, be_->partials(kCtrDot), be_->counter(kCtrDot), scal(), s);
+            be_->timedEnd(kTController);
+            const double alignment = be_->fetchScalars().v[kAlignment];
+            if (alignment <= 0.0) {
+                restarted = true;
+            } else {
+                be_->timedBegin(kTController);
+                launchLookahead(X_.get(), V_.get(), beta, B_.get(), clamp_, P_, s);
+                be_->timedEnd(kTController);
+                look = gradEval(B_.get(), Rl_.get(), Gl_.get());
+                if (look.objective > cur.objective * (1.0 + o_.hpwl_growth_restart_threshold)) {
+                    restarted = true;
+                } else {
+                    base = B_.get();
+                    base_raw = Rl_.get();
+                    base_pre = Gl_.get();
+                    base_eval = &look;
+                    momentum_used = true;
+                }
+            }
+        }
+        if (restarted) {
+            base = X_.get();
+            base_raw = Rc_.get();
+            base_pre = Gc_.get();
+            base_eval = &cur;
+            momentum_age_ = 0;
+            worse_streak_ = 0;
+            ++result_.momentum_restarts;
+        }
+
+        bool accepted = false, new_best = false;
+        int backtracks = 0;
+        double used_step = 0.0, used_curvature = 0.0, used_disp = 0.0;
+        EvalRec acc;
+        for (int pass = 0; pass < 2 && !accepted; ++pass) {
+            // estimateStep :470-492
+            double step = std::max(last_step_, kEps), curvature = 0.0;
+            if (has_previous_) {
+                be_->timedBegin(kTController);
+                launchCurvature(base, PX_.get(), base_pre, PG_.get(), P_, be_->partials(kCtrCurv), be_->counter(kCtrCurv),
+                                scal(), s);
+                be_->timedEnd(kTController);
+                const HostScalars& h = be_->fetchScalars();
+                const double dp = h.v[kCurvDp], dg = h.v[kCurvDg];
+                if (!(dp <= kEps || dg <= kEps)) {
+                    curvature = std::sqrt(dg / dp);
+                    const double fallback = step;
+                    step = clampd(1.0 / std::max(curvature, kEps), fallback * 0.25, fallback * 4.0);
+                }
+            }
+            const double ratio = clampd(base_eval->s.optimizer_density.normalized_overflow / std::max(initial_overflow_, kEps), 0.0, 1.0);
+            const double max_disp = o_.maximum_movement_in_bins * min_bin_ * (0.35 + 0.65 * ratio);
+            for (int attempt = 0; attempt <= o_.maximum_backtracks; ++attempt) {
+                be_->timedBegin(kTController);
+                launchPropose(base, base_pre, base_raw, step, max_disp, Y_.get(), clamp_, P_, be_->partials(kCtrPropose),
+                              be_->counter(kCtrPropose), scal(), s);
+                be_->timedEnd(kTController);
+                enqueueValue(Y_.get());
+                const HostScalars& h = be_->fetchScalars();
+                const double predicted = h.v[kPredicted];
+                const double disp = h.v[kMaxDisp];
+                if (predicted <= kEps) break;
+                EvalRec cand = record(h, false);
+                if (accepts(*base_eval, cand, predicted)) {
+                    accepted = true;
+                    used_step = step;
+                    used_curvature = curvature;
+                    used_disp = disp;
+                    acc = cand;
+                    break;
+                }
+                ++backtracks;
+                ++result_.rejected_candidates;
+                step *= o_.backtracking_ratio;
+            }
+            if (accepted || restarted || !momentum_used) break;
+            restarted = true;
+            ++result_.momentum_restarts;
+            momentum_age_ = 0;
+            worse_streak_ = 0;
+            base = X_.get();
+            base_raw = Rc_.get();
+            base_pre = Gc_.get();
+            base_eval = &cur;
+            momentum_used = false;
+        }
+
+        const double penalty_used = penalty_;
+        if (accepted) {
+            be_->timedBegin(kTController);
+            launchDifference(Y_.get(), X_.get(), V_.get(), P_, s);
+            be_->timedEnd(kTController);
+            last_step_ = used_step;
+            momentum_age_ = restarted ? 0 : momentum_age_ + 1;
+            const double growth = (acc.s.hpwl - cur.s.hpwl) / std::max(cur.s.hpwl, 1.0);
+            worse_streak_ = growth > o_.hpwl_growth_restart_threshold ? worse_streak_ + 1 : 0;
+            const double next = updatePenalty(penalty_, cur, acc);
+            const bool changed = std::abs(std::log(next / std::max(penalty_, kEps))) > 0.01;
+            penalty_ = next;
+            if (changed) {
+                has_previous_ = false;
+            } else {
+                copy(PX_, base);
+                copy(PG_, base_pre);
+                has_previous_ = true;
+            }
+            const double gnorm = base_eval->pre_norm;
+            X_.swapWith(Y_);
+            last_overflow_ = acc.s.optimizer_density.normalized_overflow;
+            considerCheckpoint(acc, it + 1, new_best);
+            ++result_.accepted_iterations;
+            if (acc.s.design_density.normalized_overflow <= o_.maximum_bin_overflow) ++feasible_refinement_;
+            else feasible_refinement_ = 0;
+            history_.push_back(row(it + 1, acc, penalty_used, used_step, used_disp, gnorm, used_curvature, backtracks,
+                                   restarted, true, new_best));
+        } else {
+            launchZero(reinterpret_cast<double*>(V_.get()), 2 * static_cast<std::size_t>(P_), s);
+            momentum_age_ = 0;
+            worse_streak_ = 0;
+            if (cur.s.optimizer_density.normalized_overflow > o_.maximum_bin_overflow)
+                penalty_ = clampd(penalty_ * o_.density_penalty_increase,
+                                  std::max(o_.minimum_density_penalty, reference_penalty_ * 0.02), o_.maximum_density_penalty);
+            last_overflow_ = cur.s.optimizer_density.normalized_overflow;
+            if (cur.s.design_density.normalized_overflow <= o_.maximum_bin_overflow) ++feasible_refinement_;
+            else feasible_refinement_ = 0;
+            history_.push_back(row(it + 1, cur, penalty_used, 0.0, 0.0, cur.pre_norm, 0.0, backtracks, restarted, false, false));
+        }
+        result_.completed_iterations = it + 1;
+        if ((it >= 20 && feasible_refinement_ >= o_.feasible_refinement_iterations) ||
+            result_.completed_iterations >= o_.iterations)
+            stopped_ = true;
+    }
+
+    std::shared_ptr<const Prepared> prep_;
+    mp_gp_options o_;
+    std::unique_ptr<Backend> be_;
+    int P_ = 0;
+    ClampArgs clamp_{};
+    DeviceArray<double2> X_, B_, Y_, V_, PX_, PG_, Rc_, Gc_, Rl_, Gl_, CKf_, CKb_;
+    DeviceArray<double> precond_;
+    double min_bin_ = 0, max_bin_ = 0;
+    double smoothing_ = 0, penalty_ = 1, reference_penalty_ = 1, initial_overflow_ = 0, last_overflow_ = 0, last_step_ = 0;
+    int momentum_age_ = 0, worse_streak_ = 0, feasible_refinement_ = 0;
+    bool has_previous_ = false, stopped_ = false;
+    EvalRec current_;
+    Checkpoint feasible_, fallback_;
+    mp_gp_result result_{};
+    std::vector<mp_gp_history_row> history_;
+    double setup_seconds_ = 0.0, loop_seconds_ = 0.0;
+};
+
+}  // namespace mpgp
+
+using namespace mpgp;
+
+struct mp_gp_prepared {
+    std::shared_ptr<const Prepared> p;
+};
+
+struct mp_gp_session {
+    std::unique_ptr<Session> s;
+    mp_gp_backend view;  // non-owning view of the session's backend
+};
+
+namespace {
+template <typename F>
+mp_gp_status guarded(F&& body) {
+    g_last_error.clear();
+    try {
+        body();
+        return MP_GP_OK;
+    } catch (...) {
+        return translate(std::current_exception());
+    }
+}
+}  // namespace
+
+extern "C" {
+
+void mp_gp_default_options(mp_gp_options* o) {
+    if (!o) return;
+    *o = {};
+    o->iterations = 280;
+    o->bins_x = 64;
+    o->bins_y = 64;
+    o->target_density = 0.85;
+    o->maximum_bin_overflow = 0.10;
+    o->initial_smoothing = 0.75;
+    o->final_smoothing = 0.12;
+    o->maximum_movement_in_bins = 0.22;
+    o->maximum_backtracks = 6;
+    o->backtracking_ratio = 0.5;
+    o->armijo_coefficient = 1e-4;
+    o->maximum_momentum = 0.92;
+    o->initial_movement_in_bins = 0.12;
+    o->objective_increase_for_density = 0.002;
+    o->hpwl_growth_restart_threshold = 0.02;
+    o->density_stall_threshold = 0.002;
+    o->density_penalty_increase = 1.05;
+    o->density_penalty_decrease = 0.88;
+    o->minimum_density_penalty = 1e-4;
+    o->maximum_density_penalty = 1e6;
+    o->smoothing_overflow_exponent = 1.0;
+    o->feasible_refinement_iterations = 12;
+    o->maximum_fillers = 50000;
+    o->seed = 2026;
+    o->device = 0;
+    o->dct_precision = MP_DCT_FP32;
+    o->maximum_device_bytes = 40ULL << 30;
+}
+
+mp_gp_status mp_gp_prepare(const mp_design* design, const mp_gp_options* options, mp_gp_prepared** out) {
+    return guarded([&] {
+        if (!design || !options || !out) throw std::invalid_argument("null design, options or output");
+        *out = nullptr;
+        auto h = std::make_unique<mp_gp_prepared>();
+        h->p = std::make_shared<const Prepared>(*design, *options);
+        *out = h.release();
+    });
+}
+
+void mp_gp_prepared_destroy(mp_gp_prepared* p) { delete p; }
+
+mp_gp_status mp_gp_prepared_query(const mp_gp_prepared* h, mp_gp_prepared_info* info) {
+    return guarded([&] {
+        if (!h || !info) throw std::invalid_argument("null prepared design or output");
+        const Prepared& p = *h->p;
+        *info = {};
+        info->movable_count = static_cast<int64_t>(p.net.movable.size());
+        info->filler_count = static_cast<int64_t>(p.filler_x.size());
+        info->particle_count = static_cast<int64_t>(p.pw.size());
+        info->net_count = static_cast<int64_t>(p.net_offsets.size()) - 1;
+        info->pin_count = static_cast<int64_t>(p.pin_particles.size());
+        info->bin_count = static_cast<int64_t>(p.nx) * p.ny;
+        info->core_ll_x = p.net.core[0];
+        info->core_ll_y = p.net.core[1];
+        info->core_ur_x = p.net.core[2];
+        info->core_ur_y = p.net.core[3];
+        info->nominal_row_height = p.net.nominal_row_height;
+        info->filler_width = p.filler_w;
+        info->filler_height = p.filler_h;
+    });
+}
+
+mp_gp_status mp_gp_prepared_layout(const mp_gp_prepared* h, mp_gp_static* layout) {
+    return guarded([&] {
+        if (!h || !layout) throw std::invalid_argument("null prepared design or output");
+        const int device = layout->device;
+        const int precision = layout->dct_precision;
+        const std::uint64_t bytes = layout->maximum_device_bytes ? layout->maximum_device_bytes : (40ULL << 30);
+        *layout = h->p->layout(device, precision, bytes);
+    });
+}
+
+mp_gp_status mp_gp_prepared_arrays(const mp_gp_prepared* h, int64_t* movable_ids, double* filler_centers,
+                                   double* preconditioner_base) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("null prepared design");
+        const Prepared& p = *h->p;
+        if (movable_ids) std::copy(p.net.movable.begin(), p.net.movable.end(), movable_ids);
+        if (filler_centers)
+            for (std::size_t i = 0; i < p.filler_x.size(); ++i) {
+                filler_centers[2 * i] = p.filler_x[i];
+                filler_centers[2 * i + 1] = p.filler_y[i];
+            }
+        if (preconditioner_base) std::copy(p.precond.begin(), p.precond.end(), preconditioner_base);
+    });
+}
+
+mp_gp_status mp_gp_random_placement(const mp_design* design, uint64_t seed, double* centers) {
+    return guarded([&] {
+        if (!design || !centers) throw std::invalid_argument("null design or output");
+        randomPlacement(*design, seed, centers);
+    });
+}
+
+mp_gp_status mp_gp_session_create(const mp_gp_prepared* prepared, const mp_gp_options* options, const double* centers,
+                                  mp_gp_session** out) {
+    return guarded([&] {
+        if (!prepared || !options || !out) throw std::invalid_argument("null prepared design, options or output");
+        *out = nullptr;
+        if (options->maximum_device_bytes == 0)
+            throw std::invalid_argument("HIP global placement requires a non-zero memory budget.");
+        auto h = std::make_unique<mp_gp_session>();
+        h->s = std::make_unique<Session>(prepared->p, *options, centers);
+        h->view.impl = &h->s->backend();
+        *out = h.release();
+    });
+}
+
+mp_gp_status mp_gp_session_step(mp_gp_session* h, int32_t iterations, int32_t* completed, int32_t* stopped) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("null session");
+        if (iterations < 0) throw std::invalid_argument("negative iteration count");
+        const int done = h->s->step(iterations);
+        if (completed) *completed = done;
+        if (stopped) *stopped = h->s->stopped() ? 1 : 0;
+    });
+}
+
+mp_gp_status mp_gp_session_finish(mp_gp_session* h, double* centers, mp_gp_result* result) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("null session");
+        h->s->finish(centers, result);
+    });
+}
+
+int64_t mp_gp_session_history(const mp_gp_session* h, mp_gp_history_row* rows, int64_t capacity) {
+    if (!h) return 0;
+    const auto& hist = h->s->history();
+    if (rows == nullptr) return static_cast<int64_t>(hist.size());
+    const int64_t n = std::min<int64_t>(capacity, static_cast<int64_t>(hist.size()));
+    std::copy(hist.begin(), hist.begin() + n, rows);
+    return n;
+}
+
+mp_gp_status mp_gp_session_positions(mp_gp_session* h, double* xy) {
+    return guarded([&] {
+        if (!h || !xy) throw std::invalid_argument("null session or output");
+        h->s->positions(xy);
+    });
+}
+
+mp_gp_status mp_gp_session_enable_kernel_timing(mp_gp_session* h, int enable) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("null session");
+        h->s->backend().enableTiming(enable != 0);
+    });
+}
+
+int32_t mp_gp_session_kernel_timing(mp_gp_session* h, char* names, size_t names_len, double* total_us,
+                                    int64_t* launches, double* work, int32_t capacity) {
+    if (!h) return 0;
+    try {
+        std::vector<double> t, wk;
+        std::vector<long long> c;
+        h->s->backend().timing(t, c, wk);
+        const auto nm = h->s->backend().timingNames();
+        std::string joined;
+        for (std::size_t i = 0; i < nm.size(); ++i) joined += (i ? "," : "") + nm[i];
+        if (names && names_len) {
+            const std::size_t n = std::min(names_len - 1, joined.size());
+            std::copy(joined.begin(), joined.begin() + n, names);
+            names[n] = '\0';
+        }
+        const int32_t n = std::min<int32_t>(capacity, static_cast<int32_t>(t.size()));
+        for (int32_t i = 0; i < n; ++i) {
+            if (total_us) total_us[i] = t[i];
+            if (launches) launches[i] = c[i];
+            if (work) work[i] = wk[i];
+        }
+        return static_cast<int32_t>(t.size());
+    } catch (...) {
+        translate(std::current_exception());
+        return -1;
+    }
+}
+
+mp_gp_backend* mp_gp_session_backend(mp_gp_session* h) { return h ? &h->view : nullptr; }
+
+void mp_gp_session_destroy(mp_gp_session* h) { delete h; }
+
+mp_gp_status mp_gp_global_place(const mp_design* design, const mp_gp_options* options, double* centers,
+                                mp_gp_result* result, mp_gp_history_row* history, int64_t history_capacity) {
+    return guarded([&] {
+        if (!design || !options || !centers) throw std::invalid_argument("null design, options or centres");
+        mp_design d = *design;
+        d.center = centers;
+        const auto t0 = std::chrono::steady_clock::now();
+        Netlist probe(d);
+        if (probe.movable.empty()) {
+            if (result) *result = {};
+            return;
+        }
+        validateOptions(*options);
+        const double hpwl_before = probe.hpwl(probe.cx.data(), probe.cy.data());
+        const double ovf_before = probe.designOverflow(probe.cx.data(), probe.cy.data(), options->bins_x, options->bins_y,
+                                                       options->target_density);
+        auto prep = std::make_shared<const Prepared>(d, *options);
+        Session s(prep, *options, nullptr);
+        s.step(options->iterations);
+        mp_gp_result r{};
+        s.finish(centers, &r);
+        r.hpwl_before = hpwl_before;
+        r.overflow_before = ovf_before;
+        r.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (result) *result = r;
+        if (history) {
+            const auto& hist = s.history();
+            const int64_t n = std::min<int64_t>(history_capacity, static_cast<int64_t>(hist.size()));
+            std::copy(hist.begin(), hist.begin() + n, history);
+        }
+    });
+}
+
+}  // extern "C"

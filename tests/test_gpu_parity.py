@@ -1,0 +1,177 @@
+"""GPU parity: the HIP backend against the FP64 oracle on the same inputs.
+
+Tolerances (stated here, checked below):
+  HPWL, density metrics        relative 1e-10 (FP64 on both sides; only summation order differs)
+  smooth wirelength            relative 2e-6  (exp / log in FP32 per pin, sums per net in FP32)
+  electrostatic energy         relative 2e-4  (FP32 DCT products on MFMA; energy taken in the DCT basis)
+  wire gradient                normwise relative 2e-5, elementwise 1e-4 of the largest entry
+  density gradient / field     normwise relative 5e-4
+  end-to-end adaptive run      final HPWL within 0.5 %, final design overflow within 0.005
+"""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from myplacement_amd import abi, design
+from myplacement_amd import backend as hb
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ob():
+    from oracle import bindings
+    return bindings
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def nrel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _setup(d, t, bins, seed=2026):
+    d.center = hb.random_placement(d, seed)
+    o = abi.default_options(target_density=t, bins_x=bins, bins_y=bins)
+    return d, o
+
+
+def _cases():
+    fx = os.path.join(ROOT, "tests", "golden", "bookshelf")
+    return [
+        ("tiny-8x8", lambda: (design.read_bookshelf(os.path.join(fx, "tiny.aux")), 0.85, 8)),
+        ("mixed-8x8", lambda: (design.read_bookshelf(os.path.join(fx, "mixed.aux")), 0.85, 8)),
+        ("plain-64", lambda: (design.plain_design(), 0.9, 64)),
+        ("plain-48", lambda: (design.plain_design(seed=7), 0.9, 48)),
+        ("mid-128", lambda: (design.large_design(seed=3, cells=20000, nets=21000, row_count=200, sites=400,
+                                                 macros=6), 0.8, 128)),
+    ]
+
+
+@pytest.mark.parametrize("name,make", _cases(), ids=[c[0] for c in _cases()])
+def test_evaluate_matches_oracle(ob, name, make):
+    d, t, bins = make()
+    d, o = _setup(d, t, bins)
+    orc = ob.OracleSession(d, o)
+    L = orc.layout()
+    say(f"[gpu] evaluate parity {name}: P={orc.particle_count} bins={bins}")
+    be = hb.HipPlacementBackend.create(L)
+    xy0 = orc.arrays()["positions"]
+    rng = np.random.default_rng(1)
+    region = L["region"]
+    bw = (region[2] - region[0]) / bins
+    for trial, gamma_bins in enumerate((0.75, 0.3, 0.12)):
+        xy = xy0 + rng.normal(0.0, 2.0 * bw, size=xy0.shape) * (trial > 0)
+        gamma = gamma_bins * max(bw, (region[3] - region[1]) / bins)
+        for grad in (True, False):
+            r = orc.evaluate(xy, gamma, grad)
+            h = be.evaluate(xy, gamma, grad)
+            assert rel(h.hpwl, r["hpwl"]) < 1e-10
+            assert rel(h.smooth_wirelength, r["smooth_wirelength"]) < 2e-6
+            assert rel(h.electrostatic_energy, r["electrostatic_energy"]) < 2e-4, (h.electrostatic_energy, r["electrostatic_energy"])
+            for key in ("optimizer_density", "design_density"):
+                for m, v in r[key].items():
+                    assert abs(h.__dict__[key][m] - v) <= 1e-10 * max(1.0, abs(v)), (key, m, h.__dict__[key][m], v)
+            if grad:
+                assert nrel(h.wire_gradient, r["wire_gradient"]) < 2e-5
+                assert np.abs(h.wire_gradient - r["wire_gradient"]).max() <= 1e-4 * np.abs(r["wire_gradient"]).max()
+                assert nrel(h.density_gradient, r["density_gradient"]) < 5e-4, nrel(h.density_gradient, r["density_gradient"])
+            else:
+                assert h.wire_gradient is None
+    be.close()
+
+
+def test_fields_match_oracle(ob):
+    d, o = _setup(design.plain_design(), 0.9, 64)
+    orc = ob.OracleSession(d, o)
+    L = orc.layout()
+    say("[gpu] field grids plain-64")
+    be = hb.HipPlacementBackend.create(L)
+    xy = orc.arrays()["positions"]
+    gamma = 0.75 * 360 / 64
+    orc.evaluate(xy, gamma, True)
+    be.evaluate(xy, gamma, True)
+    g_ref = orc.grids()
+    g = be.debug_grids(64 * 64)
+    assert nrel(g["source"], g_ref["source"]) < 1e-6
+    for k in ("potential", "field_x", "field_y"):
+        assert nrel(g[k], g_ref[k]) < 5e-4, (k, nrel(g[k], g_ref[k]))
+    be.close()
+
+
+def test_evaluate_is_deterministic_for_wirelength(ob):
+    d, o = _setup(design.plain_design(), 0.9, 64)
+    orc = ob.OracleSession(d, o)
+    be = hb.HipPlacementBackend.create(orc.layout())
+    xy = orc.arrays()["positions"]
+    say("[gpu] determinism plain-64")
+    a = be.evaluate(xy, 3.0, True)
+    b = be.evaluate(xy, 3.0, True)
+    assert a.hpwl == b.hpwl and a.smooth_wirelength == b.smooth_wirelength
+    assert np.array_equal(a.wire_gradient, b.wire_gradient)
+    be.close()
+
+
+def test_bad_arguments_raise():
+    d, o = _setup(design.plain_design(), 0.9, 64)
+    pp = hb.PreparedDesign(d, o)
+    be = hb.HipPlacementBackend.create(pp.layout())
+    say("[gpu] argument checks")
+    xy = pp.initial_positions()
+    with pytest.raises(ValueError):
+        be.evaluate(xy, 0.0, True)
+    with pytest.raises(ValueError):
+        be.evaluate(xy, float("nan"), False)
+    with pytest.raises(ValueError):
+        be.evaluate(xy[:-1], 1.0, False)
+    assert be.device() == 0
+    assert be.reserved_bytes() > 0
+    be.close()
+
+
+def test_adaptive_run_matches_oracle_end_state(ob):
+    d, o = _setup(design.plain_design(), 0.9, 64)
+    o.feasible_refinement_iterations = 2 ** 31 - 1
+    say("[gpu] adaptive run plain-64, 280 iterations")
+    c_h, r_h, hist_h = hb.global_place(d, o)
+    c_o, r_o, hist_o = ob.global_place(d, o)
+    say("  hip", r_h["hpwl_after"], r_h["overflow_after"], "oracle", r_o["hpwl_after"], r_o["overflow_after"])
+    assert r_h["completed_iterations"] == 280 and len(hist_h) == 280
+    assert rel(r_h["hpwl_before"], r_o["hpwl_before"]) < 1e-12
+    assert rel(r_h["hpwl_after"], r_o["hpwl_after"]) < 5e-3
+    assert abs(r_h["overflow_after"] - r_o["overflow_after"]) < 5e-3
+    assert r_h["accepted_iterations"] > 0 and r_h["restored_best_checkpoint"] == 1
+    for row in hist_h:
+        assert all(np.isfinite(v) for v in row.values())
+    # the first iterations follow the oracle closely before controller decisions can diverge
+    assert rel(hist_h[0]["hpwl"], hist_o[0]["hpwl"]) < 1e-4
+
+
+def test_session_steps_and_history():
+    d, o = _setup(design.plain_design(), 0.9, 64)
+    o.iterations = 30
+    pp = hb.PreparedDesign(d, o)
+    say("[gpu] session stepping plain-64")
+    s = hb.PlacementSession(pp, o)
+    assert s.step(10) == 10
+    assert s.step(100) == 20 and s.stopped
+    c, r = s.finish()
+    assert r["completed_iterations"] == 30 and len(s.history()) == 30
+    assert r["gradient_evaluations"] >= 32
+    core = d.core_region()
+    mov = d.movable_ids()
+    assert np.all(c[mov, 0] - d.width[mov] / 2 >= core[0] - 1e-9)
+    assert np.all(c[mov, 0] + d.width[mov] / 2 <= core[2] + 1e-9)
+    s.close()

@@ -1,0 +1,132 @@
+"""Host side of the product library, no GPU needed: the C ABI loads and exports every declared
+symbol, the struct layouts agree with C, the run set-up (layout, fillers, static density,
+preconditioner, random placement) is bit-equal to the oracle, and the error paths report the
+reference's error kinds."""
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from myplacement_amd import abi, design
+from myplacement_amd import backend as hb
+from oracle import bindings as ob
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mp_gp_backend.h")
+BOOK = os.path.join(ROOT, "tests", "golden", "bookshelf")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(mp_gp_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = hb.library()
+    missing = [n for n in _declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert len(_declared_functions()) >= 20
+    assert hb.hip_placement_backend_compiled()
+
+
+def test_struct_sizes_match_c(tmp_path):
+    src = tmp_path / "sizes.c"
+    src.write_text('#include <stdio.h>\n#include "mp_gp_backend.h"\nint main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
+                   'sizeof(mp_design),sizeof(mp_gp_static),sizeof(mp_gp_density_metrics),sizeof(mp_gp_scalars),'
+                   'sizeof(mp_gp_options),sizeof(mp_gp_history_row),sizeof(mp_gp_result),sizeof(mp_gp_prepared_info),'
+                   'offsetof(mp_gp_options, seed));return 0;}\n'.replace("#include <stdio.h>", "#include <stdio.h>\n#include <stddef.h>"))
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [C.sizeof(abi.MpDesign), C.sizeof(abi.MpStatic), C.sizeof(abi.MpDensityMetrics), C.sizeof(abi.MpScalars),
+            C.sizeof(abi.MpOptions), C.sizeof(abi.MpHistoryRow), C.sizeof(abi.MpResult), C.sizeof(abi.MpPreparedInfo),
+            abi.MpOptions.seed.offset]
+    assert got == want
+
+
+def test_default_options_match_reference_defaults():
+    a = hb.default_options()
+    b = abi.default_options()
+    for name, _ in abi.MpOptions._fields_:
+        assert getattr(a, name) == getattr(b, name), name
+
+
+def _designs():
+    out = []
+    d = design.plain_design(seed=2026)
+    d.center = hb.random_placement(d, 2026)
+    out.append(("plain", d, 0.9, 64))
+    for name in ("tiny", "mixed"):
+        out.append((name, design.read_bookshelf(os.path.join(BOOK, f"{name}.aux")), 0.85, 8))
+    d = design.large_design(seed=5, cells=15000, nets=16000, row_count=150, sites=300, macros=4)
+    d.center = hb.random_placement(d, 5)
+    out.append(("mid", d, 0.8, 100))
+    return out
+
+
+@pytest.mark.parametrize("name,d,t,bins", _designs(), ids=lambda v: v if isinstance(v, str) else "")
+def test_prepared_layout_bit_equal_oracle(name, d, t, bins):
+    o = abi.default_options(target_density=t, bins_x=bins, bins_y=bins)
+    pp = hb.PreparedDesign(d, o)
+    orc = ob.OracleSession(d, o)
+    L1, L2 = pp.layout(), orc.layout()
+    for k in L1:
+        assert np.array_equal(np.asarray(L1[k]), np.asarray(L2[k])), k
+    A1, A2 = pp.arrays(), orc.arrays()
+    for k in A1:
+        assert np.array_equal(A1[k], A2[k]), k
+    assert pp.info.filler_count == orc.info.filler_count
+    assert pp.info.filler_width == orc.info.filler_width
+
+
+def test_random_placement_bit_equal_oracle():
+    for seed in (1, 2026):
+        d = design.plain_design(seed=seed)
+        assert np.array_equal(hb.random_placement(d, seed), ob.random_placement(d, seed))
+
+
+def test_factory_without_device_reports_reason():
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK):
+        pytest.skip("a HIP device is present")
+    d = design.plain_design()
+    d.center = hb.random_placement(d)
+    pp = hb.PreparedDesign(d, abi.default_options(target_density=0.9))
+    b, reason = hb.try_create_hip_placement_backend(pp.layout())
+    assert b is None and reason
+
+
+def test_error_kinds():
+    lib = hb.library()
+    h = C.c_void_p()
+    assert lib.mp_gp_create(None, C.byref(h)) == abi.MP_GP_INVALID_ARGUMENT
+    assert lib.mp_gp_last_error()
+    d = design.plain_design()
+    o = abi.default_options(bins_x=1025, bins_y=1024)
+    with pytest.raises(ValueError):
+        hb.PreparedDesign(d, o)
+    o = abi.default_options(target_density=0.9)
+    bad = design.Design(width=[1.0], height=[9.0], center=[[1, 1]], orientation=[0], is_terminal=[0], is_fixed=[0],
+                        net_weight=[1.0], net_pin_start=[0, 1], pin_module=[5], pin_offset=[[0, 0]],
+                        rows=[[0, 9, 0, 1, 10]])
+    with pytest.raises(ValueError):
+        hb.PreparedDesign(bad, o)
+    rows = [[0.0, 9.0, 0.0, 1.0, 10.0], [0.0, 9.0, 0.0, 1.0, 10.0]]
+    overlap = design.Design(width=[1.0], height=[9.0], center=[[1, 1]], orientation=[0], is_terminal=[0], is_fixed=[0],
+                            net_weight=[], net_pin_start=[0], pin_module=[], pin_offset=np.zeros((0, 2)), rows=rows)
+    with pytest.raises(ValueError):
+        hb.PreparedDesign(overlap, abi.default_options(bins_x=2, bins_y=2))
+
+
+def test_parse_compute_backend():
+    assert hb.parse_compute_backend("hip") is hb.ComputeBackend.HIP
+    assert hb.parse_compute_backend("ROCm") is hb.ComputeBackend.HIP
+    assert hb.parse_compute_backend("gpu") is hb.ComputeBackend.CUDA
+    assert hb.parse_compute_backend("auto") is hb.ComputeBackend.AUTO
+    with pytest.raises(ValueError):
+        hb.parse_compute_backend("tpu")

@@ -1,0 +1,393 @@
+"""The FP64 oracle, pinned before it is trusted.
+
+Parity status of each oracle function (oracle/gp_oracle.cpp), and what pins it:
+  REDFT10 / REDFT01 / RODFT01        scipy.fft golden vectors (tests/golden/neumann_golden.npz);
+                                     direct O(N^2) and FFT forms agree               -> pinned
+  Neumann solve (ElectrostaticField) analytic cos-cos modes of tests/DensityFieldTests.cpp:12-100
+                                     at 2e-12; uniform source -> zero (:102-111); scipy golden fields -> pinned
+  parity sampling                    zero normal field on all four edges (:53-66); bin-centre
+                                     sampling returns the grid value                 -> pinned
+  DensityMap                         2x2 known answer (tests/DensityModelTests.cpp:10-52); the
+                                     reference's own Metrics.cpp on random rectangles; mixed
+                                     normalisation area 540 (PlacementFlowTests.cpp:305-310)  -> pinned
+  static density, fillers            bit-equal to the reference's GlobalPlacementSupport.cpp    -> pinned
+  design HPWL / overflow             the reference's calculateHpwl / calculateDensity           -> pinned
+  smooth wirelength + gradient       two-pin closed form, gamma -> 0 limit (HPWL), central
+                                     finite differences                              -> pinned
+  electrostatic energy               equals the DCT-basis (Parseval) sum computed with scipy    -> pinned
+  random initial placement           mt19937_64 10000th output (C++ standard), and the product's
+                                     independent implementation bit-equal             -> pinned
+  adaptive controller                no numeric pin exists in the reference; invariants of
+                                     PlacementFlowTests.cpp:153-216 only              -> parity unpinned
+"""
+
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+from scipy import fft as sf
+
+from myplacement_amd import abi, design
+from oracle import bindings as ob
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+BOOK = os.path.join(GOLD, "bookshelf")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(os.path.join(GOLD, "neumann_golden.npz"))
+
+
+@pytest.fixture(scope="module")
+def ref():
+    lib = ob.reference()
+    if lib is None:
+        pytest.skip("the reference checker oracle/_ref is not built here")
+    return lib
+
+
+# ---- transforms ------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [12, 16, 64, 100])
+@pytest.mark.parametrize("method", [1, 0])
+def test_transforms_match_scipy_golden(golden, n, method):
+    ob.set_method(method)
+    try:
+        x = golden[f"x{n}"]
+        for kind, key in ((0, "redft10"), (1, "redft01"), (2, "rodft01")):
+            ref_v = golden[f"{key}_{n}"]
+            got = ob.transform_1d(kind, x)
+            assert np.abs(got - ref_v).max() <= 1e-13 * np.abs(ref_v).max()
+    finally:
+        ob.set_method(0)
+
+
+def test_fft_and_direct_forms_agree():
+    rng = np.random.default_rng(5)
+    for n in (4, 8, 32, 256, 1024):
+        x = rng.standard_normal(n)
+        for kind in (0, 1, 2):
+            ob.set_method(1)
+            a = ob.transform_1d(kind, x)
+            ob.set_method(2)
+            b = ob.transform_1d(kind, x)
+            ob.set_method(0)
+            assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+
+
+def test_transform_planted_error_is_caught(golden):
+    # the golden check is sharp enough to see a single flipped sign in the sine transform
+    x = golden["x16"]
+    got = ob.transform_1d(2, x)
+    wrong = got.copy()
+    wrong[1::2] *= -1
+    assert np.abs(wrong - golden["rodft01_16"]).max() > 1e-3
+
+
+# ---- Neumann field (DensityFieldTests.cpp) -----------------------------------------------------
+
+REGION = (-3.0, 2.0, 5.0, 8.0)
+
+
+def _grid(cols, rows):
+    bw, bh = (REGION[2] - REGION[0]) / cols, (REGION[3] - REGION[1]) / rows
+    x = REGION[0] + (np.arange(cols) + 0.5) * bw
+    y = REGION[1] + (np.arange(rows) + 0.5) * bh
+    return np.meshgrid(x - REGION[0], y - REGION[1])
+
+
+@pytest.mark.parametrize("mx,my", [(2, 3), (0, 3), (2, 0)])
+def test_analytic_neumann_modes(mx, my):
+    cols, rows = 16, 12
+    X, Y = _grid(cols, rows)
+    kx, ky = np.pi * mx / 8.0, np.pi * my / 6.0
+    den = kx * kx + ky * ky
+    src = np.cos(kx * X) * np.cos(ky * Y)
+    pot, fx, fy = ob.field_solve(src, REGION)
+    assert np.abs(pot - src / den).max() < 2e-12
+    assert np.abs(fx - kx * np.sin(kx * X) * np.cos(ky * Y) / den).max() < 2e-12
+    assert np.abs(fy - ky * np.cos(kx * X) * np.sin(ky * Y) / den).max() < 2e-12
+
+
+def test_uniform_source_gives_zero():
+    pot, fx, fy = ob.field_solve(np.full((12, 16), 7.0), REGION)
+    for g in (pot, fx, fy):
+        assert np.abs(g).max() < 2e-12
+
+
+def test_zero_normal_field_on_edges():
+    cols, rows = 16, 12
+    X, Y = _grid(cols, rows)
+    src = np.cos(np.pi * 2 / 8.0 * X) * np.cos(np.pi * 3 / 6.0 * Y)
+    _, fx, fy = ob.field_solve(src, REGION)
+    bw, bh = 8.0 / cols, 6.0 / rows
+    ys = REGION[1] + (np.arange(rows) + 0.5) * bh
+    xs = REGION[0] + (np.arange(cols) + 0.5) * bw
+    left = ob.sample_field(fx, fy, REGION, np.stack([np.full(rows, REGION[0]), ys], 1))
+    right = ob.sample_field(fx, fy, REGION, np.stack([np.full(rows, REGION[2]), ys], 1))
+    bottom = ob.sample_field(fx, fy, REGION, np.stack([xs, np.full(cols, REGION[1])], 1))
+    top = ob.sample_field(fx, fy, REGION, np.stack([xs, np.full(cols, REGION[3])], 1))
+    assert np.abs(left[:, 0]).max() < 2e-12 and np.abs(right[:, 0]).max() < 2e-12
+    assert np.abs(bottom[:, 1]).max() < 2e-12 and np.abs(top[:, 1]).max() < 2e-12
+    # at bin centres the bilinear sample is the grid value itself
+    Xc, Yc = np.meshgrid(xs, ys)
+    s = ob.sample_field(fx, fy, REGION, np.stack([Xc.ravel(), Yc.ravel()], 1))
+    assert np.abs(s[:, 0] - fx.ravel()).max() < 1e-13
+    assert np.abs(s[:, 1] - fy.ravel()).max() < 1e-13
+
+
+@pytest.mark.parametrize("key", ["field_12x16", "field_64x64", "field_48x80"])
+def test_field_golden(golden, key):
+    pot, fx, fy = ob.field_solve(golden[key + "_in"], golden[key + "_region"])
+    for name, got in (("potential", pot), ("fx", fx), ("fy", fy)):
+        ref_v = golden[f"{key}_{name}"]
+        assert np.abs(got - ref_v).max() <= 1e-12 * np.abs(ref_v).max()
+
+
+def test_oversized_grid_rejected():
+    with pytest.raises(ValueError):
+        ob.field_solve(np.zeros((1024, 1025)), REGION)
+    with pytest.raises(ValueError):
+        ob.density_map(REGION, 1025, 1024, 0.85, [], [], [])
+
+
+# ---- density model ---------------------------------------------------------------------------
+
+KAT_RECTS = [(0, 0, 2, 5), (0, 5, 2, 10), (2, 0, 5, 5), (2, 5, 5, 10)]
+KAT_LAYERS = [1, 2, 0, 3]  # movable, macro, fixed, filler
+
+
+def test_density_two_by_two_known_answer():
+    w, wo, bins = ob.density_map((0, 0, 10, 10), 2, 2, 0.5, [(0, 0, 5, 10)], KAT_RECTS, KAT_LAYERS)
+    assert wo["dark_area"] == pytest.approx(50.0, abs=1e-12)
+    assert wo["placeable_area"] == pytest.approx(50.0, abs=1e-12)
+    assert wo["normalization_area"] == pytest.approx(15.0, abs=1e-12)
+    assert w["normalization_area"] == pytest.approx(15.0, abs=1e-12)
+    assert wo["total_charge_area"] == pytest.approx(47.5, abs=1e-12)
+    assert w["total_charge_area"] == pytest.approx(55.0, abs=1e-12)
+    assert wo["total_overflow_area"] == pytest.approx(5.0, abs=1e-12)
+    assert wo["normalized_overflow"] == pytest.approx(1.0 / 3.0, abs=1e-12)
+    assert bins[1, 4] == pytest.approx(25.0, abs=1e-12)  # bin (1, 0) dark area
+
+
+def test_overlapping_rows_rejected():
+    with pytest.raises(ValueError):
+        ob.density_map((0, 0, 10, 10), 2, 2, 0.5, [(0, 0, 10, 10), (0, 0, 10, 10)], [], [])
+
+
+def test_density_map_matches_reference_code(ref):
+    rng = np.random.default_rng(11)
+    for trial in range(5):
+        cols, rows = int(rng.integers(3, 40)), int(rng.integers(3, 40))
+        region = (-5.0, 3.0, 95.0, 60.0)
+        rects, layers = [], []
+        for _ in range(300):
+            cx, cy = rng.uniform(-10, 100), rng.uniform(0, 65)
+            w, h = rng.uniform(0, 9), rng.uniform(0, 9)
+            rects.append((cx - w / 2, cy - h / 2, cx + w / 2, cy + h / 2))
+            layers.append(int(rng.integers(0, 5)))
+        regions = [(-5.0, 3.0 + 5.7 * k, 95.0, 3.0 + 5.7 * (k + 1)) for k in range(10)]
+        a = ob.density_map(region, cols, rows, 0.8, regions, rects, layers)
+        b = ob.density_map(region, cols, rows, 0.8, regions, rects, layers, lib=ref)
+        assert a[0] == b[0] and a[1] == b[1]
+        assert np.array_equal(a[2], b[2])
+
+
+def _fixture(name):
+    return design.read_bookshelf(os.path.join(BOOK, f"{name}.aux"))
+
+
+def test_mixed_normalization_area_540():
+    d = _fixture("mixed")
+    _, m = ob.design_metrics(d, d.center, 8, 8, 0.85)
+    assert m["normalization_area"] == pytest.approx(540.0, abs=1e-9)
+
+
+def test_tiny_fixture_counts():
+    d = _fixture("tiny")
+    assert d.module_count == 4 and d.net_count == 2 and d.pin_count == 5
+    assert int((d.is_fixed | d.is_terminal).sum()) == 1 and d.movable_ids().shape[0] == 3
+    core = d.core_region()
+    assert core[2] - core[0] == pytest.approx(100.0) and core[3] - core[1] == pytest.approx(40.0)
+    hpwl, _ = ob.design_metrics(d, d.center, 8, 8, 0.85)
+    assert hpwl > 0
+
+
+def test_bookshelf_reader_matches_reference_parser(ref):
+    for name in ("tiny", "mixed"):
+        d = _fixture(name)
+        h = C.c_void_p()
+        assert ref.ref_parse_aux(os.path.join(BOOK, f"{name}.aux").encode(), C.byref(h)) == 0
+        sizes = np.zeros(7, dtype=np.int64)
+        ref.ref_db_sizes(h, sizes.ctypes.data_as(C.POINTER(C.c_int64)))
+        M, N, I, R = (int(v) for v in sizes[:4])
+        out = dict(width=np.zeros(M), height=np.zeros(M), center=np.zeros((M, 2)), orientation=np.zeros(M, np.int32),
+                   is_terminal=np.zeros(M, np.uint8), is_fixed=np.zeros(M, np.uint8), net_weight=np.zeros(N),
+                   net_pin_start=np.zeros(N + 1, np.int64), pin_module=np.zeros(I, np.int32),
+                   pin_offset=np.zeros((I, 2)), rows=np.zeros((R, 5)))
+        core = np.zeros(4)
+        args = [out[k].ctypes.data_as(C.c_void_p) for k in out] + [core.ctypes.data_as(C.c_void_p)]
+        assert ref.ref_db_export(h, *args) == 0
+        ref.ref_db_destroy(h)
+        for k, v in out.items():
+            assert np.array_equal(getattr(d, k), v), (name, k)
+        assert tuple(core) == d.core_region()
+
+
+def _synthetic_with_random_start(seed=2026):
+    d = design.plain_design(seed=seed)
+    d.center = ob.random_placement(d, seed)
+    return d
+
+
+def test_fillers_and_static_grids_bit_equal_reference(ref):
+    for d, t, bins in ((_synthetic_with_random_start(), 0.9, 64), (_fixture("mixed"), 0.85, 8),
+                       (design.large_design(seed=4, cells=20000, nets=21000, row_count=200, sites=400, macros=6), 0.8, 128)):
+        if d.module_count > 10000:
+            d.center = ob.random_placement(d, 9)
+        o = abi.default_options(target_density=t, bins_x=bins, bins_y=bins, maximum_fillers=50000)
+        s = ob.OracleSession(d, o)
+        L = s.layout()
+        A = s.arrays()
+        cnt = C.c_int64()
+        cap = 60000
+        F = np.zeros((cap, 4))
+        fixed, dark = np.zeros(bins * bins), np.zeros(bins * bins)
+        dc = d.to_c()
+        assert ref.ref_fillers(C.byref(dc), bins, bins, t, 50000, 2026, fixed.ctypes.data_as(C.c_void_p),
+                               dark.ctypes.data_as(C.c_void_p), C.byref(cnt), F.ctypes.data_as(C.c_void_p), cap) == 0
+        n = cnt.value
+        assert n == s.info.filler_count
+        assert np.array_equal(F[:n, :2], A["filler_centers"])
+        assert np.all(F[:n, 2] == s.info.filler_width) and np.all(F[:n, 3] == s.info.filler_height)
+        assert np.array_equal(fixed, L["fixed_area"]) and np.array_equal(dark, L["dark_area"])
+
+
+def test_design_metrics_match_reference(ref):
+    d = _synthetic_with_random_start()
+    hp, m = ob.design_metrics(d, d.center, 64, 64, 0.9)
+    hp_r = C.c_double()
+    m_r = abi.MpDensityMetrics()
+    dc = d.to_c()
+    assert ref.ref_design_metrics(C.byref(dc), 64, 64, 0.9, C.byref(hp_r), C.byref(m_r)) == 0
+    assert hp == hp_r.value
+    assert m == m_r.as_dict()
+
+
+# ---- smooth wirelength -----------------------------------------------------------------------
+
+def _two_module_design(ax, ay, bx, by):
+    return design.Design(
+        width=[2.0, 2.0], height=[10.0, 10.0], center=[[ax, ay], [bx, by]], orientation=[0, 0],
+        is_terminal=[0, 0], is_fixed=[0, 0], net_weight=[1.5], net_pin_start=[0, 2], pin_module=[0, 1],
+        pin_offset=[[0.0, 0.0], [0.0, 0.0]], rows=[[10.0 * r, 10.0, 0.0, 1.0, 100.0] for r in range(10)])
+
+
+def test_two_pin_closed_form_and_limit():
+    a, b = (20.0, 33.0), (61.5, 48.25)
+    d = _two_module_design(*a, *b)
+    o = abi.default_options(bins_x=8, bins_y=8, maximum_fillers=0)
+    s = ob.OracleSession(d, o)
+    xy = np.array([a, b])
+    for gamma in (0.5, 3.0, 20.0):
+        r = s.evaluate(xy, gamma, True)
+        def lse(u, v):
+            return gamma * np.logaddexp(u / gamma, v / gamma) + gamma * np.logaddexp(-u / gamma, -v / gamma)
+        closed = 1.5 * (lse(a[0], b[0]) + lse(a[1], b[1]))
+        assert r["smooth_wirelength"] == pytest.approx(closed, rel=1e-13)
+        assert r["hpwl"] == pytest.approx(1.5 * ((b[0] - a[0]) + (b[1] - a[1])), rel=1e-15)
+        # gradient of pin 0 along x: w * (softmax weight of a in max - softmax weight of a in min)
+        ex = 1.5 * (1 / (1 + np.exp((b[0] - a[0]) / gamma)) - 1 / (1 + np.exp((a[0] - b[0]) / gamma)))
+        assert r["wire_gradient"][0, 0] == pytest.approx(ex, rel=1e-12)
+    r = s.evaluate(xy, 1e-3, False)
+    assert r["smooth_wirelength"] == pytest.approx(r["hpwl"], rel=1e-12)
+
+
+def test_wire_gradient_finite_differences():
+    d = _synthetic_with_random_start()
+    o = abi.default_options(target_density=0.9)
+    s = ob.OracleSession(d, o)
+    xy = s.arrays()["positions"]
+    gamma = 4.0
+    g = s.evaluate(xy, gamma, True)["wire_gradient"]
+    rng = np.random.default_rng(3)
+    h = 1e-4
+    for p in rng.choice(int(s.info.movable_count), 12, replace=False):
+        for axis in (0, 1):
+            up, dn = xy.copy(), xy.copy()
+            up[p, axis] += h
+            dn[p, axis] -= h
+            fd = (s.evaluate(up, gamma, False)["smooth_wirelength"] - s.evaluate(dn, gamma, False)["smooth_wirelength"]) / (2 * h)
+            assert fd == pytest.approx(g[p, axis], abs=1e-6 * max(1.0, abs(g[p, axis])))
+
+
+# ---- energy ----------------------------------------------------------------------------------
+
+def test_energy_equals_dct_basis_sum():
+    d = _synthetic_with_random_start()
+    o = abi.default_options(target_density=0.9)
+    s = ob.OracleSession(d, o)
+    xy = s.arrays()["positions"]
+    r = s.evaluate(xy, 3.0, False)
+    src = s.grids()["source"].reshape(64, 64)
+    core = d.core_region()
+    W, H = core[2] - core[0], core[3] - core[1]
+    chat = sf.dct(sf.dct(src, type=2, axis=1), type=2, axis=0)
+    wx = np.pi * np.arange(64) / W
+    wy = np.pi * np.arange(64) / H
+    den = wy[:, None] ** 2 + wx[None, :] ** 2
+    den[0, 0] = np.inf
+    phihat = chat / (4 * 64 * 64) / den
+    c = np.where(np.arange(64) == 0, 1.0, 2.0)
+    weight = c[:, None] * c[None, :] / 4
+    energy = 0.5 * (W / 64) * (H / 64) * np.sum(weight * chat * phihat)
+    assert r["electrostatic_energy"] == pytest.approx(energy, rel=1e-10)
+
+
+# ---- random placement / controller -----------------------------------------------------------
+
+def test_random_placement_inside_core_and_deterministic():
+    d = design.plain_design()
+    a = ob.random_placement(d, 2026)
+    b = ob.random_placement(d, 2026)
+    assert np.array_equal(a, b)
+    core = d.core_region()
+    mov = d.movable_ids()
+    assert np.all(a[mov, 0] - d.width[mov] / 2 >= core[0]) and np.all(a[mov, 0] + d.width[mov] / 2 <= core[2])
+    fixed = np.setdiff1d(np.arange(d.module_count), mov)
+    assert np.array_equal(a[fixed], d.center[fixed])
+
+
+def test_adaptive_run_invariants_tiny():
+    d = _fixture("tiny")
+    o = abi.default_options(iterations=24, bins_x=8, bins_y=8, maximum_fillers=100)
+    c, r, hist = ob.global_place(d, o)
+    assert r["completed_iterations"] == 24 and len(hist) == 24
+    assert r["accepted_iterations"] > 0 and r["restored_best_checkpoint"] == 1
+    assert r["best_checkpoint_iteration"] <= r["completed_iterations"]
+    for row in hist:
+        assert all(np.isfinite(v) for v in row.values())
+        assert row["step_size"] >= 0 and row["maximum_displacement"] >= 0
+    core = d.core_region()
+    mov = d.movable_ids()
+    assert np.all(c[mov, 0] - d.width[mov] / 2 >= core[0] - 1e-9)
+
+
+def test_aggressive_controller_backtracks_and_restarts():
+    d = _fixture("tiny")
+    d.center = ob.random_placement(d, 2026)
+    o = abi.default_options(iterations=24, bins_x=8, bins_y=8, maximum_fillers=100, initial_movement_in_bins=1.0,
+                            maximum_movement_in_bins=1.0, armijo_coefficient=0.5, objective_increase_for_density=0.0)
+    _, r, _ = ob.global_place(d, o)
+    assert r["rejected_candidates"] > 0
+    assert r["momentum_restarts"] > 0
+
+
+def test_invalid_controller_option_rejected():
+    d = _fixture("tiny")
+    o = abi.default_options(iterations=1, bins_x=8, bins_y=8, feasible_refinement_iterations=0)
+    with pytest.raises(ValueError):
+        ob.global_place(d, o)
