@@ -181,7 +181,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
                                      particleBlocks(P_), (nxp_ / 32) * (nyp_ / 32), 1});
     for (int k = 0; k < kCounterCount; ++k) partials_[k].allocate(static_cast<std::size_t>(max_blocks) * 10, budget_, "reduction partials");
     counters_.allocate(kCounterCount, budget_, "reduction counters");
-    scal_.allocate(kScalarCount, budget_, "scalar block");
+    scal_.allocate(kScalarCount * kScalarBanks, budget_, "scalar block");
     wire_out_.allocate(P_, budget_, "wire gradient");
     density_out_.allocate(P_, budget_, "density gradient");
     hipCheck(hipHostMalloc(reinterpret_cast<void**>(&host_scal_), sizeof(HostScalars)), "allocate pinned scalars");
@@ -225,7 +225,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     hipCheck(hipMemsetAsync(filler_.get(), 0, G * sizeof(double), stream_), "clear density layer");
     hipCheck(hipMemsetAsync(rho_.get(), 0, Gp * sizeof(float), stream_), "clear density grid");
     hipCheck(hipMemsetAsync(counters_.get(), 0, kCounterCount * sizeof(unsigned), stream_), "clear counters");
-    hipCheck(hipMemsetAsync(scal_.get(), 0, kScalarCount * sizeof(double), stream_), "clear scalars");
+    hipCheck(hipMemsetAsync(scal_.get(), 0, kScalarCount * kScalarBanks * sizeof(double), stream_), "clear scalars");
     hipCheck(hipStreamSynchronize(stream_), "finish backend upload");
 }
 
@@ -300,9 +300,9 @@ void Backend::timing(std::vector<double>& total_us, std::vector<long long>& laun
     work = t_work_;
 }
 
-void Backend::enqueueEvaluate(const double2* pos, double gamma, bool want_grad, const SessionOut* session) {
+void Backend::enqueueEvaluate(const double2* pos, double gamma, bool want_grad, const SessionOut* session, int bank) {
     hipStream_t s = stream_;
-    double* scal = scal_.get();
+    double* scal = scal_.get() + bank * kScalarCount;
     WireArgs wa{pos, net_off_.get(), net_w_.get(), pin_part_.get(), pin_off_.get(), small_nets_.get(), n_small_,
                 large_nets_.get(), n_large_, gamma, want_grad ? 1 : 0, pin_grad_.get(), partials_[kCtrWire].get(),
                 counters_.get() + kCtrWire, scal};
@@ -381,19 +381,20 @@ const HostScalars& Backend::fetchScalars() {
     return *host_scal_;
 }
 
-void Backend::toScalars(const HostScalars& h, mp_gp_scalars* out) const {
-    out->hpwl = h.v[kHpwl];
-    out->smooth_wirelength = h.v[kSmooth];
-    out->electrostatic_energy = h.v[kEnergy];
+void Backend::toScalars(const HostScalars& hs, mp_gp_scalars* out, int bank) const {
+    const double* v = hs.bank(bank);
+    out->hpwl = v[kHpwl];
+    out->smooth_wirelength = v[kSmooth];
+    out->electrostatic_energy = v[kEnergy];
     auto fill = [&](int base, mp_gp_density_metrics& m) {
-        m.total_overflow_area = h.v[base + 0];
-        m.normalized_overflow = h.v[base + 1];
-        m.maximum_density = h.v[base + 2];
-        m.average_density = h.v[base + 3];
-        m.total_charge_area = h.v[base + 4];
-        m.normalization_area = h.v[base + 5];
-        m.placeable_area = h.v[base + 6];
-        m.dark_area = h.v[base + 7];
+        m.total_overflow_area = v[base + 0];
+        m.normalized_overflow = v[base + 1];
+        m.maximum_density = v[base + 2];
+        m.average_density = v[base + 3];
+        m.total_charge_area = v[base + 4];
+        m.normalization_area = v[base + 5];
+        m.placeable_area = v[base + 6];
+        m.dark_area = v[base + 7];
     };
     fill(kOptDensity, out->optimizer_density);
     fill(kDesignDensity, out->design_density);

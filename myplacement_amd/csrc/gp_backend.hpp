@@ -71,7 +71,8 @@ private:
 };
 
 struct HostScalars {
-    double v[kScalarCount];
+    double v[kScalarCount * kScalarBanks];
+    const double* bank(int b) const { return v + b * kScalarCount; }
 };
 
 class Backend {
@@ -89,10 +90,10 @@ public:
         double2* raw;
         double2* pre;
     };
-    void enqueueEvaluate(const double2* pos, double gamma, bool want_grad, const SessionOut* session);
+    void enqueueEvaluate(const double2* pos, double gamma, bool want_grad, const SessionOut* session, int bank = 0);
     // Copies the scalar block to host (pinned) and waits for the stream.
     const HostScalars& fetchScalars();
-    void toScalars(const HostScalars& h, mp_gp_scalars* out) const;
+    void toScalars(const HostScalars& h, mp_gp_scalars* out, int bank = 0) const;
 
     // Reference-shaped blocking evaluate (CudaPlacementBackend::evaluate).
     void evaluateHost(const double* xy, double gamma, bool want_grad, mp_gp_scalars* out, double* wire_grad,
@@ -106,7 +107,7 @@ public:
     const Region& region() const { return region_; }
     DeviceBudget& budget() { return budget_; }
     const double2* sizes() const { return size_.get(); }
-    double* scalarBlock() const { return scal_.get(); }
+    double* scalarBlock(int bank = 0) const { return scal_.get() + bank * kScalarCount; }
     double* partials(int which) const { return partials_[which].get(); }
     unsigned* counter(int which) const { return counters_.get() + which; }
 

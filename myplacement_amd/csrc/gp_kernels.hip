@@ -487,8 +487,8 @@ __global__ __launch_bounds__(kThreads) void k_dot(const double2* a, const double
 }
 
 __global__ __launch_bounds__(kThreads) void k_curvature(const double2* b, const double2* pb, const double2* g,
-                                                        const double2* pg, int n, double* partials, unsigned* counter,
-                                                        double* scal) {
+                                                        const double2* pg, int n, int slot, double* partials,
+                                                        unsigned* counter, double* scal) {
     __shared__ double lds[kWaves * 2];
     const int p = blockIdx.x * kThreads + threadIdx.x;
     double v[2] = {0.0, 0.0};
@@ -500,8 +500,8 @@ __global__ __launch_bounds__(kThreads) void k_curvature(const double2* b, const 
     }
     double tot[2];
     if (finishReduction<2>(v, 0U, 0.0, blockIdx.x, gridDim.x, partials, counter, lds, tot) && threadIdx.x == 0) {
-        scal[kCurvDp] = tot[0];
-        scal[kCurvDg] = tot[1];
+        scal[slot] = tot[0];
+        scal[slot + 1] = tot[1];
     }
 }
 
@@ -607,9 +607,9 @@ void launchDot(const double2* a, const double2* b, double sign, int slot, int n,
     check(hipGetLastError(), "launch dot kernel");
 }
 
-void launchCurvature(const double2* b, const double2* pb, const double2* g, const double2* pg, int n,
+void launchCurvature(const double2* b, const double2* pb, const double2* g, const double2* pg, int n, int slot,
                      double* partials, unsigned* counter, double* scal, hipStream_t s) {
-    hipLaunchKernelGGL(k_curvature, dim3(particleBlocks(n)), dim3(kThreads), 0, s, b, pb, g, pg, n, partials,
+    hipLaunchKernelGGL(k_curvature, dim3(particleBlocks(n)), dim3(kThreads), 0, s, b, pb, g, pg, n, slot, partials,
                        counter, scal);
     check(hipGetLastError(), "launch curvature kernel");
 }

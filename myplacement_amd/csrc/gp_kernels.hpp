@@ -24,8 +24,13 @@ enum ScalarSlot : int {
     kAlignment = 25,
     kCurvDp = 26,
     kCurvDg = 27,
+    kCurvDp2 = 28,
+    kCurvDg2 = 29,
     kScalarCount = 32
 };
+// The scalar block holds kScalarBanks banks of kScalarCount slots, so several evaluations can be
+// enqueued back to back and read with one copy.
+constexpr int kScalarBanks = 4;
 
 // Reduction counters (one per reducing kernel; reset by that kernel's last workgroup).
 enum Counter : int { kCtrWire = 0, kCtrMetrics = 1, kCtrEnergy = 2, kCtrGather = 3, kCtrPropose = 4,
@@ -146,8 +151,8 @@ void launchLookahead(const double2* x, const double2* v, double beta, double2* o
 // scal[slot] = sum(a . b) * sign
 void launchDot(const double2* a, const double2* b, double sign, int slot, int n, double* partials,
                unsigned* counter, double* scal, hipStream_t s);
-// scal[kCurvDp] = sum |b - pb|^2, scal[kCurvDg] = sum |g - pg|^2
-void launchCurvature(const double2* b, const double2* pb, const double2* g, const double2* pg, int n,
+// scal[slot] = sum |b - pb|^2, scal[slot + 1] = sum |g - pg|^2
+void launchCurvature(const double2* b, const double2* pb, const double2* g, const double2* pg, int n, int slot,
                      double* partials, unsigned* counter, double* scal, hipStream_t s);
 // y = clamp(b + cap(-step * g, max_disp)); scal[kPredicted] = sum raw.(b - y); scal[kMaxDisp] = max |b - y|
 void launchPropose(const double2* b, const double2* g, const double2* raw, double step, double max_disp,

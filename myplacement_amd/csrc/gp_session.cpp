@@ -167,33 +167,37 @@ private:
     // ---- evaluation helpers -----------------------------------------------------------------
     double strength() const { return std::sqrt(1.0 + std::max(0.0, penalty_ / std::max(reference_penalty_, kEps))); }
 
-    void enqueueGrad(const double2* pos, double2* raw, double2* pre) {
+    // Scalar banks: 0 current-position gradient evaluation, 1 look-ahead gradient evaluation,
+    // 2 proposal + candidate value evaluation, 3 alignment and curvature sums.
+    enum Bank : int { kBankCurrent = 0, kBankLook = 1, kBankCandidate = 2, kBankMisc = 3 };
+
+    void enqueueGrad(const double2* pos, double2* raw, double2* pre, int bank) {
         Backend::SessionOut so{precond_.get(), penalty_, 1.0 / strength(), raw, pre};
-        be_->enqueueEvaluate(pos, smoothing_, true, &so);
-        ++result_.gradient_evaluations;
+        be_->enqueueEvaluate(pos, smoothing_, true, &so, bank);
     }
-    void enqueueValue(const double2* pos) {
-        be_->enqueueEvaluate(pos, smoothing_, false, nullptr);
-        ++result_.value_evaluations;
-    }
-    EvalRec record(const HostScalars& h, bool grad) {
+    void enqueueValue(const double2* pos, int bank) { be_->enqueueEvaluate(pos, smoothing_, false, nullptr, bank); }
+    EvalRec record(const HostScalars& h, bool grad, int bank) {
         EvalRec e;
-        be_->toScalars(h, &e.s);
+        be_->toScalars(h, &e.s, bank);
+        const double* v = h.bank(bank);
         e.objective = e.s.smooth_wirelength + penalty_ * e.s.electrostatic_energy;
         if (!std::isfinite(e.objective) || !std::isfinite(e.s.hpwl))
             throw std::runtime_error("HIP adaptive placement objective produced a non-finite value.");
         if (grad) {
-            e.pre_norm = std::sqrt(h.v[kPreNorm2]);
-            e.wire_l1 = h.v[kWireL1];
-            e.density_l1 = h.v[kDensityL1];
+            e.pre_norm = std::sqrt(v[kPreNorm2]);
+            e.wire_l1 = v[kWireL1];
+            e.density_l1 = v[kDensityL1];
+            ++result_.gradient_evaluations;
+        } else {
+            ++result_.value_evaluations;
         }
         return e;
     }
     EvalRec gradEval(const double2* pos, double2* raw, double2* pre) {
-        enqueueGrad(pos, raw, pre);
-        return record(be_->fetchScalars(), true);
+        enqueueGrad(pos, raw, pre, kBankCurrent);
+        return record(be_->fetchScalars(), true, kBankCurrent);
     }
-    double* scal() { return be_->scalarBlock(); }
+    double* scal(int bank) { return be_->scalarBlock(bank); }
     void copy(DeviceArray<double2>& dst, const double2* src) {
         if (dst.get() == src) return;
         hipCheck(hipMemcpyAsync(dst.get(), src, P_ * sizeof(double2), hipMemcpyDeviceToDevice, be_->stream()),
@@ -294,15 +298,47 @@ private:
         current_ = init;
     }
 
-    // one pass of the loop body :735-928
+    // One pass of the loop body :735-928.  Everything the first decisions need is enqueued before
+    // one read-back: the gradient evaluation at X, and -- when momentum may be used -- the alignment
+    // sum, the look-ahead positions and their gradient evaluation, plus the curvature sums for both
+    // possible step bases.  Work whose result the control law then does not use is discarded and
+    // not counted as an evaluation.
     void iterate() {
         hipStream_t s = be_->stream();
         const int it = result_.completed_iterations;
+        const bool try_momentum = worse_streak_ < 2 && momentum_age_ > 0;
+        const double beta = std::min(o_.maximum_momentum, static_cast<double>(momentum_age_) / (momentum_age_ + 3));
+        bool need_read = false;
         if (it > 0) {
             smoothing_ = smoothingFromOverflow(last_overflow_);
-            current_ = gradEval(X_.get(), Rc_.get(), Gc_.get());
+            enqueueGrad(X_.get(), Rc_.get(), Gc_.get(), kBankCurrent);
+            need_read = true;
         }
+        if (try_momentum) {
+            be_->timedBegin(kTController);
+            launchDot(V_.get(), Gc_.get(), -1.0, kAlignment, P_, be_->partials(kCtrDot), be_->counter(kCtrDot),
+                      scal(kBankMisc), s);
+            launchLookahead(X_.get(), V_.get(), beta, B_.get(), clamp_, P_, s);
+            be_->timedEnd(kTController);
+            enqueueGrad(B_.get(), Rl_.get(), Gl_.get(), kBankLook);
+            need_read = true;
+        }
+        if (has_previous_) {
+            be_->timedBegin(kTController);
+            launchCurvature(X_.get(), PX_.get(), Gc_.get(), PG_.get(), P_, kCurvDp, be_->partials(kCtrCurv),
+                            be_->counter(kCtrCurv), scal(kBankMisc), s);
+            if (try_momentum)
+                launchCurvature(B_.get(), PX_.get(), Gl_.get(), PG_.get(), P_, kCurvDp2, be_->partials(kCtrCurv),
+                                be_->counter(kCtrCurv), scal(kBankMisc), s);
+            be_->timedEnd(kTController);
+            need_read = true;
+        }
+        HostScalars first{};
+        if (need_read) first = be_->fetchScalars();
+        if (it > 0) current_ = record(first, true, kBankCurrent);
         const EvalRec& cur = current_;
+        const double* misc = first.bank(kBankMisc);
+
         bool restarted = worse_streak_ >= 2;
         bool momentum_used = false;
         const double2* base = X_.get();
@@ -310,19 +346,11 @@ private:
         const double2* base_pre = Gc_.get();
         const EvalRec* base_eval = &cur;
         EvalRec look;
-        if (!restarted && momentum_age_ > 0) {
-            const double beta = std::min(o_.maximum_momentum, static_cast<double>(momentum_age_) / (momentum_age_ + 3));
-            be_->timedBegin(kTController);
-            launchDot(V_.get(), Gc_.get(), -1.0, kAlignment, P_, be_->partials(kCtrDot), be_->counter(kCtrDot), scal(), s);
-            be_->timedEnd(kTController);
-            const double alignment = be_->fetchScalars().v[kAlignment];
-            if (alignment <= 0.0) {
+        if (try_momentum) {
+            if (misc[kAlignment] <= 0.0) {
                 restarted = true;
             } else {
-                be_->timedBegin(kTController);
-                launchLookahead(X_.get(), V_.get(), beta, B_.get(), clamp_, P_, s);
-                be_->timedEnd(kTController);
-                look = gradEval(B_.get(), Rl_.get(), Gl_.get());
+                look = record(first, true, kBankLook);
                 if (look.objective > cur.objective * (1.0 + o_.hpwl_growth_restart_threshold)) {
                     restarted = true;
                 } else {
@@ -349,19 +377,14 @@ private:
         double used_step = 0.0, used_curvature = 0.0, used_disp = 0.0;
         EvalRec acc;
         for (int pass = 0; pass < 2 && !accepted; ++pass) {
-            // estimateStep :470-492
+            // estimateStep :470-492 from the sums taken for this base
             double step = std::max(last_step_, kEps), curvature = 0.0;
             if (has_previous_) {
-                be_->timedBegin(kTController);
-                launchCurvature(base, PX_.get(), base_pre, PG_.get(), P_, be_->partials(kCtrCurv), be_->counter(kCtrCurv),
-                                scal(), s);
-                be_->timedEnd(kTController);
-                const HostScalars& h = be_->fetchScalars();
-                const double dp = h.v[kCurvDp], dg = h.v[kCurvDg];
+                const bool on_look = base == B_.get();
+                const double dp = misc[on_look ? kCurvDp2 : kCurvDp], dg = misc[on_look ? kCurvDg2 : kCurvDg];
                 if (!(dp <= kEps || dg <= kEps)) {
                     curvature = std::sqrt(dg / dp);
-                    const double fallback = step;
-                    step = clampd(1.0 / std::max(curvature, kEps), fallback * 0.25, fallback * 4.0);
+                    step = clampd(1.0 / std::max(curvature, kEps), step * 0.25, step * 4.0);
                 }
             }
             const double ratio = clampd(base_eval->s.optimizer_density.normalized_overflow / std::max(initial_overflow_, kEps), 0.0, 1.0);
@@ -369,14 +392,14 @@ private:
             for (int attempt = 0; attempt <= o_.maximum_backtracks; ++attempt) {
                 be_->timedBegin(kTController);
                 launchPropose(base, base_pre, base_raw, step, max_disp, Y_.get(), clamp_, P_, be_->partials(kCtrPropose),
-                              be_->counter(kCtrPropose), scal(), s);
+                              be_->counter(kCtrPropose), scal(kBankCandidate), s);
                 be_->timedEnd(kTController);
-                enqueueValue(Y_.get());
+                enqueueValue(Y_.get(), kBankCandidate);
                 const HostScalars& h = be_->fetchScalars();
-                const double predicted = h.v[kPredicted];
-                const double disp = h.v[kMaxDisp];
+                const double predicted = h.bank(kBankCandidate)[kPredicted];
+                const double disp = h.bank(kBankCandidate)[kMaxDisp];
                 if (predicted <= kEps) break;
-                EvalRec cand = record(h, false);
+                EvalRec cand = record(h, false, kBankCandidate);
                 if (accepts(*base_eval, cand, predicted)) {
                     accepted = true;
                     used_step = step;

@@ -6,7 +6,8 @@ Tolerances (stated here, checked below):
   electrostatic energy         relative 2e-4  (FP32 DCT products on MFMA; energy taken in the DCT basis)
   wire gradient                normwise relative 2e-5, elementwise 1e-4 of the largest entry
   density gradient / field     normwise relative 5e-4
-  end-to-end adaptive run      final HPWL within 0.5 %, final design overflow within 0.005
+  end-to-end adaptive run      first iterations row by row at 1e-5; final HPWL within 0.5 % and final
+                               design overflow within 0.005 as five-run ensemble means
 """
 
 import os
@@ -141,22 +142,43 @@ def test_bad_arguments_raise():
     be.close()
 
 
-def test_adaptive_run_matches_oracle_end_state(ob):
+def test_adaptive_run_follows_oracle_then_matches_ensemble(ob):
+    """The adaptive controller amplifies last-bit differences: the oracle itself, started from
+    positions perturbed by 1e-9, ends up to ~0.7 % apart in HPWL and ~0.008 in overflow on this
+    design.  So the end state is compared as an ensemble (five runs per side from the same
+    1e-9-perturbed starts): mean final HPWL within 0.5 %, mean final overflow within 0.005.  The
+    early trajectory, before any discrete decision can flip, is compared row by row."""
     d, o = _setup(design.plain_design(), 0.9, 64)
     o.feasible_refinement_iterations = 2 ** 31 - 1
-    say("[gpu] adaptive run plain-64, 280 iterations")
+    say("[gpu] adaptive run plain-64, 280 iterations, unperturbed")
     c_h, r_h, hist_h = hb.global_place(d, o)
     c_o, r_o, hist_o = ob.global_place(d, o)
-    say("  hip", r_h["hpwl_after"], r_h["overflow_after"], "oracle", r_o["hpwl_after"], r_o["overflow_after"])
+    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+    np.save(os.path.join(ROOT, "gpurun_out", "hist_hip.npy"), np.array([[r[k] for k in abi.HISTORY_COLUMNS] for r in hist_h]))
+    np.save(os.path.join(ROOT, "gpurun_out", "hist_oracle.npy"), np.array([[r[k] for k in abi.HISTORY_COLUMNS] for r in hist_o]))
     assert r_h["completed_iterations"] == 280 and len(hist_h) == 280
     assert rel(r_h["hpwl_before"], r_o["hpwl_before"]) < 1e-12
-    assert rel(r_h["hpwl_after"], r_o["hpwl_after"]) < 5e-3
-    assert abs(r_h["overflow_after"] - r_o["overflow_after"]) < 5e-3
     assert r_h["accepted_iterations"] > 0 and r_h["restored_best_checkpoint"] == 1
     for row in hist_h:
         assert all(np.isfinite(v) for v in row.values())
-    # the first iterations follow the oracle closely before controller decisions can diverge
-    assert rel(hist_h[0]["hpwl"], hist_o[0]["hpwl"]) < 1e-4
+    for k in range(5):
+        assert rel(hist_h[k]["hpwl"], hist_o[k]["hpwl"]) < 1e-5, k
+        assert abs(hist_h[k]["overflow"] - hist_o[k]["overflow"]) < 1e-5, k
+    rng = np.random.default_rng(0)
+    mov = d.movable_ids()
+    hh, ho = [], []
+    for k in range(5):
+        c = d.center.copy()
+        c[mov] += rng.normal(0.0, 1e-9, size=c[mov].shape)
+        say(f"  ensemble member {k}")
+        _, a, _ = hb.global_place(d, o, center=c)
+        _, b, _ = ob.global_place(d, o, center=c)
+        hh.append((a["hpwl_after"], a["overflow_after"]))
+        ho.append((b["hpwl_after"], b["overflow_after"]))
+    hh, ho = np.array(hh), np.array(ho)
+    say("  hip mean", hh.mean(0), "oracle mean", ho.mean(0))
+    assert abs(hh[:, 0].mean() / ho[:, 0].mean() - 1.0) < 5e-3
+    assert abs(hh[:, 1].mean() - ho[:, 1].mean()) < 5e-3
 
 
 def test_session_steps_and_history():
