@@ -101,34 +101,47 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     // ---- host-side derived arrays ----
     if (N_ > 0 && (L.net_offsets == nullptr || L.net_offsets[0] != 0 || L.net_offsets[N_] != I_))
         throw InvalidArgument("net offsets do not cover the pin array");
-    std::vector<int> small, large;
-    for (int n = 0; n < N_; ++n) {
-        const int deg = L.net_offsets[n + 1] - L.net_offsets[n];
-        if (deg < 0) throw InvalidArgument("net offsets are not monotone");
-        if (deg < 2) continue;
-        (deg <= kSmallDegree ? small : large).push_back(n);
-    }
-    n_small_ = static_cast<int>(small.size());
-    n_large_ = static_cast<int>(large.size());
-    // particle -> pin CSR over movable pins of nets with >= 2 pins, pins in net order
-    std::vector<int> count(static_cast<std::size_t>(P_) + 1, 0);
-    for (int n = 0; n < N_; ++n) {
-        if (L.net_offsets[n + 1] - L.net_offsets[n] < 2) continue;
+    // Internal net order for the wirelength kernel: degree classes 2..8, 9..32, >32 (nets with fewer
+    // than two pins contribute nothing and are dropped, .cu:251-253, :324-326); pins follow.
+    std::vector<int> order;
+    for (int cls = 0; cls < 3; ++cls)
+        for (int n = 0; n < N_; ++n) {
+            const int deg = L.net_offsets[n + 1] - L.net_offsets[n];
+            if (deg < 0) throw InvalidArgument("net offsets are not monotone");
+            if (deg < 2) continue;
+            const int c = deg <= kSmallDegree ? 0 : (deg <= kMediumDegree ? 1 : 2);
+            if (c != cls) continue;
+            order.push_back(n);
+            (cls == 0 ? n_small_ : (cls == 1 ? n_medium_ : n_large_)) += 1;
+        }
+    std::vector<int> inet_off(order.size() + 1, 0);
+    std::vector<double> inet_w(order.size());
+    std::vector<int> ipin_part;
+    std::vector<double2> ipin_off;
+    ipin_part.reserve(I_);
+    ipin_off.reserve(I_);
+    for (std::size_t k = 0; k < order.size(); ++k) {
+        const int n = order[k];
+        inet_w[k] = L.net_weights[n];
         for (int p = L.net_offsets[n]; p < L.net_offsets[n + 1]; ++p) {
             const int part = L.pin_particles[p];
             if (part >= P_) throw InvalidArgument("pin refers to an unknown particle");
-            if (part >= 0) ++count[part + 1];
+            ipin_part.push_back(part);
+            ipin_off.push_back(make_double2(L.pin_offsets_or_static[2 * p], L.pin_offsets_or_static[2 * p + 1]));
         }
+        inet_off[k + 1] = static_cast<int>(ipin_part.size());
     }
+    const std::size_t Ii = ipin_part.size();
+    // particle -> pin CSR over the movable pins, pins in internal order
+    std::vector<int> count(static_cast<std::size_t>(P_) + 1, 0);
+    for (std::size_t q = 0; q < Ii; ++q)
+        if (ipin_part[q] >= 0) ++count[ipin_part[q] + 1];
     std::partial_sum(count.begin(), count.end(), count.begin());
     std::vector<int> ppin_idx(static_cast<std::size_t>(count[P_]));
     {
         std::vector<int> fill(count.begin(), count.end() - 1);
-        for (int n = 0; n < N_; ++n) {
-            if (L.net_offsets[n + 1] - L.net_offsets[n] < 2) continue;
-            for (int p = L.net_offsets[n]; p < L.net_offsets[n + 1]; ++p)
-                if (L.pin_particles[p] >= 0) ppin_idx[fill[L.pin_particles[p]]++] = p;
-        }
+        for (std::size_t q = 0; q < Ii; ++q)
+            if (ipin_part[q] >= 0) ppin_idx[fill[ipin_part[q]]++] = static_cast<int>(q);
     }
     std::vector<double2> sizes(P_);
     for (int i = 0; i < P_; ++i) sizes[i] = make_double2(L.particle_width[i], L.particle_height[i]);
@@ -152,17 +165,35 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     size_.allocate(P_, budget_, "particle sizes");
     charge_.allocate(P_, budget_, "particle charge areas");
     layer_.allocate(P_, budget_, "particle layers");
-    net_off_.allocate(static_cast<std::size_t>(N_) + 1, budget_, "net offsets");
-    net_w_.allocate(N_, budget_, "net weights");
-    small_nets_.allocate(n_small_, budget_, "small-net list");
-    large_nets_.allocate(n_large_, budget_, "large-net list");
-    pin_part_.allocate(I_, budget_, "pin-particle map");
-    pin_off_.allocate(I_, budget_, "pin offsets");
-    pin_grad_.allocate(I_, budget_, "per-pin wirelength gradient");
-    ppin_off_.allocate(static_cast<std::size_t>(P_) + 1, budget_, "particle-pin offsets");
+    net_off_.allocate(order.size() + 1, budget_, "net offsets");
+    net_w_.allocate(order.size(), budget_, "net weights");
+    pin_part_.allocate(Ii, budget_, "pin-particle map");
+    pin_off_.allocate(Ii, budget_, "pin offsets");
+    pin_grad_.allocate(Ii, budget_, "per-pin wirelength gradient");
     ppin_idx_.allocate(ppin_idx.size(), budget_, "particle-pin index");
-    movable_.allocate(G, budget_, "movable density layer");
-    filler_.allocate(G, budget_, "filler density layer");
+    ppin_off_.allocate(static_cast<std::size_t>(P_) + 1, budget_, "particle-pin offsets");
+    // density band geometry: bands of band_rows_ bin rows, one workgroup each, both fixed-point
+    // layers of a band in LDS (<= 64 KiB); particles taller than two bin rows take the residual path
+    if (nx_ > 4096) throw InvalidArgument("density grids wider than 4096 bins are not supported by the band kernel");
+    if (ny_ > 8192) throw InvalidArgument("density grids taller than 8192 bins are not supported by the row sort");
+    band_rows_ = std::max(1, std::min(ny_ / 256, 4096 / nx_));
+    bands_ = (ny_ + band_rows_ - 1) / band_rows_;
+    fix_ = std::ldexp(1.0, 40) / (region_.bw * region_.bh);
+    std::vector<unsigned char> tall(P_);
+    for (int i = 0; i < P_; ++i) {
+        tall[i] = L.particle_height[i] > 2.0 * region_.bh ? 1 : 0;
+        has_tall_ = has_tall_ || tall[i];
+    }
+    const int hist_blocks = densityHistBlocks(P_);
+    key_.allocate(P_, budget_, "density sort keys");
+    hist_.allocate(static_cast<std::size_t>(hist_blocks) * ny_, budget_, "density row histograms");
+    colprefix_.allocate(static_cast<std::size_t>(hist_blocks) * ny_, budget_, "density row prefixes");
+    keytotal_.allocate(ny_, budget_, "density row totals");
+    keystart_.allocate(static_cast<std::size_t>(ny_) + 1, budget_, "density row starts");
+    sorted_.allocate(P_, budget_, "row-sorted clipped rectangles");
+    sorted_span_.allocate(P_, budget_, "row-sorted bin spans");
+    tall_.allocate(P_, budget_, "tall-particle flags");
+    residual_.allocate(has_tall_ ? 2 * G : 1, budget_, "tall-particle residual layers");
     static_charge_.allocate(G, budget_, "static density charge");
     const std::size_t Gp = static_cast<std::size_t>(nxp_) * nyp_;
     rho_.allocate(Gp, budget_, "density deviation grid");
@@ -177,10 +208,11 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     bxi_.allocate(static_cast<std::size_t>(nxp_) * 2 * nxp_, budget_, "inverse x factors");
     ayi_.allocate(static_cast<std::size_t>(nyp_) * nyp_, budget_, "inverse y cosine factor");
     syi_.allocate(static_cast<std::size_t>(nyp_) * nyp_, budget_, "inverse y sine factor");
-    const int max_blocks = std::max({wireBlocks(n_small_, n_large_), metricsBlocks(static_cast<int>(G)),
+    const int max_blocks = std::max({wireBlocks(n_small_, n_medium_, n_large_), bands_,
                                      particleBlocks(P_), (nxp_ / 32) * (nyp_ / 32), 1});
-    for (int k = 0; k < kCounterCount; ++k) partials_[k].allocate(static_cast<std::size_t>(max_blocks) * 10, budget_, "reduction partials");
-    counters_.allocate(kCounterCount, budget_, "reduction counters");
+    for (int k = 0; k < kRedKinds * kScalarBanks; ++k)
+        partials_[k].allocate(static_cast<std::size_t>(max_blocks) * redWidth(k / kScalarBanks), budget_,
+                              "reduction partials");
     scal_.allocate(kScalarCount * kScalarBanks, budget_, "scalar block");
     wire_out_.allocate(P_, budget_, "wire gradient");
     density_out_.allocate(P_, budget_, "density gradient");
@@ -192,14 +224,10 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     size_.upload(sizes.data(), P_, stream_, "upload particle sizes");
     charge_.upload(L.particle_charge_area, P_, stream_, "upload particle charges");
     layer_.upload(layers.data(), P_, stream_, "upload particle layers");
-    if (N_ > 0) {
-        net_off_.upload(L.net_offsets, static_cast<std::size_t>(N_) + 1, stream_, "upload net offsets");
-        net_w_.upload(L.net_weights, N_, stream_, "upload net weights");
-    }
-    small_nets_.upload(small.data(), small.size(), stream_, "upload small nets");
-    large_nets_.upload(large.data(), large.size(), stream_, "upload large nets");
-    pin_part_.upload(L.pin_particles, I_, stream_, "upload pin-particle map");
-    pin_off_.upload(reinterpret_cast<const double2*>(L.pin_offsets_or_static), I_, stream_, "upload pin offsets");
+    net_off_.upload(inet_off.data(), inet_off.size(), stream_, "upload net offsets");
+    net_w_.upload(inet_w.data(), inet_w.size(), stream_, "upload net weights");
+    pin_part_.upload(ipin_part.data(), Ii, stream_, "upload pin-particle map");
+    pin_off_.upload(ipin_off.data(), Ii, stream_, "upload pin offsets");
     ppin_off_.upload(count.data(), count.size(), stream_, "upload particle-pin offsets");
     ppin_idx_.upload(ppin_idx.data(), ppin_idx.size(), stream_, "upload particle-pin index");
     static_charge_.upload(static_charge.data(), G, stream_, "upload static density");
@@ -221,10 +249,10 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     bxi_.upload(bx2.data(), bx2.size(), stream_, "upload inverse x factors");
     ayi_.upload(iy.data(), iy.size(), stream_, "upload inverse y factor");
     syi_.upload(sy.data(), sy.size(), stream_, "upload inverse y sine factor");
-    hipCheck(hipMemsetAsync(movable_.get(), 0, G * sizeof(double), stream_), "clear density layer");
-    hipCheck(hipMemsetAsync(filler_.get(), 0, G * sizeof(double), stream_), "clear density layer");
+    tall_.upload(tall.data(), tall.size(), stream_, "upload tall-particle flags");
+    hipCheck(hipMemsetAsync(residual_.get(), 0, residual_.size() * sizeof(unsigned long long), stream_),
+             "clear residual density layers");
     hipCheck(hipMemsetAsync(rho_.get(), 0, Gp * sizeof(float), stream_), "clear density grid");
-    hipCheck(hipMemsetAsync(counters_.get(), 0, kCounterCount * sizeof(unsigned), stream_), "clear counters");
     hipCheck(hipMemsetAsync(scal_.get(), 0, kScalarCount * kScalarBanks * sizeof(double), stream_), "clear scalars");
     hipCheck(hipStreamSynchronize(stream_), "finish backend upload");
 }
@@ -271,7 +299,7 @@ void Backend::enableTiming(bool on) {
 }
 
 std::vector<std::string> Backend::timingNames() const {
-    return {"k_wire", "k_scatter", "k_metrics", "k_gemm", "k_gather", "controller"};
+    return {"k_wire", "k_density", "k_gemm", "k_gather", "controller"};
 }
 
 void Backend::timing(std::vector<double>& total_us, std::vector<long long>& launches, std::vector<double>& work) {
@@ -303,28 +331,52 @@ void Backend::timing(std::vector<double>& total_us, std::vector<long long>& laun
 void Backend::enqueueEvaluate(const double2* pos, double gamma, bool want_grad, const SessionOut* session, int bank) {
     hipStream_t s = stream_;
     double* scal = scal_.get() + bank * kScalarCount;
-    WireArgs wa{pos, net_off_.get(), net_w_.get(), pin_part_.get(), pin_off_.get(), small_nets_.get(), n_small_,
-                large_nets_.get(), n_large_, gamma, want_grad ? 1 : 0, pin_grad_.get(), partials_[kCtrWire].get(),
-                counters_.get() + kCtrWire, scal};
+    WireArgs wa{pos, net_off_.get(), net_w_.get(), pin_part_.get(), pin_off_.get(), n_small_, n_medium_, n_large_,
+                gamma, want_grad ? 1 : 0, pin_grad_.get(), partials(kRedWire, bank)};
     timedBegin(kTWire);
-    if (n_small_ + n_large_ > 0) launchWire(wa, s);
-    else launchZero(scal + kHpwl, 2, s);
+    if (n_small_ + n_medium_ + n_large_ > 0) {
+        launchWire(wa, s);
+        addReduction(kRedWire, bank, wireBlocks(n_small_, n_medium_, n_large_));
+    } else {
+        launchZero(scal + kHpwl, 2, s);
+    }
     // algorithmic bytes: 12 per pin, 8 per net, 16 (8 value-only) per particle
     timedEnd(kTWire, 12.0 * I_ + 8.0 * N_ + (want_grad ? 16.0 : 8.0) * P_);
 
-    ScatterArgs sa{pos, size_.get(), layer_.get(), P_, region_, t_, movable_.get(), filler_.get()};
-    timedBegin(kTScatter);
-    launchScatter(sa, s);
-    timedEnd(kTScatter, 20.0 * P_ + 12.0 * static_cast<double>(nx_) * ny_);
-    MetricsArgs ma{movable_.get(), filler_.get(), static_charge_.get(), region_, t_, dark_total_, placeable_total_,
-                   rho_.get(), nxp_, partials_[kCtrMetrics].get(), counters_.get() + kCtrMetrics, scal};
-    timedBegin(kTMetrics);
-    launchMetrics(ma, s);
-    timedEnd(kTMetrics, 24.0 * static_cast<double>(nx_) * ny_);
+    DensityArgs da{};
+    da.pos = pos;
+    da.size = size_.get();
+    da.layer = layer_.get();
+    da.tall = tall_.get();
+    da.n = P_;
+    da.r = region_;
+    da.t = t_;
+    da.fix = fix_;
+    da.inv_fix = 1.0 / fix_;
+    da.band_rows = band_rows_;
+    da.bands = bands_;
+    da.lookback = kBandLookback;
+    da.has_tall = has_tall_ ? 1 : 0;
+    da.key = key_.get();
+    da.hist = hist_.get();
+    da.colprefix = colprefix_.get();
+    da.keytotal = keytotal_.get();
+    da.keystart = keystart_.get();
+    da.sorted = sorted_.get();
+    da.sorted_span = sorted_span_.get();
+    da.residual = residual_.get();
+    da.static_charge = static_charge_.get();
+    da.rho = rho_.get();
+    da.ldrho = nxp_;
+    da.partials = partials(kRedMetrics, bank);
+    timedBegin(kTDensity);
+    launchDensity(da, s);
+    // algorithmic bytes of the scatter (20 per particle, 12 per bin) and the metrics (24 per bin)
+    timedEnd(kTDensity, 20.0 * P_ + 36.0 * static_cast<double>(nx_) * ny_);
+    addReduction(kRedMetrics, bank, bands_, 0, static_cast<double>(nx_) * ny_, dark_total_, placeable_total_);
 
     const double W = region_.urx - region_.llx, H = region_.ury - region_.lly;
-    SpectralArgs sp{nx_, ny_, W, H, 1.0 / (4.0 * nx_ * static_cast<double>(ny_)), 0.5 * region_.bw * region_.bh,
-                    partials_[kCtrEnergy].get(), counters_.get() + kCtrEnergy, scal};
+    SpectralArgs sp{nx_, ny_, W, H, 1.0 / (4.0 * nx_ * static_cast<double>(ny_)), partials(kRedEnergy, bank)};
     auto gemm = [&](const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epi) {
         timedBegin(kTDct);
         launchGemm(g, batch, M, N, K, lda, ldb, ldc, epi, sp, dct_precision_, s);
@@ -335,6 +387,7 @@ void Backend::enqueueEvaluate(const double2* pos, double gamma, bool want_grad, 
     gemm(f1, 1, nyp_, nxp_, nxp_, nxp_, nxp_, nxp_, kEpiStore);
     GemmBatch f2{{ayf_.get()}, {tmp_.get()}, {phi_.get()}};
     gemm(f2, 1, nyp_, nxp_, nyp_, nyp_, nxp_, nxp_, kEpiSpectral);
+    addReduction(kRedEnergy, bank, (nxp_ / 32) * (nyp_ / 32), 0, 0.5 * region_.bw * region_.bh);
     if (want_grad) {
         // [T1 | T2] = phihat * [Cx_inv | Sx_inv] ; Ex = Ay_inv * T2 ; Ey = Sy_inv * T1
         GemmBatch i1{{phi_.get()}, {bxi_.get()}, {t12_.get()}};
@@ -366,15 +419,21 @@ void Backend::enqueueEvaluate(const double2* pos, double gamma, bool want_grad, 
         ga.wire_out = wire_out_.get();
         ga.density_out = density_out_.get();
     }
-    ga.partials = partials_[kCtrGather].get();
-    ga.counter = counters_.get() + kCtrGather;
-    ga.scal = scal;
+    ga.partials = partials(kRedGather, bank);
     timedBegin(kTGather);
     launchGather(ga, s);
+    addReduction(kRedGather, bank, particleBlocks(P_));
     timedEnd(kTGather, 20.0 * P_ + 8.0 * static_cast<double>(nx_) * ny_);
 }
 
+void Backend::addReduction(int kind, int bank, int blocks, int slot, double p0, double p1, double p2) {
+    if (pending_.n == kMaxRedJobs) throw std::logic_error("too many pending reductions");
+    pending_.job[pending_.n++] = RedJob{partials(kind, bank), blocks, kind, scalarBlock(bank), slot, p0, p1, p2};
+}
+
 const HostScalars& Backend::fetchScalars() {
+    launchFinalize(pending_, stream_);
+    pending_.n = 0;
     hipCheck(hipMemcpyAsync(host_scal_, scal_.get(), sizeof(HostScalars), hipMemcpyDeviceToHost, stream_),
              "read scalar block");
     hipCheck(hipStreamSynchronize(stream_), "synchronize placement evaluation");

@@ -108,8 +108,10 @@ public:
     DeviceBudget& budget() { return budget_; }
     const double2* sizes() const { return size_.get(); }
     double* scalarBlock(int bank = 0) const { return scal_.get() + bank * kScalarCount; }
-    double* partials(int which) const { return partials_[which].get(); }
-    unsigned* counter(int which) const { return counters_.get() + which; }
+    // Partial rows of reduction `kind` destined for scalar bank `bank`.
+    double* partials(int kind, int bank) const { return partials_[kind * kScalarBanks + bank].get(); }
+    // Queues the final reduction of `blocks` partial rows; done by the next fetchScalars().
+    void addReduction(int kind, int bank, int blocks, int slot = 0, double p0 = 0, double p1 = 0, double p2 = 0);
 
     // Optional per-kernel timing with HIP events on the backend's stream.
     void enableTiming(bool on);
@@ -135,22 +137,30 @@ private:
     DeviceArray<double2> pos_, size_;
     DeviceArray<double> charge_;
     DeviceArray<int> layer_;
-    DeviceArray<int> net_off_, small_nets_, large_nets_, pin_part_, ppin_off_, ppin_idx_;
-    int n_small_ = 0, n_large_ = 0;
+    DeviceArray<int> net_off_, pin_part_, ppin_off_, ppin_idx_;
+    int n_small_ = 0, n_medium_ = 0, n_large_ = 0;
     DeviceArray<double> net_w_;
     DeviceArray<double2> pin_off_;
     DeviceArray<float2> pin_grad_;
-    DeviceArray<double> movable_, filler_, static_charge_;
+    DeviceArray<double> static_charge_;
+    DeviceArray<int> key_, hist_, colprefix_, keytotal_, keystart_;
+    DeviceArray<double4> sorted_;
+    DeviceArray<int4> sorted_span_;
+    DeviceArray<unsigned char> tall_;
+    DeviceArray<unsigned long long> residual_;
+    int band_rows_ = 1, bands_ = 1;
+    bool has_tall_ = false;
+    double fix_ = 1.0;
     DeviceArray<float> rho_, tmp_, phi_, t12_, ex_, ey_, pot_;
     DeviceArray<float> bxf_, ayf_, bxi_, ayi_, syi_;
-    DeviceArray<double> partials_[kCounterCount];
-    DeviceArray<unsigned> counters_;
+    DeviceArray<double> partials_[kRedKinds * kScalarBanks];
+    RedList pending_{};
     DeviceArray<double> scal_;
     DeviceArray<double2> wire_out_, density_out_;
     HostScalars* host_scal_ = nullptr;
 
     // timing
-    static constexpr int kTimingClasses = 6;
+    static constexpr int kTimingClasses = 5;
     bool timing_on_ = false;
     std::vector<hipEvent_t> ev_begin_[kTimingClasses], ev_end_[kTimingClasses];
     std::vector<double> t_total_, t_work_;
@@ -158,7 +168,7 @@ private:
     std::vector<long long> t_count_;
 };
 
-enum TimingClass : int { kTWire = 0, kTScatter = 1, kTMetrics = 2, kTDct = 3, kTGather = 4, kTController = 5 };
+enum TimingClass : int { kTWire = 0, kTDensity = 1, kTDct = 2, kTGather = 3, kTController = 4 };
 
 }  // namespace mpgp
 

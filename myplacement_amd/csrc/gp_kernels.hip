@@ -3,8 +3,9 @@
 // Reference counterparts (src/placement/cuda/CudaPlacementBackend.cu unless noted):
 //   k_wire      buildPinPositions + computeNetBounds + computeNetSums + computePinGradient (:194-337),
 //               CPU form AdaptiveGlobalPlacer.cpp:29-82
-//   k_scatter   depositParticlesKernel (:339-375), CPU DensityMap::addRectangle Metrics.cpp:74-99
-//   k_metrics   buildDensityMetricsKernel + 10 reductions (:377-398, :1022-1038), Metrics.cpp:101-119
+//   k_dhist / k_dscan / k_dplace / k_dband
+//               depositParticlesKernel + buildDensityMetricsKernel + 10 reductions (:339-398,
+//               :1022-1038); CPU DensityMap::addRectangle / metrics, Metrics.cpp:74-119
 //   k_gemm      forwardDct / solveSpectralKernel / inverse2d (:547-567, :679-742); CPU
 //               ElectrostaticField.cpp:205-237 (FFTW REDFT10 / REDFT01 / RODFT01 as dense products)
 //   k_gather    sampleDensityGradientKernel (:575-591) + the gradient combine of
@@ -12,8 +13,10 @@
 //   controller  proposeStep / estimateStep / momentumAlignment / look-ahead (AdaptiveGlobalPlacer.cpp
 //               :470-519, :610-617, :776-780)
 //
-// Every reduction writes one partial per workgroup; the last workgroup to finish (atomic ticket)
-// reduces them in index order, so the scalars are deterministic for a given set of partials.
+// Every reduction writes one partial row per workgroup; k_finalize, launched once before each
+// read-back, reduces them in index order, so the scalars are deterministic for given partials.
+// No hot kernel issues a device-scope fence or atomic: on MI355X those reach across the eight
+// XCDs' L2 caches and cost more than the kernels' own work.
 
 #include "gp_kernels.hpp"
 
@@ -27,6 +30,8 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
+constexpr int kFinalizeThreads = 1024;
+constexpr int kGemmGroup = 4;
 constexpr double kEps = 1e-9;
 constexpr double kPi = 3.141592653589793238462643383279502884;
 
@@ -55,10 +60,8 @@ __device__ inline float waveSumF(float v) {
     return v;
 }
 
-__device__ inline double loadVolatile(const double* p) { return *reinterpret_cast<const volatile double*>(p); }
-
 // Sum (or max, bit k of max_mask) of K values over a 256-thread block; result in thread 0.
-template <int K>
+template <int K, int NW = kWaves>
 __device__ void blockReduce(double (&v)[K], unsigned max_mask, double* lds) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -67,44 +70,51 @@ __device__ void blockReduce(double (&v)[K], unsigned max_mask, double* lds) {
 #pragma unroll
         for (int k = 0; k < K; ++k) lds[wave * K + k] = v[k];
     __syncthreads();
+    // thread k < K combines value k over the wavefronts (in wave order), then thread 0 collects
+    if (threadIdx.x < K) {
+        const int k = threadIdx.x;
+        double a = lds[k];
+        for (int w = 1; w < NW; ++w) a = ((max_mask >> k) & 1U) ? fmax(a, lds[w * K + k]) : a + lds[w * K + k];
+        lds[NW * K + k] = a;
+    }
+    __syncthreads();
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            double a = lds[k];
-            for (int w = 1; w < kWaves; ++w) a = ((max_mask >> k) & 1U) ? fmax(a, lds[w * K + k]) : a + lds[w * K + k];
-            v[k] = a;
-        }
+        for (int k = 0; k < K; ++k) v[k] = lds[NW * K + k];
     __syncthreads();
 }
 
-// Publishes this block's K partials; the last block reduces all of them (index order) into tot.
-// Returns true in every thread of the last block; tot is valid in thread 0.
-template <int K>
-__device__ bool finishReduction(double (&v)[K], unsigned max_mask, double init_max, int block, int blocks,
-                                double* partials, unsigned* counter, double* lds, double (&tot)[K]) {
-    __shared__ int last;
-    blockReduce<K>(v, max_mask, lds);
-    if (threadIdx.x == 0) {
+// Writes this block's K partials (one row per workgroup).  No fence: the read happens in a later
+// kernel (k_finalize), after the kernel boundary has made the writes visible to every XCD.
+template <int K, int NW = kWaves>
+__device__ void publishPartials(double (&v)[K], unsigned max_mask, int block, double* partials, double* lds) {
+    blockReduce<K, NW>(v, max_mask, lds);
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < K; ++k) partials[static_cast<size_t>(block) * K + k] = v[k];
-        __threadfence();
-        const unsigned ticket = atomicAdd(counter, 1U);
-        last = ticket == static_cast<unsigned>(blocks - 1);
+}
+
+// Inclusive scan of one int per thread over a block of NW wavefronts; returns the thread's
+// inclusive prefix, *total receives the block sum.
+template <int NW>
+__device__ int blockScan(int v, int* lds, int* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o);
+        if (lane >= o) v += u;
+    }
+    if (lane == 63) lds[wave] = v;
+    __syncthreads();
+    int before = 0, sum = 0;
+    for (int w = 0; w < NW; ++w) {
+        const int t = lds[w];
+        if (w < wave) before += t;
+        sum += t;
     }
     __syncthreads();
-    if (!last) return false;
-    __threadfence();
-#pragma unroll
-    for (int k = 0; k < K; ++k) tot[k] = ((max_mask >> k) & 1U) ? init_max : 0.0;
-    for (int b = threadIdx.x; b < blocks; b += kThreads)
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const double x = loadVolatile(partials + static_cast<size_t>(b) * K + k);
-            tot[k] = ((max_mask >> k) & 1U) ? fmax(tot[k], x) : tot[k] + x;
-        }
-    blockReduce<K>(tot, max_mask, lds);
-    if (threadIdx.x == 0) *counter = 0U;
-    return true;
+    *total = sum;
+    return v + before;
 }
 
 __device__ inline double2 pinPosition(const double2* pos, const int* pin_part, const double2* pin_off, int p) {
@@ -116,30 +126,103 @@ __device__ inline double2 pinPosition(const double2* pos, const int* pin_part, c
 }
 
 // ---------------------------------------------------------------------------------------------
-// Wirelength: small nets one thread each, large nets one wavefront each (same launch).
-__global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int small_blocks) {
-    __shared__ double lds[kWaves * 2];
+// Wirelength.  Nets arrive ordered by degree class (the host permutes them once): nets of degree
+// 2..8 one thread each with their pins held in registers; 9..32 one half-wavefront (32 lanes)
+// each; larger nets one wavefront each.  Max/min and positions in FP64; exponentials, their sums
+// and the log in FP32; per-net HPWL and smooth WL accumulated in FP64.
+
+__device__ inline float ex32(double d, double inv_gamma) { return __expf(static_cast<float>(d * inv_gamma)); }
+
+template <int W>
+__device__ inline double groupMax(double v) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, W));
+    return v;
+}
+template <int W>
+__device__ inline double groupMin(double v) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, W));
+    return v;
+}
+template <int W>
+__device__ inline float groupSum(float v) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+    return v;
+}
+
+// One net on a group of W lanes (W = 32 or 64); returns (hpwl, smooth) in lane 0 of the group.
+template <int W>
+__device__ void wireGroup(const WireArgs& a, int net, int sub, double inv_gamma, float gamma_f, double& hp, double& wl) {
+    const int b = a.net_off[net], e = a.net_off[net + 1];
+    double mx = -INFINITY, nx = INFINITY, my = -INFINITY, ny = INFINITY;
+    for (int p = b + sub; p < e; p += W) {
+        const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, p);
+        mx = fmax(mx, q.x); nx = fmin(nx, q.x);
+        my = fmax(my, q.y); ny = fmin(ny, q.y);
+    }
+    mx = groupMax<W>(mx); nx = groupMin<W>(nx); my = groupMax<W>(my); ny = groupMin<W>(ny);
+    float spx = 0.f, snx = 0.f, spy = 0.f, sny = 0.f;
+    for (int p = b + sub; p < e; p += W) {
+        const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, p);
+        spx += ex32(q.x - mx, inv_gamma);
+        snx += ex32(nx - q.x, inv_gamma);
+        spy += ex32(q.y - my, inv_gamma);
+        sny += ex32(ny - q.y, inv_gamma);
+    }
+    spx = groupSum<W>(spx); snx = groupSum<W>(snx); spy = groupSum<W>(spy); sny = groupSum<W>(sny);
+    const double w = a.net_w[net];
+    if (sub == 0) {
+        const double lse = static_cast<double>(gamma_f * (__logf(spx) + __logf(snx) + __logf(spy) + __logf(sny)));
+        wl = w * ((mx - nx) + (my - ny) + lse);
+        hp = w * ((mx - nx) + (my - ny));
+    }
+    if (a.want_grad) {
+        const float wf = static_cast<float>(w);
+        const float ipx = wf / spx, inx = wf / snx, ipy = wf / spy, iny = wf / sny;
+        for (int p = b + sub; p < e; p += W) {
+            const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, p);
+            a.pin_grad[p] = make_float2(ex32(q.x - mx, inv_gamma) * ipx - ex32(nx - q.x, inv_gamma) * inx,
+                                        ex32(q.y - my, inv_gamma) * ipy - ex32(ny - q.y, inv_gamma) * iny);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int small_blocks, int medium_blocks) {
+    __shared__ double lds[(kWaves + 1) * 2];
     double wl = 0.0, hp = 0.0;
     const double inv_gamma = 1.0 / a.gamma;
     const float gamma_f = static_cast<float>(a.gamma);
-    if (static_cast<int>(blockIdx.x) < small_blocks) {
-        const int idx = blockIdx.x * kThreads + threadIdx.x;
-        if (idx < a.n_small) {
-            const int net = a.small_nets[idx];
-            const int b = a.net_off[net], e = a.net_off[net + 1];
+    const int block = blockIdx.x;
+    if (block < small_blocks) {
+        const int net = block * kThreads + threadIdx.x;
+        if (net < a.n_small) {
+            const int b = a.net_off[net];
+            const int deg = a.net_off[net + 1] - b;
+            double qx[kSmallDegree], qy[kSmallDegree];
             double mx = -INFINITY, nx = INFINITY, my = -INFINITY, ny = INFINITY;
-            for (int p = b; p < e; ++p) {
-                const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, p);
-                mx = fmax(mx, q.x); nx = fmin(nx, q.x);
-                my = fmax(my, q.y); ny = fmin(ny, q.y);
+#pragma unroll
+            for (int i = 0; i < kSmallDegree; ++i) {
+                if (i < deg) {
+                    const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, b + i);
+                    qx[i] = q.x;
+                    qy[i] = q.y;
+                    mx = fmax(mx, q.x); nx = fmin(nx, q.x);
+                    my = fmax(my, q.y); ny = fmin(ny, q.y);
+                }
             }
+            float epx[kSmallDegree], enx[kSmallDegree], epy[kSmallDegree], eny[kSmallDegree];
             float spx = 0.f, snx = 0.f, spy = 0.f, sny = 0.f;
-            for (int p = b; p < e; ++p) {
-                const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, p);
-                spx += __expf(static_cast<float>((q.x - mx) * inv_gamma));
-                snx += __expf(static_cast<float>((nx - q.x) * inv_gamma));
-                spy += __expf(static_cast<float>((q.y - my) * inv_gamma));
-                sny += __expf(static_cast<float>((ny - q.y) * inv_gamma));
+#pragma unroll
+            for (int i = 0; i < kSmallDegree; ++i) {
+                if (i < deg) {
+                    epx[i] = ex32(qx[i] - mx, inv_gamma);
+                    enx[i] = ex32(nx - qx[i], inv_gamma);
+                    epy[i] = ex32(qy[i] - my, inv_gamma);
+                    eny[i] = ex32(ny - qy[i], inv_gamma);
+                    spx += epx[i]; snx += enx[i]; spy += epy[i]; sny += eny[i];
+                }
             }
             const double w = a.net_w[net];
             const double lse = static_cast<double>(gamma_f * (__logf(spx) + __logf(snx) + __logf(spy) + __logf(sny)));
@@ -147,120 +230,203 @@ __global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int small_blocks)
             hp = w * ((mx - nx) + (my - ny));
             if (a.want_grad) {
                 const float wf = static_cast<float>(w);
-                const float ipx = 1.f / spx, inx = 1.f / snx, ipy = 1.f / spy, iny = 1.f / sny;
-                for (int p = b; p < e; ++p) {
-                    const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, p);
-                    const float gx = __expf(static_cast<float>((q.x - mx) * inv_gamma)) * ipx -
-                                     __expf(static_cast<float>((nx - q.x) * inv_gamma)) * inx;
-                    const float gy = __expf(static_cast<float>((q.y - my) * inv_gamma)) * ipy -
-                                     __expf(static_cast<float>((ny - q.y) * inv_gamma)) * iny;
-                    a.pin_grad[p] = make_float2(wf * gx, wf * gy);
-                }
+                const float ipx = wf / spx, inx = wf / snx, ipy = wf / spy, iny = wf / sny;
+#pragma unroll
+                for (int i = 0; i < kSmallDegree; ++i)
+                    if (i < deg)
+                        a.pin_grad[b + i] = make_float2(epx[i] * ipx - enx[i] * inx, epy[i] * ipy - eny[i] * iny);
             }
         }
-    } else {
+    } else if (block < small_blocks + medium_blocks) {
         const int lane = threadIdx.x & 63;
-        const int item = (blockIdx.x - small_blocks) * kWaves + (threadIdx.x >> 6);
-        if (item < a.n_large) {
-            const int net = a.large_nets[item];
-            const int b = a.net_off[net], e = a.net_off[net + 1];
-            double mx = -INFINITY, nx = INFINITY, my = -INFINITY, ny = INFINITY;
-            for (int p = b + lane; p < e; p += 64) {
-                const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, p);
-                mx = fmax(mx, q.x); nx = fmin(nx, q.x);
-                my = fmax(my, q.y); ny = fmin(ny, q.y);
-            }
-            mx = waveMax(mx); nx = waveMin(nx); my = waveMax(my); ny = waveMin(ny);
-            float spx = 0.f, snx = 0.f, spy = 0.f, sny = 0.f;
-            for (int p = b + lane; p < e; p += 64) {
-                const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, p);
-                spx += __expf(static_cast<float>((q.x - mx) * inv_gamma));
-                snx += __expf(static_cast<float>((nx - q.x) * inv_gamma));
-                spy += __expf(static_cast<float>((q.y - my) * inv_gamma));
-                sny += __expf(static_cast<float>((ny - q.y) * inv_gamma));
-            }
-            spx = waveSumF(spx); snx = waveSumF(snx); spy = waveSumF(spy); sny = waveSumF(sny);
-            const double w = a.net_w[net];
-            if (lane == 0) {
-                const double lse = static_cast<double>(gamma_f * (__logf(spx) + __logf(snx) + __logf(spy) + __logf(sny)));
-                wl = w * ((mx - nx) + (my - ny) + lse);
-                hp = w * ((mx - nx) + (my - ny));
-            }
-            if (a.want_grad) {
-                const float wf = static_cast<float>(w);
-                const float ipx = 1.f / spx, inx = 1.f / snx, ipy = 1.f / spy, iny = 1.f / sny;
-                for (int p = b + lane; p < e; p += 64) {
-                    const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, p);
-                    const float gx = __expf(static_cast<float>((q.x - mx) * inv_gamma)) * ipx -
-                                     __expf(static_cast<float>((nx - q.x) * inv_gamma)) * inx;
-                    const float gy = __expf(static_cast<float>((q.y - my) * inv_gamma)) * ipy -
-                                     __expf(static_cast<float>((ny - q.y) * inv_gamma)) * iny;
-                    a.pin_grad[p] = make_float2(wf * gx, wf * gy);
-                }
-            }
-        }
+        const int net = a.n_small + (block - small_blocks) * (2 * kWaves) + 2 * (threadIdx.x >> 6) + (lane >> 5);
+        if (net < a.n_small + a.n_medium) wireGroup<32>(a, net, lane & 31, inv_gamma, gamma_f, hp, wl);
+    } else {
+        const int net = a.n_small + a.n_medium + (block - small_blocks - medium_blocks) * kWaves + (threadIdx.x >> 6);
+        if (net < a.n_small + a.n_medium + a.n_large) wireGroup<64>(a, net, threadIdx.x & 63, inv_gamma, gamma_f, hp, wl);
     }
     double v[2] = {hp, wl};
-    double tot[2];
-    if (finishReduction<2>(v, 0U, 0.0, blockIdx.x, gridDim.x, a.partials, a.counter, lds, tot) &&
-        threadIdx.x == 0) {
-        a.scal[kHpwl] = tot[0];
-        a.scal[kSmooth] = tot[1];
-    }
+    publishPartials<2>(v, 0U, blockIdx.x, a.partials, lds);
 }
 
 // ---------------------------------------------------------------------------------------------
-// Density scatter: exact rectangle / bin overlap in FP64 (bin index math as Metrics.cpp:78-85).
-__global__ __launch_bounds__(kThreads) void k_scatter(ScatterArgs a) {
-    const int p = blockIdx.x * kThreads + threadIdx.x;
+// Density: counting sort of the particles by the first bin row they cover, then one workgroup per
+// band of bin rows accumulates its particles' exact rectangle / bin overlaps (FP64 geometry, bin
+// index math of Metrics.cpp:78-85) into LDS as 2^40-per-bin-area fixed point -- order-independent,
+// so the density is bit-reproducible -- and evaluates the bin metrics and the FP32 deviation grid
+// for the DCT in the same kernel.  Particles taller than the band look-back (rare: movable macros)
+// are deposited with global 64-bit integer atomics into a residual layer that the bands fold in.
+
+__device__ inline void clippedSpan(double2 c, double2 sz, const Region& r, double& left, double& right,
+                                   double& bottom, double& top, int& c0, int& c1, int& r0, int& r1, bool& empty) {
+    left = fmax(c.x - sz.x * 0.5, r.llx);
+    right = fmin(c.x + sz.x * 0.5, r.urx);
+    bottom = fmax(c.y - sz.y * 0.5, r.lly);
+    top = fmin(c.y + sz.y * 0.5, r.ury);
+    empty = fmax(0.0, right - left) * fmax(0.0, top - bottom) <= kEps;
+    c0 = max(0, min(r.nx - 1, static_cast<int>(floor((left - r.llx) / r.bw))));
+    c1 = max(0, min(r.nx - 1, static_cast<int>(ceil((right - r.llx) / r.bw) - 1.0)));
+    r0 = max(0, min(r.ny - 1, static_cast<int>(floor((bottom - r.lly) / r.bh))));
+    r1 = max(0, min(r.ny - 1, static_cast<int>(ceil((top - r.lly) / r.bh) - 1.0)));
+}
+
+__device__ inline unsigned long long toFixed(double area, double fix) {
+    return static_cast<unsigned long long>(__double2ll_rn(area * fix));
+}
+
+// keys: first covered row, or ny (nothing to deposit) / ny + 1 (tall: residual path)
+__global__ __launch_bounds__(kSortThreads) void k_dhist(DensityArgs a) {
+    extern __shared__ int hist[];
+    const int ny = a.r.ny;
+    for (int k = threadIdx.x; k < ny; k += kSortThreads) hist[k] = 0;
+    __syncthreads();
+    const int p = blockIdx.x * kSortThreads + threadIdx.x;
+    if (p < a.n) {
+        double l, rr, bo, to;
+        int c0, c1, r0, r1;
+        bool empty;
+        clippedSpan(a.pos[p], a.size[p], a.r, l, rr, bo, to, c0, c1, r0, r1, empty);
+        const int key = empty ? ny : (a.tall[p] ? ny + 1 : r0);
+        a.key[p] = key;
+        if (key < ny) atomicAdd(&hist[key], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < ny; k += kSortThreads) a.hist[static_cast<size_t>(blockIdx.x) * ny + k] = hist[k];
+}
+
+// one workgroup per key (bin row): exclusive prefix of that key's count over the histogram rows
+__global__ __launch_bounds__(kThreads) void k_dscan(DensityArgs a, int rows) {
+    __shared__ int lds[kWaves];
+    const int ny = a.r.ny, k = blockIdx.x;
+    int carry = 0;
+    for (int h0 = 0; h0 < rows; h0 += kThreads) {
+        const int h = h0 + threadIdx.x;
+        const size_t i = static_cast<size_t>(h) * ny + k;
+        const int c = h < rows ? a.hist[i] : 0;
+        int total;
+        const int incl = blockScan<kWaves>(c, lds, &total);
+        if (h < rows) a.colprefix[i] = carry + incl - c;
+        carry += total;
+    }
+    if (threadIdx.x == 0) a.keytotal[k] = carry;
+}
+
+// row starts from the key totals (every workgroup, in LDS), then each particle's slot in row order
+__global__ __launch_bounds__(kSortThreads) void k_dplace(DensityArgs a) {
+    extern __shared__ int sh[];
+    int* start = sh;             // [ny + 1]
+    int* rank = sh + a.r.ny + 1; // [ny]
+    __shared__ int lds[kSortThreads / 64];
+    const int ny = a.r.ny;
+    int carry = 0;
+    for (int k0 = 0; k0 < ny; k0 += kSortThreads) {
+        const int k = k0 + threadIdx.x;
+        const int c = k < ny ? a.keytotal[k] : 0;
+        int total;
+        const int incl = blockScan<kSortThreads / 64>(c, lds, &total);
+        if (k < ny) {
+            start[k] = carry + incl - c;
+            rank[k] = 0;
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) start[ny] = carry;
+    __syncthreads();
+    if (blockIdx.x == 0)
+        for (int k = threadIdx.x; k <= ny; k += kSortThreads) a.keystart[k] = start[k];
+    const int p = blockIdx.x * kSortThreads + threadIdx.x;
     if (p >= a.n) return;
-    const double2 c = a.pos[p];
-    const double2 sz = a.size[p];
+    const int key = a.key[p];
+    if (key > ny) {  // tall particle: straight into the residual layer
+        double l, rr, bo, to;
+        int c0, c1, r0, r1;
+        bool empty;
+        clippedSpan(a.pos[p], a.size[p], a.r, l, rr, bo, to, c0, c1, r0, r1, empty);
+        const int layer = a.layer[p];
+        const double scale = layer == 1 ? a.t : 1.0;
+        unsigned long long* dst = a.residual + (layer == 2 ? static_cast<size_t>(a.r.nx) * ny : 0);
+        for (int row = r0; row <= r1; ++row) {
+            const double bin_bottom = a.r.lly + static_cast<double>(row) * a.r.bh;
+            const double oy = fmax(0.0, fmin(to, bin_bottom + a.r.bh) - fmax(bo, bin_bottom));
+            for (int col = c0; col <= c1; ++col) {
+                const double bin_left = a.r.llx + static_cast<double>(col) * a.r.bw;
+                const double ox = fmax(0.0, fmin(rr, bin_left + a.r.bw) - fmax(l, bin_left));
+                const double area = ox * oy;
+                if (area <= kEps) continue;
+                atomicAdd(dst + static_cast<size_t>(row) * a.r.nx + col, toFixed(scale * area, a.fix));
+            }
+        }
+        return;
+    }
+    if (key == ny) return;
+    const int slot = start[key] + a.colprefix[static_cast<size_t>(blockIdx.x) * ny + key] + atomicAdd(&rank[key], 1);
+    double l, rr, bo, to;
+    int c0, c1, r0, r1;
+    bool empty;
+    clippedSpan(a.pos[p], a.size[p], a.r, l, rr, bo, to, c0, c1, r0, r1, empty);
+    a.sorted[slot] = make_double4(l, rr, bo, to);
+    a.sorted_span[slot] = make_int4(c0 | (a.layer[p] << 28), c1, r0, r1);
+}
+
+__global__ __launch_bounds__(kBandThreads) void k_dband(DensityArgs a) {
+    extern __shared__ unsigned long long band[];
+    __shared__ double lds[(kBandThreads / 64 + 1) * 9];
     const Region& r = a.r;
-    const double left = fmax(c.x - sz.x * 0.5, r.llx);
-    const double right = fmin(c.x + sz.x * 0.5, r.urx);
-    const double bottom = fmax(c.y - sz.y * 0.5, r.lly);
-    const double top = fmin(c.y + sz.y * 0.5, r.ury);
-    if (fmax(0.0, right - left) * fmax(0.0, top - bottom) <= kEps) return;
-    const int c0 = max(0, min(r.nx - 1, static_cast<int>(floor((left - r.llx) / r.bw))));
-    const int c1 = max(0, min(r.nx - 1, static_cast<int>(ceil((right - r.llx) / r.bw) - 1.0)));
-    const int r0 = max(0, min(r.ny - 1, static_cast<int>(floor((bottom - r.lly) / r.bh))));
-    const int r1 = max(0, min(r.ny - 1, static_cast<int>(ceil((top - r.lly) / r.bh) - 1.0)));
-    const int layer = a.layer[p];
-    const double scale = layer == 1 ? a.t : 1.0;
-    double* dst = layer == 2 ? a.filler : a.movable;
-    for (int row = r0; row <= r1; ++row) {
-        const double bin_bottom = r.lly + static_cast<double>(row) * r.bh;
-        const double oy = fmax(0.0, fmin(top, bin_bottom + r.bh) - fmax(bottom, bin_bottom));
-        for (int col = c0; col <= c1; ++col) {
-            const double bin_left = r.llx + static_cast<double>(col) * r.bw;
-            const double ox = fmax(0.0, fmin(right, bin_left + r.bw) - fmax(left, bin_left));
-            const double area = ox * oy;
-            if (area <= kEps) continue;
-            atomicAdd(dst + static_cast<size_t>(row) * r.nx + col, scale * area);
+    const int nx = r.nx, ny = r.ny;
+    const int R0 = blockIdx.x * a.band_rows;
+    const int R1 = min(R0 + a.band_rows, ny);
+    const int cells = (R1 - R0) * nx;
+    unsigned long long* lmov = band;
+    unsigned long long* lfil = band + static_cast<size_t>(a.band_rows) * nx;
+    const size_t G = static_cast<size_t>(nx) * ny;
+    for (int i = threadIdx.x; i < cells; i += kBandThreads) {
+        const size_t gi = static_cast<size_t>(R0) * nx + i;
+        if (a.has_tall) {
+            lmov[i] = a.residual[gi];
+            lfil[i] = a.residual[G + gi];
+            a.residual[gi] = 0ULL;
+            a.residual[G + gi] = 0ULL;
+        } else {
+            lmov[i] = 0ULL;
+            lfil[i] = 0ULL;
         }
     }
-}
-
-// ---------------------------------------------------------------------------------------------
-// Per-bin charge, overflow, density and the FP32 density deviation for the DCT; clears the
-// dynamic layers for the next scatter.
-__global__ __launch_bounds__(kThreads) void k_metrics(MetricsArgs a) {
-    __shared__ double lds[kWaves * 9];
-    const Region& r = a.r;
-    const int G = r.nx * r.ny;
-    // sums: ovf_w, ovf_wo, q_w, q_wo, norm, dens_w, dens_wo ; max: dmax_w, dmax_wo
+    __syncthreads();
+    if (a.n > 0) {
+        const int lo = max(0, R0 - a.lookback + 1);
+        const int q0 = a.keystart[lo], q1 = a.keystart[R1];
+        for (int q = q0 + threadIdx.x; q < q1; q += kBandThreads) {
+            const int4 sp = a.sorted_span[q];
+            const int ra = max(sp.z, R0), rb = min(sp.w, R1 - 1);
+            if (ra > rb) continue;
+            const double4 rc = a.sorted[q];  // clipped left, right, bottom, top
+            const int layer = sp.x >> 28, c0 = sp.x & 0x0fffffff;
+            const double scale = layer == 1 ? a.t : 1.0;
+            unsigned long long* dst = layer == 2 ? lfil : lmov;
+            for (int row = ra; row <= rb; ++row) {
+                const double bin_bottom = r.lly + static_cast<double>(row) * r.bh;
+                const double oy = fmax(0.0, fmin(rc.w, bin_bottom + r.bh) - fmax(rc.z, bin_bottom));
+                for (int col = c0; col <= sp.y; ++col) {
+                    const double bin_left = r.llx + static_cast<double>(col) * r.bw;
+                    const double ox = fmax(0.0, fmin(rc.y, bin_left + r.bw) - fmax(rc.x, bin_left));
+                    const double area = ox * oy;
+                    if (area <= kEps) continue;
+                    atomicAdd(dst + (row - R0) * nx + col, toFixed(scale * area, a.fix));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // bin metrics (Metrics.cpp:101-119 per bin) and the deviation grid
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < G; i += gridDim.x * kThreads) {
-        const int row = i / r.nx, col = i - row * r.nx;
+    for (int i = threadIdx.x; i < cells; i += kBandThreads) {
+        const int row = R0 + i / nx, col = i - (i / nx) * nx;
         const double left = r.llx + static_cast<double>(col) * r.bw;
         const double bottom = r.lly + static_cast<double>(row) * r.bh;
         const double area = fmax(0.0, (left + r.bw) - left) * fmax(0.0, (bottom + r.bh) - bottom);
-        const double mov = a.movable[i];
-        const double fil = a.filler[i];
-        a.movable[i] = 0.0;
-        a.filler[i] = 0.0;
-        const double q_wo = mov + a.static_charge[i];
+        const double mov = static_cast<double>(lmov[i]) * a.inv_fix;
+        const double fil = static_cast<double>(lfil[i]) * a.inv_fix;
+        const double q_wo = mov + a.static_charge[static_cast<size_t>(row) * nx + col];
         const double q_w = q_wo + a.t * fil;
         const double cap = a.t * area;
         const double inv_area = 1.0 / fmax(area, kEps);
@@ -276,23 +442,7 @@ __global__ __launch_bounds__(kThreads) void k_metrics(MetricsArgs a) {
         v[8] = fmax(v[8], d_wo);
         a.rho[static_cast<size_t>(row) * a.ldrho + col] = static_cast<float>(d_w - a.t);
     }
-    double tot[9];
-    const unsigned max_mask = (1U << 7) | (1U << 8);
-    if (finishReduction<9>(v, max_mask, 0.0, blockIdx.x, gridDim.x, a.partials, a.counter, lds, tot) &&
-        threadIdx.x == 0) {
-        const double Gd = static_cast<double>(G);
-        for (int f = 0; f < 2; ++f) {
-            double* m = a.scal + (f == 0 ? kOptDensity : kDesignDensity);
-            m[0] = tot[f];                                     // total_overflow_area
-            m[5] = tot[4];                                     // normalization_area
-            m[1] = m[0] / fmax(m[5], kEps);                    // normalized_overflow
-            m[2] = tot[7 + f];                                 // maximum_density (starts at 0)
-            m[3] = tot[5 + f] / Gd;                            // average_density
-            m[4] = tot[2 + f];                                 // total_charge_area
-            m[6] = a.placeable_total;                          // placeable_area
-            m[7] = a.dark_total;                               // dark_area
-        }
-    }
+    publishPartials<9, kBandThreads / 64>(v, (1U << 7) | (1U << 8), blockIdx.x, a.partials, lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -304,7 +454,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 template <int EPI>
 __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int K, int lda, int ldb, int ldc, SpectralArgs sp) {
     __shared__ float red[kWaves][32][33];
-    __shared__ double lds[kWaves];
+    __shared__ double lds[kWaves + 1];
     const int z = blockIdx.z;
     const float* __restrict__ A = g.A[z];
     const float* __restrict__ B = g.B[z];
@@ -315,26 +465,60 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int K, int lda, 
     const float* a_row0 = A + static_cast<size_t>(m0 + r16) * lda + 4 * grp;
     const float* a_row1 = a_row0 + static_cast<size_t>(16) * lda;
     const float* b_col = B + static_cast<size_t>(4 * grp) * ldb + n0 + r16;
-    for (int k0 = wave * 16; k0 < K; k0 += 16 * kWaves) {
-        const float4 a0 = *reinterpret_cast<const float4*>(a_row0 + k0);
-        const float4 a1 = *reinterpret_cast<const float4*>(a_row1 + k0);
-        const float* bk = b_col + static_cast<size_t>(k0) * ldb;
-        float b0[4], b1[4];
+    // Register pipelining: a wave owns every fourth 16-wide K chunk and keeps two groups of
+    // kGemmGroup chunks in registers, loading one group while the MFMAs of the other run.
+    // Chunks past K load as zeros and add nothing.
+    constexpr int step = 16 * kWaves;
+    float4 ca0[kGemmGroup], ca1[kGemmGroup], na0[kGemmGroup], na1[kGemmGroup];
+    float cb0[kGemmGroup][4], cb1[kGemmGroup][4], nb0[kGemmGroup][4], nb1[kGemmGroup][4];
+#define MP_GEMM_LOAD(A0, A1, B0, B1, KK)                                                   \
+    {                                                                                   \
+        const int kk_ = (KK);                                                           \
+        if (kk_ < K) {                                                                  \
+            A0 = *reinterpret_cast<const float4*>(a_row0 + kk_);                        \
+            A1 = *reinterpret_cast<const float4*>(a_row1 + kk_);                        \
+            const float* bk_ = b_col + static_cast<size_t>(kk_) * ldb;                  \
+            _Pragma("unroll") for (int j = 0; j < 4; ++j) {                             \
+                B0[j] = bk_[static_cast<size_t>(j) * ldb];                              \
+                B1[j] = bk_[static_cast<size_t>(j) * ldb + 16];                         \
+            }                                                                           \
+        } else {                                                                        \
+            A0 = make_float4(0.f, 0.f, 0.f, 0.f);                                       \
+            A1 = A0;                                                                    \
+            _Pragma("unroll") for (int j = 0; j < 4; ++j) B0[j] = B1[j] = 0.f;          \
+        }                                                                               \
+    }
+    int k0 = wave * 16;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            b0[j] = bk[static_cast<size_t>(j) * ldb];
-            b1[j] = bk[static_cast<size_t>(j) * ldb + 16];
+    for (int g = 0; g < kGemmGroup; ++g) MP_GEMM_LOAD(ca0[g], ca1[g], cb0[g], cb1[g], k0 + g * step)
+    for (; k0 < K; k0 += kGemmGroup * step) {
+#pragma unroll
+        for (int g = 0; g < kGemmGroup; ++g)
+            MP_GEMM_LOAD(na0[g], na1[g], nb0[g], nb1[g], k0 + (kGemmGroup + g) * step)
+#pragma unroll
+        for (int g = 0; g < kGemmGroup; ++g) {
+            const float av0[4] = {ca0[g].x, ca0[g].y, ca0[g].z, ca0[g].w};
+            const float av1[4] = {ca1[g].x, ca1[g].y, ca1[g].z, ca1[g].w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(av0[j], cb0[g][j], acc00, 0, 0, 0);
+                acc01 = __builtin_amdgcn_mfma_f32_16x16x4f32(av0[j], cb1[g][j], acc01, 0, 0, 0);
+                acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(av1[j], cb0[g][j], acc10, 0, 0, 0);
+                acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(av1[j], cb1[g][j], acc11, 0, 0, 0);
+            }
         }
-        const float av0[4] = {a0.x, a0.y, a0.z, a0.w};
-        const float av1[4] = {a1.x, a1.y, a1.z, a1.w};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(av0[j], b0[j], acc00, 0, 0, 0);
-            acc01 = __builtin_amdgcn_mfma_f32_16x16x4f32(av0[j], b1[j], acc01, 0, 0, 0);
-            acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(av1[j], b0[j], acc10, 0, 0, 0);
-            acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(av1[j], b1[j], acc11, 0, 0, 0);
+        for (int g = 0; g < kGemmGroup; ++g) {
+            ca0[g] = na0[g];
+            ca1[g] = na1[g];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                cb0[g][j] = nb0[g][j];
+                cb1[g][j] = nb1[g][j];
+            }
         }
     }
+#undef MP_GEMM_LOAD
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         red[wave][4 * grp + v][r16] = acc00[v];
@@ -370,11 +554,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int K, int lda, 
     }
     if (EPI == kEpiSpectral) {
         double v[1] = {energy};
-        double tot[1];
-        const int block = blockIdx.y * gridDim.x + blockIdx.x;
-        if (finishReduction<1>(v, 0U, 0.0, block, gridDim.x * gridDim.y, sp.partials, sp.counter, lds, tot) &&
-            threadIdx.x == 0)
-            sp.scal[kEnergy] = fmax(0.0, sp.half_bin_area * tot[0]);
+        publishPartials<1>(v, 0U, blockIdx.y * gridDim.x + blockIdx.x, sp.partials, lds);
     }
 }
 
@@ -413,7 +593,7 @@ __device__ inline double sampleParity(const float* __restrict__ v, int ld, doubl
 }
 
 __global__ __launch_bounds__(kThreads) void k_gather(GatherArgs a) {
-    __shared__ double lds[kWaves * 4];
+    __shared__ double lds[(kWaves + 1) * 4];
     const int p = blockIdx.x * kThreads + threadIdx.x;
     double v[4] = {0, 0, 0, 0};  // wire L1, density L1, |raw|^2, |pre|^2
     if (p < a.n) {
@@ -444,14 +624,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(GatherArgs a) {
             v[3] = pre.x * pre.x + pre.y * pre.y;
         }
     }
-    double tot[4];
-    if (finishReduction<4>(v, 0U, 0.0, blockIdx.x, gridDim.x, a.partials, a.counter, lds, tot) &&
-        threadIdx.x == 0) {
-        a.scal[kWireL1] = tot[0];
-        a.scal[kDensityL1] = tot[1];
-        a.scal[kRawNorm2] = tot[2];
-        a.scal[kPreNorm2] = tot[3];
-    }
+    publishPartials<4>(v, 0U, blockIdx.x, a.partials, lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -472,24 +645,20 @@ __global__ __launch_bounds__(kThreads) void k_lookahead(const double2* x, const 
     out[p] = clampCenter(make_double2(a.x + b.x * beta, a.y + b.y * beta), c.size[p], c.r);
 }
 
-__global__ __launch_bounds__(kThreads) void k_dot(const double2* a, const double2* b, double sign, int slot, int n,
-                                                  double* partials, unsigned* counter, double* scal) {
-    __shared__ double lds[kWaves];
+__global__ __launch_bounds__(kThreads) void k_dot(const double2* a, const double2* b, int n, double* partials) {
+    __shared__ double lds[kWaves + 1];
     const int p = blockIdx.x * kThreads + threadIdx.x;
     double v[1] = {0.0};
     if (p < n) {
         const double2 x = a[p], y = b[p];
         v[0] = x.x * y.x + x.y * y.y;
     }
-    double tot[1];
-    if (finishReduction<1>(v, 0U, 0.0, blockIdx.x, gridDim.x, partials, counter, lds, tot) && threadIdx.x == 0)
-        scal[slot] = sign * tot[0];
+    publishPartials<1>(v, 0U, blockIdx.x, partials, lds);
 }
 
 __global__ __launch_bounds__(kThreads) void k_curvature(const double2* b, const double2* pb, const double2* g,
-                                                        const double2* pg, int n, int slot, double* partials,
-                                                        unsigned* counter, double* scal) {
-    __shared__ double lds[kWaves * 2];
+                                                        const double2* pg, int n, double* partials) {
+    __shared__ double lds[(kWaves + 1) * 2];
     const int p = blockIdx.x * kThreads + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (p < n) {
@@ -498,17 +667,13 @@ __global__ __launch_bounds__(kThreads) void k_curvature(const double2* b, const 
         v[0] = dx * dx + dy * dy;
         v[1] = gx * gx + gy * gy;
     }
-    double tot[2];
-    if (finishReduction<2>(v, 0U, 0.0, blockIdx.x, gridDim.x, partials, counter, lds, tot) && threadIdx.x == 0) {
-        scal[slot] = tot[0];
-        scal[slot + 1] = tot[1];
-    }
+    publishPartials<2>(v, 0U, blockIdx.x, partials, lds);
 }
 
 __global__ __launch_bounds__(kThreads) void k_propose(const double2* b, const double2* g, const double2* raw, double step,
                                                       double max_disp, double2* y, ClampArgs c, int n,
-                                                      double* partials, unsigned* counter, double* scal) {
-    __shared__ double lds[kWaves * 2];
+                                                      double* partials) {
+    __shared__ double lds[(kWaves + 1) * 2];
     const int p = blockIdx.x * kThreads + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (p < n) {
@@ -527,11 +692,7 @@ __global__ __launch_bounds__(kThreads) void k_propose(const double2* b, const do
         v[0] = r.x * ax + r.y * ay;
         v[1] = sqrt(ax * ax + ay * ay);
     }
-    double tot[2];
-    if (finishReduction<2>(v, 2U, 0.0, blockIdx.x, gridDim.x, partials, counter, lds, tot) && threadIdx.x == 0) {
-        scal[kPredicted] = tot[0];
-        scal[kMaxDisp] = tot[1];
-    }
+    publishPartials<2>(v, 2U, blockIdx.x, partials, lds);
 }
 
 __global__ __launch_bounds__(kThreads) void k_difference(const double2* y, const double2* x, double2* v, int n) {
@@ -547,31 +708,99 @@ __global__ __launch_bounds__(kThreads) void k_zero(double* p, size_t count) {
         p[i] = 0.0;
 }
 
+// ---------------------------------------------------------------------------------------------
+// One workgroup per pending reduction: sums (or maxima) of the partial rows in index order, then
+// the reference's post-processing of each scalar (DensityMap::metrics Metrics.cpp:101-119, energy
+// clamp .cu:1008, alignment sign AdaptiveGlobalPlacer.cpp:610-617).
+__device__ const int kRedWidths[kRedKinds] = {2, 9, 1, 4, 2, 1, 2, 2};
+
+__global__ __launch_bounds__(kFinalizeThreads) void k_finalize(RedList l) {
+    __shared__ double lds[(kFinalizeThreads / 64 + 1) * 9];
+    const RedJob j = l.job[blockIdx.x];
+    const int K = kRedWidths[j.kind];
+    const unsigned max_mask = j.kind == kRedMetrics ? ((1U << 7) | (1U << 8)) : (j.kind == kRedPropose ? 2U : 0U);
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < j.blocks; b += kFinalizeThreads)
+        for (int k = 0; k < K; ++k) {
+            const double x = j.partials[static_cast<size_t>(b) * K + k];
+            acc[k] = ((max_mask >> k) & 1U) ? fmax(acc[k], x) : acc[k] + x;
+        }
+    blockReduce<9, kFinalizeThreads / 64>(acc, max_mask, lds);
+    if (threadIdx.x != 0) return;
+    double* o = j.out;
+    switch (j.kind) {
+        case kRedWire:
+            o[kHpwl] = acc[0];
+            o[kSmooth] = acc[1];
+            break;
+        case kRedMetrics:
+            for (int f = 0; f < 2; ++f) {
+                double* m = o + (f == 0 ? kOptDensity : kDesignDensity);
+                m[0] = acc[f];                        // total_overflow_area
+                m[5] = acc[4];                        // normalization_area
+                m[1] = m[0] / fmax(m[5], kEps);       // normalized_overflow
+                m[2] = acc[7 + f];                    // maximum_density (starts at 0)
+                m[3] = acc[5 + f] / j.p0;             // average_density
+                m[4] = acc[2 + f];                    // total_charge_area
+                m[6] = j.p2;                          // placeable_area
+                m[7] = j.p1;                          // dark_area
+            }
+            break;
+        case kRedEnergy:
+            o[kEnergy] = fmax(0.0, j.p0 * acc[0]);
+            break;
+        case kRedGather:
+            o[kWireL1] = acc[0];
+            o[kDensityL1] = acc[1];
+            o[kRawNorm2] = acc[2];
+            o[kPreNorm2] = acc[3];
+            break;
+        case kRedPropose:
+            o[kPredicted] = acc[0];
+            o[kMaxDisp] = acc[1];
+            break;
+        case kRedDot:
+            o[j.slot] = j.p0 * acc[0];
+            break;
+        default:  // curvature pairs
+            o[j.slot] = acc[0];
+            o[j.slot + 1] = acc[1];
+            break;
+    }
+}
+
 int blocksFor(long long n) { return static_cast<int>((n + kThreads - 1) / kThreads); }
 
 }  // namespace
 
-int wireBlocks(int n_small, int n_large) { return blocksFor(n_small) + (n_large + kWaves - 1) / kWaves; }
+int wireBlocks(int n_small, int n_medium, int n_large) {
+    return blocksFor(n_small) + (n_medium + 2 * kWaves - 1) / (2 * kWaves) + (n_large + kWaves - 1) / kWaves;
+}
 
 void launchWire(const WireArgs& a, hipStream_t s) {
     const int small_blocks = blocksFor(a.n_small);
-    const int blocks = wireBlocks(a.n_small, a.n_large);
+    const int medium_blocks = (a.n_medium + 2 * kWaves - 1) / (2 * kWaves);
+    const int blocks = wireBlocks(a.n_small, a.n_medium, a.n_large);
     if (blocks == 0) return;
-    hipLaunchKernelGGL(k_wire, dim3(blocks), dim3(kThreads), 0, s, a, small_blocks);
+    hipLaunchKernelGGL(k_wire, dim3(blocks), dim3(kThreads), 0, s, a, small_blocks, medium_blocks);
     check(hipGetLastError(), "launch wirelength kernel");
 }
 
-void launchScatter(const ScatterArgs& a, hipStream_t s) {
-    if (a.n == 0) return;
-    hipLaunchKernelGGL(k_scatter, dim3(blocksFor(a.n)), dim3(kThreads), 0, s, a);
-    check(hipGetLastError(), "launch density scatter kernel");
-}
+int densityHistBlocks(int n) { return std::max(1, (n + kSortThreads - 1) / kSortThreads); }
 
-int metricsBlocks(int G) { return std::min(blocksFor(G), 1024); }
-
-void launchMetrics(const MetricsArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_metrics, dim3(metricsBlocks(a.r.nx * a.r.ny)), dim3(kThreads), 0, s, a);
-    check(hipGetLastError(), "launch density metrics kernel");
+void launchDensity(const DensityArgs& a, hipStream_t s) {
+    const int hb = densityHistBlocks(a.n);
+    if (a.n > 0) {
+        hipLaunchKernelGGL(k_dhist, dim3(hb), dim3(kSortThreads), static_cast<size_t>(a.r.ny) * sizeof(int), s, a);
+        check(hipGetLastError(), "launch density histogram kernel");
+        hipLaunchKernelGGL(k_dscan, dim3(a.r.ny), dim3(kThreads), 0, s, a, hb);
+        check(hipGetLastError(), "launch density scan kernel");
+        hipLaunchKernelGGL(k_dplace, dim3(hb), dim3(kSortThreads), (2 * static_cast<size_t>(a.r.ny) + 1) * sizeof(int), s, a);
+        check(hipGetLastError(), "launch density placement kernel");
+    }
+    const size_t band_lds = 2 * static_cast<size_t>(a.band_rows) * a.r.nx * sizeof(unsigned long long);
+    hipLaunchKernelGGL(k_dband, dim3(a.bands), dim3(kBandThreads), band_lds, s, a);
+    check(hipGetLastError(), "launch density band kernel");
 }
 
 void launchGemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
@@ -600,24 +829,21 @@ void launchLookahead(const double2* x, const double2* v, double beta, double2* o
     check(hipGetLastError(), "launch look-ahead kernel");
 }
 
-void launchDot(const double2* a, const double2* b, double sign, int slot, int n, double* partials, unsigned* counter,
-               double* scal, hipStream_t s) {
-    hipLaunchKernelGGL(k_dot, dim3(particleBlocks(n)), dim3(kThreads), 0, s, a, b, sign, slot, n, partials, counter,
-                       scal);
+void launchDot(const double2* a, const double2* b, int n, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_dot, dim3(particleBlocks(n)), dim3(kThreads), 0, s, a, b, n, partials);
     check(hipGetLastError(), "launch dot kernel");
 }
 
-void launchCurvature(const double2* b, const double2* pb, const double2* g, const double2* pg, int n, int slot,
-                     double* partials, unsigned* counter, double* scal, hipStream_t s) {
-    hipLaunchKernelGGL(k_curvature, dim3(particleBlocks(n)), dim3(kThreads), 0, s, b, pb, g, pg, n, slot, partials,
-                       counter, scal);
+void launchCurvature(const double2* b, const double2* pb, const double2* g, const double2* pg, int n,
+                     double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_curvature, dim3(particleBlocks(n)), dim3(kThreads), 0, s, b, pb, g, pg, n, partials);
     check(hipGetLastError(), "launch curvature kernel");
 }
 
 void launchPropose(const double2* b, const double2* g, const double2* raw, double step, double max_disp, double2* y,
-                   const ClampArgs& c, int n, double* partials, unsigned* counter, double* scal, hipStream_t s) {
+                   const ClampArgs& c, int n, double* partials, hipStream_t s) {
     hipLaunchKernelGGL(k_propose, dim3(particleBlocks(n)), dim3(kThreads), 0, s, b, g, raw, step, max_disp, y, c, n,
-                       partials, counter, scal);
+                       partials);
     check(hipGetLastError(), "launch propose kernel");
 }
 
@@ -625,6 +851,17 @@ void launchDifference(const double2* y, const double2* x, double2* v, int n, hip
     if (n == 0) return;
     hipLaunchKernelGGL(k_difference, dim3(blocksFor(n)), dim3(kThreads), 0, s, y, x, v, n);
     check(hipGetLastError(), "launch difference kernel");
+}
+
+int redWidth(int kind) {
+    static const int w[kRedKinds] = {2, 9, 1, 4, 2, 1, 2, 2};
+    return w[kind];
+}
+
+void launchFinalize(const RedList& l, hipStream_t s) {
+    if (l.n == 0) return;
+    hipLaunchKernelGGL(k_finalize, dim3(l.n), dim3(kFinalizeThreads), 0, s, l);
+    check(hipGetLastError(), "launch reduction finalize kernel");
 }
 
 void launchZero(double* p, size_t count, hipStream_t s) {

@@ -316,8 +316,8 @@ private:
         }
         if (try_momentum) {
             be_->timedBegin(kTController);
-            launchDot(V_.get(), Gc_.get(), -1.0, kAlignment, P_, be_->partials(kCtrDot), be_->counter(kCtrDot),
-                      scal(kBankMisc), s);
+            launchDot(V_.get(), Gc_.get(), P_, be_->partials(kRedDot, kBankMisc), s);
+            be_->addReduction(kRedDot, kBankMisc, particleBlocks(P_), kAlignment, -1.0);
             launchLookahead(X_.get(), V_.get(), beta, B_.get(), clamp_, P_, s);
             be_->timedEnd(kTController);
             enqueueGrad(B_.get(), Rl_.get(), Gl_.get(), kBankLook);
@@ -325,11 +325,12 @@ private:
         }
         if (has_previous_) {
             be_->timedBegin(kTController);
-            launchCurvature(X_.get(), PX_.get(), Gc_.get(), PG_.get(), P_, kCurvDp, be_->partials(kCtrCurv),
-                            be_->counter(kCtrCurv), scal(kBankMisc), s);
-            if (try_momentum)
-                launchCurvature(B_.get(), PX_.get(), Gl_.get(), PG_.get(), P_, kCurvDp2, be_->partials(kCtrCurv),
-                                be_->counter(kCtrCurv), scal(kBankMisc), s);
+            launchCurvature(X_.get(), PX_.get(), Gc_.get(), PG_.get(), P_, be_->partials(kRedCurv, kBankMisc), s);
+            be_->addReduction(kRedCurv, kBankMisc, particleBlocks(P_), kCurvDp);
+            if (try_momentum) {
+                launchCurvature(B_.get(), PX_.get(), Gl_.get(), PG_.get(), P_, be_->partials(kRedCurv2, kBankMisc), s);
+                be_->addReduction(kRedCurv2, kBankMisc, particleBlocks(P_), kCurvDp2);
+            }
             be_->timedEnd(kTController);
             need_read = true;
         }
@@ -391,8 +392,9 @@ private:
             const double max_disp = o_.maximum_movement_in_bins * min_bin_ * (0.35 + 0.65 * ratio);
             for (int attempt = 0; attempt <= o_.maximum_backtracks; ++attempt) {
                 be_->timedBegin(kTController);
-                launchPropose(base, base_pre, base_raw, step, max_disp, Y_.get(), clamp_, P_, be_->partials(kCtrPropose),
-                              be_->counter(kCtrPropose), scal(kBankCandidate), s);
+                launchPropose(base, base_pre, base_raw, step, max_disp, Y_.get(), clamp_, P_,
+                              be_->partials(kRedPropose, kBankCandidate), s);
+                be_->addReduction(kRedPropose, kBankCandidate, particleBlocks(P_));
                 be_->timedEnd(kTController);
                 enqueueValue(Y_.get(), kBankCandidate);
                 const HostScalars& h = be_->fetchScalars();
