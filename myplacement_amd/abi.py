@@ -131,6 +131,7 @@ class MpOptions(C.Structure):
         ("device", C.c_int32),
         ("dct_precision", C.c_int32),
         ("maximum_device_bytes", C.c_uint64),
+        ("density_order_reuse_rows", C.c_double),
     ]
 
 
@@ -207,7 +208,7 @@ def default_options(**overrides) -> MpOptions:
         density_penalty_increase=1.05, density_penalty_decrease=0.88, minimum_density_penalty=1e-4,
         maximum_density_penalty=1e6, smoothing_overflow_exponent=1.0, feasible_refinement_iterations=12,
         maximum_fillers=50000, seed=2026, device=0, dct_precision=MP_DCT_FP32,
-        maximum_device_bytes=40 * 1024 ** 3,
+        maximum_device_bytes=40 * 1024 ** 3, density_order_reuse_rows=2.5,
     )
     for key, value in overrides.items():
         setattr(o, key, value)

@@ -3,7 +3,12 @@
 //
 // Reference: CudaPlacementBackendImpl and tryCreateCudaPlacementBackend,
 // src/placement/cuda/CudaPlacementBackend.cu:841-1216.  Differences by design:
-//   * one stream, no blocking scalar copy inside the evaluation (the reference does 14);
+//   * no blocking scalar copy inside the evaluation (the reference does 14): one k_finalize
+//     writes every scalar into host-mapped memory, and the host waits once;
+//   * the controller's two gradient evaluations (current and look-ahead positions) run as one
+//     chain of batched launches: twice the workgroups per launch, half the launches;
+//   * the particle row order of the density bands is reused across evaluations while every
+//     particle stays within a stated number of bin rows of where it was sorted;
 //   * the gradient scatter is atomic-free: per-pin terms, then a particle -> pin CSR sum;
 //   * the DCT stages are dense bin-grid products on MFMA; the electrostatic energy is taken in
 //     the cosine basis (sum_lk c_l c_k / 4 * chat * phihat), which equals the reference's grid sum
@@ -161,6 +166,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     // ---- device ----
     setDevice();
     hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "create HIP stream");
+    hipStream_t stream = stream_;
     pos_.allocate(P_, budget_, "particle positions");
     size_.allocate(P_, budget_, "particle sizes");
     charge_.allocate(P_, budget_, "particle charge areas");
@@ -169,39 +175,45 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     net_w_.allocate(order.size(), budget_, "net weights");
     pin_part_.allocate(Ii, budget_, "pin-particle map");
     pin_off_.allocate(Ii, budget_, "pin offsets");
-    pin_grad_.allocate(Ii, budget_, "per-pin wirelength gradient");
     ppin_idx_.allocate(ppin_idx.size(), budget_, "particle-pin index");
     ppin_off_.allocate(static_cast<std::size_t>(P_) + 1, budget_, "particle-pin offsets");
     // density band geometry: bands of band_rows_ bin rows, one workgroup each, both fixed-point
-    // layers of a band in LDS (<= 64 KiB); particles taller than two bin rows take the residual path
+    // layers of a band in LDS (<= 64 KiB); particles taller than two bin rows are not row-sorted
+    // and every band visits them
     if (nx_ > 4096) throw InvalidArgument("density grids wider than 4096 bins are not supported by the band kernel");
     if (ny_ > 8192) throw InvalidArgument("density grids taller than 8192 bins are not supported by the row sort");
+    if (static_cast<unsigned>(P_) > kPermIdMask) throw InvalidArgument("HIP backend cannot order more than 2^30 - 1 particles");
     band_rows_ = std::max(1, std::min(ny_ / 256, 4096 / nx_));
     bands_ = (ny_ + band_rows_ - 1) / band_rows_;
     fix_ = std::ldexp(1.0, 40) / (region_.bw * region_.bh);
     std::vector<unsigned char> tall(P_);
+    std::vector<int> tall_ids;
     for (int i = 0; i < P_; ++i) {
         tall[i] = L.particle_height[i] > 2.0 * region_.bh ? 1 : 0;
-        has_tall_ = has_tall_ || tall[i];
+        if (tall[i]) tall_ids.push_back(i);
     }
+    n_tall_ = static_cast<int>(tall_ids.size());
     const int hist_blocks = densityHistBlocks(P_);
     key_.allocate(P_, budget_, "density sort keys");
     hist_.allocate(static_cast<std::size_t>(hist_blocks) * ny_, budget_, "density row histograms");
     colprefix_.allocate(static_cast<std::size_t>(hist_blocks) * ny_, budget_, "density row prefixes");
     keytotal_.allocate(ny_, budget_, "density row totals");
     keystart_.allocate(static_cast<std::size_t>(ny_) + 1, budget_, "density row starts");
-    sorted_.allocate(P_, budget_, "row-sorted clipped rectangles");
-    sorted_span_.allocate(P_, budget_, "row-sorted bin spans");
+    perm_.allocate(P_, budget_, "particle row order");
+    sorted_size_.allocate(P_, budget_, "row-ordered particle sizes");
     tall_.allocate(P_, budget_, "tall-particle flags");
-    residual_.allocate(has_tall_ ? 2 * G : 1, budget_, "tall-particle residual layers");
+    tall_ids_.allocate(tall_ids.size(), budget_, "tall-particle ids");
     static_charge_.allocate(G, budget_, "static density charge");
     const std::size_t Gp = static_cast<std::size_t>(nxp_) * nyp_;
-    rho_.allocate(Gp, budget_, "density deviation grid");
-    tmp_.allocate(Gp, budget_, "DCT intermediate");
-    phi_.allocate(Gp, budget_, "potential coefficients");
-    t12_.allocate(2 * Gp, budget_, "inverse DCT intermediate");
-    ex_.allocate(Gp, budget_, "horizontal field");
-    ey_.allocate(Gp, budget_, "vertical field");
+    for (Slot& sl : slots_) {
+        sl.pin_grad.allocate(Ii, budget_, "per-pin wirelength gradient");
+        sl.rho.allocate(Gp, budget_, "density deviation grid");
+        sl.tmp.allocate(Gp, budget_, "DCT intermediate");
+        sl.phi.allocate(Gp, budget_, "potential coefficients");
+        sl.t12.allocate(2 * Gp, budget_, "inverse DCT intermediate");
+        sl.ex.allocate(Gp, budget_, "horizontal field");
+        sl.ey.allocate(Gp, budget_, "vertical field");
+    }
     pot_.allocate(Gp, budget_, "potential grid");
     bxf_.allocate(static_cast<std::size_t>(nxp_) * nxp_, budget_, "forward x factor");
     ayf_.allocate(static_cast<std::size_t>(nyp_) * nyp_, budget_, "forward y factor");
@@ -213,24 +225,28 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     for (int k = 0; k < kRedKinds * kScalarBanks; ++k)
         partials_[k].allocate(static_cast<std::size_t>(max_blocks) * redWidth(k / kScalarBanks), budget_,
                               "reduction partials");
-    scal_.allocate(kScalarCount * kScalarBanks, budget_, "scalar block");
     wire_out_.allocate(P_, budget_, "wire gradient");
     density_out_.allocate(P_, budget_, "density gradient");
-    hipCheck(hipHostMalloc(reinterpret_cast<void**>(&host_scal_), sizeof(HostScalars)), "allocate pinned scalars");
+    // coherent host memory: k_finalize's stores go straight to the host, no copy after it
+    hipCheck(hipHostMalloc(reinterpret_cast<void**>(&host_scal_), sizeof(HostScalars),
+                           hipHostMallocMapped | hipHostMallocCoherent),
+             "allocate host-mapped scalars");
+    std::memset(host_scal_, 0, sizeof(HostScalars));
+    hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&dev_scal_), host_scal_, 0), "map scalar block");
 
     std::vector<int> layers(L.particle_layer, L.particle_layer + P_);
     for (int v : layers)
         if (v < 0 || v > 2) throw InvalidArgument("unknown particle layer");
-    size_.upload(sizes.data(), P_, stream_, "upload particle sizes");
-    charge_.upload(L.particle_charge_area, P_, stream_, "upload particle charges");
-    layer_.upload(layers.data(), P_, stream_, "upload particle layers");
-    net_off_.upload(inet_off.data(), inet_off.size(), stream_, "upload net offsets");
-    net_w_.upload(inet_w.data(), inet_w.size(), stream_, "upload net weights");
-    pin_part_.upload(ipin_part.data(), Ii, stream_, "upload pin-particle map");
-    pin_off_.upload(ipin_off.data(), Ii, stream_, "upload pin offsets");
-    ppin_off_.upload(count.data(), count.size(), stream_, "upload particle-pin offsets");
-    ppin_idx_.upload(ppin_idx.data(), ppin_idx.size(), stream_, "upload particle-pin index");
-    static_charge_.upload(static_charge.data(), G, stream_, "upload static density");
+    size_.upload(sizes.data(), P_, stream, "upload particle sizes");
+    charge_.upload(L.particle_charge_area, P_, stream, "upload particle charges");
+    layer_.upload(layers.data(), P_, stream, "upload particle layers");
+    net_off_.upload(inet_off.data(), inet_off.size(), stream, "upload net offsets");
+    net_w_.upload(inet_w.data(), inet_w.size(), stream, "upload net weights");
+    pin_part_.upload(ipin_part.data(), Ii, stream, "upload pin-particle map");
+    pin_off_.upload(ipin_off.data(), Ii, stream, "upload pin offsets");
+    ppin_off_.upload(count.data(), count.size(), stream, "upload particle-pin offsets");
+    ppin_idx_.upload(ppin_idx.data(), ppin_idx.size(), stream, "upload particle-pin index");
+    static_charge_.upload(static_charge.data(), G, stream, "upload static density");
     const auto fx = basis(nxp_, nxp_, nx_, 0, W);
     const auto fy = basis(nyp_, nyp_, ny_, 1, H);
     const auto ix = basis(nxp_, nxp_, nx_, 2, W);
@@ -244,17 +260,16 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
             bx2[static_cast<std::size_t>(k) * 2 * nxp_ + i] = ix[static_cast<std::size_t>(i) * nxp_ + k];
             bx2[static_cast<std::size_t>(k) * 2 * nxp_ + nxp_ + i] = sx[static_cast<std::size_t>(i) * nxp_ + k];
         }
-    bxf_.upload(fx.data(), fx.size(), stream_, "upload forward x factor");
-    ayf_.upload(fy.data(), fy.size(), stream_, "upload forward y factor");
-    bxi_.upload(bx2.data(), bx2.size(), stream_, "upload inverse x factors");
-    ayi_.upload(iy.data(), iy.size(), stream_, "upload inverse y factor");
-    syi_.upload(sy.data(), sy.size(), stream_, "upload inverse y sine factor");
-    tall_.upload(tall.data(), tall.size(), stream_, "upload tall-particle flags");
-    hipCheck(hipMemsetAsync(residual_.get(), 0, residual_.size() * sizeof(unsigned long long), stream_),
-             "clear residual density layers");
-    hipCheck(hipMemsetAsync(rho_.get(), 0, Gp * sizeof(float), stream_), "clear density grid");
-    hipCheck(hipMemsetAsync(scal_.get(), 0, kScalarCount * kScalarBanks * sizeof(double), stream_), "clear scalars");
-    hipCheck(hipStreamSynchronize(stream_), "finish backend upload");
+    bxf_.upload(fx.data(), fx.size(), stream, "upload forward x factor");
+    ayf_.upload(fy.data(), fy.size(), stream, "upload forward y factor");
+    bxi_.upload(bx2.data(), bx2.size(), stream, "upload inverse x factors");
+    ayi_.upload(iy.data(), iy.size(), stream, "upload inverse y factor");
+    syi_.upload(sy.data(), sy.size(), stream, "upload inverse y sine factor");
+    tall_.upload(tall.data(), tall.size(), stream, "upload tall-particle flags");
+    tall_ids_.upload(tall_ids.data(), tall_ids.size(), stream, "upload tall-particle ids");
+    // the padding of the deviation grids stays zero: the band kernel writes only the nx x ny part
+    for (Slot& sl : slots_) hipCheck(hipMemsetAsync(sl.rho.get(), 0, Gp * sizeof(float), stream), "clear density grid");
+    hipCheck(hipStreamSynchronize(stream), "finish backend upload");
 }
 
 Backend::~Backend() {
@@ -328,26 +343,13 @@ void Backend::timing(std::vector<double>& total_us, std::vector<long long>& laun
     work = t_work_;
 }
 
-void Backend::enqueueEvaluate(const double2* pos, double gamma, bool want_grad, const SessionOut* session, int bank) {
-    hipStream_t s = stream_;
-    double* scal = scal_.get() + bank * kScalarCount;
-    WireArgs wa{pos, net_off_.get(), net_w_.get(), pin_part_.get(), pin_off_.get(), n_small_, n_medium_, n_large_,
-                gamma, want_grad ? 1 : 0, pin_grad_.get(), partials(kRedWire, bank)};
-    timedBegin(kTWire);
-    if (n_small_ + n_medium_ + n_large_ > 0) {
-        launchWire(wa, s);
-        addReduction(kRedWire, bank, wireBlocks(n_small_, n_medium_, n_large_));
-    } else {
-        launchZero(scal + kHpwl, 2, s);
-    }
-    // algorithmic bytes: 12 per pin, 8 per net, 16 (8 value-only) per particle
-    timedEnd(kTWire, 12.0 * I_ + 8.0 * N_ + (want_grad ? 16.0 : 8.0) * P_);
-
+DensityArgs Backend::densityArgs() const {
     DensityArgs da{};
-    da.pos = pos;
     da.size = size_.get();
     da.layer = layer_.get();
     da.tall = tall_.get();
+    da.tall_ids = tall_ids_.get();
+    da.n_tall = n_tall_;
     da.n = P_;
     da.r = region_;
     da.t = t_;
@@ -356,86 +358,162 @@ void Backend::enqueueEvaluate(const double2* pos, double gamma, bool want_grad, 
     da.band_rows = band_rows_;
     da.bands = bands_;
     da.lookback = kBandLookback;
-    da.has_tall = has_tall_ ? 1 : 0;
     da.key = key_.get();
     da.hist = hist_.get();
     da.colprefix = colprefix_.get();
     da.keytotal = keytotal_.get();
     da.keystart = keystart_.get();
-    da.sorted = sorted_.get();
-    da.sorted_span = sorted_span_.get();
-    da.residual = residual_.get();
+    da.perm = perm_.get();
+    da.sorted_size = sorted_size_.get();
     da.static_charge = static_charge_.get();
-    da.rho = rho_.get();
     da.ldrho = nxp_;
-    da.partials = partials(kRedMetrics, bank);
+    return da;
+}
+
+void Backend::sortParticles(const double2* pos) {
+    DensityArgs da = densityArgs();
+    da.pos[0] = pos;
     timedBegin(kTDensity);
-    launchDensity(da, s);
-    // algorithmic bytes of the scatter (20 per particle, 12 per bin) and the metrics (24 per bin)
-    timedEnd(kTDensity, 20.0 * P_ + 36.0 * static_cast<double>(nx_) * ny_);
-    addReduction(kRedMetrics, bank, bands_, 0, static_cast<double>(nx_) * ny_, dark_total_, placeable_total_);
+    launchDensitySort(da, stream_);
+    // algorithmic bytes: positions and sizes read twice, keys written and read, order and sizes written
+    timedEnd(kTDensity, 64.0 * P_);
+}
+
+void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bool want_grad) {
+    if (count < 1 || count > kSlots) throw std::logic_error("one or two evaluations per batch");
+    for (int e = 0; e < count; ++e) {
+        if (specs[e].margin == kFreshOrder && count != 1) throw std::logic_error("a fresh row order needs a single evaluation");
+        if (!specs[e].session && count != 1) throw std::logic_error("backend-mode evaluations are not batched");
+    }
+    hipStream_t s = stream_;
+    if (specs[0].margin == kFreshOrder) sortParticles(specs[0].pos);
+
+    WireArgs wa{};
+    wa.net_off = net_off_.get();
+    wa.net_w = net_w_.get();
+    wa.pin_part = pin_part_.get();
+    wa.pin_off = pin_off_.get();
+    wa.n_small = n_small_;
+    wa.n_medium = n_medium_;
+    wa.n_large = n_large_;
+    wa.gamma = gamma;
+    wa.want_grad = want_grad ? 1 : 0;
+    for (int e = 0; e < count; ++e) {
+        wa.pos[e] = specs[e].pos;
+        wa.pin_grad[e] = slots_[e].pin_grad.get();
+        wa.partials[e] = partials(kRedWire, specs[e].bank);
+    }
+    timedBegin(kTWire);
+    if (n_small_ + n_medium_ + n_large_ > 0) {
+        launchWire(wa, count, s);
+        for (int e = 0; e < count; ++e) addReduction(kRedWire, specs[e].bank, wireBlocks(n_small_, n_medium_, n_large_));
+    } else {
+        for (int e = 0; e < count; ++e) launchZero(dev_scal_ + specs[e].bank * kScalarCount + kHpwl, 2, s);
+    }
+    // algorithmic bytes: 12 per pin, 8 per net, 16 (8 value-only) per particle, per evaluation
+    timedEnd(kTWire, count * (12.0 * I_ + 8.0 * N_ + (want_grad ? 16.0 : 8.0) * P_));
+
+    DensityArgs da = densityArgs();
+    for (int e = 0; e < count; ++e) {
+        da.pos[e] = specs[e].pos;
+        da.margin[e] = specs[e].margin == kFreshOrder ? 0 : specs[e].margin;
+        da.rho[e] = slots_[e].rho.get();
+        da.partials[e] = partials(kRedMetrics, specs[e].bank);
+    }
+    timedBegin(kTDensity);
+    launchDensityBands(da, count, s);
+    // algorithmic bytes: 32 per particle (position, size) and 36 per bin (static charge, deviation, metrics)
+    timedEnd(kTDensity, count * (32.0 * P_ + 36.0 * static_cast<double>(nx_) * ny_));
+    for (int e = 0; e < count; ++e)
+        addReduction(kRedMetrics, specs[e].bank, bands_, static_cast<double>(nx_) * ny_, dark_total_, placeable_total_);
 
     const double W = region_.urx - region_.llx, H = region_.ury - region_.lly;
-    SpectralArgs sp{nx_, ny_, W, H, 1.0 / (4.0 * nx_ * static_cast<double>(ny_)), partials(kRedEnergy, bank)};
+    SpectralArgs sp{nx_, ny_, W, H, 1.0 / (4.0 * nx_ * static_cast<double>(ny_)), {}};
+    for (int e = 0; e < count; ++e) sp.partials[e] = partials(kRedEnergy, specs[e].bank);
     auto gemm = [&](const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epi) {
         timedBegin(kTDct);
         launchGemm(g, batch, M, N, K, lda, ldb, ldc, epi, sp, dct_precision_, s);
         timedEnd(kTDct, 2.0 * M * static_cast<double>(N) * K * batch);
     };
     // forward: T = rho * Bx ; chat = Ay * T -> phihat (+ energy)
-    GemmBatch f1{{rho_.get()}, {bxf_.get()}, {tmp_.get()}};
-    gemm(f1, 1, nyp_, nxp_, nxp_, nxp_, nxp_, nxp_, kEpiStore);
-    GemmBatch f2{{ayf_.get()}, {tmp_.get()}, {phi_.get()}};
-    gemm(f2, 1, nyp_, nxp_, nyp_, nyp_, nxp_, nxp_, kEpiSpectral);
-    addReduction(kRedEnergy, bank, (nxp_ / 32) * (nyp_ / 32), 0, 0.5 * region_.bw * region_.bh);
-    if (want_grad) {
-        // [T1 | T2] = phihat * [Cx_inv | Sx_inv] ; Ex = Ay_inv * T2 ; Ey = Sy_inv * T1
-        GemmBatch i1{{phi_.get()}, {bxi_.get()}, {t12_.get()}};
-        gemm(i1, 1, nyp_, 2 * nxp_, nxp_, nxp_, 2 * nxp_, 2 * nxp_, kEpiStore);
-        GemmBatch i2{{ayi_.get(), syi_.get()}, {t12_.get() + nxp_, t12_.get()}, {ex_.get(), ey_.get()}};
-        gemm(i2, 2, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiStore);
+    GemmBatch f1{}, f2{};
+    for (int e = 0; e < count; ++e) {
+        f1.A[e] = slots_[e].rho.get();
+        f1.B[e] = bxf_.get();
+        f1.C[e] = slots_[e].tmp.get();
+        f2.A[e] = ayf_.get();
+        f2.B[e] = slots_[e].tmp.get();
+        f2.C[e] = slots_[e].phi.get();
     }
-    have_fields_ = want_grad;
-
+    gemm(f1, count, nyp_, nxp_, nxp_, nxp_, nxp_, nxp_, kEpiStore);
+    gemm(f2, count, nyp_, nxp_, nyp_, nyp_, nxp_, nxp_, kEpiSpectral);
+    for (int e = 0; e < count; ++e)
+        addReduction(kRedEnergy, specs[e].bank, (nxp_ / 32) * (nyp_ / 32), 0.5 * region_.bw * region_.bh);
     if (!want_grad) return;
+    // [T1 | T2] = phihat * [Cx_inv | Sx_inv] ; Ex = Ay_inv * T2 ; Ey = Sy_inv * T1
+    GemmBatch i1{}, i2{};
+    for (int e = 0; e < count; ++e) {
+        const Slot& sl = slots_[e];
+        i1.A[e] = sl.phi.get();
+        i1.B[e] = bxi_.get();
+        i1.C[e] = sl.t12.get();
+        i2.A[2 * e] = ayi_.get();
+        i2.B[2 * e] = sl.t12.get() + nxp_;
+        i2.C[2 * e] = sl.ex.get();
+        i2.A[2 * e + 1] = syi_.get();
+        i2.B[2 * e + 1] = sl.t12.get();
+        i2.C[2 * e + 1] = sl.ey.get();
+    }
+    gemm(i1, count, nyp_, 2 * nxp_, nxp_, nxp_, 2 * nxp_, 2 * nxp_, kEpiStore);
+    gemm(i2, 2 * count, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiStore);
+    have_fields_ = true;
+
     GatherArgs ga{};
-    ga.pos = pos;
     ga.charge = charge_.get();
-    ga.ex = ex_.get();
-    ga.ey = ey_.get();
     ga.ldf = nxp_;
     ga.r = region_;
     ga.ppin_off = ppin_off_.get();
     ga.ppin_idx = ppin_idx_.get();
-    ga.pin_grad = pin_grad_.get();
     ga.n = P_;
-    if (session) {
-        ga.precond = session->precond;
-        ga.penalty = session->penalty;
-        ga.inv_strength = session->inv_strength;
-        ga.raw_out = session->raw;
-        ga.pre_out = session->pre;
-    } else {
-        ga.wire_out = wire_out_.get();
-        ga.density_out = density_out_.get();
+    for (int e = 0; e < count; ++e) {
+        const Slot& sl = slots_[e];
+        ga.pos[e] = specs[e].pos;
+        ga.ex[e] = sl.ex.get();
+        ga.ey[e] = sl.ey.get();
+        ga.pin_grad[e] = sl.pin_grad.get();
+        ga.partials[e] = partials(kRedGather, specs[e].bank);
+        const SessionOut* so = specs[e].session;
+        if (so) {
+            ga.precond = so->precond;
+            ga.penalty = so->penalty;
+            ga.inv_strength = so->inv_strength;
+            ga.raw_out[e] = so->raw;
+            ga.pre_out[e] = so->pre;
+            ga.prev_pos[e] = so->prev_pos;
+            ga.px = so->px;
+            ga.pg = so->pg;
+            ga.ck_out[e] = so->ck_out;
+        } else {
+            ga.wire_out = wire_out_.get();
+            ga.density_out = density_out_.get();
+        }
     }
-    ga.partials = partials(kRedGather, bank);
     timedBegin(kTGather);
-    launchGather(ga, s);
-    addReduction(kRedGather, bank, particleBlocks(P_));
-    timedEnd(kTGather, 20.0 * P_ + 8.0 * static_cast<double>(nx_) * ny_);
+    launchGather(ga, count, s);
+    for (int e = 0; e < count; ++e) addReduction(kRedGather, specs[e].bank, particleBlocks(P_));
+    // algorithmic bytes: position, charge, two gradient outputs (40 + 32 per particle in session
+    // mode with the preconditioner), the CSR walk (8 per pin) and the two field grids
+    timedEnd(kTGather, count * (48.0 * P_ + 8.0 * I_ + 8.0 * static_cast<double>(nx_) * ny_));
 }
 
-void Backend::addReduction(int kind, int bank, int blocks, int slot, double p0, double p1, double p2) {
+void Backend::addReduction(int kind, int bank, int blocks, double p0, double p1, double p2) {
     if (pending_.n == kMaxRedJobs) throw std::logic_error("too many pending reductions");
-    pending_.job[pending_.n++] = RedJob{partials(kind, bank), blocks, kind, scalarBlock(bank), slot, p0, p1, p2};
+    pending_.job[pending_.n++] = RedJob{partials(kind, bank), blocks, kind, dev_scal_ + bank * kScalarCount, p0, p1, p2};
 }
 
 const HostScalars& Backend::fetchScalars() {
     launchFinalize(pending_, stream_);
     pending_.n = 0;
-    hipCheck(hipMemcpyAsync(host_scal_, scal_.get(), sizeof(HostScalars), hipMemcpyDeviceToHost, stream_),
-             "read scalar block");
     hipCheck(hipStreamSynchronize(stream_), "synchronize placement evaluation");
     return *host_scal_;
 }
@@ -467,12 +545,14 @@ void Backend::evaluateHost(const double* xy, double gamma, bool want_grad, mp_gp
     if (want_grad && P_ > 0 && (wire_grad == nullptr || density_grad == nullptr))
         throw InvalidArgument("gradient outputs are required when want_grad is set");
     setDevice();
-    pos_.upload(reinterpret_cast<const double2*>(xy), P_, stream_, "upload particle positions");
-    enqueueEvaluate(pos_.get(), gamma, want_grad, nullptr);
+    hipStream_t s = stream_;
+    pos_.upload(reinterpret_cast<const double2*>(xy), P_, s, "upload particle positions");
+    const EvalSpec spec{pos_.get(), nullptr, 0, kFreshOrder};
+    enqueueEvaluate(&spec, 1, gamma, want_grad);
     if (want_grad && P_ > 0) {
-        hipCheck(hipMemcpyAsync(wire_grad, wire_out_.get(), P_ * sizeof(double2), hipMemcpyDeviceToHost, stream_),
+        hipCheck(hipMemcpyAsync(wire_grad, wire_out_.get(), P_ * sizeof(double2), hipMemcpyDeviceToHost, s),
                  "download wire gradients");
-        hipCheck(hipMemcpyAsync(density_grad, density_out_.get(), P_ * sizeof(double2), hipMemcpyDeviceToHost, stream_),
+        hipCheck(hipMemcpyAsync(density_grad, density_out_.get(), P_ * sizeof(double2), hipMemcpyDeviceToHost, s),
                  "download density gradients");
     }
     const HostScalars& h = fetchScalars();
@@ -485,9 +565,10 @@ void Backend::debugGrids(double* source, double* potential, double* field_x, dou
     setDevice();
     if (!have_fields_) throw std::runtime_error("no gradient evaluation has produced field grids yet");
     const std::size_t Gp = static_cast<std::size_t>(nxp_) * nyp_;
+    const Slot& sl = slots_[0];
     if (potential) {
         SpectralArgs sp{};
-        GemmBatch g{{ayi_.get()}, {t12_.get()}, {pot_.get()}};
+        GemmBatch g{{ayi_.get()}, {sl.t12.get()}, {pot_.get()}};
         launchGemm(g, 1, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiStore, sp, dct_precision_, stream_);
     }
     std::vector<float> buf(Gp);
@@ -499,7 +580,7 @@ void Backend::debugGrids(double* source, double* potential, double* field_x, dou
             for (int c = 0; c < nx_; ++c)
                 dst[static_cast<std::size_t>(r) * nx_ + c] = buf[static_cast<std::size_t>(r) * nxp_ + c];
     };
-    pull(rho_, source);
+    pull(sl.rho, source);
     if (source) {  // centre it as ElectrostaticField::solve does
         double mean = 0.0;
         for (std::size_t i = 0; i < static_cast<std::size_t>(nx_) * ny_; ++i) mean += source[i];
@@ -507,8 +588,8 @@ void Backend::debugGrids(double* source, double* potential, double* field_x, dou
         for (std::size_t i = 0; i < static_cast<std::size_t>(nx_) * ny_; ++i) source[i] -= mean;
     }
     pull(pot_, potential);
-    pull(ex_, field_x);
-    pull(ey_, field_y);
+    pull(sl.ex, field_x);
+    pull(sl.ey, field_y);
 }
 
 // ---------------------------------------------------------------------------------------------

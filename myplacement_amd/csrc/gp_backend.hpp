@@ -2,8 +2,9 @@
 //
 // One Backend owns every device buffer of one placement problem (the role of
 // CudaPlacementBackendImpl, src/placement/cuda/CudaPlacementBackend.cu:841-1148).  Static data is
-// uploaded once; an evaluation enqueues its kernels on the backend's stream and the host reads one
-// 256-byte scalar block back.
+// uploaded once.  One or two evaluations (the controller's current and look-ahead positions) are
+// enqueued as one chain of batched launches on one stream, each with its own work buffers; every
+// reduction result lands in a small host-mapped scalar block that k_finalize writes directly.
 #pragma once
 
 #include "mp_gp_backend.h"
@@ -77,21 +78,41 @@ struct HostScalars {
 
 class Backend {
 public:
+    // `margin` of an evaluation: rebuild the particle row order at its positions first.
+    static constexpr int kFreshOrder = -1;
+
     Backend(const mp_gp_static& layout, std::size_t allocation_limit);
     ~Backend();
     Backend(const Backend&) = delete;
     Backend& operator=(const Backend&) = delete;
 
-    // Enqueues one evaluation at device positions `pos`.  With `session` set, the gather writes the
-    // combined raw / preconditioned gradient instead of the two separate gradients.
+    // Session-mode outputs of a gradient evaluation: the gather writes the combined raw /
+    // preconditioned gradient and, where the pointers are set, the controller sums and a
+    // checkpoint copy of the positions.
     struct SessionOut {
         const double* precond;
         double penalty, inv_strength;
         double2* raw;
         double2* pre;
+        const double2* prev_pos = nullptr;
+        const double2* px = nullptr;
+        const double2* pg = nullptr;
+        double2* ck_out = nullptr;
     };
-    void enqueueEvaluate(const double2* pos, double gamma, bool want_grad, const SessionOut* session, int bank = 0);
-    // Copies the scalar block to host (pinned) and waits for the stream.
+    struct EvalSpec {
+        const double2* pos;          // device positions
+        const SessionOut* session;   // null: write the separate wire / density gradients
+        int bank;                    // scalar bank of the results
+        int margin;                  // bin rows the shared row order may be stale by, or kFreshOrder
+    };
+    // Enqueues `count` (1 or 2) evaluations with smoothing `gamma` as one chain of batched
+    // launches.  The density bands read the shared particle row order, which must be within each
+    // evaluation's `margin` bin rows of its positions for every particle (kFreshOrder: rebuild it
+    // first, single evaluations only).
+    void enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bool want_grad);
+    // Rebuilds the particle row order at `pos`.
+    void sortParticles(const double2* pos);
+    // Reduces the pending partials into the scalar block and waits for the stream.
     const HostScalars& fetchScalars();
     void toScalars(const HostScalars& h, mp_gp_scalars* out, int bank = 0) const;
 
@@ -107,13 +128,12 @@ public:
     const Region& region() const { return region_; }
     DeviceBudget& budget() { return budget_; }
     const double2* sizes() const { return size_.get(); }
-    double* scalarBlock(int bank = 0) const { return scal_.get() + bank * kScalarCount; }
     // Partial rows of reduction `kind` destined for scalar bank `bank`.
     double* partials(int kind, int bank) const { return partials_[kind * kScalarBanks + bank].get(); }
     // Queues the final reduction of `blocks` partial rows; done by the next fetchScalars().
-    void addReduction(int kind, int bank, int blocks, int slot = 0, double p0 = 0, double p1 = 0, double p2 = 0);
+    void addReduction(int kind, int bank, int blocks, double p0 = 0, double p1 = 0, double p2 = 0);
 
-    // Optional per-kernel timing with HIP events on the backend's stream.
+    // Optional per-kernel-class timing with HIP events on the backend's stream.
     void enableTiming(bool on);
     std::vector<std::string> timingNames() const;
     void timing(std::vector<double>& total_us, std::vector<long long>& launches, std::vector<double>& work);
@@ -121,7 +141,14 @@ public:
     void timedEnd(int cls, double work = 0.0);
 
 private:
+    // work buffers of one evaluation of a batch
+    struct Slot {
+        DeviceArray<float2> pin_grad;
+        DeviceArray<float> rho, tmp, phi, t12, ex, ey;
+    };
+    static constexpr int kSlots = kMaxEval;
     void setDevice() const;
+    DensityArgs densityArgs() const;
 
     int device_ = 0;
     hipStream_t stream_ = nullptr;
@@ -133,6 +160,7 @@ private:
     int dct_precision_ = 0;
     bool have_fields_ = false;
     DeviceBudget budget_;
+    Slot slots_[kSlots];
 
     DeviceArray<double2> pos_, size_;
     DeviceArray<double> charge_;
@@ -141,23 +169,21 @@ private:
     int n_small_ = 0, n_medium_ = 0, n_large_ = 0;
     DeviceArray<double> net_w_;
     DeviceArray<double2> pin_off_;
-    DeviceArray<float2> pin_grad_;
     DeviceArray<double> static_charge_;
-    DeviceArray<int> key_, hist_, colprefix_, keytotal_, keystart_;
-    DeviceArray<double4> sorted_;
-    DeviceArray<int4> sorted_span_;
+    DeviceArray<int> key_, hist_, colprefix_, keytotal_, keystart_, tall_ids_;
+    DeviceArray<unsigned> perm_;
+    DeviceArray<double2> sorted_size_;
     DeviceArray<unsigned char> tall_;
-    DeviceArray<unsigned long long> residual_;
+    int n_tall_ = 0;
     int band_rows_ = 1, bands_ = 1;
-    bool has_tall_ = false;
     double fix_ = 1.0;
-    DeviceArray<float> rho_, tmp_, phi_, t12_, ex_, ey_, pot_;
+    DeviceArray<float> pot_;
     DeviceArray<float> bxf_, ayf_, bxi_, ayi_, syi_;
     DeviceArray<double> partials_[kRedKinds * kScalarBanks];
     RedList pending_{};
-    DeviceArray<double> scal_;
     DeviceArray<double2> wire_out_, density_out_;
-    HostScalars* host_scal_ = nullptr;
+    HostScalars* host_scal_ = nullptr;  // host-mapped, written by k_finalize
+    double* dev_scal_ = nullptr;        // device address of host_scal_
 
     // timing
     static constexpr int kTimingClasses = 5;

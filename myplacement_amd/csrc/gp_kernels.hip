@@ -154,18 +154,19 @@ __device__ inline float groupSum(float v) {
 
 // One net on a group of W lanes (W = 32 or 64); returns (hpwl, smooth) in lane 0 of the group.
 template <int W>
-__device__ void wireGroup(const WireArgs& a, int net, int sub, double inv_gamma, float gamma_f, double& hp, double& wl) {
+__device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_grad, int net, int sub, double inv_gamma,
+                          float gamma_f, double& hp, double& wl) {
     const int b = a.net_off[net], e = a.net_off[net + 1];
     double mx = -INFINITY, nx = INFINITY, my = -INFINITY, ny = INFINITY;
     for (int p = b + sub; p < e; p += W) {
-        const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, p);
+        const double2 q = pinPosition(pos, a.pin_part, a.pin_off, p);
         mx = fmax(mx, q.x); nx = fmin(nx, q.x);
         my = fmax(my, q.y); ny = fmin(ny, q.y);
     }
     mx = groupMax<W>(mx); nx = groupMin<W>(nx); my = groupMax<W>(my); ny = groupMin<W>(ny);
     float spx = 0.f, snx = 0.f, spy = 0.f, sny = 0.f;
     for (int p = b + sub; p < e; p += W) {
-        const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, p);
+        const double2 q = pinPosition(pos, a.pin_part, a.pin_off, p);
         spx += ex32(q.x - mx, inv_gamma);
         snx += ex32(nx - q.x, inv_gamma);
         spy += ex32(q.y - my, inv_gamma);
@@ -182,8 +183,8 @@ __device__ void wireGroup(const WireArgs& a, int net, int sub, double inv_gamma,
         const float wf = static_cast<float>(w);
         const float ipx = wf / spx, inx = wf / snx, ipy = wf / spy, iny = wf / sny;
         for (int p = b + sub; p < e; p += W) {
-            const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, p);
-            a.pin_grad[p] = make_float2(ex32(q.x - mx, inv_gamma) * ipx - ex32(nx - q.x, inv_gamma) * inx,
+            const double2 q = pinPosition(pos, a.pin_part, a.pin_off, p);
+            pin_grad[p] = make_float2(ex32(q.x - mx, inv_gamma) * ipx - ex32(nx - q.x, inv_gamma) * inx,
                                         ex32(q.y - my, inv_gamma) * ipy - ex32(ny - q.y, inv_gamma) * iny);
         }
     }
@@ -195,6 +196,8 @@ __global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int small_blocks,
     const double inv_gamma = 1.0 / a.gamma;
     const float gamma_f = static_cast<float>(a.gamma);
     const int block = blockIdx.x;
+    const double2* pos = a.pos[blockIdx.y];
+    float2* pin_grad = a.pin_grad[blockIdx.y];
     if (block < small_blocks) {
         const int net = block * kThreads + threadIdx.x;
         if (net < a.n_small) {
@@ -205,7 +208,7 @@ __global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int small_blocks,
 #pragma unroll
             for (int i = 0; i < kSmallDegree; ++i) {
                 if (i < deg) {
-                    const double2 q = pinPosition(a.pos, a.pin_part, a.pin_off, b + i);
+                    const double2 q = pinPosition(pos, a.pin_part, a.pin_off, b + i);
                     qx[i] = q.x;
                     qy[i] = q.y;
                     mx = fmax(mx, q.x); nx = fmin(nx, q.x);
@@ -234,19 +237,19 @@ __global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int small_blocks,
 #pragma unroll
                 for (int i = 0; i < kSmallDegree; ++i)
                     if (i < deg)
-                        a.pin_grad[b + i] = make_float2(epx[i] * ipx - enx[i] * inx, epy[i] * ipy - eny[i] * iny);
+                        pin_grad[b + i] = make_float2(epx[i] * ipx - enx[i] * inx, epy[i] * ipy - eny[i] * iny);
             }
         }
     } else if (block < small_blocks + medium_blocks) {
         const int lane = threadIdx.x & 63;
         const int net = a.n_small + (block - small_blocks) * (2 * kWaves) + 2 * (threadIdx.x >> 6) + (lane >> 5);
-        if (net < a.n_small + a.n_medium) wireGroup<32>(a, net, lane & 31, inv_gamma, gamma_f, hp, wl);
+        if (net < a.n_small + a.n_medium) wireGroup<32>(a, pos, pin_grad, net, lane & 31, inv_gamma, gamma_f, hp, wl);
     } else {
         const int net = a.n_small + a.n_medium + (block - small_blocks - medium_blocks) * kWaves + (threadIdx.x >> 6);
-        if (net < a.n_small + a.n_medium + a.n_large) wireGroup<64>(a, net, threadIdx.x & 63, inv_gamma, gamma_f, hp, wl);
+        if (net < a.n_small + a.n_medium + a.n_large) wireGroup<64>(a, pos, pin_grad, net, threadIdx.x & 63, inv_gamma, gamma_f, hp, wl);
     }
     double v[2] = {hp, wl};
-    publishPartials<2>(v, 0U, blockIdx.x, a.partials, lds);
+    publishPartials<2>(v, 0U, blockIdx.x, a.partials[blockIdx.y], lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -254,8 +257,10 @@ __global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int small_blocks,
 // band of bin rows accumulates its particles' exact rectangle / bin overlaps (FP64 geometry, bin
 // index math of Metrics.cpp:78-85) into LDS as 2^40-per-bin-area fixed point -- order-independent,
 // so the density is bit-reproducible -- and evaluates the bin metrics and the FP32 deviation grid
-// for the DCT in the same kernel.  Particles taller than the band look-back (rare: movable macros)
-// are deposited with global 64-bit integer atomics into a residual layer that the bands fold in.
+// for the DCT in the same kernel.  The row order may be stale by `margin` rows (positions move a
+// fraction of a bin per evaluation), so it is rebuilt only when the accumulated movement could
+// exceed that margin; every band recomputes its particles' spans from the current positions.
+// Particles taller than two bin rows (movable macros) are not sorted: every band visits them.
 
 __device__ inline void clippedSpan(double2 c, double2 sz, const Region& r, double& left, double& right,
                                    double& bottom, double& top, int& c0, int& c1, int& r0, int& r1, bool& empty) {
@@ -274,7 +279,9 @@ __device__ inline unsigned long long toFixed(double area, double fix) {
     return static_cast<unsigned long long>(__double2ll_rn(area * fix));
 }
 
-// keys: first covered row, or ny (nothing to deposit) / ny + 1 (tall: residual path)
+// key: first covered row (clamped into the grid), or ny for the tall particles the bands visit
+// directly.  Particles whose clipped area is empty keep their row key: they may become non-empty
+// while the order is reused, and the bands test emptiness at the positions they deposit.
 __global__ __launch_bounds__(kSortThreads) void k_dhist(DensityArgs a) {
     extern __shared__ int hist[];
     const int ny = a.r.ny;
@@ -285,8 +292,8 @@ __global__ __launch_bounds__(kSortThreads) void k_dhist(DensityArgs a) {
         double l, rr, bo, to;
         int c0, c1, r0, r1;
         bool empty;
-        clippedSpan(a.pos[p], a.size[p], a.r, l, rr, bo, to, c0, c1, r0, r1, empty);
-        const int key = empty ? ny : (a.tall[p] ? ny + 1 : r0);
+        clippedSpan(a.pos[0][p], a.size[p], a.r, l, rr, bo, to, c0, c1, r0, r1, empty);
+        const int key = a.tall[p] ? ny : r0;
         a.key[p] = key;
         if (key < ny) atomicAdd(&hist[key], 1);
     }
@@ -337,37 +344,11 @@ __global__ __launch_bounds__(kSortThreads) void k_dplace(DensityArgs a) {
     const int p = blockIdx.x * kSortThreads + threadIdx.x;
     if (p >= a.n) return;
     const int key = a.key[p];
-    if (key > ny) {  // tall particle: straight into the residual layer
-        double l, rr, bo, to;
-        int c0, c1, r0, r1;
-        bool empty;
-        clippedSpan(a.pos[p], a.size[p], a.r, l, rr, bo, to, c0, c1, r0, r1, empty);
-        const int layer = a.layer[p];
-        const double scale = layer == 1 ? a.t : 1.0;
-        unsigned long long* dst = a.residual + (layer == 2 ? static_cast<size_t>(a.r.nx) * ny : 0);
-        for (int row = r0; row <= r1; ++row) {
-            const double bin_bottom = a.r.lly + static_cast<double>(row) * a.r.bh;
-            const double oy = fmax(0.0, fmin(to, bin_bottom + a.r.bh) - fmax(bo, bin_bottom));
-            for (int col = c0; col <= c1; ++col) {
-                const double bin_left = a.r.llx + static_cast<double>(col) * a.r.bw;
-                const double ox = fmax(0.0, fmin(rr, bin_left + a.r.bw) - fmax(l, bin_left));
-                const double area = ox * oy;
-                if (area <= kEps) continue;
-                atomicAdd(dst + static_cast<size_t>(row) * a.r.nx + col, toFixed(scale * area, a.fix));
-            }
-        }
-        return;
-    }
-    if (key == ny) return;
+    if (key >= ny) return;
     const int slot = start[key] + a.colprefix[static_cast<size_t>(blockIdx.x) * ny + key] + atomicAdd(&rank[key], 1);
-    double l, rr, bo, to;
-    int c0, c1, r0, r1;
-    bool empty;
-    clippedSpan(a.pos[p], a.size[p], a.r, l, rr, bo, to, c0, c1, r0, r1, empty);
-    a.sorted[slot] = make_double4(l, rr, bo, to);
-    a.sorted_span[slot] = make_int4(c0 | (a.layer[p] << 28), c1, r0, r1);
+    a.perm[slot] = static_cast<unsigned>(p) | (static_cast<unsigned>(a.layer[p]) << kPermLayerShift);
+    a.sorted_size[slot] = a.size[p];
 }
-
 __global__ __launch_bounds__(kBandThreads) void k_dband(DensityArgs a) {
     extern __shared__ unsigned long long band[];
     __shared__ double lds[(kBandThreads / 64 + 1) * 9];
@@ -378,47 +359,50 @@ __global__ __launch_bounds__(kBandThreads) void k_dband(DensityArgs a) {
     const int cells = (R1 - R0) * nx;
     unsigned long long* lmov = band;
     unsigned long long* lfil = band + static_cast<size_t>(a.band_rows) * nx;
-    const size_t G = static_cast<size_t>(nx) * ny;
     for (int i = threadIdx.x; i < cells; i += kBandThreads) {
-        const size_t gi = static_cast<size_t>(R0) * nx + i;
-        if (a.has_tall) {
-            lmov[i] = a.residual[gi];
-            lfil[i] = a.residual[G + gi];
-            a.residual[gi] = 0ULL;
-            a.residual[G + gi] = 0ULL;
-        } else {
-            lmov[i] = 0ULL;
-            lfil[i] = 0ULL;
-        }
+        lmov[i] = 0ULL;
+        lfil[i] = 0ULL;
     }
     __syncthreads();
-    if (a.n > 0) {
-        const int lo = max(0, R0 - a.lookback + 1);
-        const int q0 = a.keystart[lo], q1 = a.keystart[R1];
-        for (int q = q0 + threadIdx.x; q < q1; q += kBandThreads) {
-            const int4 sp = a.sorted_span[q];
-            const int ra = max(sp.z, R0), rb = min(sp.w, R1 - 1);
-            if (ra > rb) continue;
-            const double4 rc = a.sorted[q];  // clipped left, right, bottom, top
-            const int layer = sp.x >> 28, c0 = sp.x & 0x0fffffff;
-            const double scale = layer == 1 ? a.t : 1.0;
-            unsigned long long* dst = layer == 2 ? lfil : lmov;
-            for (int row = ra; row <= rb; ++row) {
-                const double bin_bottom = r.lly + static_cast<double>(row) * r.bh;
-                const double oy = fmax(0.0, fmin(rc.w, bin_bottom + r.bh) - fmax(rc.z, bin_bottom));
-                for (int col = c0; col <= sp.y; ++col) {
-                    const double bin_left = r.llx + static_cast<double>(col) * r.bw;
-                    const double ox = fmax(0.0, fmin(rc.y, bin_left + r.bw) - fmax(rc.x, bin_left));
-                    const double area = ox * oy;
-                    if (area <= kEps) continue;
-                    atomicAdd(dst + (row - R0) * nx + col, toFixed(scale * area, a.fix));
-                }
+    auto deposit = [&](double2 c, double2 sz, int layer) {
+        double left, right, bottom, top;
+        int c0, c1, r0, r1;
+        bool empty;
+        clippedSpan(c, sz, r, left, right, bottom, top, c0, c1, r0, r1, empty);
+        const int ra = max(r0, R0), rb = min(r1, R1 - 1);
+        if (empty || ra > rb) return;
+        const double scale = layer == 1 ? a.t : 1.0;
+        unsigned long long* dst = layer == 2 ? lfil : lmov;
+        for (int row = ra; row <= rb; ++row) {
+            const double bin_bottom = r.lly + static_cast<double>(row) * r.bh;
+            const double oy = fmax(0.0, fmin(top, bin_bottom + r.bh) - fmax(bottom, bin_bottom));
+            for (int col = c0; col <= c1; ++col) {
+                const double bin_left = r.llx + static_cast<double>(col) * r.bw;
+                const double ox = fmax(0.0, fmin(right, bin_left + r.bw) - fmax(left, bin_left));
+                const double area = ox * oy;
+                if (area <= kEps) continue;
+                atomicAdd(dst + (row - R0) * nx + col, toFixed(scale * area, a.fix));
             }
+        }
+    };
+    if (a.n > 0) {
+        const double2* pos = a.pos[blockIdx.y];
+        const int margin = a.margin[blockIdx.y];
+        const int lo = max(0, R0 - a.lookback + 1 - margin);
+        const int hi = min(ny, R1 + margin);
+        for (int q = a.keystart[lo] + threadIdx.x; q < a.keystart[hi]; q += kBandThreads) {
+            const unsigned e = a.perm[q];
+            deposit(pos[e & kPermIdMask], a.sorted_size[q], static_cast<int>(e >> kPermLayerShift));
+        }
+        for (int k = threadIdx.x; k < a.n_tall; k += kBandThreads) {
+            const int p = a.tall_ids[k];
+            deposit(pos[p], a.size[p], a.layer[p]);
         }
     }
     __syncthreads();
     // bin metrics (Metrics.cpp:101-119 per bin) and the deviation grid
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float* rho = a.rho[blockIdx.y];
     for (int i = threadIdx.x; i < cells; i += kBandThreads) {
         const int row = R0 + i / nx, col = i - (i / nx) * nx;
         const double left = r.llx + static_cast<double>(col) * r.bw;
@@ -440,9 +424,9 @@ __global__ __launch_bounds__(kBandThreads) void k_dband(DensityArgs a) {
         v[6] += d_wo;
         v[7] = fmax(v[7], d_w);
         v[8] = fmax(v[8], d_wo);
-        a.rho[static_cast<size_t>(row) * a.ldrho + col] = static_cast<float>(d_w - a.t);
+        rho[static_cast<size_t>(row) * a.ldrho + col] = static_cast<float>(d_w - a.t);
     }
-    publishPartials<9, kBandThreads / 64>(v, (1U << 7) | (1U << 8), blockIdx.x, a.partials, lds);
+    publishPartials<9, kBandThreads / 64>(v, (1U << 7) | (1U << 8), blockIdx.x, a.partials[blockIdx.y], lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -554,7 +538,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int K, int lda, 
     }
     if (EPI == kEpiSpectral) {
         double v[1] = {energy};
-        publishPartials<1>(v, 0U, blockIdx.y * gridDim.x + blockIdx.x, sp.partials, lds);
+        publishPartials<1>(v, 0U, blockIdx.y * gridDim.x + blockIdx.x, sp.partials[blockIdx.z], lds);
     }
 }
 
@@ -593,18 +577,20 @@ __device__ inline double sampleParity(const float* __restrict__ v, int ld, doubl
 }
 
 __global__ __launch_bounds__(kThreads) void k_gather(GatherArgs a) {
-    __shared__ double lds[(kWaves + 1) * 4];
+    __shared__ double lds[(kWaves + 1) * 7];
     const int p = blockIdx.x * kThreads + threadIdx.x;
-    double v[4] = {0, 0, 0, 0};  // wire L1, density L1, |raw|^2, |pre|^2
+    // wire L1, density L1, |raw|^2, |pre|^2, V.pre, |pos - px|^2, |pre - pg|^2
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    const int e = blockIdx.y;
     if (p < a.n) {
-        const double2 c = a.pos[p];
+        const double2 c = a.pos[e][p];
         const double q = a.charge[p];
-        const double ex = sampleParity(a.ex, a.ldf, c.x, c.y, a.r, true, false);
-        const double ey = sampleParity(a.ey, a.ldf, c.x, c.y, a.r, false, true);
+        const double ex = sampleParity(a.ex[e], a.ldf, c.x, c.y, a.r, true, false);
+        const double ey = sampleParity(a.ey[e], a.ldf, c.x, c.y, a.r, false, true);
         const double2 dg = make_double2(-q * ex, -q * ey);
         double2 wg = make_double2(0.0, 0.0);
         for (int j = a.ppin_off[p]; j < a.ppin_off[p + 1]; ++j) {
-            const float2 g = a.pin_grad[a.ppin_idx[j]];
+            const float2 g = a.pin_grad[e][a.ppin_idx[j]];
             wg.x += static_cast<double>(g.x);
             wg.y += static_cast<double>(g.y);
         }
@@ -614,17 +600,28 @@ __global__ __launch_bounds__(kThreads) void k_gather(GatherArgs a) {
             a.wire_out[p] = wg;
             a.density_out[p] = dg;
         }
-        if (a.raw_out) {
+        if (a.raw_out[e]) {
             const double2 raw = make_double2(wg.x + dg.x * a.penalty, wg.y + dg.y * a.penalty);
             const double s = a.precond[p] * a.inv_strength;
             const double2 pre = make_double2(raw.x * s, raw.y * s);
-            a.raw_out[p] = raw;
-            a.pre_out[p] = pre;
+            a.raw_out[e][p] = raw;
+            a.pre_out[e][p] = pre;
             v[2] = raw.x * raw.x + raw.y * raw.y;
             v[3] = pre.x * pre.x + pre.y * pre.y;
+            if (a.prev_pos[e]) {  // momentumAlignment, AdaptiveGlobalPlacer.cpp:610-617 (V = X - X_prev)
+                const double2 o = a.prev_pos[e][p];
+                v[4] = (c.x - o.x) * pre.x + (c.y - o.y) * pre.y;
+            }
+            if (a.px) {  // estimateStep sums, :480-486
+                const double2 b = a.px[p], g = a.pg[p];
+                const double dx = c.x - b.x, dy = c.y - b.y, gx = pre.x - g.x, gy = pre.y - g.y;
+                v[5] = dx * dx + dy * dy;
+                v[6] = gx * gx + gy * gy;
+            }
         }
+        if (a.ck_out[e]) a.ck_out[e][p] = c;
     }
-    publishPartials<4>(v, 0U, blockIdx.x, a.partials, lds);
+    publishPartials<7>(v, 0U, blockIdx.x, a.partials[e], lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -637,42 +634,18 @@ __device__ inline double2 clampCenter(double2 c, double2 sz, const Region& r) { 
     return c;
 }
 
-__global__ __launch_bounds__(kThreads) void k_lookahead(const double2* x, const double2* v, double beta,
+// B = clamp(X + beta * (X - X_prev)): the look-ahead of :776-780 with the velocity of :861-863
+__global__ __launch_bounds__(kThreads) void k_lookahead(const double2* x, const double2* x_prev, double beta,
                                                         double2* out, ClampArgs c, int n) {
     const int p = blockIdx.x * kThreads + threadIdx.x;
     if (p >= n) return;
-    const double2 a = x[p], b = v[p];
-    out[p] = clampCenter(make_double2(a.x + b.x * beta, a.y + b.y * beta), c.size[p], c.r);
-}
-
-__global__ __launch_bounds__(kThreads) void k_dot(const double2* a, const double2* b, int n, double* partials) {
-    __shared__ double lds[kWaves + 1];
-    const int p = blockIdx.x * kThreads + threadIdx.x;
-    double v[1] = {0.0};
-    if (p < n) {
-        const double2 x = a[p], y = b[p];
-        v[0] = x.x * y.x + x.y * y.y;
-    }
-    publishPartials<1>(v, 0U, blockIdx.x, partials, lds);
-}
-
-__global__ __launch_bounds__(kThreads) void k_curvature(const double2* b, const double2* pb, const double2* g,
-                                                        const double2* pg, int n, double* partials) {
-    __shared__ double lds[(kWaves + 1) * 2];
-    const int p = blockIdx.x * kThreads + threadIdx.x;
-    double v[2] = {0.0, 0.0};
-    if (p < n) {
-        const double2 x = b[p], px = pb[p], y = g[p], py = pg[p];
-        const double dx = x.x - px.x, dy = x.y - px.y, gx = y.x - py.x, gy = y.y - py.y;
-        v[0] = dx * dx + dy * dy;
-        v[1] = gx * gx + gy * gy;
-    }
-    publishPartials<2>(v, 0U, blockIdx.x, partials, lds);
+    const double2 a = x[p], o = x_prev[p];
+    out[p] = clampCenter(make_double2(a.x + (a.x - o.x) * beta, a.y + (a.y - o.y) * beta), c.size[p], c.r);
 }
 
 __global__ __launch_bounds__(kThreads) void k_propose(const double2* b, const double2* g, const double2* raw, double step,
-                                                      double max_disp, double2* y, ClampArgs c, int n,
-                                                      double* partials) {
+                                                      double max_disp, double2* y, double2* b_copy, double2* g_copy,
+                                                      ClampArgs c, int n, double* partials) {
     __shared__ double lds[(kWaves + 1) * 2];
     const int p = blockIdx.x * kThreads + threadIdx.x;
     double v[2] = {0.0, 0.0};
@@ -687,19 +660,16 @@ __global__ __launch_bounds__(kThreads) void k_propose(const double2* b, const do
         }
         const double2 out = clampCenter(make_double2(base.x + dx, base.y + dy), c.size[p], c.r);
         y[p] = out;
+        if (b_copy) {  // staged copies of the step base, kept if the step is accepted (:886-888)
+            b_copy[p] = base;
+            g_copy[p] = gr;
+        }
         const double ax = base.x - out.x, ay = base.y - out.y;
         const double2 r = raw[p];
         v[0] = r.x * ax + r.y * ay;
         v[1] = sqrt(ax * ax + ay * ay);
     }
     publishPartials<2>(v, 2U, blockIdx.x, partials, lds);
-}
-
-__global__ __launch_bounds__(kThreads) void k_difference(const double2* y, const double2* x, double2* v, int n) {
-    const int p = blockIdx.x * kThreads + threadIdx.x;
-    if (p >= n) return;
-    const double2 a = y[p], b = x[p];
-    v[p] = make_double2(a.x - b.x, a.y - b.y);
 }
 
 __global__ __launch_bounds__(kThreads) void k_zero(double* p, size_t count) {
@@ -712,7 +682,7 @@ __global__ __launch_bounds__(kThreads) void k_zero(double* p, size_t count) {
 // One workgroup per pending reduction: sums (or maxima) of the partial rows in index order, then
 // the reference's post-processing of each scalar (DensityMap::metrics Metrics.cpp:101-119, energy
 // clamp .cu:1008, alignment sign AdaptiveGlobalPlacer.cpp:610-617).
-__device__ const int kRedWidths[kRedKinds] = {2, 9, 1, 4, 2, 1, 2, 2};
+__device__ const int kRedWidths[kRedKinds] = {2, 9, 1, 7, 2};
 
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(RedList l) {
     __shared__ double lds[(kFinalizeThreads / 64 + 1) * 9];
@@ -754,17 +724,13 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(RedList l) {
             o[kDensityL1] = acc[1];
             o[kRawNorm2] = acc[2];
             o[kPreNorm2] = acc[3];
+            o[kAlignment] = -acc[4];
+            o[kCurvDp] = acc[5];
+            o[kCurvDg] = acc[6];
             break;
-        case kRedPropose:
+        default:  // kRedPropose
             o[kPredicted] = acc[0];
             o[kMaxDisp] = acc[1];
-            break;
-        case kRedDot:
-            o[j.slot] = j.p0 * acc[0];
-            break;
-        default:  // curvature pairs
-            o[j.slot] = acc[0];
-            o[j.slot + 1] = acc[1];
             break;
     }
 }
@@ -777,29 +743,33 @@ int wireBlocks(int n_small, int n_medium, int n_large) {
     return blocksFor(n_small) + (n_medium + 2 * kWaves - 1) / (2 * kWaves) + (n_large + kWaves - 1) / kWaves;
 }
 
-void launchWire(const WireArgs& a, hipStream_t s) {
+void launchWire(const WireArgs& a, int count, hipStream_t s) {
     const int small_blocks = blocksFor(a.n_small);
     const int medium_blocks = (a.n_medium + 2 * kWaves - 1) / (2 * kWaves);
     const int blocks = wireBlocks(a.n_small, a.n_medium, a.n_large);
     if (blocks == 0) return;
-    hipLaunchKernelGGL(k_wire, dim3(blocks), dim3(kThreads), 0, s, a, small_blocks, medium_blocks);
+    if (count < 1 || count > kMaxEval) throw std::invalid_argument("bad evaluation batch");
+    hipLaunchKernelGGL(k_wire, dim3(blocks, count), dim3(kThreads), 0, s, a, small_blocks, medium_blocks);
     check(hipGetLastError(), "launch wirelength kernel");
 }
 
 int densityHistBlocks(int n) { return std::max(1, (n + kSortThreads - 1) / kSortThreads); }
 
-void launchDensity(const DensityArgs& a, hipStream_t s) {
+void launchDensitySort(const DensityArgs& a, hipStream_t s) {
     const int hb = densityHistBlocks(a.n);
-    if (a.n > 0) {
-        hipLaunchKernelGGL(k_dhist, dim3(hb), dim3(kSortThreads), static_cast<size_t>(a.r.ny) * sizeof(int), s, a);
-        check(hipGetLastError(), "launch density histogram kernel");
-        hipLaunchKernelGGL(k_dscan, dim3(a.r.ny), dim3(kThreads), 0, s, a, hb);
-        check(hipGetLastError(), "launch density scan kernel");
-        hipLaunchKernelGGL(k_dplace, dim3(hb), dim3(kSortThreads), (2 * static_cast<size_t>(a.r.ny) + 1) * sizeof(int), s, a);
-        check(hipGetLastError(), "launch density placement kernel");
-    }
+    if (a.n == 0) return;
+    hipLaunchKernelGGL(k_dhist, dim3(hb), dim3(kSortThreads), static_cast<size_t>(a.r.ny) * sizeof(int), s, a);
+    check(hipGetLastError(), "launch density histogram kernel");
+    hipLaunchKernelGGL(k_dscan, dim3(a.r.ny), dim3(kThreads), 0, s, a, hb);
+    check(hipGetLastError(), "launch density scan kernel");
+    hipLaunchKernelGGL(k_dplace, dim3(hb), dim3(kSortThreads), (2 * static_cast<size_t>(a.r.ny) + 1) * sizeof(int), s, a);
+    check(hipGetLastError(), "launch density placement kernel");
+}
+
+void launchDensityBands(const DensityArgs& a, int count, hipStream_t s) {
+    if (count < 1 || count > kMaxEval) throw std::invalid_argument("bad evaluation batch");
     const size_t band_lds = 2 * static_cast<size_t>(a.band_rows) * a.r.nx * sizeof(unsigned long long);
-    hipLaunchKernelGGL(k_dband, dim3(a.bands), dim3(kBandThreads), band_lds, s, a);
+    hipLaunchKernelGGL(k_dband, dim3(a.bands, count), dim3(kBandThreads), band_lds, s, a);
     check(hipGetLastError(), "launch density band kernel");
 }
 
@@ -807,6 +777,7 @@ void launchGemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int
                 const SpectralArgs& sp, int precision, hipStream_t s) {
     if (M % 32 || N % 32 || K % 16) throw std::invalid_argument("DCT product dimensions are not tile multiples");
     if (precision != 0) throw std::invalid_argument("only the FP32 DCT products are built");
+    if (batch < 1 || batch > kMaxGemmBatch) throw std::invalid_argument("bad DCT product batch");
     const dim3 grid(N / 32, M / 32, batch);
     if (epilogue == kEpiSpectral)
         hipLaunchKernelGGL(k_gemm<kEpiSpectral>, grid, dim3(kThreads), 0, s, g, K, lda, ldb, ldc, sp);
@@ -817,44 +788,28 @@ void launchGemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int
 
 int particleBlocks(int n) { return std::max(1, blocksFor(n)); }
 
-void launchGather(const GatherArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_gather, dim3(particleBlocks(a.n)), dim3(kThreads), 0, s, a);
+void launchGather(const GatherArgs& a, int count, hipStream_t s) {
+    if (count < 1 || count > kMaxEval) throw std::invalid_argument("bad evaluation batch");
+    hipLaunchKernelGGL(k_gather, dim3(particleBlocks(a.n), count), dim3(kThreads), 0, s, a);
     check(hipGetLastError(), "launch field gather kernel");
 }
 
-void launchLookahead(const double2* x, const double2* v, double beta, double2* out, const ClampArgs& c, int n,
+void launchLookahead(const double2* x, const double2* x_prev, double beta, double2* out, const ClampArgs& c, int n,
                      hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_lookahead, dim3(blocksFor(n)), dim3(kThreads), 0, s, x, v, beta, out, c, n);
+    hipLaunchKernelGGL(k_lookahead, dim3(blocksFor(n)), dim3(kThreads), 0, s, x, x_prev, beta, out, c, n);
     check(hipGetLastError(), "launch look-ahead kernel");
 }
 
-void launchDot(const double2* a, const double2* b, int n, double* partials, hipStream_t s) {
-    hipLaunchKernelGGL(k_dot, dim3(particleBlocks(n)), dim3(kThreads), 0, s, a, b, n, partials);
-    check(hipGetLastError(), "launch dot kernel");
-}
-
-void launchCurvature(const double2* b, const double2* pb, const double2* g, const double2* pg, int n,
-                     double* partials, hipStream_t s) {
-    hipLaunchKernelGGL(k_curvature, dim3(particleBlocks(n)), dim3(kThreads), 0, s, b, pb, g, pg, n, partials);
-    check(hipGetLastError(), "launch curvature kernel");
-}
-
 void launchPropose(const double2* b, const double2* g, const double2* raw, double step, double max_disp, double2* y,
-                   const ClampArgs& c, int n, double* partials, hipStream_t s) {
-    hipLaunchKernelGGL(k_propose, dim3(particleBlocks(n)), dim3(kThreads), 0, s, b, g, raw, step, max_disp, y, c, n,
-                       partials);
+                   double2* b_copy, double2* g_copy, const ClampArgs& c, int n, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_propose, dim3(particleBlocks(n)), dim3(kThreads), 0, s, b, g, raw, step, max_disp, y, b_copy,
+                       g_copy, c, n, partials);
     check(hipGetLastError(), "launch propose kernel");
 }
 
-void launchDifference(const double2* y, const double2* x, double2* v, int n, hipStream_t s) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_difference, dim3(blocksFor(n)), dim3(kThreads), 0, s, y, x, v, n);
-    check(hipGetLastError(), "launch difference kernel");
-}
-
 int redWidth(int kind) {
-    static const int w[kRedKinds] = {2, 9, 1, 4, 2, 1, 2, 2};
+    static const int w[kRedKinds] = {2, 9, 1, 7, 2};
     return w[kind];
 }
 

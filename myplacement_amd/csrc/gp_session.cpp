@@ -3,9 +3,10 @@
 // resident on the device.  The scalar control law runs on the host in FP64 exactly as the reference
 // writes it; each decision reads one small scalar block back from the device.
 //
-// Device vectors (double2[P]): X current positions, B look-ahead base, Y proposal, V velocity,
-// PX / PG previous base positions and preconditioned gradient, Rc/Gc and Rl/Gl raw and
-// preconditioned gradients of the current and look-ahead evaluations, two checkpoint copies.
+// Device vectors (double2[P]): X current and XP previous positions (the velocity is X - XP),
+// B look-ahead base, Y proposal, PX / PG previous step base and its preconditioned gradient (PXn /
+// PGn staged by the propose kernel), Rc/Gc and Rl/Gl raw and preconditioned gradients of the
+// current and look-ahead evaluations, two checkpoint copies.  Accepting a step swaps pointers.
 //
 // One difference in sequencing, not in result: a proposal and its value-only evaluation are
 // enqueued together and read back with one synchronisation; when the predicted reduction is
@@ -59,7 +60,8 @@ void validateOptions(const mp_gp_options& o) {  // AdaptiveGlobalPlacer.cpp:413-
         o.smoothing_overflow_exponent <= 0.0 || o.feasible_refinement_iterations <= 0 ||
         !std::isfinite(o.target_density) || !std::isfinite(o.maximum_bin_overflow) ||
         !std::isfinite(o.initial_smoothing) || !std::isfinite(o.final_smoothing) ||
-        !std::isfinite(o.maximum_density_penalty))
+        !std::isfinite(o.maximum_density_penalty) || !(o.density_order_reuse_rows >= 0.0) ||
+        !std::isfinite(o.density_order_reuse_rows))
         throw std::invalid_argument("Adaptive global placement options are outside their valid range.");
     if (static_cast<std::size_t>(o.bins_x) > (std::size_t{1} << 20) / static_cast<std::size_t>(o.bins_y))
         throw std::invalid_argument("Density grid exceeds the safe one-million-bin workspace limit.");
@@ -87,16 +89,27 @@ public:
         }
         for (std::size_t i = 0; i < p.filler_x.size(); ++i) x[Mv + i] = make_double2(p.filler_x[i], p.filler_y[i]);
         DeviceBudget& b = be_->budget();
-        for (DeviceArray<double2>* a : {&X_, &B_, &Y_, &V_, &PX_, &PG_, &Rc_, &Gc_, &Rl_, &Gl_, &CKf_, &CKb_})
+        for (DeviceArray<double2>* a : {&X_, &XP_, &B_, &Y_, &PX_, &PG_, &PXn_, &PGn_, &Rc_, &Gc_, &Rl_, &Gl_, &CKf_, &CKb_})
             a->allocate(P_, b, "controller vector");
         precond_.allocate(P_, b, "preconditioner");
         hipStream_t s = be_->stream();
         X_.upload(x.data(), P_, s, "upload positions");
         precond_.upload(p.precond.data(), P_, s, "upload preconditioner");
-        launchZero(reinterpret_cast<double*>(V_.get()), 2 * static_cast<std::size_t>(P_), s);
         min_bin_ = std::min(p.bw, p.bh);
         max_bin_ = std::max(p.bw, p.bh);
         clamp_ = ClampArgs{be_->sizes(), be_->region()};
+        bin_height_ = be_->region().bh;
+        // The movement bounds behind the reuse of the density row order assume positions inside
+        // their clamp boxes; until the first accepted step (whose positions are clamped) every
+        // evaluation rebuilds the order if the starting positions are not.
+        const Region& r = be_->region();
+        for (int i = 0; i < P_ && !outside_; ++i) {
+            const double w = p.pw[i], h = p.ph[i];
+            const double lx = r.llx + w * 0.5, ux = r.urx - w * 0.5, ly = r.lly + h * 0.5, uy = r.ury - h * 0.5;
+            const double cx = lx <= ux ? std::max(lx, std::min(x[i].x, ux)) : (r.llx + r.urx) * 0.5;
+            const double cy = ly <= uy ? std::max(ly, std::min(x[i].y, uy)) : (r.lly + r.ury) * 0.5;
+            outside_ = cx != x[i].x || cy != x[i].y;
+        }
         const auto t0 = std::chrono::steady_clock::now();
         initialize();
         setup_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -126,6 +139,7 @@ public:
 
     void finish(double* centers, mp_gp_result* r) {
         hipCheck(hipSetDevice(o_.device), "select HIP device");
+        flushCheckpoint();
         const Prepared& p = *prep_;
         const Checkpoint& chosen = feasible_.found ? feasible_ : fallback_;
         std::vector<double2> pos(P_);
@@ -168,14 +182,25 @@ private:
     double strength() const { return std::sqrt(1.0 + std::max(0.0, penalty_ / std::max(reference_penalty_, kEps))); }
 
     // Scalar banks: 0 current-position gradient evaluation, 1 look-ahead gradient evaluation,
-    // 2 proposal + candidate value evaluation, 3 alignment and curvature sums.
-    enum Bank : int { kBankCurrent = 0, kBankLook = 1, kBankCandidate = 2, kBankMisc = 3 };
+    // 2 proposal + candidate value evaluation.
+    enum Bank : int { kBankCurrent = 0, kBankLook = 1, kBankCandidate = 2 };
 
-    void enqueueGrad(const double2* pos, double2* raw, double2* pre, int bank) {
-        Backend::SessionOut so{precond_.get(), penalty_, 1.0 / strength(), raw, pre};
-        be_->enqueueEvaluate(pos, smoothing_, true, &so, bank);
+    // Bin rows of reuse margin covering a movement bound `d` (rounding-safe).
+    int marginRows(double d) const {
+        if (d <= 0.0) return 0;
+        const double rows = std::floor(d / bin_height_) + 1.0;
+        return rows > 1e6 ? 1000000 : static_cast<int>(rows);
     }
-    void enqueueValue(const double2* pos, int bank) { be_->enqueueEvaluate(pos, smoothing_, false, nullptr, bank); }
+    Backend::SessionOut sessionOut(double2* raw, double2* pre, const double2* prev, double2* ck) const {
+        Backend::SessionOut so{precond_.get(), penalty_, 1.0 / strength(), raw, pre};
+        so.prev_pos = prev;
+        if (has_previous_) {
+            so.px = PX_.get();
+            so.pg = PG_.get();
+        }
+        so.ck_out = ck;
+        return so;
+    }
     EvalRec record(const HostScalars& h, bool grad, int bank) {
         EvalRec e;
         be_->toScalars(h, &e.s, bank);
@@ -193,15 +218,21 @@ private:
         }
         return e;
     }
-    EvalRec gradEval(const double2* pos, double2* raw, double2* pre) {
-        enqueueGrad(pos, raw, pre, kBankCurrent);
+    EvalRec gradEval() {
+        const Backend::SessionOut so = sessionOut(Rc_.get(), Gc_.get(), nullptr, nullptr);
+        const Backend::EvalSpec spec{X_.get(), &so, kBankCurrent, Backend::kFreshOrder};
+        be_->enqueueEvaluate(&spec, 1, smoothing_, true);
+        dX_ = 0.0;
         return record(be_->fetchScalars(), true, kBankCurrent);
     }
-    double* scal(int bank) { return be_->scalarBlock(bank); }
-    void copy(DeviceArray<double2>& dst, const double2* src) {
-        if (dst.get() == src) return;
-        hipCheck(hipMemcpyAsync(dst.get(), src, P_ * sizeof(double2), hipMemcpyDeviceToDevice, be_->stream()),
-                 "copy controller vector");
+    // A new best position is copied by the next gradient evaluation at X (its gather writes the
+    // copy); a copy still pending when that evaluation will not come is made here.
+    void flushCheckpoint() {
+        if (!pending_ck_) return;
+        hipCheck(hipMemcpyAsync(pending_ck_->get(), X_.get(), P_ * sizeof(double2), hipMemcpyDeviceToDevice,
+                                be_->stream()),
+                 "copy checkpoint");
+        pending_ck_ = nullptr;
     }
 
     // ---- control law (AdaptiveGlobalPlacer.cpp:456-596) ------------------------------------------
@@ -243,13 +274,13 @@ private:
         if (ovf <= o_.maximum_bin_overflow &&
             (!feasible_.found || score < feasible_.score - 1e-10 ||
              (std::abs(score - feasible_.score) <= 1e-10 && e.s.hpwl < feasible_.hpwl))) {
-            copy(CKf_, X_.get());
+            pending_ck_ = &CKf_;
             feasible_ = {iteration, e.s.hpwl, ovf, score, true};
             is_new = true;
         }
         if (!feasible_.found && (!fallback_.found || ovf < fallback_.overflow - 1e-12 ||
                                  (std::abs(ovf - fallback_.overflow) <= 1e-12 && e.s.hpwl < fallback_.hpwl))) {
-            copy(CKb_, X_.get());
+            pending_ck_ = &CKb_;
             fallback_ = {iteration, e.s.hpwl, ovf, score, true};
             is_new = true;
         }
@@ -282,13 +313,13 @@ private:
         smoothing_ = o_.initial_smoothing * max_bin_;
         penalty_ = 1.0;
         reference_penalty_ = 1.0;
-        EvalRec init = gradEval(X_.get(), Rc_.get(), Gc_.get());
+        EvalRec init = gradEval();
         initial_overflow_ = std::max(init.s.optimizer_density.normalized_overflow, kEps);
         smoothing_ = smoothingFromOverflow(init.s.optimizer_density.normalized_overflow);
         const double raw_penalty = init.density_l1 > kEps ? init.wire_l1 / init.density_l1 : 1.0;
         penalty_ = clampd(raw_penalty, o_.minimum_density_penalty, o_.maximum_density_penalty);
         reference_penalty_ = penalty_;
-        init = gradEval(X_.get(), Rc_.get(), Gc_.get());
+        init = gradEval();
         last_overflow_ = init.s.optimizer_density.normalized_overflow;
         const double rms = init.pre_norm / std::sqrt(static_cast<double>(std::max(1, P_)));
         last_step_ = o_.initial_movement_in_bins * min_bin_ / std::max(rms, kEps);
@@ -299,46 +330,60 @@ private:
     }
 
     // One pass of the loop body :735-928.  Everything the first decisions need is enqueued before
-    // one read-back: the gradient evaluation at X, and -- when momentum may be used -- the alignment
-    // sum, the look-ahead positions and their gradient evaluation, plus the curvature sums for both
-    // possible step bases.  Work whose result the control law then does not use is discarded and
-    // not counted as an evaluation.
+    // one read-back: the gradient evaluation at X (slot 0) with the momentum alignment and the
+    // curvature sums fused into its gather, and -- when momentum may be used -- the look-ahead
+    // positions and their gradient evaluation (slot 1, concurrently).  Work whose result the
+    // control law then does not use is discarded and not counted as an evaluation.
+    //
+    // The density bands reuse one particle row order while the evaluated positions stay within a
+    // bounded distance of where it was built: |X - S| <= dX_ (accumulated accepted displacements),
+    // |B - X| <= beta * mX_ (mX_ bounds |X - X_prev|), |Y - base| <= max_disp (clamping to the
+    // boxes is non-expansive once the positions are inside them).
     void iterate() {
         hipStream_t s = be_->stream();
         const int it = result_.completed_iterations;
         const bool try_momentum = worse_streak_ < 2 && momentum_age_ > 0;
         const double beta = std::min(o_.maximum_momentum, static_cast<double>(momentum_age_) / (momentum_age_ + 3));
-        bool need_read = false;
-        if (it > 0) {
-            smoothing_ = smoothingFromOverflow(last_overflow_);
-            enqueueGrad(X_.get(), Rc_.get(), Gc_.get(), kBankCurrent);
-            need_read = true;
+        const double look_drift = try_momentum ? beta * mX_ : 0.0;
+        if (it == 0) flushCheckpoint();
+        if (it > 0) smoothing_ = smoothingFromOverflow(last_overflow_);
+        const double cap = o_.maximum_movement_in_bins * min_bin_;
+        // rebuild the row order when the positions about to be evaluated may have moved more than
+        // the reuse limit from where it was built
+        // (a reuse limit of 0 rebuilds the order for every evaluation and runs them one after another)
+        const bool fresh = o_.density_order_reuse_rows <= 0.0;
+        if (!fresh && it > 0 && (outside_ || dX_ + look_drift + cap > o_.density_order_reuse_rows * bin_height_)) {
+            be_->timedBegin(kTController);
+            be_->sortParticles(X_.get());
+            be_->timedEnd(kTController);
+            dX_ = 0.0;
         }
+        // try_momentum implies it > 0: momentum_age_ only grows on an accepted step
+        const Backend::SessionOut so_cur =
+            sessionOut(Rc_.get(), Gc_.get(), try_momentum ? XP_.get() : nullptr, pending_ck_ ? pending_ck_->get() : nullptr);
+        const Backend::SessionOut so_look = sessionOut(Rl_.get(), Gl_.get(), nullptr, nullptr);
+        Backend::EvalSpec specs[2] = {
+            {X_.get(), &so_cur, kBankCurrent, fresh ? Backend::kFreshOrder : marginRows(dX_)},
+            {B_.get(), &so_look, kBankLook, fresh ? Backend::kFreshOrder : marginRows(dX_ + look_drift)}};
         if (try_momentum) {
             be_->timedBegin(kTController);
-            launchDot(V_.get(), Gc_.get(), P_, be_->partials(kRedDot, kBankMisc), s);
-            be_->addReduction(kRedDot, kBankMisc, particleBlocks(P_), kAlignment, -1.0);
-            launchLookahead(X_.get(), V_.get(), beta, B_.get(), clamp_, P_, s);
+            launchLookahead(X_.get(), XP_.get(), beta, B_.get(), clamp_, P_, s);
             be_->timedEnd(kTController);
-            enqueueGrad(B_.get(), Rl_.get(), Gl_.get(), kBankLook);
-            need_read = true;
         }
-        if (has_previous_) {
-            be_->timedBegin(kTController);
-            launchCurvature(X_.get(), PX_.get(), Gc_.get(), PG_.get(), P_, be_->partials(kRedCurv, kBankMisc), s);
-            be_->addReduction(kRedCurv, kBankMisc, particleBlocks(P_), kCurvDp);
-            if (try_momentum) {
-                launchCurvature(B_.get(), PX_.get(), Gl_.get(), PG_.get(), P_, be_->partials(kRedCurv2, kBankMisc), s);
-                be_->addReduction(kRedCurv2, kBankMisc, particleBlocks(P_), kCurvDp2);
+        if (it > 0) {
+            pending_ck_ = nullptr;
+            if (try_momentum && !fresh) {
+                be_->enqueueEvaluate(specs, 2, smoothing_, true);
+            } else {
+                be_->enqueueEvaluate(&specs[0], 1, smoothing_, true);
+                if (try_momentum) be_->enqueueEvaluate(&specs[1], 1, smoothing_, true);
             }
-            be_->timedEnd(kTController);
-            need_read = true;
         }
+        const bool need_read = it > 0;
         HostScalars first{};
         if (need_read) first = be_->fetchScalars();
         if (it > 0) current_ = record(first, true, kBankCurrent);
         const EvalRec& cur = current_;
-        const double* misc = first.bank(kBankMisc);
 
         bool restarted = worse_streak_ >= 2;
         bool momentum_used = false;
@@ -346,9 +391,11 @@ private:
         const double2* base_raw = Rc_.get();
         const double2* base_pre = Gc_.get();
         const EvalRec* base_eval = &cur;
+        int base_bank = kBankCurrent;
+        double base_drift = dX_;
         EvalRec look;
         if (try_momentum) {
-            if (misc[kAlignment] <= 0.0) {
+            if (first.bank(kBankCurrent)[kAlignment] <= 0.0) {
                 restarted = true;
             } else {
                 look = record(first, true, kBankLook);
@@ -359,6 +406,8 @@ private:
                     base_raw = Rl_.get();
                     base_pre = Gl_.get();
                     base_eval = &look;
+                    base_bank = kBankLook;
+                    base_drift = dX_ + look_drift;
                     momentum_used = true;
                 }
             }
@@ -368,21 +417,22 @@ private:
             base_raw = Rc_.get();
             base_pre = Gc_.get();
             base_eval = &cur;
+            base_bank = kBankCurrent;
+            base_drift = dX_;
             momentum_age_ = 0;
             worse_streak_ = 0;
             ++result_.momentum_restarts;
         }
 
-        bool accepted = false, new_best = false;
+        bool accepted = false, new_best = false, reordered = false;
         int backtracks = 0;
         double used_step = 0.0, used_curvature = 0.0, used_disp = 0.0;
         EvalRec acc;
         for (int pass = 0; pass < 2 && !accepted; ++pass) {
-            // estimateStep :470-492 from the sums taken for this base
+            // estimateStep :470-492 from the sums the base's gather took
             double step = std::max(last_step_, kEps), curvature = 0.0;
             if (has_previous_) {
-                const bool on_look = base == B_.get();
-                const double dp = misc[on_look ? kCurvDp2 : kCurvDp], dg = misc[on_look ? kCurvDg2 : kCurvDg];
+                const double dp = first.bank(base_bank)[kCurvDp], dg = first.bank(base_bank)[kCurvDg];
                 if (!(dp <= kEps || dg <= kEps)) {
                     curvature = std::sqrt(dg / dp);
                     step = clampd(1.0 / std::max(curvature, kEps), step * 0.25, step * 4.0);
@@ -392,11 +442,15 @@ private:
             const double max_disp = o_.maximum_movement_in_bins * min_bin_ * (0.35 + 0.65 * ratio);
             for (int attempt = 0; attempt <= o_.maximum_backtracks; ++attempt) {
                 be_->timedBegin(kTController);
-                launchPropose(base, base_pre, base_raw, step, max_disp, Y_.get(), clamp_, P_,
+                launchPropose(base, base_pre, base_raw, step, max_disp, Y_.get(), PXn_.get(), PGn_.get(), clamp_, P_,
                               be_->partials(kRedPropose, kBankCandidate), s);
                 be_->addReduction(kRedPropose, kBankCandidate, particleBlocks(P_));
                 be_->timedEnd(kTController);
-                enqueueValue(Y_.get(), kBankCandidate);
+                const Backend::SessionOut so_cand = sessionOut(nullptr, nullptr, nullptr, nullptr);
+                const Backend::EvalSpec cand_spec{Y_.get(), &so_cand, kBankCandidate,
+                                                  fresh || outside_ ? Backend::kFreshOrder : marginRows(base_drift + max_disp)};
+                be_->enqueueEvaluate(&cand_spec, 1, smoothing_, false);
+                reordered = reordered || outside_;
                 const HostScalars& h = be_->fetchScalars();
                 const double predicted = h.bank(kBankCandidate)[kPredicted];
                 const double disp = h.bank(kBankCandidate)[kMaxDisp];
@@ -423,14 +477,13 @@ private:
             base_raw = Rc_.get();
             base_pre = Gc_.get();
             base_eval = &cur;
+            base_bank = kBankCurrent;
+            base_drift = dX_;
             momentum_used = false;
         }
 
         const double penalty_used = penalty_;
         if (accepted) {
-            be_->timedBegin(kTController);
-            launchDifference(Y_.get(), X_.get(), V_.get(), P_, s);
-            be_->timedEnd(kTController);
             last_step_ = used_step;
             momentum_age_ = restarted ? 0 : momentum_age_ + 1;
             const double growth = (acc.s.hpwl - cur.s.hpwl) / std::max(cur.s.hpwl, 1.0);
@@ -440,13 +493,18 @@ private:
             penalty_ = next;
             if (changed) {
                 has_previous_ = false;
-            } else {
-                copy(PX_, base);
-                copy(PG_, base_pre);
+            } else {  // the propose kernel staged the accepted step's base and gradient
+                PX_.swapWith(PXn_);
+                PG_.swapWith(PGn_);
                 has_previous_ = true;
             }
             const double gnorm = base_eval->pre_norm;
+            // X_prev <- X, X <- Y (velocity X - X_prev, :861-863)
             X_.swapWith(Y_);
+            XP_.swapWith(Y_);
+            mX_ = used_disp + (momentum_used ? look_drift : 0.0);
+            dX_ = outside_ ? 0.0 : base_drift + used_disp;
+            outside_ = false;
             last_overflow_ = acc.s.optimizer_density.normalized_overflow;
             considerCheckpoint(acc, it + 1, new_best);
             ++result_.accepted_iterations;
@@ -455,7 +513,7 @@ private:
             history_.push_back(row(it + 1, acc, penalty_used, used_step, used_disp, gnorm, used_curvature, backtracks,
                                    restarted, true, new_best));
         } else {
-            launchZero(reinterpret_cast<double*>(V_.get()), 2 * static_cast<std::size_t>(P_), s);
+            if (reordered) dX_ = std::numeric_limits<double>::infinity();  // the order was rebuilt at a rejected Y
             momentum_age_ = 0;
             worse_streak_ = 0;
             if (cur.s.optimizer_density.normalized_overflow > o_.maximum_bin_overflow)
@@ -477,10 +535,13 @@ private:
     std::unique_ptr<Backend> be_;
     int P_ = 0;
     ClampArgs clamp_{};
-    DeviceArray<double2> X_, B_, Y_, V_, PX_, PG_, Rc_, Gc_, Rl_, Gl_, CKf_, CKb_;
+    DeviceArray<double2> X_, XP_, B_, Y_, PX_, PG_, PXn_, PGn_, Rc_, Gc_, Rl_, Gl_, CKf_, CKb_;
     DeviceArray<double> precond_;
-    double min_bin_ = 0, max_bin_ = 0;
+    DeviceArray<double2>* pending_ck_ = nullptr;
+    double min_bin_ = 0, max_bin_ = 0, bin_height_ = 1;
     double smoothing_ = 0, penalty_ = 1, reference_penalty_ = 1, initial_overflow_ = 0, last_overflow_ = 0, last_step_ = 0;
+    double dX_ = 0.0, mX_ = 0.0;  // movement bounds: X from the row order's positions, X from X_prev
+    bool outside_ = false;        // starting positions outside their clamp boxes
     int momentum_age_ = 0, worse_streak_ = 0, feasible_refinement_ = 0;
     bool has_previous_ = false, stopped_ = false;
     EvalRec current_;
@@ -548,6 +609,7 @@ void mp_gp_default_options(mp_gp_options* o) {
     o->device = 0;
     o->dct_precision = MP_DCT_FP32;
     o->maximum_device_bytes = 40ULL << 30;
+    o->density_order_reuse_rows = 2.5;
 }
 
 mp_gp_status mp_gp_prepare(const mp_design* design, const mp_gp_options* options, mp_gp_prepared** out) {

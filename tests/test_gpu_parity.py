@@ -186,6 +186,36 @@ def test_adaptive_run_follows_oracle_then_matches_ensemble(ob):
     assert abs(hh[:, 1].mean() - ho[:, 1].mean()) < 5e-3
 
 
+@pytest.mark.parametrize("start", ["inside", "outside"])
+def test_reused_row_order_is_bit_identical_to_rebuilding_it(start):
+    """The density bands may reuse a particle row order built at earlier positions (within a
+    tracked movement margin) and the current and look-ahead evaluations run concurrently.  Neither
+    may change a single bit: the fixed-point density sum does not depend on the order, and every
+    other reduction has a fixed order.  So whole runs with the order rebuilt for every evaluation
+    (reuse limit 0, evaluations one after another), the default limit, and a limit so large that
+    the order is never rebuilt must agree exactly.  The "outside" start places some cells beyond
+    the core, where the movement bounds do not hold until the first accepted step."""
+    d, o = _setup(design.plain_design(), 0.9, 64)
+    o.iterations = 120
+    center = d.center.copy()
+    if start == "outside":
+        core = d.core_region()
+        mov = d.movable_ids()
+        center[mov[:40], 0] = core[2] + 5.0
+        center[mov[40:80], 1] = core[1] - 3.0
+    runs = []
+    for rows in (0.0, 2.5, 1e6):
+        o.density_order_reuse_rows = rows
+        say(f"[gpu] row-order reuse {start} start, limit {rows}")
+        c, r, hist = hb.global_place(d, o, center=center)
+        runs.append((c, r, np.array([[row[k] for k in abi.HISTORY_COLUMNS] for row in hist])))
+    for c, r, h in runs[1:]:
+        assert r["completed_iterations"] == runs[0][1]["completed_iterations"]
+        assert np.array_equal(h, runs[0][2])
+        assert np.array_equal(c, runs[0][0])
+    assert runs[0][1]["accepted_iterations"] > 10
+
+
 def test_session_steps_and_history():
     d, o = _setup(design.plain_design(), 0.9, 64)
     o.iterations = 30
