@@ -33,6 +33,12 @@ namespace {
 constexpr double kPiD = 3.141592653589793238462643383279502884;
 constexpr std::size_t kMaximumBinCount = 1U << 20U;  // core/ResourceLimits.hpp:10
 constexpr std::size_t kDeviceReserveBytes = 4ULL << 30;
+#ifndef MP_GP_TARGET_BANDS
+#define MP_GP_TARGET_BANDS 128
+#endif
+// density bands per evaluation the band height aims at (fewer, taller bands read each particle
+// fewer times; more bands fill more CUs)
+constexpr int kTargetBands = MP_GP_TARGET_BANDS;
 
 int roundUp32(int n) { return (n + 31) / 32 * 32; }
 
@@ -183,7 +189,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     if (nx_ > 4096) throw InvalidArgument("density grids wider than 4096 bins are not supported by the band kernel");
     if (ny_ > 8192) throw InvalidArgument("density grids taller than 8192 bins are not supported by the row sort");
     if (static_cast<unsigned>(P_) > kPermIdMask) throw InvalidArgument("HIP backend cannot order more than 2^30 - 1 particles");
-    band_rows_ = std::max(1, std::min(ny_ / 256, 4096 / nx_));
+    band_rows_ = std::max(1, std::min(ny_ / kTargetBands, 4096 / nx_));
     bands_ = (ny_ + band_rows_ - 1) / band_rows_;
     fix_ = std::ldexp(1.0, 40) / (region_.bw * region_.bh);
     std::vector<unsigned char> tall(P_);
@@ -211,8 +217,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
         sl.tmp.allocate(Gp, budget_, "DCT intermediate");
         sl.phi.allocate(Gp, budget_, "potential coefficients");
         sl.t12.allocate(2 * Gp, budget_, "inverse DCT intermediate");
-        sl.ex.allocate(Gp, budget_, "horizontal field");
-        sl.ey.allocate(Gp, budget_, "vertical field");
+        sl.field.allocate(2 * Gp, budget_, "field grid");
     }
     pot_.allocate(Gp, budget_, "potential grid");
     bxf_.allocate(static_cast<std::size_t>(nxp_) * nxp_, budget_, "forward x factor");
@@ -459,13 +464,13 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
         i1.C[e] = sl.t12.get();
         i2.A[2 * e] = ayi_.get();
         i2.B[2 * e] = sl.t12.get() + nxp_;
-        i2.C[2 * e] = sl.ex.get();
+        i2.C[2 * e] = sl.field.get();
         i2.A[2 * e + 1] = syi_.get();
         i2.B[2 * e + 1] = sl.t12.get();
-        i2.C[2 * e + 1] = sl.ey.get();
+        i2.C[2 * e + 1] = sl.field.get() + 1;
     }
     gemm(i1, count, nyp_, 2 * nxp_, nxp_, nxp_, 2 * nxp_, 2 * nxp_, kEpiStore);
-    gemm(i2, 2 * count, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiStore);
+    gemm(i2, 2 * count, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiInterleave);
     have_fields_ = true;
 
     GatherArgs ga{};
@@ -478,8 +483,7 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
     for (int e = 0; e < count; ++e) {
         const Slot& sl = slots_[e];
         ga.pos[e] = specs[e].pos;
-        ga.ex[e] = sl.ex.get();
-        ga.ey[e] = sl.ey.get();
+        ga.field[e] = reinterpret_cast<const float2*>(sl.field.get());
         ga.pin_grad[e] = sl.pin_grad.get();
         ga.partials[e] = partials(kRedGather, specs[e].bank);
         const SessionOut* so = specs[e].session;
@@ -571,25 +575,26 @@ void Backend::debugGrids(double* source, double* potential, double* field_x, dou
         GemmBatch g{{ayi_.get()}, {sl.t12.get()}, {pot_.get()}};
         launchGemm(g, 1, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiStore, sp, dct_precision_, stream_);
     }
-    std::vector<float> buf(Gp);
-    auto pull = [&](const DeviceArray<float>& a, double* dst) {
+    std::vector<float> buf(2 * Gp);
+    auto pull = [&](const DeviceArray<float>& a, double* dst, int stride, int offset) {
         if (!dst) return;
-        hipCheck(hipMemcpyAsync(buf.data(), a.get(), Gp * sizeof(float), hipMemcpyDeviceToHost, stream_), "read grid");
+        hipCheck(hipMemcpyAsync(buf.data(), a.get(), stride * Gp * sizeof(float), hipMemcpyDeviceToHost, stream_),
+                 "read grid");
         hipCheck(hipStreamSynchronize(stream_), "read grid");
         for (int r = 0; r < ny_; ++r)
             for (int c = 0; c < nx_; ++c)
-                dst[static_cast<std::size_t>(r) * nx_ + c] = buf[static_cast<std::size_t>(r) * nxp_ + c];
+                dst[static_cast<std::size_t>(r) * nx_ + c] = buf[stride * (static_cast<std::size_t>(r) * nxp_ + c) + offset];
     };
-    pull(sl.rho, source);
+    pull(sl.rho, source, 1, 0);
     if (source) {  // centre it as ElectrostaticField::solve does
         double mean = 0.0;
         for (std::size_t i = 0; i < static_cast<std::size_t>(nx_) * ny_; ++i) mean += source[i];
         mean /= static_cast<double>(nx_) * ny_;
         for (std::size_t i = 0; i < static_cast<std::size_t>(nx_) * ny_; ++i) source[i] -= mean;
     }
-    pull(pot_, potential);
-    pull(sl.ex, field_x);
-    pull(sl.ey, field_y);
+    pull(pot_, potential, 1, 0);
+    pull(sl.field, field_x, 2, 0);
+    pull(sl.field, field_y, 2, 1);
 }
 
 // ---------------------------------------------------------------------------------------------

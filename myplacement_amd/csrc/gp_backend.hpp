@@ -144,7 +144,8 @@ private:
     // work buffers of one evaluation of a batch
     struct Slot {
         DeviceArray<float2> pin_grad;
-        DeviceArray<float> rho, tmp, phi, t12, ex, ey;
+        DeviceArray<float> rho, tmp, phi, t12;
+        DeviceArray<float> field;  // interleaved (Ex, Ey)
     };
     static constexpr int kSlots = kMaxEval;
     void setDevice() const;

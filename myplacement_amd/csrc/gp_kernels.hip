@@ -534,7 +534,8 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int K, int lda, 
                 }
             }
         }
-        C[static_cast<size_t>(m0 + row) * ldc + n0 + col] = out;
+        if (EPI == kEpiInterleave) C[2 * (static_cast<size_t>(m0 + row) * ldc + n0 + col)] = out;
+        else C[static_cast<size_t>(m0 + row) * ldc + n0 + col] = out;
     }
     if (EPI == kEpiSpectral) {
         double v[1] = {energy};
@@ -552,8 +553,9 @@ __device__ inline int reflectIndex(int index, int size, bool odd, bool& negate) 
     return index;
 }
 
-__device__ inline double sampleParity(const float* __restrict__ v, int ld, double x, double y, const Region& r,
-                                      bool odd_x, bool odd_y) {
+// Bilinear sample of the interleaved (Ex, Ey) grid at (x, y).  Taps outside the grid reflect into
+// it; a reflection across a vertical edge flips Ex, across a horizontal edge flips Ey.
+__device__ inline double2 sampleField(const float2* __restrict__ f, int ld, double x, double y, const Region& r) {
     x = fmax(r.llx, fmin(x, r.urx));
     y = fmax(r.lly, fmin(y, r.ury));
     const double gx = (x - r.llx) / r.bw - 0.5;
@@ -562,18 +564,19 @@ __device__ inline double sampleParity(const float* __restrict__ v, int ld, doubl
     const int bottom = static_cast<int>(floor(gy));
     const double tx = gx - static_cast<double>(left);
     const double ty = gy - static_cast<double>(bottom);
-    double val[4];
+    double vx[4], vy[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        bool neg = false;
-        const int cc = reflectIndex(left + (q & 1), r.nx, odd_x, neg);
-        const int rr = reflectIndex(bottom + (q >> 1), r.ny, odd_y, neg);
-        const double f = static_cast<double>(v[static_cast<size_t>(rr) * ld + cc]);
-        val[q] = neg ? -f : f;
+        bool flip_x = false, flip_y = false;
+        const int cc = reflectIndex(left + (q & 1), r.nx, true, flip_x);
+        const int rr = reflectIndex(bottom + (q >> 1), r.ny, true, flip_y);
+        const float2 v = f[static_cast<size_t>(rr) * ld + cc];
+        vx[q] = flip_x ? -static_cast<double>(v.x) : static_cast<double>(v.x);
+        vy[q] = flip_y ? -static_cast<double>(v.y) : static_cast<double>(v.y);
     }
-    const double lower = val[0] * (1.0 - tx) + val[1] * tx;
-    const double upper = val[2] * (1.0 - tx) + val[3] * tx;
-    return lower * (1.0 - ty) + upper * ty;
+    const double lx = vx[0] * (1.0 - tx) + vx[1] * tx, ux = vx[2] * (1.0 - tx) + vx[3] * tx;
+    const double ly = vy[0] * (1.0 - tx) + vy[1] * tx, uy = vy[2] * (1.0 - tx) + vy[3] * tx;
+    return make_double2(lx * (1.0 - ty) + ux * ty, ly * (1.0 - ty) + uy * ty);
 }
 
 __global__ __launch_bounds__(kThreads) void k_gather(GatherArgs a) {
@@ -585,9 +588,8 @@ __global__ __launch_bounds__(kThreads) void k_gather(GatherArgs a) {
     if (p < a.n) {
         const double2 c = a.pos[e][p];
         const double q = a.charge[p];
-        const double ex = sampleParity(a.ex[e], a.ldf, c.x, c.y, a.r, true, false);
-        const double ey = sampleParity(a.ey[e], a.ldf, c.x, c.y, a.r, false, true);
-        const double2 dg = make_double2(-q * ex, -q * ey);
+        const double2 f = sampleField(a.field[e], a.ldf, c.x, c.y, a.r);
+        const double2 dg = make_double2(-q * f.x, -q * f.y);
         double2 wg = make_double2(0.0, 0.0);
         for (int j = a.ppin_off[p]; j < a.ppin_off[p + 1]; ++j) {
             const float2 g = a.pin_grad[e][a.ppin_idx[j]];
@@ -781,6 +783,8 @@ void launchGemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int
     const dim3 grid(N / 32, M / 32, batch);
     if (epilogue == kEpiSpectral)
         hipLaunchKernelGGL(k_gemm<kEpiSpectral>, grid, dim3(kThreads), 0, s, g, K, lda, ldb, ldc, sp);
+    else if (epilogue == kEpiInterleave)
+        hipLaunchKernelGGL(k_gemm<kEpiInterleave>, grid, dim3(kThreads), 0, s, g, K, lda, ldb, ldc, sp);
     else
         hipLaunchKernelGGL(k_gemm<kEpiStore>, grid, dim3(kThreads), 0, s, g, K, lda, ldb, ldc, sp);
     check(hipGetLastError(), "launch DCT product kernel");

@@ -114,7 +114,8 @@ void launchDensitySort(const DensityArgs& a, hipStream_t s);
 void launchDensityBands(const DensityArgs& a, int count, hipStream_t s);
 
 // ---- DCT products on MFMA (FP32) ---------------------------------------------------------------
-enum Epilogue : int { kEpiStore = 0, kEpiSpectral = 1 };
+// kEpiInterleave stores C[i][j] at C + 2 * (i * ldc + j) (two products into one interleaved grid)
+enum Epilogue : int { kEpiStore = 0, kEpiSpectral = 1, kEpiInterleave = 2 };
 constexpr int kMaxGemmBatch = 2 * kMaxEval;
 struct GemmBatch {
     const float* A[kMaxGemmBatch];
@@ -135,8 +136,7 @@ void launchGemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int
 struct GatherArgs {
     const double2* pos[kMaxEval];
     const double* charge;
-    const float* ex[kMaxEval];
-    const float* ey[kMaxEval];
+    const float2* field[kMaxEval];  // interleaved (Ex, Ey), row stride ldf
     int ldf;
     Region r;
     const int* ppin_off;   // [P+1] particle -> pin CSR
