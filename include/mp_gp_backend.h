@@ -260,9 +260,11 @@ int64_t mp_gp_session_history(const mp_gp_session* session, mp_gp_history_row* r
 /* Current particle positions [2P] (host copy). */
 mp_gp_status mp_gp_session_positions(mp_gp_session* session, double* xy);
 
-/* Per-kernel timing of the session's stream with HIP events (enable, then read).  Returns the class
- * count; names: comma-separated kernel classes; per class the summed event time, the launch count
- * and the summed algorithmic work (bytes, or flops for the DCT products) of the timed launches. */
+/* Per-kernel timing of the session's stream with HIP events (enable, then read).  enable: 0 off,
+ * otherwise a bit mask of the kernel classes to time (bit i: the i-th name; -1: all).  The read
+ * returns the class count; names: comma-separated kernel classes; per class the summed event time,
+ * the launch count and the summed algorithmic work (bytes, or flops for the DCT products) of the
+ * timed launches. */
 mp_gp_status mp_gp_session_enable_kernel_timing(mp_gp_session* session, int enable);
 int32_t mp_gp_session_kernel_timing(mp_gp_session* session, char* names, size_t names_len, double* total_us,
                                     int64_t* launches, double* work, int32_t capacity);

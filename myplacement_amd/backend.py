@@ -324,8 +324,14 @@ class PlacementSession:
         _raise(library().mp_gp_session_positions(self._h, _p(xy)))
         return xy
 
-    def enable_kernel_timing(self, on: bool = True) -> None:
-        _raise(library().mp_gp_session_enable_kernel_timing(self._h, int(on)))
+    TIMING_CLASSES = ("k_wire", "k_dband", "k_density_sort", "k_gemm", "k_gather", "controller")
+
+    def enable_kernel_timing(self, on: bool = True, classes=None) -> None:
+        """Times the named kernel classes (all when `classes` is None) with HIP events."""
+        mask = 0
+        if on:
+            mask = -1 if classes is None else sum(1 << self.TIMING_CLASSES.index(c) for c in classes)
+        _raise(library().mp_gp_session_enable_kernel_timing(self._h, mask))
 
     def kernel_timing(self) -> dict:
         names = C.create_string_buffer(256)

@@ -143,6 +143,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
         inet_off[k + 1] = static_cast<int>(ipin_part.size());
     }
     const std::size_t Ii = ipin_part.size();
+
     // particle -> pin CSR over the movable pins, pins in internal order
     std::vector<int> count(static_cast<std::size_t>(P_) + 1, 0);
     for (std::size_t q = 0; q < Ii; ++q)
@@ -281,10 +282,7 @@ Backend::~Backend() {
     if (stream_) {
         (void)hipSetDevice(device_);
         (void)hipStreamSynchronize(stream_);
-        for (int c = 0; c < kTimingClasses; ++c) {
-            for (auto e : ev_begin_[c]) (void)hipEventDestroy(e);
-            for (auto e : ev_end_[c]) (void)hipEventDestroy(e);
-        }
+        for (auto e : ev_pool_) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(stream_);
     }
     if (host_scal_) (void)hipHostFree(host_scal_);
@@ -292,25 +290,32 @@ Backend::~Backend() {
 
 void Backend::setDevice() const { hipCheck(hipSetDevice(device_), "select HIP device"); }
 
+hipEvent_t Backend::poolEvent() {
+    if (ev_next_ == ev_pool_.size()) {
+        hipEvent_t e;
+        hipCheck(hipEventCreate(&e), "create timing event");
+        ev_pool_.push_back(e);
+    }
+    return ev_pool_[ev_next_++];
+}
+
 void Backend::timedBegin(int cls) {
-    if (!timing_on_) return;
-    hipEvent_t e;
-    hipCheck(hipEventCreate(&e), "create timing event");
+    if (!((timing_mask_ >> cls) & 1U)) return;
+    const hipEvent_t e = poolEvent();
     hipCheck(hipEventRecord(e, stream_), "record timing event");
     ev_begin_[cls].push_back(e);
 }
 
 void Backend::timedEnd(int cls, double work) {
-    if (!timing_on_) return;
+    if (!((timing_mask_ >> cls) & 1U)) return;
     pending_work_[cls].push_back(work);
-    hipEvent_t e;
-    hipCheck(hipEventCreate(&e), "create timing event");
+    const hipEvent_t e = poolEvent();
     hipCheck(hipEventRecord(e, stream_), "record timing event");
     ev_end_[cls].push_back(e);
 }
 
-void Backend::enableTiming(bool on) {
-    timing_on_ = on;
+void Backend::enableTiming(unsigned class_mask) {
+    timing_mask_ = class_mask;
     if (t_total_.empty()) {
         t_total_.assign(kTimingClasses, 0.0);
         t_work_.assign(kTimingClasses, 0.0);
@@ -319,7 +324,7 @@ void Backend::enableTiming(bool on) {
 }
 
 std::vector<std::string> Backend::timingNames() const {
-    return {"k_wire", "k_density", "k_gemm", "k_gather", "controller"};
+    return {"k_wire", "k_dband", "k_density_sort", "k_gemm", "k_gather", "controller"};
 }
 
 void Backend::timing(std::vector<double>& total_us, std::vector<long long>& launches, std::vector<double>& work) {
@@ -338,11 +343,10 @@ void Backend::timing(std::vector<double>& total_us, std::vector<long long>& laun
             if (i < pending_work_[c].size()) t_work_[c] += pending_work_[c][i];
         }
         pending_work_[c].clear();
-        for (auto e : ev_begin_[c]) (void)hipEventDestroy(e);
-        for (auto e : ev_end_[c]) (void)hipEventDestroy(e);
         ev_begin_[c].clear();
         ev_end_[c].clear();
     }
+    ev_next_ = 0;  // every pooled event has completed
     total_us = t_total_;
     launches = t_count_;
     work = t_work_;
@@ -378,10 +382,10 @@ DensityArgs Backend::densityArgs() const {
 void Backend::sortParticles(const double2* pos) {
     DensityArgs da = densityArgs();
     da.pos[0] = pos;
-    timedBegin(kTDensity);
+    timedBegin(kTSort);
     launchDensitySort(da, stream_);
     // algorithmic bytes: positions and sizes read twice, keys written and read, order and sizes written
-    timedEnd(kTDensity, 64.0 * P_);
+    timedEnd(kTSort, 64.0 * P_);
 }
 
 void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bool want_grad) {
@@ -415,7 +419,9 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
     } else {
         for (int e = 0; e < count; ++e) launchZero(dev_scal_ + specs[e].bank * kScalarCount + kHpwl, 2, s);
     }
-    // algorithmic bytes: 12 per pin, 8 per net, 16 (8 value-only) per particle, per evaluation
+    // algorithmic bytes per evaluation, by the convention of one pass over each array at the path's
+    // nominal widths (4-byte indices and scalars, 8-byte positions): 12 per pin, 8 per net, 16 per
+    // particle with the gradient and 8 without
     timedEnd(kTWire, count * (12.0 * I_ + 8.0 * N_ + (want_grad ? 16.0 : 8.0) * P_));
 
     DensityArgs da = densityArgs();
@@ -425,10 +431,11 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
         da.rho[e] = slots_[e].rho.get();
         da.partials[e] = partials(kRedMetrics, specs[e].bank);
     }
-    timedBegin(kTDensity);
+    timedBegin(kTBand);
     launchDensityBands(da, count, s);
-    // algorithmic bytes: 32 per particle (position, size) and 36 per bin (static charge, deviation, metrics)
-    timedEnd(kTDensity, count * (32.0 * P_ + 36.0 * static_cast<double>(nx_) * ny_));
+    // algorithmic bytes per evaluation (same convention): scatter 20 per particle and 12 per bin,
+    // metrics and deviation 24 per bin
+    timedEnd(kTBand, count * (20.0 * P_ + 36.0 * static_cast<double>(nx_) * ny_));
     for (int e = 0; e < count; ++e)
         addReduction(kRedMetrics, specs[e].bank, bands_, static_cast<double>(nx_) * ny_, dark_total_, placeable_total_);
 
@@ -505,9 +512,8 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
     timedBegin(kTGather);
     launchGather(ga, count, s);
     for (int e = 0; e < count; ++e) addReduction(kRedGather, specs[e].bank, particleBlocks(P_));
-    // algorithmic bytes: position, charge, two gradient outputs (40 + 32 per particle in session
-    // mode with the preconditioner), the CSR walk (8 per pin) and the two field grids
-    timedEnd(kTGather, count * (48.0 * P_ + 8.0 * I_ + 8.0 * static_cast<double>(nx_) * ny_));
+    // algorithmic bytes per evaluation (same convention): 20 per particle and 8 per bin
+    timedEnd(kTGather, count * (20.0 * P_ + 8.0 * static_cast<double>(nx_) * ny_));
 }
 
 void Backend::addReduction(int kind, int bank, int blocks, double p0, double p1, double p2) {

@@ -133,8 +133,9 @@ public:
     // Queues the final reduction of `blocks` partial rows; done by the next fetchScalars().
     void addReduction(int kind, int bank, int blocks, double p0 = 0, double p1 = 0, double p2 = 0);
 
-    // Optional per-kernel-class timing with HIP events on the backend's stream.
-    void enableTiming(bool on);
+    // Optional per-kernel-class timing with HIP events on the backend's stream; bit i of the mask
+    // times class i of timingNames() (0: off).
+    void enableTiming(unsigned class_mask);
     std::vector<std::string> timingNames() const;
     void timing(std::vector<double>& total_us, std::vector<long long>& launches, std::vector<double>& work);
     void timedBegin(int cls);
@@ -186,16 +187,19 @@ private:
     HostScalars* host_scal_ = nullptr;  // host-mapped, written by k_finalize
     double* dev_scal_ = nullptr;        // device address of host_scal_
 
-    // timing
-    static constexpr int kTimingClasses = 5;
-    bool timing_on_ = false;
+    // timing (events come from a pool that is reused once they have been read)
+    static constexpr int kTimingClasses = 6;
+    hipEvent_t poolEvent();
+    unsigned timing_mask_ = 0;
+    std::vector<hipEvent_t> ev_pool_;
+    std::size_t ev_next_ = 0;
     std::vector<hipEvent_t> ev_begin_[kTimingClasses], ev_end_[kTimingClasses];
     std::vector<double> t_total_, t_work_;
     std::vector<double> pending_work_[kTimingClasses];
     std::vector<long long> t_count_;
 };
 
-enum TimingClass : int { kTWire = 0, kTDensity = 1, kTDct = 2, kTGather = 3, kTController = 4 };
+enum TimingClass : int { kTWire = 0, kTBand = 1, kTSort = 2, kTDct = 3, kTGather = 4, kTController = 5 };
 
 }  // namespace mpgp
 

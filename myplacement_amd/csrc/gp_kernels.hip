@@ -39,6 +39,15 @@ void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// (evaluation, block) of a launch batched over blockIdx.y.  With two evaluations the even
+// workgroups serve the first and the odd ones the second: workgroups go to the eight XCDs round
+// robin, so each XCD's L2 then holds the gathered data of one evaluation only.
+__device__ inline int2 batchBlock() {
+    if (gridDim.y == 1) return make_int2(0, blockIdx.x);
+    const int lin = blockIdx.x + blockIdx.y * gridDim.x;
+    return make_int2(lin % static_cast<int>(gridDim.y), lin / static_cast<int>(gridDim.y));
+}
+
 __device__ inline double waveSum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -185,7 +194,7 @@ __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_gra
         for (int p = b + sub; p < e; p += W) {
             const double2 q = pinPosition(pos, a.pin_part, a.pin_off, p);
             pin_grad[p] = make_float2(ex32(q.x - mx, inv_gamma) * ipx - ex32(nx - q.x, inv_gamma) * inx,
-                                        ex32(q.y - my, inv_gamma) * ipy - ex32(ny - q.y, inv_gamma) * iny);
+                                      ex32(q.y - my, inv_gamma) * ipy - ex32(ny - q.y, inv_gamma) * iny);
         }
     }
 }
@@ -195,9 +204,10 @@ __global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int small_blocks,
     double wl = 0.0, hp = 0.0;
     const double inv_gamma = 1.0 / a.gamma;
     const float gamma_f = static_cast<float>(a.gamma);
-    const int block = blockIdx.x;
-    const double2* pos = a.pos[blockIdx.y];
-    float2* pin_grad = a.pin_grad[blockIdx.y];
+    const int2 eb = batchBlock();
+    const int block = eb.y;
+    const double2* pos = a.pos[eb.x];
+    float2* pin_grad = a.pin_grad[eb.x];
     if (block < small_blocks) {
         const int net = block * kThreads + threadIdx.x;
         if (net < a.n_small) {
@@ -249,7 +259,7 @@ __global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int small_blocks,
         if (net < a.n_small + a.n_medium + a.n_large) wireGroup<64>(a, pos, pin_grad, net, threadIdx.x & 63, inv_gamma, gamma_f, hp, wl);
     }
     double v[2] = {hp, wl};
-    publishPartials<2>(v, 0U, blockIdx.x, a.partials[blockIdx.y], lds);
+    publishPartials<2>(v, 0U, block, a.partials[eb.x], lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -354,7 +364,8 @@ __global__ __launch_bounds__(kBandThreads) void k_dband(DensityArgs a) {
     __shared__ double lds[(kBandThreads / 64 + 1) * 9];
     const Region& r = a.r;
     const int nx = r.nx, ny = r.ny;
-    const int R0 = blockIdx.x * a.band_rows;
+    const int2 eb = batchBlock();
+    const int R0 = eb.y * a.band_rows;
     const int R1 = min(R0 + a.band_rows, ny);
     const int cells = (R1 - R0) * nx;
     unsigned long long* lmov = band;
@@ -386,8 +397,8 @@ __global__ __launch_bounds__(kBandThreads) void k_dband(DensityArgs a) {
         }
     };
     if (a.n > 0) {
-        const double2* pos = a.pos[blockIdx.y];
-        const int margin = a.margin[blockIdx.y];
+        const double2* pos = a.pos[eb.x];
+        const int margin = a.margin[eb.x];
         const int lo = max(0, R0 - a.lookback + 1 - margin);
         const int hi = min(ny, R1 + margin);
         for (int q = a.keystart[lo] + threadIdx.x; q < a.keystart[hi]; q += kBandThreads) {
@@ -402,7 +413,7 @@ __global__ __launch_bounds__(kBandThreads) void k_dband(DensityArgs a) {
     __syncthreads();
     // bin metrics (Metrics.cpp:101-119 per bin) and the deviation grid
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    float* rho = a.rho[blockIdx.y];
+    float* rho = a.rho[eb.x];
     for (int i = threadIdx.x; i < cells; i += kBandThreads) {
         const int row = R0 + i / nx, col = i - (i / nx) * nx;
         const double left = r.llx + static_cast<double>(col) * r.bw;
@@ -426,7 +437,7 @@ __global__ __launch_bounds__(kBandThreads) void k_dband(DensityArgs a) {
         v[8] = fmax(v[8], d_wo);
         rho[static_cast<size_t>(row) * a.ldrho + col] = static_cast<float>(d_w - a.t);
     }
-    publishPartials<9, kBandThreads / 64>(v, (1U << 7) | (1U << 8), blockIdx.x, a.partials[blockIdx.y], lds);
+    publishPartials<9, kBandThreads / 64>(v, (1U << 7) | (1U << 8), eb.y, a.partials[eb.x], lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -581,10 +592,11 @@ __device__ inline double2 sampleField(const float2* __restrict__ f, int ld, doub
 
 __global__ __launch_bounds__(kThreads) void k_gather(GatherArgs a) {
     __shared__ double lds[(kWaves + 1) * 7];
-    const int p = blockIdx.x * kThreads + threadIdx.x;
+    const int2 eb = batchBlock();
+    const int e = eb.x;
+    const int p = eb.y * kThreads + threadIdx.x;
     // wire L1, density L1, |raw|^2, |pre|^2, V.pre, |pos - px|^2, |pre - pg|^2
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
-    const int e = blockIdx.y;
     if (p < a.n) {
         const double2 c = a.pos[e][p];
         const double q = a.charge[p];
@@ -623,7 +635,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(GatherArgs a) {
         }
         if (a.ck_out[e]) a.ck_out[e][p] = c;
     }
-    publishPartials<7>(v, 0U, blockIdx.x, a.partials[e], lds);
+    publishPartials<7>(v, 0U, eb.y, a.partials[e], lds);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -727,7 +727,7 @@ mp_gp_status mp_gp_session_positions(mp_gp_session* h, double* xy) {
 mp_gp_status mp_gp_session_enable_kernel_timing(mp_gp_session* h, int enable) {
     return guarded([&] {
         if (!h) throw std::invalid_argument("null session");
-        h->s->backend().enableTiming(enable != 0);
+        h->s->backend().enableTiming(static_cast<unsigned>(enable));
     });
 }
 
