@@ -238,6 +238,35 @@ mp_gp_status mp_gp_prepared_arrays(const mp_gp_prepared* prepared, int64_t* mova
  * uniform centre per movable module, then clampModuleToCore.  centers: [2M] in/out. */
 mp_gp_status mp_gp_random_placement(const mp_design* design, uint64_t seed, double* centers);
 
+/* InitialPlacementOptions, quadratic fields (include/myplacement/placement/InitialPlacer.hpp:21-25). */
+typedef struct mp_qp_options {
+    int32_t outer_iterations;      /* 5 */
+    int32_t solver_iterations;     /* 100: BiCGSTAB iterations per axis and outer iteration */
+    double tolerance;              /* 1e-5: relative residual */
+    double minimum_distance;       /* 1.0 */
+    uint64_t clique_degree_limit;  /* 96: larger nets use the star model */
+    int32_t device;                /* HIP device ordinal (0) */
+} mp_qp_options;
+
+/* InitialPlacementResult (InitialPlacer.hpp:28-34) for the quadratic method, plus solver figures. */
+typedef struct mp_qp_result {
+    double hpwl_before, hpwl_after;
+    double elapsed_seconds;
+    int32_t iterations;            /* completed outer iterations */
+    int32_t solver_iterations;     /* BiCGSTAB iterations summed over axes and outer iterations */
+    double relative_residual[2];   /* final |b - Ax| / |b| of the last outer iteration, x then y */
+} mp_qp_result;
+
+void mp_gp_default_qp_options(mp_qp_options* options);
+
+/* InitialPlacer quadraticPlacement, src/placement/InitialPlacer.cpp:315-402: movable modules start
+ * at the core centre; each outer iteration builds the clique (degree <= limit) / star bound-to-bound
+ * system from the current pin positions with a 1e-4 anchor and solves x and y with unpreconditioned
+ * BiCGSTAB on the device from the current positions, then clamps into the core.  centers: [2M]
+ * in/out (fixed modules are read, movable ones written). */
+mp_gp_status mp_gp_quadratic_placement(const mp_design* design, const mp_qp_options* options, double* centers,
+                                       mp_qp_result* result);
+
 /* ---- device-resident adaptive run (runAdaptiveGlobalPlacement with --compute-backend hip) ---- */
 
 /* Creates the backend and the device-resident optimizer state from a prepared design.

@@ -180,6 +180,42 @@ class MpResult(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class MpQpOptions(C.Structure):
+    """mp_qp_options: InitialPlacementOptions, quadratic fields (InitialPlacer.hpp:21-25)."""
+    _fields_ = [
+        ("outer_iterations", C.c_int32),
+        ("solver_iterations", C.c_int32),
+        ("tolerance", C.c_double),
+        ("minimum_distance", C.c_double),
+        ("clique_degree_limit", C.c_uint64),
+        ("device", C.c_int32),
+    ]
+
+
+def default_qp_options(**overrides) -> MpQpOptions:
+    o = MpQpOptions(outer_iterations=5, solver_iterations=100, tolerance=1e-5, minimum_distance=1.0,
+                    clique_degree_limit=96, device=0)
+    for key, value in overrides.items():
+        setattr(o, key, value)
+    return o
+
+
+class MpQpResult(C.Structure):
+    _fields_ = [
+        ("hpwl_before", C.c_double),
+        ("hpwl_after", C.c_double),
+        ("elapsed_seconds", C.c_double),
+        ("iterations", C.c_int32),
+        ("solver_iterations", C.c_int32),
+        ("relative_residual", C.c_double * 2),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "relative_residual"}
+        d["relative_residual"] = (self.relative_residual[0], self.relative_residual[1])
+        return d
+
+
 class MpPreparedInfo(C.Structure):
     _fields_ = [
         ("movable_count", C.c_int64),

@@ -57,6 +57,9 @@ def library():
         lib.mp_gp_prepared_layout.argtypes = [V, C.POINTER(abi.MpStatic)]
         lib.mp_gp_prepared_arrays.argtypes = [V, V, V, V]
         lib.mp_gp_random_placement.argtypes = [C.POINTER(abi.MpDesign), C.c_uint64, V]
+        lib.mp_gp_default_qp_options.argtypes = [C.POINTER(abi.MpQpOptions)]
+        lib.mp_gp_quadratic_placement.argtypes = [C.POINTER(abi.MpDesign), C.POINTER(abi.MpQpOptions), V,
+                                                  C.POINTER(abi.MpQpResult)]
         lib.mp_gp_session_create.argtypes = [V, C.POINTER(abi.MpOptions), V, C.POINTER(V)]
         lib.mp_gp_session_step.argtypes = [V, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         lib.mp_gp_session_finish.argtypes = [V, V, C.POINTER(abi.MpResult)]
@@ -356,6 +359,17 @@ def random_placement(design, seed: int = 2026) -> np.ndarray:
     d = design.to_c()
     _raise(library().mp_gp_random_placement(C.byref(d), seed, _p(c)))
     return c
+
+
+def quadratic_placement(design, options: abi.MpQpOptions | None = None):
+    """InitialPlacer quadraticPlacement (InitialPlacer.cpp:315-402) with the BiCGSTAB solves on the
+    GPU; returns (centres, result dict)."""
+    o = abi.default_qp_options() if options is None else options
+    c = np.ascontiguousarray(design.center, dtype=np.float64).copy()
+    d = design.to_c(c)
+    r = abi.MpQpResult()
+    _raise(library().mp_gp_quadratic_placement(C.byref(d), C.byref(o), _p(c), C.byref(r)))
+    return c, r.as_dict()
 
 
 def global_place(design, options: abi.MpOptions, center=None, history_capacity: int = 100000):

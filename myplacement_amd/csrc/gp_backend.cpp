@@ -7,8 +7,8 @@
 //     writes every scalar into host-mapped memory, and the host waits once;
 //   * the controller's two gradient evaluations (current and look-ahead positions) run as one
 //     chain of batched launches: twice the workgroups per launch, half the launches;
-//   * the particle row order of the density bands is reused across evaluations while every
-//     particle stays within a stated number of bin rows of where it was sorted;
+//   * the density is a fixed-point deposit through per-workgroup LDS windows over a particle
+//     order sorted by bin row, reused across evaluations (the order affects speed, not results);
 //   * the gradient scatter is atomic-free: per-pin terms, then a particle -> pin CSR sum;
 //   * the DCT stages are dense bin-grid products on MFMA; the electrostatic energy is taken in
 //     the cosine basis (sum_lk c_l c_k / 4 * chat * phihat), which equals the reference's grid sum
@@ -33,12 +33,7 @@ namespace {
 constexpr double kPiD = 3.141592653589793238462643383279502884;
 constexpr std::size_t kMaximumBinCount = 1U << 20U;  // core/ResourceLimits.hpp:10
 constexpr std::size_t kDeviceReserveBytes = 4ULL << 30;
-#ifndef MP_GP_TARGET_BANDS
-#define MP_GP_TARGET_BANDS 128
-#endif
-// density bands per evaluation the band height aims at (fewer, taller bands read each particle
-// fewer times; more bands fill more CUs)
-constexpr int kTargetBands = MP_GP_TARGET_BANDS;
+
 
 int roundUp32(int n) { return (n + 31) / 32 * 32; }
 
@@ -184,14 +179,10 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     pin_off_.allocate(Ii, budget_, "pin offsets");
     ppin_idx_.allocate(ppin_idx.size(), budget_, "particle-pin index");
     ppin_off_.allocate(static_cast<std::size_t>(P_) + 1, budget_, "particle-pin offsets");
-    // density band geometry: bands of band_rows_ bin rows, one workgroup each, both fixed-point
-    // layers of a band in LDS (<= 64 KiB); particles taller than two bin rows are not row-sorted
-    // and every band visits them
-    if (nx_ > 4096) throw InvalidArgument("density grids wider than 4096 bins are not supported by the band kernel");
+    // density: particles taller than two bin rows are not row-sorted (their own deposit workgroups
+    // add them to the global layers)
     if (ny_ > 8192) throw InvalidArgument("density grids taller than 8192 bins are not supported by the row sort");
     if (static_cast<unsigned>(P_) > kPermIdMask) throw InvalidArgument("HIP backend cannot order more than 2^30 - 1 particles");
-    band_rows_ = std::max(1, std::min(ny_ / kTargetBands, 4096 / nx_));
-    bands_ = (ny_ + band_rows_ - 1) / band_rows_;
     fix_ = std::ldexp(1.0, 40) / (region_.bw * region_.bh);
     std::vector<unsigned char> tall(P_);
     std::vector<int> tall_ids;
@@ -219,6 +210,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
         sl.phi.allocate(Gp, budget_, "potential coefficients");
         sl.t12.allocate(2 * Gp, budget_, "inverse DCT intermediate");
         sl.field.allocate(2 * Gp, budget_, "field grid");
+        sl.grid.allocate(2 * G, budget_, "fixed-point density layers");
     }
     pot_.allocate(Gp, budget_, "potential grid");
     bxf_.allocate(static_cast<std::size_t>(nxp_) * nxp_, budget_, "forward x factor");
@@ -226,7 +218,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     bxi_.allocate(static_cast<std::size_t>(nxp_) * 2 * nxp_, budget_, "inverse x factors");
     ayi_.allocate(static_cast<std::size_t>(nyp_) * nyp_, budget_, "inverse y cosine factor");
     syi_.allocate(static_cast<std::size_t>(nyp_) * nyp_, budget_, "inverse y sine factor");
-    const int max_blocks = std::max({wireBlocks(n_small_, n_medium_, n_large_), bands_,
+    const int max_blocks = std::max({wireBlocks(n_small_, n_medium_, n_large_), densityBinBlocks(region_),
                                      particleBlocks(P_), (nxp_ / 32) * (nyp_ / 32), 1});
     for (int k = 0; k < kRedKinds * kScalarBanks; ++k)
         partials_[k].allocate(static_cast<std::size_t>(max_blocks) * redWidth(k / kScalarBanks), budget_,
@@ -274,7 +266,11 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     tall_.upload(tall.data(), tall.size(), stream, "upload tall-particle flags");
     tall_ids_.upload(tall_ids.data(), tall_ids.size(), stream, "upload tall-particle ids");
     // the padding of the deviation grids stays zero: the band kernel writes only the nx x ny part
-    for (Slot& sl : slots_) hipCheck(hipMemsetAsync(sl.rho.get(), 0, Gp * sizeof(float), stream), "clear density grid");
+    for (Slot& sl : slots_) {
+        hipCheck(hipMemsetAsync(sl.rho.get(), 0, Gp * sizeof(float), stream), "clear density grid");
+        // the bin kernel clears the layers after reading them, so they start every evaluation at zero
+        hipCheck(hipMemsetAsync(sl.grid.get(), 0, 2 * G * sizeof(unsigned long long), stream), "clear density layers");
+    }
     hipCheck(hipStreamSynchronize(stream), "finish backend upload");
 }
 
@@ -324,7 +320,7 @@ void Backend::enableTiming(unsigned class_mask) {
 }
 
 std::vector<std::string> Backend::timingNames() const {
-    return {"k_wire", "k_dband", "k_density_sort", "k_gemm", "k_gather", "controller"};
+    return {"k_wire", "k_density", "k_density_sort", "k_gemm", "k_gather", "controller"};
 }
 
 void Backend::timing(std::vector<double>& total_us, std::vector<long long>& launches, std::vector<double>& work) {
@@ -364,9 +360,6 @@ DensityArgs Backend::densityArgs() const {
     da.t = t_;
     da.fix = fix_;
     da.inv_fix = 1.0 / fix_;
-    da.band_rows = band_rows_;
-    da.bands = bands_;
-    da.lookback = kBandLookback;
     da.key = key_.get();
     da.hist = hist_.get();
     da.colprefix = colprefix_.get();
@@ -390,12 +383,10 @@ void Backend::sortParticles(const double2* pos) {
 
 void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bool want_grad) {
     if (count < 1 || count > kSlots) throw std::logic_error("one or two evaluations per batch");
-    for (int e = 0; e < count; ++e) {
-        if (specs[e].margin == kFreshOrder && count != 1) throw std::logic_error("a fresh row order needs a single evaluation");
+    for (int e = 0; e < count; ++e)
         if (!specs[e].session && count != 1) throw std::logic_error("backend-mode evaluations are not batched");
-    }
     hipStream_t s = stream_;
-    if (specs[0].margin == kFreshOrder) sortParticles(specs[0].pos);
+    if (specs[0].reorder) sortParticles(specs[0].pos);
 
     WireArgs wa{};
     wa.net_off = net_off_.get();
@@ -427,17 +418,18 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
     DensityArgs da = densityArgs();
     for (int e = 0; e < count; ++e) {
         da.pos[e] = specs[e].pos;
-        da.margin[e] = specs[e].margin == kFreshOrder ? 0 : specs[e].margin;
+        da.grid[e] = slots_[e].grid.get();
         da.rho[e] = slots_[e].rho.get();
         da.partials[e] = partials(kRedMetrics, specs[e].bank);
     }
     timedBegin(kTBand);
-    launchDensityBands(da, count, s);
+    launchDensity(da, count, s);
     // algorithmic bytes per evaluation (same convention): scatter 20 per particle and 12 per bin,
     // metrics and deviation 24 per bin
     timedEnd(kTBand, count * (20.0 * P_ + 36.0 * static_cast<double>(nx_) * ny_));
     for (int e = 0; e < count; ++e)
-        addReduction(kRedMetrics, specs[e].bank, bands_, static_cast<double>(nx_) * ny_, dark_total_, placeable_total_);
+        addReduction(kRedMetrics, specs[e].bank, densityBinBlocks(region_), static_cast<double>(nx_) * ny_, dark_total_,
+                     placeable_total_);
 
     const double W = region_.urx - region_.llx, H = region_.ury - region_.lly;
     SpectralArgs sp{nx_, ny_, W, H, 1.0 / (4.0 * nx_ * static_cast<double>(ny_)), {}};
@@ -557,7 +549,7 @@ void Backend::evaluateHost(const double* xy, double gamma, bool want_grad, mp_gp
     setDevice();
     hipStream_t s = stream_;
     pos_.upload(reinterpret_cast<const double2*>(xy), P_, s, "upload particle positions");
-    const EvalSpec spec{pos_.get(), nullptr, 0, kFreshOrder};
+    const EvalSpec spec{pos_.get(), nullptr, 0, true};
     enqueueEvaluate(&spec, 1, gamma, want_grad);
     if (want_grad && P_ > 0) {
         hipCheck(hipMemcpyAsync(wire_grad, wire_out_.get(), P_ * sizeof(double2), hipMemcpyDeviceToHost, s),

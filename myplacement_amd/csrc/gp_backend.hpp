@@ -78,9 +78,6 @@ struct HostScalars {
 
 class Backend {
 public:
-    // `margin` of an evaluation: rebuild the particle row order at its positions first.
-    static constexpr int kFreshOrder = -1;
-
     Backend(const mp_gp_static& layout, std::size_t allocation_limit);
     ~Backend();
     Backend(const Backend&) = delete;
@@ -103,12 +100,11 @@ public:
         const double2* pos;          // device positions
         const SessionOut* session;   // null: write the separate wire / density gradients
         int bank;                    // scalar bank of the results
-        int margin;                  // bin rows the shared row order may be stale by, or kFreshOrder
+        bool reorder;                // first rebuild the particle row order at these positions (first spec only)
     };
     // Enqueues `count` (1 or 2) evaluations with smoothing `gamma` as one chain of batched
-    // launches.  The density bands read the shared particle row order, which must be within each
-    // evaluation's `margin` bin rows of its positions for every particle (kFreshOrder: rebuild it
-    // first, single evaluations only).
+    // launches.  The density deposit visits particles in the shared row order, which only has to
+    // be roughly current for speed: results do not depend on it.
     void enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bool want_grad);
     // Rebuilds the particle row order at `pos`.
     void sortParticles(const double2* pos);
@@ -147,6 +143,7 @@ private:
         DeviceArray<float2> pin_grad;
         DeviceArray<float> rho, tmp, phi, t12;
         DeviceArray<float> field;  // interleaved (Ex, Ey)
+        DeviceArray<unsigned long long> grid;  // fixed-point density layers, zero between evaluations
     };
     static constexpr int kSlots = kMaxEval;
     void setDevice() const;
@@ -177,7 +174,6 @@ private:
     DeviceArray<double2> sorted_size_;
     DeviceArray<unsigned char> tall_;
     int n_tall_ = 0;
-    int band_rows_ = 1, bands_ = 1;
     double fix_ = 1.0;
     DeviceArray<float> pot_;
     DeviceArray<float> bxf_, ayf_, bxi_, ayi_, syi_;

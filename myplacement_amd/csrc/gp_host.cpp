@@ -42,6 +42,8 @@ Box boxAround(double cx, double cy, double w, double h) {
     return {cx - w * 0.5, cy - h * 0.5, cx + w * 0.5, cy + h * 0.5};
 }
 
+}  // namespace
+
 void orient(double& x, double& y, int o) {  // PlacementDatabase.cpp:55-68
     const double ox = x, oy = y;
     switch (o) {
@@ -55,6 +57,8 @@ void orient(double& x, double& y, int o) {  // PlacementDatabase.cpp:55-68
         default: break;
     }
 }
+
+namespace {
 
 // Bin span of a clipped rectangle (Metrics.cpp:78-85).
 void binSpan(const Box& b, const Box& region, double bw, double bh, int nx, int ny, int& c0, int& c1, int& r0, int& r1) {

@@ -52,4 +52,7 @@ struct Prepared {
 
 void randomPlacement(const mp_design& d, std::uint64_t seed, double* centers);
 
+// transformOffset, PlacementDatabase.cpp:55-68 (orientation codes N,S,E,W,FN,FS,FE,FW = 0..7)
+void orient(double& x, double& y, int o);
+
 }  // namespace mpgp

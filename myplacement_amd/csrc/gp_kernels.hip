@@ -3,7 +3,7 @@
 // Reference counterparts (src/placement/cuda/CudaPlacementBackend.cu unless noted):
 //   k_wire      buildPinPositions + computeNetBounds + computeNetSums + computePinGradient (:194-337),
 //               CPU form AdaptiveGlobalPlacer.cpp:29-82
-//   k_dhist / k_dscan / k_dplace / k_dband
+//   k_dhist / k_dscan / k_dplace / k_ddeposit / k_dbins
 //               depositParticlesKernel + buildDensityMetricsKernel + 10 reductions (:339-398,
 //               :1022-1038); CPU DensityMap::addRectangle / metrics, Metrics.cpp:74-119
 //   k_gemm      forwardDct / solveSpectralKernel / inverse2d (:547-567, :679-742); CPU
@@ -21,6 +21,7 @@
 #include "gp_kernels.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <stdexcept>
 #include <string>
@@ -263,14 +264,14 @@ __global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int small_blocks,
 }
 
 // ---------------------------------------------------------------------------------------------
-// Density: counting sort of the particles by the first bin row they cover, then one workgroup per
-// band of bin rows accumulates its particles' exact rectangle / bin overlaps (FP64 geometry, bin
-// index math of Metrics.cpp:78-85) into LDS as 2^40-per-bin-area fixed point -- order-independent,
-// so the density is bit-reproducible -- and evaluates the bin metrics and the FP32 deviation grid
-// for the DCT in the same kernel.  The row order may be stale by `margin` rows (positions move a
-// fraction of a bin per evaluation), so it is rebuilt only when the accumulated movement could
-// exceed that margin; every band recomputes its particles' spans from the current positions.
-// Particles taller than two bin rows (movable macros) are not sorted: every band visits them.
+// Density.  The exact rectangle / bin overlaps (FP64 geometry, bin index math of Metrics.cpp:78-85)
+// are added as unsigned 2^40-per-bin-area fixed point, so the density does not depend on the order
+// of the additions and is bit-reproducible.  The particles are visited in an order sorted by the
+// first bin row they cover (a counting sort, rebuilt only when positions may have moved far): the
+// particles of one deposit workgroup then cover a few bin rows, which it accumulates in an LDS
+// window before adding the window to the global layers.  The order only affects speed, never the
+// result.  Particles taller than two bin rows (movable macros) are not sorted; their own
+// workgroups add them straight to the global layers.
 
 __device__ inline void clippedSpan(double2 c, double2 sz, const Region& r, double& left, double& right,
                                    double& bottom, double& top, int& c0, int& c1, int& r0, int& r1, bool& empty) {
@@ -359,31 +360,69 @@ __global__ __launch_bounds__(kSortThreads) void k_dplace(DensityArgs a) {
     a.perm[slot] = static_cast<unsigned>(p) | (static_cast<unsigned>(a.layer[p]) << kPermLayerShift);
     a.sorted_size[slot] = a.size[p];
 }
-__global__ __launch_bounds__(kBandThreads) void k_dband(DensityArgs a) {
-    extern __shared__ unsigned long long band[];
-    __shared__ double lds[(kBandThreads / 64 + 1) * 9];
-    const Region& r = a.r;
-    const int nx = r.nx, ny = r.ny;
+// Deposit: one workgroup per kDepositThreads consecutive particles of the row order (then the tall
+// particles).  The workgroup finds the bin rows its particles cover at the current positions and
+// walks them in slabs that fit its LDS window: accumulate the slab in LDS, then add its non-zero
+// cells to the global fixed-point layers.  Integer addition keeps the result independent of the
+// order in which workgroups and lanes arrive.
+__global__ __launch_bounds__(kDepositThreads) void k_ddeposit(DensityArgs a, int sorted_blocks) {
+    __shared__ unsigned long long win[2 * kWindowCells];
+    __shared__ int lds[2 * (kDepositThreads / 64)];
     const int2 eb = batchBlock();
-    const int R0 = eb.y * a.band_rows;
-    const int R1 = min(R0 + a.band_rows, ny);
-    const int cells = (R1 - R0) * nx;
-    unsigned long long* lmov = band;
-    unsigned long long* lfil = band + static_cast<size_t>(a.band_rows) * nx;
-    for (int i = threadIdx.x; i < cells; i += kBandThreads) {
-        lmov[i] = 0ULL;
-        lfil[i] = 0ULL;
+    const Region& r = a.r;
+    const int nx = r.nx;
+    const size_t G = static_cast<size_t>(nx) * r.ny;
+    const double2* pos = a.pos[eb.x];
+    unsigned long long* grid = a.grid[eb.x];
+    int p = -1, layer = 0;
+    double2 sz = make_double2(0.0, 0.0);
+    if (eb.y < sorted_blocks) {
+        const int q = eb.y * kDepositThreads + threadIdx.x;
+        if (q < a.keystart[r.ny]) {
+            const unsigned e = a.perm[q];
+            p = static_cast<int>(e & kPermIdMask);
+            layer = static_cast<int>(e >> kPermLayerShift);
+            sz = a.sorted_size[q];
+        }
+    } else {
+        const int k = (eb.y - sorted_blocks) * kDepositThreads + threadIdx.x;
+        if (k < a.n_tall) {
+            p = a.tall_ids[k];
+            sz = a.size[p];
+            layer = a.layer[p];
+        }
+    }
+    double left = 0.0, right = 0.0, bottom = 0.0, top = 0.0;
+    int c0 = 0, c1 = -1, r0 = 0, r1 = -1;
+    bool live = false;
+    if (p >= 0) {
+        bool empty;
+        clippedSpan(pos[p], sz, r, left, right, bottom, top, c0, c1, r0, r1, empty);
+        live = !empty;
+    }
+    // rows covered by the workgroup
+    int lo = live ? r0 : INT_MAX, hi = live ? r1 : INT_MIN;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        lds[wave] = lo;
+        lds[kDepositThreads / 64 + wave] = hi;
     }
     __syncthreads();
-    auto deposit = [&](double2 c, double2 sz, int layer) {
-        double left, right, bottom, top;
-        int c0, c1, r0, r1;
-        bool empty;
-        clippedSpan(c, sz, r, left, right, bottom, top, c0, c1, r0, r1, empty);
-        const int ra = max(r0, R0), rb = min(r1, R1 - 1);
-        if (empty || ra > rb) return;
-        const double scale = layer == 1 ? a.t : 1.0;
-        unsigned long long* dst = layer == 2 ? lfil : lmov;
+    lo = INT_MAX;
+    hi = INT_MIN;
+    for (int w = 0; w < kDepositThreads / 64; ++w) {
+        lo = min(lo, lds[w]);
+        hi = max(hi, lds[kDepositThreads / 64 + w]);
+    }
+    if (lo > hi) return;  // nothing to deposit (uniform across the workgroup)
+    const double scale = layer == 1 ? a.t : 1.0;
+    const int L = layer == 2 ? 1 : 0;
+    auto deposit = [&](int ra, int rb, unsigned long long* dst, int dst_row0, size_t layer_stride) {
         for (int row = ra; row <= rb; ++row) {
             const double bin_bottom = r.lly + static_cast<double>(row) * r.bh;
             const double oy = fmax(0.0, fmin(top, bin_bottom + r.bh) - fmax(bottom, bin_bottom));
@@ -392,52 +431,71 @@ __global__ __launch_bounds__(kBandThreads) void k_dband(DensityArgs a) {
                 const double ox = fmax(0.0, fmin(right, bin_left + r.bw) - fmax(left, bin_left));
                 const double area = ox * oy;
                 if (area <= kEps) continue;
-                atomicAdd(dst + (row - R0) * nx + col, toFixed(scale * area, a.fix));
+                atomicAdd(dst + L * layer_stride + static_cast<size_t>(row - dst_row0) * nx + col,
+                          toFixed(scale * area, a.fix));
             }
         }
     };
-    if (a.n > 0) {
-        const double2* pos = a.pos[eb.x];
-        const int margin = a.margin[eb.x];
-        const int lo = max(0, R0 - a.lookback + 1 - margin);
-        const int hi = min(ny, R1 + margin);
-        for (int q = a.keystart[lo] + threadIdx.x; q < a.keystart[hi]; q += kBandThreads) {
-            const unsigned e = a.perm[q];
-            deposit(pos[e & kPermIdMask], a.sorted_size[q], static_cast<int>(e >> kPermLayerShift));
-        }
-        for (int k = threadIdx.x; k < a.n_tall; k += kBandThreads) {
-            const int p = a.tall_ids[k];
-            deposit(pos[p], a.size[p], a.layer[p]);
-        }
+    const int slab_rows = kWindowCells / nx;
+    if (slab_rows == 0) {  // grids wider than the window: straight to the global layers
+        if (live) deposit(r0, r1, grid, 0, G);
+        return;
     }
-    __syncthreads();
-    // bin metrics (Metrics.cpp:101-119 per bin) and the deviation grid
+    // the covered rows in slabs of the window's height: accumulate in LDS, add non-zero cells
+    for (int s0 = lo; s0 <= hi; s0 += slab_rows) {
+        const int s1 = min(hi, s0 + slab_rows - 1);
+        const int cells = (s1 - s0 + 1) * nx;
+        for (int i = threadIdx.x; i < 2 * cells; i += kDepositThreads) win[i] = 0ULL;
+        __syncthreads();
+        if (live && r0 <= s1 && r1 >= s0) deposit(max(r0, s0), min(r1, s1), win, s0, static_cast<size_t>(cells));
+        __syncthreads();
+        const size_t base = static_cast<size_t>(s0) * nx;
+        for (int i = threadIdx.x; i < cells; i += kDepositThreads) {
+            const unsigned long long m = win[i], f = win[cells + i];
+            if (m) atomicAdd(grid + base + i, m);
+            if (f) atomicAdd(grid + G + base + i, f);
+        }
+        __syncthreads();
+    }
+}
+
+// Bin metrics (Metrics.cpp:101-119 per bin) and the FP32 deviation grid for the DCT from the
+// fixed-point layers, which it clears for the next evaluation.
+__global__ __launch_bounds__(kThreads) void k_dbins(DensityArgs a) {
+    __shared__ double lds[(kWaves + 1) * 9];
+    const int2 eb = batchBlock();
+    const Region& r = a.r;
+    const int nx = r.nx;
+    const size_t G = static_cast<size_t>(nx) * r.ny;
+    unsigned long long* grid = a.grid[eb.x];
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    float* rho = a.rho[eb.x];
-    for (int i = threadIdx.x; i < cells; i += kBandThreads) {
-        const int row = R0 + i / nx, col = i - (i / nx) * nx;
+    const size_t i = static_cast<size_t>(eb.y) * kThreads + threadIdx.x;
+    if (i < G) {
+        const int row = static_cast<int>(i / nx), col = static_cast<int>(i - static_cast<size_t>(row) * nx);
         const double left = r.llx + static_cast<double>(col) * r.bw;
         const double bottom = r.lly + static_cast<double>(row) * r.bh;
         const double area = fmax(0.0, (left + r.bw) - left) * fmax(0.0, (bottom + r.bh) - bottom);
-        const double mov = static_cast<double>(lmov[i]) * a.inv_fix;
-        const double fil = static_cast<double>(lfil[i]) * a.inv_fix;
-        const double q_wo = mov + a.static_charge[static_cast<size_t>(row) * nx + col];
+        const double mov = static_cast<double>(grid[i]) * a.inv_fix;
+        const double fil = static_cast<double>(grid[G + i]) * a.inv_fix;
+        grid[i] = 0ULL;
+        grid[G + i] = 0ULL;
+        const double q_wo = mov + a.static_charge[i];
         const double q_w = q_wo + a.t * fil;
         const double cap = a.t * area;
         const double inv_area = 1.0 / fmax(area, kEps);
         const double d_w = q_w * inv_area, d_wo = q_wo * inv_area;
-        v[0] += fmax(0.0, q_w - cap);
-        v[1] += fmax(0.0, q_wo - cap);
-        v[2] += q_w;
-        v[3] += q_wo;
-        v[4] += mov;
-        v[5] += d_w;
-        v[6] += d_wo;
-        v[7] = fmax(v[7], d_w);
-        v[8] = fmax(v[8], d_wo);
-        rho[static_cast<size_t>(row) * a.ldrho + col] = static_cast<float>(d_w - a.t);
+        v[0] = fmax(0.0, q_w - cap);
+        v[1] = fmax(0.0, q_wo - cap);
+        v[2] = q_w;
+        v[3] = q_wo;
+        v[4] = mov;
+        v[5] = d_w;
+        v[6] = d_wo;
+        v[7] = d_w;
+        v[8] = d_wo;
+        a.rho[eb.x][static_cast<size_t>(row) * a.ldrho + col] = static_cast<float>(d_w - a.t);
     }
-    publishPartials<9, kBandThreads / 64>(v, (1U << 7) | (1U << 8), eb.y, a.partials[eb.x], lds);
+    publishPartials<9>(v, (1U << 7) | (1U << 8), eb.y, a.partials[eb.x], lds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -780,11 +838,21 @@ void launchDensitySort(const DensityArgs& a, hipStream_t s) {
     check(hipGetLastError(), "launch density placement kernel");
 }
 
-void launchDensityBands(const DensityArgs& a, int count, hipStream_t s) {
+int densityBinBlocks(const Region& r) {
+    return static_cast<int>((static_cast<long long>(r.nx) * r.ny + kThreads - 1) / kThreads);
+}
+
+void launchDensity(const DensityArgs& a, int count, hipStream_t s) {
     if (count < 1 || count > kMaxEval) throw std::invalid_argument("bad evaluation batch");
-    const size_t band_lds = 2 * static_cast<size_t>(a.band_rows) * a.r.nx * sizeof(unsigned long long);
-    hipLaunchKernelGGL(k_dband, dim3(a.bands, count), dim3(kBandThreads), band_lds, s, a);
-    check(hipGetLastError(), "launch density band kernel");
+    const int sorted_blocks = (a.n + kDepositThreads - 1) / kDepositThreads;
+    const int tall_blocks = (a.n_tall + kDepositThreads - 1) / kDepositThreads;
+    if (sorted_blocks + tall_blocks > 0) {
+        hipLaunchKernelGGL(k_ddeposit, dim3(sorted_blocks + tall_blocks, count), dim3(kDepositThreads), 0, s, a,
+                           sorted_blocks);
+        check(hipGetLastError(), "launch density deposit kernel");
+    }
+    hipLaunchKernelGGL(k_dbins, dim3(densityBinBlocks(a.r), count), dim3(kThreads), 0, s, a);
+    check(hipGetLastError(), "launch density bin kernel");
 }
 
 void launchGemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,

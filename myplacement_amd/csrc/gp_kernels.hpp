@@ -78,25 +78,23 @@ constexpr int kMediumDegree = 32;
 int wireBlocks(int n_small, int n_medium, int n_large);
 void launchWire(const WireArgs& a, int count, hipStream_t s);
 
-// ---- density: counting sort by first bin row, band accumulation in LDS, bin metrics ------------
-constexpr int kSortThreads = 1024;   // particles per histogram / placement workgroup
-constexpr int kBandThreads = 1024;
-constexpr int kBandLookback = 4;     // bin rows a band looks back for particles that reach into it
-constexpr int kPermLayerShift = 30;  // row-order entries pack the particle layer above the id
+// ---- density: row order, fixed-point deposit, bin metrics ---------------------------------------
+constexpr int kSortThreads = 1024;    // particles per histogram / placement workgroup
+constexpr int kDepositThreads = 512;  // particles per deposit workgroup
+constexpr int kWindowCells = 2048;    // LDS window of a deposit workgroup, cells per layer
+constexpr int kPermLayerShift = 30;   // row-order entries pack the particle layer above the id
 constexpr unsigned kPermIdMask = (1U << kPermLayerShift) - 1U;
 struct DensityArgs {
     const double2* pos[kMaxEval];  // the order is built from pos[0]
     const double2* size;   // (w, h)
     const int* layer;      // 0 std, 1 macro, 2 filler
-    const unsigned char* tall;  // 1: taller than two bin rows (not row-sorted; visited by every band)
+    const unsigned char* tall;  // 1: taller than two bin rows (not row-sorted)
     const int* tall_ids;
     int n_tall;
     int n;
     Region r;
     double t;
     double fix, inv_fix;   // fixed-point scale: 2^40 per bin area
-    int band_rows, bands, lookback;
-    int margin[kMaxEval];  // rows the row order may be stale by
     int* key;              // [P]
     int* hist;             // [histogram blocks][ny]
     int* colprefix;        // [histogram blocks][ny]
@@ -104,14 +102,17 @@ struct DensityArgs {
     int* keystart;         // [ny + 1]
     unsigned* perm;        // [P] particle id | layer << kPermLayerShift, in row order
     double2* sorted_size;  // [P] particle sizes in row order
+    unsigned long long* grid[kMaxEval];  // [2][G] fixed point: movable + t*macro, filler; kept zero between evaluations
     const double* static_charge;   // [G] t*(fixed + dark)
     float* rho[kMaxEval];  // [nyp*nxp] density deviation, FP32, padded
     int ldrho;
-    double* partials[kMaxEval];  // one metrics row per band
+    double* partials[kMaxEval];  // one metrics row per bin block
 };
 int densityHistBlocks(int n);
+int densityBinBlocks(const Region& r);
 void launchDensitySort(const DensityArgs& a, hipStream_t s);
-void launchDensityBands(const DensityArgs& a, int count, hipStream_t s);
+// Deposit and bin metrics of `count` evaluations (partials: densityBinBlocks rows each).
+void launchDensity(const DensityArgs& a, int count, hipStream_t s);
 
 // ---- DCT products on MFMA (FP32) ---------------------------------------------------------------
 // kEpiInterleave stores C[i][j] at C + 2 * (i * ldc + j) (two products into one interleaved grid)

@@ -99,17 +99,6 @@ public:
         max_bin_ = std::max(p.bw, p.bh);
         clamp_ = ClampArgs{be_->sizes(), be_->region()};
         bin_height_ = be_->region().bh;
-        // The movement bounds behind the reuse of the density row order assume positions inside
-        // their clamp boxes; until the first accepted step (whose positions are clamped) every
-        // evaluation rebuilds the order if the starting positions are not.
-        const Region& r = be_->region();
-        for (int i = 0; i < P_ && !outside_; ++i) {
-            const double w = p.pw[i], h = p.ph[i];
-            const double lx = r.llx + w * 0.5, ux = r.urx - w * 0.5, ly = r.lly + h * 0.5, uy = r.ury - h * 0.5;
-            const double cx = lx <= ux ? std::max(lx, std::min(x[i].x, ux)) : (r.llx + r.urx) * 0.5;
-            const double cy = ly <= uy ? std::max(ly, std::min(x[i].y, uy)) : (r.lly + r.ury) * 0.5;
-            outside_ = cx != x[i].x || cy != x[i].y;
-        }
         const auto t0 = std::chrono::steady_clock::now();
         initialize();
         setup_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -185,12 +174,6 @@ private:
     // 2 proposal + candidate value evaluation.
     enum Bank : int { kBankCurrent = 0, kBankLook = 1, kBankCandidate = 2 };
 
-    // Bin rows of reuse margin covering a movement bound `d` (rounding-safe).
-    int marginRows(double d) const {
-        if (d <= 0.0) return 0;
-        const double rows = std::floor(d / bin_height_) + 1.0;
-        return rows > 1e6 ? 1000000 : static_cast<int>(rows);
-    }
     Backend::SessionOut sessionOut(double2* raw, double2* pre, const double2* prev, double2* ck) const {
         Backend::SessionOut so{precond_.get(), penalty_, 1.0 / strength(), raw, pre};
         so.prev_pos = prev;
@@ -220,7 +203,7 @@ private:
     }
     EvalRec gradEval() {
         const Backend::SessionOut so = sessionOut(Rc_.get(), Gc_.get(), nullptr, nullptr);
-        const Backend::EvalSpec spec{X_.get(), &so, kBankCurrent, Backend::kFreshOrder};
+        const Backend::EvalSpec spec{X_.get(), &so, kBankCurrent, true};
         be_->enqueueEvaluate(&spec, 1, smoothing_, true);
         dX_ = 0.0;
         return record(be_->fetchScalars(), true, kBankCurrent);
@@ -335,10 +318,12 @@ private:
     // positions and their gradient evaluation (slot 1, concurrently).  Work whose result the
     // control law then does not use is discarded and not counted as an evaluation.
     //
-    // The density bands reuse one particle row order while the evaluated positions stay within a
-    // bounded distance of where it was built: |X - S| <= dX_ (accumulated accepted displacements),
-    // |B - X| <= beta * mX_ (mX_ bounds |X - X_prev|), |Y - base| <= max_disp (clamping to the
-    // boxes is non-expansive once the positions are inside them).
+    // The density deposit visits particles in a row order built at earlier positions; it stays
+    // good (its workgroups' particles cover few bin rows) while positions stay close to where it was
+    // built.  The movement is bounded from the accepted displacements: |X - S| <= dX_, |B - X| <=
+    // beta * mX_ (mX_ bounds |X - X_prev|), |Y - base| <= max_disp, and the order is rebuilt when
+    // the bound passes the reuse limit.  The bound only steers speed: results do not depend on the
+    // order.
     void iterate() {
         hipStream_t s = be_->stream();
         const int it = result_.completed_iterations;
@@ -349,22 +334,17 @@ private:
         if (it > 0) smoothing_ = smoothingFromOverflow(last_overflow_);
         const double cap = o_.maximum_movement_in_bins * min_bin_;
         // rebuild the row order when the positions about to be evaluated may have moved more than
-        // the reuse limit from where it was built
-        // (a reuse limit of 0 rebuilds the order for every evaluation and runs them one after another)
+        // the reuse limit from where it was built (a limit of 0 rebuilds it for every evaluation,
+        // which then run one after another)
         const bool fresh = o_.density_order_reuse_rows <= 0.0;
-        if (!fresh && it > 0 && (outside_ || dX_ + look_drift + cap > o_.density_order_reuse_rows * bin_height_)) {
-            be_->timedBegin(kTController);
-            be_->sortParticles(X_.get());
-            be_->timedEnd(kTController);
-            dX_ = 0.0;
-        }
+        const bool reorder = fresh || dX_ + look_drift + cap > o_.density_order_reuse_rows * bin_height_;
+        if (it > 0 && reorder) dX_ = 0.0;
         // try_momentum implies it > 0: momentum_age_ only grows on an accepted step
         const Backend::SessionOut so_cur =
             sessionOut(Rc_.get(), Gc_.get(), try_momentum ? XP_.get() : nullptr, pending_ck_ ? pending_ck_->get() : nullptr);
         const Backend::SessionOut so_look = sessionOut(Rl_.get(), Gl_.get(), nullptr, nullptr);
-        Backend::EvalSpec specs[2] = {
-            {X_.get(), &so_cur, kBankCurrent, fresh ? Backend::kFreshOrder : marginRows(dX_)},
-            {B_.get(), &so_look, kBankLook, fresh ? Backend::kFreshOrder : marginRows(dX_ + look_drift)}};
+        Backend::EvalSpec specs[2] = {{X_.get(), &so_cur, kBankCurrent, reorder},
+                                      {B_.get(), &so_look, kBankLook, fresh}};
         if (try_momentum) {
             be_->timedBegin(kTController);
             launchLookahead(X_.get(), XP_.get(), beta, B_.get(), clamp_, P_, s);
@@ -424,7 +404,7 @@ private:
             ++result_.momentum_restarts;
         }
 
-        bool accepted = false, new_best = false, reordered = false;
+        bool accepted = false, new_best = false;
         int backtracks = 0;
         double used_step = 0.0, used_curvature = 0.0, used_disp = 0.0;
         EvalRec acc;
@@ -447,10 +427,8 @@ private:
                 be_->addReduction(kRedPropose, kBankCandidate, particleBlocks(P_));
                 be_->timedEnd(kTController);
                 const Backend::SessionOut so_cand = sessionOut(nullptr, nullptr, nullptr, nullptr);
-                const Backend::EvalSpec cand_spec{Y_.get(), &so_cand, kBankCandidate,
-                                                  fresh || outside_ ? Backend::kFreshOrder : marginRows(base_drift + max_disp)};
+                const Backend::EvalSpec cand_spec{Y_.get(), &so_cand, kBankCandidate, fresh};
                 be_->enqueueEvaluate(&cand_spec, 1, smoothing_, false);
-                reordered = reordered || outside_;
                 const HostScalars& h = be_->fetchScalars();
                 const double predicted = h.bank(kBankCandidate)[kPredicted];
                 const double disp = h.bank(kBankCandidate)[kMaxDisp];
@@ -503,8 +481,7 @@ private:
             X_.swapWith(Y_);
             XP_.swapWith(Y_);
             mX_ = used_disp + (momentum_used ? look_drift : 0.0);
-            dX_ = outside_ ? 0.0 : base_drift + used_disp;
-            outside_ = false;
+            dX_ = base_drift + used_disp;
             last_overflow_ = acc.s.optimizer_density.normalized_overflow;
             considerCheckpoint(acc, it + 1, new_best);
             ++result_.accepted_iterations;
@@ -513,7 +490,6 @@ private:
             history_.push_back(row(it + 1, acc, penalty_used, used_step, used_disp, gnorm, used_curvature, backtracks,
                                    restarted, true, new_best));
         } else {
-            if (reordered) dX_ = std::numeric_limits<double>::infinity();  // the order was rebuilt at a rejected Y
             momentum_age_ = 0;
             worse_streak_ = 0;
             if (cur.s.optimizer_density.normalized_overflow > o_.maximum_bin_overflow)
@@ -541,7 +517,6 @@ private:
     double min_bin_ = 0, max_bin_ = 0, bin_height_ = 1;
     double smoothing_ = 0, penalty_ = 1, reference_penalty_ = 1, initial_overflow_ = 0, last_overflow_ = 0, last_step_ = 0;
     double dX_ = 0.0, mX_ = 0.0;  // movement bounds: X from the row order's positions, X from X_prev
-    bool outside_ = false;        // starting positions outside their clamp boxes
     int momentum_age_ = 0, worse_streak_ = 0, feasible_refinement_ = 0;
     bool has_previous_ = false, stopped_ = false;
     EvalRec current_;

@@ -54,6 +54,8 @@ def oracle():
         lib.orc_global_place.argtypes = [C.POINTER(abi.MpDesign), C.POINTER(abi.MpOptions), _dp,
                                          C.POINTER(abi.MpResult), C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]
         lib.orc_random_placement.argtypes = [C.POINTER(abi.MpDesign), C.c_uint64, _dp]
+        lib.orc_quadratic_placement.argtypes = [C.POINTER(abi.MpDesign), C.POINTER(abi.MpQpOptions), _dp,
+                                                C.POINTER(abi.MpQpResult)]
         lib.orc_design_metrics.argtypes = [C.POINTER(abi.MpDesign), C.c_int, C.c_int, C.c_double, _dp,
                                            C.POINTER(abi.MpDensityMetrics)]
         lib.orc_transform_1d.argtypes = [C.c_int, C.c_int, _dp, _dp]
@@ -163,6 +165,16 @@ def random_placement(design, seed: int = 2026) -> np.ndarray:
     d = design.to_c()
     _check(oracle().orc_random_placement(C.byref(d), seed, abi.dptr(c)), oracle())
     return c
+
+
+def quadratic_placement(design, options=None):
+    """InitialPlacer quadraticPlacement (InitialPlacer.cpp:315-402) on the CPU; returns (centres, result)."""
+    o = abi.default_qp_options() if options is None else options
+    c = np.ascontiguousarray(design.center, dtype=np.float64).copy()
+    d = design.to_c(c)
+    r = abi.MpQpResult()
+    _check(oracle().orc_quadratic_placement(C.byref(d), C.byref(o), abi.dptr(c), C.byref(r)), oracle())
+    return c, r.as_dict()
 
 
 def design_metrics(design, center, cols, rows, t) -> tuple:

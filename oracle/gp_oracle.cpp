@@ -13,6 +13,7 @@
 //   adaptive controller                src/placement/AdaptiveGlobalPlacer.cpp:413-953
 //   HostLayout                         src/placement/cuda/CudaPlacementBackend.cu:784-839
 //   random initial placement           src/placement/InitialPlacer.cpp:102-120
+//   quadratic initial placement        src/placement/InitialPlacer.cpp:266-402 (BiCGSTAB restated)
 //
 // The reference CPU path uses FFTW r2r (REDFT10/REDFT01/RODFT01), which is not in this image.
 // Its transforms are restated here from their definitions: a direct O(N^2) sum for any N and a
@@ -1217,6 +1218,211 @@ double nowSeconds() {
 }  // namespace
 }  // namespace oracle
 
+namespace oracle {
+
+// ---- quadratic initial placement (InitialPlacer.cpp:266-402) -------------------------------------
+
+// Sparse symmetric system assembled from (row, col, value) entries; entries with equal (row, col)
+// are summed in the order they were added, as a triplet assembly does.
+struct Triplets {
+    std::vector<int64_t> row, col;
+    std::vector<double> val;
+    void add(int64_t r, int64_t c, double v) {
+        row.push_back(r);
+        col.push_back(c);
+        val.push_back(v);
+    }
+};
+
+struct Csr {
+    std::vector<int64_t> off;
+    std::vector<int64_t> col;
+    std::vector<double> val;
+    std::vector<double> multiply(const std::vector<double>& x) const {
+        std::vector<double> y(off.size() - 1, 0.0);
+        for (std::size_t r = 0; r + 1 < off.size(); ++r) {
+            double a = 0.0;
+            for (int64_t k = off[r]; k < off[r + 1]; ++k) a += val[k] * x[col[k]];
+            y[r] = a;
+        }
+        return y;
+    }
+};
+
+Csr assemble(std::size_t n, const Triplets& t) {
+    std::vector<std::size_t> order(t.val.size());
+    std::iota(order.begin(), order.end(), 0U);
+    std::stable_sort(order.begin(), order.end(), [&](std::size_t a, std::size_t b) {
+        return t.row[a] != t.row[b] ? t.row[a] < t.row[b] : t.col[a] < t.col[b];
+    });
+    Csr m;
+    m.off.assign(n + 1, 0);
+    for (std::size_t k = 0; k < order.size(); ++k) {
+        const std::size_t i = order[k];
+        if (!m.col.empty() && k > 0 && t.row[order[k - 1]] == t.row[i] && t.col[order[k - 1]] == t.col[i]) {
+            m.val.back() += t.val[i];
+            continue;
+        }
+        m.col.push_back(t.col[i]);
+        m.val.push_back(t.val[i]);
+        ++m.off[static_cast<std::size_t>(t.row[i]) + 1];
+    }
+    std::partial_sum(m.off.begin(), m.off.end(), m.off.begin());
+    return m;
+}
+
+inline double dotv(const std::vector<double>& a, const std::vector<double>& b) {
+    double s = 0.0;
+    for (std::size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+// Unpreconditioned BiCGSTAB (van der Vorst, SIAM J. Sci. Stat. Comput. 13(2), 1992) with the
+// conventions of Eigen 3.4's BiCGSTAB, which the reference calls with an identity preconditioner
+// (InitialPlacer.cpp:386-390): shadow residual r0 = b - A x0; stop when |r|^2 <= tol^2 |b|^2 or after
+// max_iterations; a zero right-hand side gives x = 0; when |rho| < eps^2 |r0|^2 the shadow residual
+// is reset to the current residual (and on the first such reset the iteration count restarts);
+// omega = 0 when |t| = 0.  Returns the iterations run and the final relative residual.
+int bicgstab(const Csr& A, const std::vector<double>& b, std::vector<double>& x, int max_iterations, double tol,
+             double& relative_residual) {
+    const std::size_t n = b.size();
+    const double b2 = dotv(b, b);
+    if (b2 == 0.0) {
+        std::fill(x.begin(), x.end(), 0.0);
+        relative_residual = 0.0;
+        return 0;
+    }
+    std::vector<double> r = A.multiply(x);
+    for (std::size_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    std::vector<double> r0 = r, p(n, 0.0), v(n, 0.0), s(n), t(n);
+    double r0_2 = dotv(r0, r0);
+    double rho = 1.0, alpha = 1.0, omega = 1.0;
+    const double tol2 = tol * tol * b2;
+    const double eps = std::numeric_limits<double>::epsilon();
+    int i = 0, restarts = 0;
+    while (dotv(r, r) > tol2 && i < max_iterations) {
+        const double rho_old = rho;
+        rho = dotv(r0, r);
+        if (std::abs(rho) < eps * eps * r0_2) {
+            r = A.multiply(x);
+            for (std::size_t k = 0; k < n; ++k) r[k] = b[k] - r[k];
+            r0 = r;
+            rho = r0_2 = dotv(r, r);
+            if (restarts++ == 0) i = 0;
+        }
+        const double beta = (rho / rho_old) * (alpha / omega);
+        for (std::size_t k = 0; k < n; ++k) p[k] = r[k] + beta * (p[k] - omega * v[k]);
+        v = A.multiply(p);
+        alpha = rho / dotv(r0, v);
+        for (std::size_t k = 0; k < n; ++k) s[k] = r[k] - alpha * v[k];
+        t = A.multiply(s);
+        const double t2 = dotv(t, t);
+        omega = t2 > 0.0 ? dotv(t, s) / t2 : 0.0;
+        for (std::size_t k = 0; k < n; ++k) {
+            x[k] += alpha * p[k] + omega * s[k];
+            r[k] = s[k] - omega * t[k];
+        }
+        ++i;
+    }
+    relative_residual = std::sqrt(dotv(r, r) / b2);
+    return i;
+}
+
+struct QuadraticRun {
+    int iterations = 0, solver_iterations = 0;
+    double residual[2] = {0.0, 0.0};
+};
+
+// quadraticPlacement, InitialPlacer.cpp:315-402, with addPairToQuadraticSystem :266-313.
+QuadraticRun quadraticPlacement(Db& db, const mp_qp_options& o) {
+    QuadraticRun run;
+    if (db.movable.empty()) return run;
+    const V2 centre = db.core.center();
+    for (const int64_t id : db.movable) {
+        db.c[static_cast<std::size_t>(id)] = centre;
+        db.clampModuleToCore(static_cast<std::size_t>(id));
+    }
+    const std::size_t n = db.movable.size();
+    std::vector<int64_t> index(db.modules(), -1);
+    for (std::size_t k = 0; k < n; ++k) index[static_cast<std::size_t>(db.movable[k])] = static_cast<int64_t>(k);
+    std::vector<double> xs(n), ys(n);
+    for (std::size_t k = 0; k < n; ++k) {
+        xs[k] = db.c[static_cast<std::size_t>(db.movable[k])].x;
+        ys[k] = db.c[static_cast<std::size_t>(db.movable[k])].y;
+    }
+    for (int outer = 0; outer < o.outer_iterations; ++outer) {
+        Triplets tx, ty;
+        std::vector<double> bx(n, 0.0), by(n, 0.0);
+        auto addPair = [&](int64_t lp, int64_t rp, double base) {  // :266-313
+            const int lm = db.pin_module[static_cast<std::size_t>(lp)], rm = db.pin_module[static_cast<std::size_t>(rp)];
+            if (lm == rm) return;
+            const int64_t li = index[static_cast<std::size_t>(lm)], ri = index[static_cast<std::size_t>(rm)];
+            if (li < 0 && ri < 0) return;
+            const V2 lo = transformOffset(db.pin_offset[static_cast<std::size_t>(lp)], db.orient[static_cast<std::size_t>(lm)]);
+            const V2 ro = transformOffset(db.pin_offset[static_cast<std::size_t>(rp)], db.orient[static_cast<std::size_t>(rm)]);
+            const V2 lpos = db.c[static_cast<std::size_t>(lm)] + lo, rpos = db.c[static_cast<std::size_t>(rm)] + ro;
+            const double wx = base / std::max(std::abs(lpos.x - rpos.x), o.minimum_distance);
+            const double wy = base / std::max(std::abs(lpos.y - rpos.y), o.minimum_distance);
+            if (li >= 0 && ri >= 0) {
+                tx.add(li, li, wx); ty.add(li, li, wy);
+                tx.add(ri, ri, wx); ty.add(ri, ri, wy);
+                tx.add(li, ri, -wx); tx.add(ri, li, -wx);
+                ty.add(li, ri, -wy); ty.add(ri, li, -wy);
+                const V2 d = lo - ro;
+                bx[li] -= wx * d.x; bx[ri] += wx * d.x;
+                by[li] -= wy * d.y; by[ri] += wy * d.y;
+            } else if (li >= 0) {
+                tx.add(li, li, wx); ty.add(li, li, wy);
+                bx[li] += wx * (db.c[static_cast<std::size_t>(rm)].x + ro.x - lo.x);
+                by[li] += wy * (db.c[static_cast<std::size_t>(rm)].y + ro.y - lo.y);
+            } else {
+                tx.add(ri, ri, wx); ty.add(ri, ri, wy);
+                bx[ri] += wx * (db.c[static_cast<std::size_t>(lm)].x + lo.x - ro.x);
+                by[ri] += wy * (db.c[static_cast<std::size_t>(lm)].y + lo.y - ro.y);
+            }
+        };
+        for (std::size_t net = 0; net < db.net_weight.size(); ++net) {
+            const int64_t b = db.net_start[net], e = db.net_start[net + 1];
+            const int64_t degree = e - b;
+            if (degree < 2) continue;
+            const double w = db.net_weight[net];
+            if (static_cast<uint64_t>(degree) <= o.clique_degree_limit) {
+                const double cw = 2.0 * w / static_cast<double>(degree - 1);
+                for (int64_t l = b; l < e; ++l)
+                    for (int64_t r = l + 1; r < e; ++r) addPair(l, r, cw);
+            } else {
+                const double sw = w / static_cast<double>(degree - 1);
+                for (int64_t k = b + 1; k < e; ++k) addPair(b, k, sw);
+            }
+        }
+        constexpr double kAnchor = 1e-4;
+        for (std::size_t k = 0; k < n; ++k) {
+            const V2 c = db.c[static_cast<std::size_t>(db.movable[k])];
+            tx.add(static_cast<int64_t>(k), static_cast<int64_t>(k), kAnchor);
+            ty.add(static_cast<int64_t>(k), static_cast<int64_t>(k), kAnchor);
+            bx[k] += kAnchor * c.x;
+            by[k] += kAnchor * c.y;
+        }
+        const Csr ax = assemble(n, tx), ay = assemble(n, ty);
+        run.solver_iterations += bicgstab(ax, bx, xs, o.solver_iterations, o.tolerance, run.residual[0]);
+        run.solver_iterations += bicgstab(ay, by, ys, o.solver_iterations, o.tolerance, run.residual[1]);
+        for (std::size_t k = 0; k < n; ++k)
+            if (!std::isfinite(xs[k]) || !std::isfinite(ys[k]))
+                throw std::runtime_error("Quadratic placement failed to produce finite coordinates.");
+        for (std::size_t k = 0; k < n; ++k) {
+            const std::size_t id = static_cast<std::size_t>(db.movable[k]);
+            db.c[id] = {xs[k], ys[k]};
+            db.clampModuleToCore(id);
+            xs[k] = db.c[id].x;
+            ys[k] = db.c[id].y;
+        }
+        run.iterations = outer + 1;
+    }
+    return run;
+}
+
+}  // namespace oracle
+
 using namespace oracle;
 
 struct orc_session {
@@ -1324,6 +1530,34 @@ int orc_random_placement(const mp_design* design, uint64_t seed, double* centers
             centers[2 * m] = db.c[m].x;
             centers[2 * m + 1] = db.c[m].y;
         }
+    });
+}
+
+// InitialPlacer::run with the quadratic method (InitialPlacer.cpp:315-402, 423-436).
+int orc_quadratic_placement(const mp_design* design, const mp_qp_options* options, double* centers,
+                            mp_qp_result* result) {
+    return guarded([&] {
+        if (!options || !centers) throw std::invalid_argument("null options or centres");
+        const mp_qp_options& o = *options;
+        if (o.outer_iterations <= 0 || o.solver_iterations <= 0 || !std::isfinite(o.tolerance) || o.tolerance <= 0.0 ||
+            !std::isfinite(o.minimum_distance) || o.minimum_distance <= 0.0)  // :52-57
+            throw std::invalid_argument("Quadratic initial-placement options are outside their valid range.");
+        const auto t0 = std::chrono::steady_clock::now();
+        Db db = loadDesign(design);
+        mp_qp_result r{};
+        r.hpwl_before = designHpwl(db);
+        const QuadraticRun run = quadraticPlacement(db, o);
+        r.hpwl_after = designHpwl(db);
+        r.iterations = run.iterations;
+        r.solver_iterations = run.solver_iterations;
+        r.relative_residual[0] = run.residual[0];
+        r.relative_residual[1] = run.residual[1];
+        r.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (std::size_t m = 0; m < db.modules(); ++m) {
+            centers[2 * m] = db.c[m].x;
+            centers[2 * m + 1] = db.c[m].y;
+        }
+        if (result) *result = r;
     });
 }
 

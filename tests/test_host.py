@@ -40,13 +40,13 @@ def test_struct_sizes_match_c(tmp_path):
     src.write_text('#include <stdio.h>\n#include "mp_gp_backend.h"\nint main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
                    'sizeof(mp_design),sizeof(mp_gp_static),sizeof(mp_gp_density_metrics),sizeof(mp_gp_scalars),'
                    'sizeof(mp_gp_options),sizeof(mp_gp_history_row),sizeof(mp_gp_result),sizeof(mp_gp_prepared_info),'
-                   'offsetof(mp_gp_options, seed));return 0;}\n'.replace("#include <stdio.h>", "#include <stdio.h>\n#include <stddef.h>"))
+                   'offsetof(mp_gp_options, seed));printf("%zu %zu\\n",sizeof(mp_qp_options),sizeof(mp_qp_result));return 0;}\n'.replace("#include <stdio.h>", "#include <stdio.h>\n#include <stddef.h>"))
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [C.sizeof(abi.MpDesign), C.sizeof(abi.MpStatic), C.sizeof(abi.MpDensityMetrics), C.sizeof(abi.MpScalars),
             C.sizeof(abi.MpOptions), C.sizeof(abi.MpHistoryRow), C.sizeof(abi.MpResult), C.sizeof(abi.MpPreparedInfo),
-            abi.MpOptions.seed.offset]
+            abi.MpOptions.seed.offset, C.sizeof(abi.MpQpOptions), C.sizeof(abi.MpQpResult)]
     assert got == want
 
 
@@ -55,6 +55,21 @@ def test_default_options_match_reference_defaults():
     b = abi.default_options()
     for name, _ in abi.MpOptions._fields_:
         assert getattr(a, name) == getattr(b, name), name
+    q = abi.MpQpOptions()
+    hb.library().mp_gp_default_qp_options(C.byref(q))
+    r = abi.default_qp_options()
+    for name, _ in abi.MpQpOptions._fields_:  # InitialPlacer.hpp:21-25
+        assert getattr(q, name) == getattr(r, name), name
+    assert (r.outer_iterations, r.solver_iterations, r.tolerance, r.minimum_distance, r.clique_degree_limit) == \
+        (5, 100, 1e-5, 1.0, 96)
+
+
+def test_quadratic_placement_rejects_bad_options_without_a_device():
+    d = design.plain_design()
+    with pytest.raises(ValueError):
+        hb.quadratic_placement(d, abi.default_qp_options(tolerance=0.0))
+    with pytest.raises(ValueError):
+        hb.quadratic_placement(d, abi.default_qp_options(outer_iterations=0))
 
 
 def _designs():

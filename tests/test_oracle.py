@@ -17,6 +17,11 @@ Parity status of each oracle function (oracle/gp_oracle.cpp), and what pins it:
   electrostatic energy               equals the DCT-basis (Parseval) sum computed with scipy    -> pinned
   random initial placement           mt19937_64 10000th output (C++ standard), and the product's
                                      independent implementation bit-equal             -> pinned
+  quadratic initial placement        one-cell closed form; one outer iteration against a dense
+                                     system built here from the net list and solved by numpy
+                                     (LAPACK); BiCGSTAB on a random SPD system against numpy;
+                                     the reference's own invariants (PlacementFlowTests.cpp:121-129)
+                                                                                      -> pinned
   adaptive controller                no numeric pin exists in the reference; invariants of
                                      PlacementFlowTests.cpp:153-216 only              -> parity unpinned
 """
@@ -391,3 +396,119 @@ def test_invalid_controller_option_rejected():
     o = abi.default_options(iterations=1, bins_x=8, bins_y=8, feasible_refinement_iterations=0)
     with pytest.raises(ValueError):
         ob.global_place(d, o)
+
+
+# ---- quadratic initial placement (InitialPlacer.cpp:266-402) ---------------------------------------
+
+def _tiny_design(cells_xy, terminals_xy, nets, W=100.0, H=40.0, offsets=None):
+    """Unit cells and 1x1 terminals on one 40-unit-high row region, nets as lists of module ids."""
+    m = len(cells_xy) + len(terminals_xy)
+    pins = [p for n in nets for p in n]
+    start = np.concatenate([[0], np.cumsum([len(n) for n in nets])])
+    off = np.zeros((len(pins), 2)) if offsets is None else np.asarray(offsets, dtype=np.float64)
+    return design.Design(
+        width=np.ones(m), height=np.ones(m), center=np.array(list(cells_xy) + list(terminals_xy), dtype=np.float64),
+        orientation=np.zeros(m, dtype=np.int32),
+        is_terminal=np.array([0] * len(cells_xy) + [1] * len(terminals_xy), dtype=np.uint8),
+        is_fixed=np.array([0] * len(cells_xy) + [1] * len(terminals_xy), dtype=np.uint8),
+        net_weight=np.ones(len(nets)), net_pin_start=start, pin_module=np.array(pins, dtype=np.int32),
+        pin_offset=off, rows=np.array([[0.0, H, 0.0, 1.0, W]]))
+
+
+def test_quadratic_one_cell_closed_form():
+    """One cell tied to one terminal: each outer iteration solves (w + a) x = w X + a x_prev with the
+    bound-to-bound weight w = 2 / max(|x_prev - X|, 1) (clique weight 2w/(d-1) = 2) and anchor a = 1e-4."""
+    d = _tiny_design([(10.0, 10.0)], [(80.0, 30.0)], [[0, 1]])
+    o = abi.default_qp_options(outer_iterations=1, tolerance=1e-14)
+    c, r = ob.quadratic_placement(d, o)
+    x0, y0 = 50.0, 20.0  # core centre
+    wx, wy = 2.0 / max(abs(x0 - 80.0), 1.0), 2.0 / max(abs(y0 - 30.0), 1.0)
+    ex = (wx * 80.0 + 1e-4 * x0) / (wx + 1e-4)
+    ey = (wy * 30.0 + 1e-4 * y0) / (wy + 1e-4)
+    assert abs(c[0, 0] - ex) < 1e-10 and abs(c[0, 1] - ey) < 1e-10
+    assert r["iterations"] == 1 and np.all(c[1] == [80.0, 30.0])
+
+
+def _dense_system(d, cx, cy, limit=96, min_dist=1.0):
+    """The clique / star bound-to-bound system written densely from the net list."""
+    mov = d.movable_ids()
+    idx = {int(m): k for k, m in enumerate(mov)}
+    n = len(mov)
+    Ax, Ay, bx, by = np.zeros((n, n)), np.zeros((n, n)), np.zeros(n), np.zeros(n)
+    pos = lambda p: (cx[d.pin_module[p]] + d.pin_offset[p, 0], cy[d.pin_module[p]] + d.pin_offset[p, 1])
+    for k in range(d.net_count):
+        pins = list(range(d.net_pin_start[k], d.net_pin_start[k + 1]))
+        deg = len(pins)
+        if deg < 2:
+            continue
+        if deg <= limit:
+            pairs, base = [(a, b) for i, a in enumerate(pins) for b in pins[i + 1:]], 2.0 * d.net_weight[k] / (deg - 1)
+        else:
+            pairs, base = [(pins[0], b) for b in pins[1:]], d.net_weight[k] / (deg - 1)
+        for a, b in pairs:
+            ma, mb = d.pin_module[a], d.pin_module[b]
+            if ma == mb or (ma not in idx and mb not in idx):
+                continue
+            (ax_, ay_), (bx_, by_) = pos(a), pos(b)
+            for A, rhs, pa, pb, oa, ob_, ca, cb in ((Ax, bx, ax_, bx_, d.pin_offset[a, 0], d.pin_offset[b, 0], cx, cx),
+                                                    (Ay, by, ay_, by_, d.pin_offset[a, 1], d.pin_offset[b, 1], cy, cy)):
+                w = base / max(abs(pa - pb), min_dist)
+                # energy w (xa + oa - xb - ob)^2 with fixed terms moved to the right-hand side
+                if ma in idx and mb in idx:
+                    i, j = idx[ma], idx[mb]
+                    A[i, i] += w; A[j, j] += w; A[i, j] -= w; A[j, i] -= w
+                    rhs[i] -= w * (oa - ob_); rhs[j] += w * (oa - ob_)
+                elif ma in idx:
+                    i = idx[ma]
+                    A[i, i] += w
+                    rhs[i] += w * (cb[mb] + ob_ - oa)
+                else:
+                    j = idx[mb]
+                    A[j, j] += w
+                    rhs[j] += w * (ca[ma] + oa - ob_)
+    for k, m in enumerate(mov):
+        Ax[k, k] += 1e-4; Ay[k, k] += 1e-4
+        bx[k] += 1e-4 * cx[m]; by[k] += 1e-4 * cy[m]
+    return Ax, Ay, bx, by
+
+
+def test_quadratic_one_iteration_matches_dense_solve():
+    d = design.plain_design(cells=120, nets=150, row_count=10, sites=100, terminals=16)
+    o = abi.default_qp_options(outer_iterations=1, solver_iterations=2000, tolerance=1e-13)
+    c, r = ob.quadratic_placement(d, o)
+    mov = d.movable_ids()
+    core = d.core_region()
+    cx, cy = d.center[:, 0].copy(), d.center[:, 1].copy()
+    cx[mov], cy[mov] = (core[0] + core[2]) / 2, (core[1] + core[3]) / 2
+    Ax, Ay, bx, by = _dense_system(d, cx, cy)
+    sx, sy = np.linalg.solve(Ax, bx), np.linalg.solve(Ay, by)
+    w = d.width[mov] / 2
+    h = d.height[mov] / 2
+    ex = np.clip(sx, core[0] + w, core[2] - w)
+    ey = np.clip(sy, core[1] + h, core[3] - h)
+    assert r["relative_residual"][0] < 1e-12 and r["relative_residual"][1] < 1e-12
+    assert np.abs(c[mov, 0] - ex).max() < 1e-6 and np.abs(c[mov, 1] - ey).max() < 1e-6
+
+
+def test_quadratic_invariants_like_the_reference():
+    """PlacementFlowTests.cpp:121-129: three outer iterations on tiny.aux keep every movable module in
+    the core with a finite HPWL; the plain design's HPWL is below that of the centre start."""
+    d = design.read_bookshelf(os.path.join(BOOK, "tiny.aux"))
+    c, r = ob.quadratic_placement(d, abi.default_qp_options(outer_iterations=3))
+    core = d.core_region()
+    mov = d.movable_ids()
+    assert r["iterations"] == 3 and np.isfinite(r["hpwl_after"])
+    assert np.all(c[mov, 0] - d.width[mov] / 2 >= core[0] - 1e-9)
+    assert np.all(c[mov, 0] + d.width[mov] / 2 <= core[2] + 1e-9)
+    assert np.all(c[mov, 1] - d.height[mov] / 2 >= core[1] - 1e-9)
+    assert np.all(c[mov, 1] + d.height[mov] / 2 <= core[3] + 1e-9)
+    p = design.plain_design()
+    cp, rp = ob.quadratic_placement(p)
+    centre = p.center.copy()
+    pm = p.movable_ids()
+    pc = p.core_region()
+    centre[pm] = [(pc[0] + pc[2]) / 2, (pc[1] + pc[3]) / 2]
+    h_centre, _ = ob.design_metrics(p, centre, 8, 8, 0.9)
+    assert rp["hpwl_after"] < h_centre
+    with pytest.raises(ValueError):
+        ob.quadratic_placement(p, abi.default_qp_options(minimum_distance=0.0))
