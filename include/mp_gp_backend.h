@@ -139,7 +139,7 @@ typedef struct mp_gp_options {
     int32_t device;                        /* HIP device ordinal (0) */
     int32_t dct_precision;                 /* MP_DCT_* */
     uint64_t maximum_device_bytes;         /* budget for device allocations */
-    double density_order_reuse_rows;       /* 2.5: the density bands reuse their particle row order
+    double density_order_reuse_rows;       /* 2.5: the density deposit reuses its particle row order
                                               until positions may have moved this many bin rows;
                                               0 rebuilds it for every evaluation (results are
                                               identical either way; this is a speed setting) */

@@ -188,10 +188,10 @@ def test_adaptive_run_follows_oracle_then_matches_ensemble(ob):
 
 @pytest.mark.parametrize("start", ["inside", "outside"])
 def test_reused_row_order_is_bit_identical_to_rebuilding_it(start):
-    """The density bands may reuse a particle row order built at earlier positions (within a
-    tracked movement margin) and the current and look-ahead evaluations run concurrently.  Neither
-    may change a single bit: the fixed-point density sum does not depend on the order, and every
-    other reduction has a fixed order.  So whole runs with the order rebuilt for every evaluation
+    """The density deposit may reuse a particle row order built at earlier positions, and the
+    current and look-ahead evaluations run as one batched chain.  Neither may change a single bit:
+    the fixed-point density sum does not depend on the order, and every other reduction has a fixed
+    order.  So whole runs with the order rebuilt for every evaluation
     (reuse limit 0, evaluations one after another), the default limit, and a limit so large that
     the order is never rebuilt must agree exactly.  The "outside" start places some cells beyond
     the core, where the movement bounds do not hold until the first accepted step."""
