@@ -15,6 +15,7 @@
  *   GlobalPlacementOptions (adaptive)     include/myplacement/placement/GlobalPlacer.hpp:26-80
  *   GlobalPlacementIteration / Result     include/myplacement/placement/GlobalPlacer.hpp:82-129
  *   runAdaptiveGlobalPlacement            src/placement/AdaptiveGlobalPlacer.cpp:652-953
+ *   randomPlacement / quadraticPlacement  src/placement/InitialPlacer.cpp:102-120, 266-402
  */
 #ifndef MP_GP_BACKEND_H
 #define MP_GP_BACKEND_H
@@ -307,6 +308,38 @@ void mp_gp_session_destroy(mp_gp_session* session);
  * backend: prepare, create, loop, restore.  centers: [2M] in/out. history may be NULL. */
 mp_gp_status mp_gp_global_place(const mp_design* design, const mp_gp_options* options, double* centers,
                                  mp_gp_result* result, mp_gp_history_row* history, int64_t history_capacity);
+
+/* ---- placement service (no counterpart in the reference, which places one design per process) --
+ * Jobs of mixed sizes are admitted against a device-memory budget into `max_active_jobs` slots;
+ * each runs mp_gp_global_place to completion on its own stream, so admitted jobs overlap on the
+ * device.  A job's result does not depend on what runs beside it.                               */
+typedef struct mp_gp_server_options {
+    int32_t device;                /* HIP device ordinal (0) */
+    int32_t max_active_jobs;       /* 8: jobs running at once */
+    uint64_t memory_budget_bytes;  /* 200 GiB: admission budget for the jobs' device footprints */
+} mp_gp_server_options;
+
+typedef struct mp_gp_job_report {
+    double submit_seconds, start_seconds, finish_seconds;  /* since the service was created */
+    double queue_seconds;          /* submit -> admitted */
+    double latency_seconds;        /* submit -> result */
+    uint64_t estimated_device_bytes;
+    mp_gp_result result;
+} mp_gp_job_report;
+
+typedef struct mp_gp_server mp_gp_server;
+
+void mp_gp_default_server_options(mp_gp_server_options* options);
+/* Upper bound of the device bytes one mp_gp_global_place run of this design reserves. */
+uint64_t mp_gp_estimate_device_bytes(const mp_design* design, const mp_gp_options* options);
+mp_gp_status mp_gp_server_create(const mp_gp_server_options* options, mp_gp_server** out);
+/* Copies the design; the caller may free it on return.  job: id for mp_gp_server_wait. */
+mp_gp_status mp_gp_server_submit(mp_gp_server* server, const mp_design* design, const mp_gp_options* options,
+                                 int64_t* job);
+/* Blocks until the job is done; centers: [2M] placed centres (may be NULL).  Collects the job. */
+mp_gp_status mp_gp_server_wait(mp_gp_server* server, int64_t job, double* centers, mp_gp_job_report* report);
+/* Finishes the running jobs, drops the waiting ones and frees the service. */
+void mp_gp_server_destroy(mp_gp_server* server);
 
 #ifdef __cplusplus
 }

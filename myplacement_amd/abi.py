@@ -216,6 +216,31 @@ class MpQpResult(C.Structure):
         return d
 
 
+class MpServerOptions(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("max_active_jobs", C.c_int32),
+        ("memory_budget_bytes", C.c_uint64),
+    ]
+
+
+class MpJobReport(C.Structure):
+    _fields_ = [
+        ("submit_seconds", C.c_double),
+        ("start_seconds", C.c_double),
+        ("finish_seconds", C.c_double),
+        ("queue_seconds", C.c_double),
+        ("latency_seconds", C.c_double),
+        ("estimated_device_bytes", C.c_uint64),
+        ("result", MpResult),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "result"}
+        d["result"] = self.result.as_dict()
+        return d
+
+
 class MpPreparedInfo(C.Structure):
     _fields_ = [
         ("movable_count", C.c_int64),

@@ -58,6 +58,14 @@ def library():
         lib.mp_gp_prepared_arrays.argtypes = [V, V, V, V]
         lib.mp_gp_random_placement.argtypes = [C.POINTER(abi.MpDesign), C.c_uint64, V]
         lib.mp_gp_default_qp_options.argtypes = [C.POINTER(abi.MpQpOptions)]
+        lib.mp_gp_default_server_options.argtypes = [C.POINTER(abi.MpServerOptions)]
+        lib.mp_gp_estimate_device_bytes.argtypes = [C.POINTER(abi.MpDesign), C.POINTER(abi.MpOptions)]
+        lib.mp_gp_estimate_device_bytes.restype = C.c_uint64
+        lib.mp_gp_server_create.argtypes = [C.POINTER(abi.MpServerOptions), C.POINTER(V)]
+        lib.mp_gp_server_submit.argtypes = [V, C.POINTER(abi.MpDesign), C.POINTER(abi.MpOptions),
+                                            C.POINTER(C.c_int64)]
+        lib.mp_gp_server_wait.argtypes = [V, C.c_int64, V, C.POINTER(abi.MpJobReport)]
+        lib.mp_gp_server_destroy.argtypes = [V]
         lib.mp_gp_quadratic_placement.argtypes = [C.POINTER(abi.MpDesign), C.POINTER(abi.MpQpOptions), V,
                                                   C.POINTER(abi.MpQpResult)]
         lib.mp_gp_session_create.argtypes = [V, C.POINTER(abi.MpOptions), V, C.POINTER(V)]
@@ -381,3 +389,49 @@ def global_place(design, options: abi.MpOptions, center=None, history_capacity: 
     _raise(library().mp_gp_global_place(C.byref(d), C.byref(options), _p(c), C.byref(r), hist, history_capacity))
     n = min(r.completed_iterations, history_capacity)
     return c, r.as_dict(), [hist[i].as_dict() for i in range(n)]
+
+
+def estimate_device_bytes(design, options: abi.MpOptions) -> int:
+    """Upper bound of the device bytes one global_place run of `design` reserves."""
+    d = design.to_c()
+    return int(library().mp_gp_estimate_device_bytes(C.byref(d), C.byref(options)))
+
+
+class PlacementServer:
+    """The placement service of mp_gp_server_*: jobs of mixed sizes admitted against a device
+    memory budget into `max_active_jobs` slots, each run to completion on its own stream.  `wait`
+    releases the GIL while it blocks, so several Python threads may wait at once."""
+
+    def __init__(self, max_active_jobs: int = 8, memory_budget_bytes: int = 200 << 30, device: int = 0):
+        o = abi.MpServerOptions()
+        library().mp_gp_default_server_options(C.byref(o))
+        o.max_active_jobs = max_active_jobs
+        o.memory_budget_bytes = memory_budget_bytes
+        o.device = device
+        h = C.c_void_p()
+        _raise(library().mp_gp_server_create(C.byref(o), C.byref(h)))
+        self._h = h
+
+    def submit(self, design, options: abi.MpOptions) -> int:
+        d = design.to_c()
+        job = C.c_int64()
+        _raise(library().mp_gp_server_submit(self._h, C.byref(d), C.byref(options), C.byref(job)))
+        return job.value
+
+    def wait(self, job: int, module_count: int):
+        """Blocks until `job` is done; returns (centres [M, 2], report dict)."""
+        c = np.zeros((module_count, 2))
+        r = abi.MpJobReport()
+        _raise(library().mp_gp_server_wait(self._h, job, _p(c), C.byref(r)))
+        return c, r.as_dict()
+
+    def close(self) -> None:
+        if self._h:
+            library().mp_gp_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

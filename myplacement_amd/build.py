@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmp_gp_hip.so")
-SOURCES = ["gp_kernels.hip", "gp_backend.cpp", "gp_host.cpp", "gp_session.cpp", "gp_quadratic.hip"]
+SOURCES = ["gp_kernels.hip", "gp_backend.cpp", "gp_host.cpp", "gp_session.cpp", "gp_quadratic.hip", "gp_serve.cpp"]
 ARCH = os.environ.get("MP_GP_ARCH", "gfx950")
 REFERENCE = "/root/reference"
 
@@ -59,7 +59,7 @@ def build_hip(verbose: bool = False) -> str:
     if failed:
         raise RuntimeError("HIP build failed:\n" + "\n".join(failed))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("HIP link failed:\n" + r.stdout + r.stderr)

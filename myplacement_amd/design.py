@@ -416,3 +416,26 @@ def large_design(seed: int = 2026, cells: int = 210_000, nets: int = 220_000, ro
     degrees[hf] = rng.integers(50, 501, size=int(hf.sum()))
     hosts = np.arange(cells + macros)
     return _assemble(rng, width, height, center, is_term, is_term, degrees, hosts, rows)
+
+
+def served_bins(cells: int) -> int:
+    """Bin grid side of a served or batched job: the largest of 32, 64, 128, 256 that keeps at least
+    4 cells per bin on average (32 when none does)."""
+    side = 32
+    for s_ in (64, 128, 256):
+        if cells / (s_ * s_) >= 4.0:
+            side = s_
+    return side
+
+
+def served_design(seed: int, cells: int) -> Design:
+    """One job of the served mixed-size stream: the large design's recipe scaled to `cells` movable
+    cells -- nets = 1.05 x cells, rows and sites scaled by sqrt(cells / 210,000) from 1,200 x 1,200
+    (at least 40), max(1, round(12 x cells / 210,000)) macros covering 15 % of the core -- with a
+    random initial placement from the same seed."""
+    scale = math.sqrt(cells / 210_000.0)
+    side = max(40, int(round(1200 * scale)))
+    macros = max(1, int(round(12 * cells / 210_000)))
+    d = large_design(seed=seed, cells=cells, nets=int(round(1.05 * cells)), row_count=side, sites=side,
+                     macros=macros)
+    return d

@@ -40,13 +40,14 @@ def test_struct_sizes_match_c(tmp_path):
     src.write_text('#include <stdio.h>\n#include "mp_gp_backend.h"\nint main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
                    'sizeof(mp_design),sizeof(mp_gp_static),sizeof(mp_gp_density_metrics),sizeof(mp_gp_scalars),'
                    'sizeof(mp_gp_options),sizeof(mp_gp_history_row),sizeof(mp_gp_result),sizeof(mp_gp_prepared_info),'
-                   'offsetof(mp_gp_options, seed));printf("%zu %zu\\n",sizeof(mp_qp_options),sizeof(mp_qp_result));return 0;}\n'.replace("#include <stdio.h>", "#include <stdio.h>\n#include <stddef.h>"))
+                   'offsetof(mp_gp_options, seed));printf("%zu %zu %zu %zu\\n",sizeof(mp_qp_options),sizeof(mp_qp_result),sizeof(mp_gp_server_options),sizeof(mp_gp_job_report));return 0;}\n'.replace("#include <stdio.h>", "#include <stdio.h>\n#include <stddef.h>"))
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [C.sizeof(abi.MpDesign), C.sizeof(abi.MpStatic), C.sizeof(abi.MpDensityMetrics), C.sizeof(abi.MpScalars),
             C.sizeof(abi.MpOptions), C.sizeof(abi.MpHistoryRow), C.sizeof(abi.MpResult), C.sizeof(abi.MpPreparedInfo),
-            abi.MpOptions.seed.offset, C.sizeof(abi.MpQpOptions), C.sizeof(abi.MpQpResult)]
+            abi.MpOptions.seed.offset, C.sizeof(abi.MpQpOptions), C.sizeof(abi.MpQpResult),
+            C.sizeof(abi.MpServerOptions), C.sizeof(abi.MpJobReport)]
     assert got == want
 
 
@@ -145,3 +146,10 @@ def test_parse_compute_backend():
     assert hb.parse_compute_backend("auto") is hb.ComputeBackend.AUTO
     with pytest.raises(ValueError):
         hb.parse_compute_backend("tpu")
+
+
+def test_device_footprint_estimate_grows_with_the_design():
+    o = abi.default_options(bins_x=64, bins_y=64)
+    small = hb.estimate_device_bytes(design.plain_design(cells=500, nets=600), o)
+    large = hb.estimate_device_bytes(design.plain_design(cells=4000, nets=4600), o)
+    assert 0 < small < large
