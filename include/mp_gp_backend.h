@@ -287,8 +287,12 @@ mp_gp_status mp_gp_session_finish(mp_gp_session* session, double* centers, mp_gp
 /* History rows so far; returns the row count when rows == NULL. */
 int64_t mp_gp_session_history(const mp_gp_session* session, mp_gp_history_row* rows, int64_t capacity);
 
-/* Current particle positions [2P] (host copy). */
+/* Current particle positions [2P] (host copy), in the prepared particle order. */
 mp_gp_status mp_gp_session_positions(mp_gp_session* session, double* xy);
+
+/* The session keeps its particles renumbered on the device (along a Z-order curve of the starting
+ * positions, for locality).  order[k] = the prepared particle index of device particle k. */
+mp_gp_status mp_gp_session_particle_order(const mp_gp_session* session, int64_t* order);
 
 /* Per-kernel timing of the session's stream with HIP events (enable, then read).  enable: 0 off,
  * otherwise a bit mask of the kernel classes to time (bit i: the i-th name; -1: all).  The read
@@ -299,7 +303,8 @@ mp_gp_status mp_gp_session_enable_kernel_timing(mp_gp_session* session, int enab
 int32_t mp_gp_session_kernel_timing(mp_gp_session* session, char* names, size_t names_len, double* total_us,
                                     int64_t* launches, double* work, int32_t capacity);
 
-/* Access to the backend owned by a session (for evaluate-level checks). */
+/* Access to the backend owned by a session (for evaluate-level checks); it takes and returns
+ * particles in the device order of mp_gp_session_particle_order. */
 mp_gp_backend* mp_gp_session_backend(mp_gp_session* session);
 
 void mp_gp_session_destroy(mp_gp_session* session);

@@ -77,6 +77,7 @@ def library():
         lib.mp_gp_session_enable_kernel_timing.argtypes = [V, C.c_int]
         lib.mp_gp_session_kernel_timing.argtypes = [V, C.c_char_p, C.c_size_t, V, V, V, C.c_int32]
         lib.mp_gp_session_kernel_timing.restype = C.c_int32
+        lib.mp_gp_session_particle_order.argtypes = [V, V]
         lib.mp_gp_session_backend.argtypes = [V]
         lib.mp_gp_session_backend.restype = V
         lib.mp_gp_session_destroy.argtypes = [V]
@@ -335,7 +336,7 @@ class PlacementSession:
         _raise(library().mp_gp_session_positions(self._h, _p(xy)))
         return xy
 
-    TIMING_CLASSES = ("k_wire", "k_dband", "k_density_sort", "k_gemm", "k_gather", "controller")
+    TIMING_CLASSES = ("k_wire", "k_density", "k_density_sort", "k_gemm", "k_gather", "controller")
 
     def enable_kernel_timing(self, on: bool = True, classes=None) -> None:
         """Times the named kernel classes (all when `classes` is None) with HIP events."""
@@ -355,7 +356,14 @@ class PlacementSession:
         keys = names.value.decode().split(",")
         return {k: dict(total_us=float(tot[i]), launches=int(cnt[i]), work=float(work[i])) for i, k in enumerate(keys[:n])}
 
+    def particle_order(self) -> np.ndarray:
+        """order[k] = prepared particle index of the session's device particle k."""
+        o = np.zeros(int(self._prepared.info.particle_count), dtype=np.int64)
+        _raise(library().mp_gp_session_particle_order(self._h, _p(o)))
+        return o
+
     def backend(self) -> HipPlacementBackend:
+        """The session's backend; it evaluates positions in the device order of particle_order()."""
         b = HipPlacementBackend(C.c_void_p(library().mp_gp_session_backend(self._h)), owned=False)
         b._particles = int(self._prepared.info.particle_count)
         return b

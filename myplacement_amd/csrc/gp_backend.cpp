@@ -108,9 +108,16 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     if (N_ > 0 && (L.net_offsets == nullptr || L.net_offsets[0] != 0 || L.net_offsets[N_] != I_))
         throw InvalidArgument("net offsets do not cover the pin array");
     // Internal net order for the wirelength kernel: degree classes 2..8, 9..32, >32 (nets with fewer
-    // than two pins contribute nothing and are dropped, .cu:251-253, :324-326); pins follow.
+    // than two pins contribute nothing and are dropped, .cu:251-253, :324-326), within a class by the
+    // lowest particle index among the net's pins, so the nets of one wavefront gather nearby
+    // particles; pins follow.
     std::vector<int> order;
-    for (int cls = 0; cls < 3; ++cls)
+    std::vector<int> lowest(static_cast<std::size_t>(N_), std::numeric_limits<int>::max());
+    for (int n = 0; n < N_; ++n)
+        for (int q = L.net_offsets[n]; q < L.net_offsets[n + 1]; ++q)
+            if (L.pin_particles[q] >= 0) lowest[n] = std::min(lowest[n], static_cast<int>(L.pin_particles[q]));
+    for (int cls = 0; cls < 3; ++cls) {
+        const std::size_t first = order.size();
         for (int n = 0; n < N_; ++n) {
             const int deg = L.net_offsets[n + 1] - L.net_offsets[n];
             if (deg < 0) throw InvalidArgument("net offsets are not monotone");
@@ -120,6 +127,8 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
             order.push_back(n);
             (cls == 0 ? n_small_ : (cls == 1 ? n_medium_ : n_large_)) += 1;
         }
+        std::stable_sort(order.begin() + first, order.end(), [&](int a, int b) { return lowest[a] < lowest[b]; });
+    }
     std::vector<int> inet_off(order.size() + 1, 0);
     std::vector<double> inet_w(order.size());
     std::vector<int> ipin_part;

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <limits>
 #include <memory>
+#include <numeric>
 #include <utility>
 #include <vector>
 
@@ -78,23 +79,66 @@ public:
         std::string reason;
         const std::size_t limit = allocationLimit(o_.device, o_.maximum_device_bytes, reason);
         if (limit == 0) throw std::runtime_error("HIP global-placement backend is unavailable: " + reason);
-        const mp_gp_static L = p.layout(o_.device, o_.dct_precision, o_.maximum_device_bytes);
-        be_ = std::make_unique<Backend>(L, limit);
-        P_ = be_->particles();
         const std::size_t Mv = p.net.movable.size();
-        std::vector<double2> x(P_);
+        const std::size_t P = p.pw.size();
+        std::vector<double2> x0(P);  // starting positions in the prepared particle order
         for (std::size_t i = 0; i < Mv; ++i) {
             const int64_t m = p.net.movable[i];
-            x[i] = centers ? make_double2(centers[2 * m], centers[2 * m + 1]) : make_double2(p.net.cx[m], p.net.cy[m]);
+            x0[i] = centers ? make_double2(centers[2 * m], centers[2 * m + 1]) : make_double2(p.net.cx[m], p.net.cy[m]);
         }
-        for (std::size_t i = 0; i < p.filler_x.size(); ++i) x[Mv + i] = make_double2(p.filler_x[i], p.filler_y[i]);
+        for (std::size_t i = 0; i < p.filler_x.size(); ++i) x0[Mv + i] = make_double2(p.filler_x[i], p.filler_y[i]);
+        // The device keeps the particles renumbered along a Z-order curve of their starting
+        // positions (movable modules first, then fillers): particles close on the chip get close
+        // indices, so the kernels' gathers by particle and by pin hit fewer cache lines.  Results are
+        // mapped back to the prepared order on the way out.
+        order_.resize(P);
+        std::iota(order_.begin(), order_.end(), 0U);
+        auto zkey = [&](std::size_t i) {
+            const double fx = (x0[i].x - p.net.core[0]) / std::max(p.net.core[2] - p.net.core[0], 1e-9);
+            const double fy = (x0[i].y - p.net.core[1]) / std::max(p.net.core[3] - p.net.core[1], 1e-9);
+            const uint32_t ix = static_cast<uint32_t>(clampd(fx, 0.0, 1.0) * 1023.0);
+            const uint32_t iy = static_cast<uint32_t>(clampd(fy, 0.0, 1.0) * 1023.0);
+            uint64_t k = 0;
+            for (int b = 9; b >= 0; --b) k = (k << 2) | (((iy >> b) & 1U) << 1) | ((ix >> b) & 1U);
+            return k;
+        };
+        std::vector<uint64_t> keys(P);
+        for (std::size_t i = 0; i < P; ++i) keys[i] = zkey(i);
+        auto byKey = [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; };
+        std::stable_sort(order_.begin(), order_.begin() + Mv, byKey);
+        std::stable_sort(order_.begin() + Mv, order_.end(), byKey);
+        std::vector<int32_t> rank(P);
+        for (std::size_t k = 0; k < P; ++k) rank[order_[k]] = static_cast<int32_t>(k);
+        std::vector<double> pw(P), ph(P), pq(P), precond(P);
+        std::vector<int32_t> player(P), pin_particles(p.pin_particles.size());
+        std::vector<double2> x(P);
+        for (std::size_t k = 0; k < P; ++k) {
+            const uint32_t i = order_[k];
+            pw[k] = p.pw[i];
+            ph[k] = p.ph[i];
+            pq[k] = p.pq[i];
+            player[k] = p.player[i];
+            precond[k] = p.precond[i];
+            x[k] = x0[i];
+        }
+        for (std::size_t q = 0; q < pin_particles.size(); ++q)
+            pin_particles[q] = p.pin_particles[q] < 0 ? -1 : rank[p.pin_particles[q]];
+        mp_gp_static L = p.layout(o_.device, o_.dct_precision, o_.maximum_device_bytes);
+        L.particle_width = pw.data();
+        L.particle_height = ph.data();
+        L.particle_charge_area = pq.data();
+        L.particle_layer = player.data();
+        L.pin_particles = pin_particles.data();
+        be_ = std::make_unique<Backend>(L, limit);
+        P_ = be_->particles();
         DeviceBudget& b = be_->budget();
         for (DeviceArray<double2>* a : {&X_, &XP_, &B_, &Y_, &PX_, &PG_, &PXn_, &PGn_, &Rc_, &Gc_, &Rl_, &Gl_, &CKf_, &CKb_})
             a->allocate(P_, b, "controller vector");
         precond_.allocate(P_, b, "preconditioner");
         hipStream_t s = be_->stream();
         X_.upload(x.data(), P_, s, "upload positions");
-        precond_.upload(p.precond.data(), P_, s, "upload preconditioner");
+        precond_.upload(precond.data(), P_, s, "upload preconditioner");
+        hipCheck(hipStreamSynchronize(s), "upload session vectors");  // the host vectors above go out of scope
         min_bin_ = std::min(p.bw, p.bh);
         max_bin_ = std::max(p.bw, p.bh);
         clamp_ = ClampArgs{be_->sizes(), be_->region()};
@@ -119,11 +163,17 @@ public:
         return done;
     }
 
+    // positions in the prepared particle order
     void positions(double* xy) {
         hipCheck(hipSetDevice(o_.device), "select HIP device");
-        hipCheck(hipMemcpyAsync(xy, X_.get(), P_ * sizeof(double2), hipMemcpyDeviceToHost, be_->stream()),
+        std::vector<double2> pos(P_);
+        hipCheck(hipMemcpyAsync(pos.data(), X_.get(), P_ * sizeof(double2), hipMemcpyDeviceToHost, be_->stream()),
                  "download positions");
         hipCheck(hipStreamSynchronize(be_->stream()), "download positions");
+        for (int k = 0; k < P_; ++k) {
+            xy[2 * order_[k]] = pos[k].x;
+            xy[2 * order_[k] + 1] = pos[k].y;
+        }
     }
 
     void finish(double* centers, mp_gp_result* r) {
@@ -137,10 +187,12 @@ public:
                  "download checkpoint");
         hipCheck(hipStreamSynchronize(be_->stream()), "download checkpoint");
         std::vector<double> cx = p.net.cx, cy = p.net.cy;
-        for (std::size_t i = 0; i < p.net.movable.size(); ++i) {
+        for (int k = 0; k < P_; ++k) {
+            const std::size_t i = order_[k];
+            if (i >= p.net.movable.size()) continue;  // filler
             const int64_t m = p.net.movable[i];
-            cx[m] = pos[i].x;
-            cy[m] = pos[i].y;
+            cx[m] = pos[k].x;
+            cy[m] = pos[k].y;
         }
         if (centers)
             for (std::size_t m = 0; m < p.net.M; ++m) {
@@ -165,6 +217,7 @@ public:
     }
 
     mp_gp_result& result() { return result_; }
+    const std::vector<uint32_t>& order() const { return order_; }
 
 private:
     // ---- evaluation helpers -----------------------------------------------------------------
@@ -513,6 +566,7 @@ private:
     ClampArgs clamp_{};
     DeviceArray<double2> X_, XP_, B_, Y_, PX_, PG_, PXn_, PGn_, Rc_, Gc_, Rl_, Gl_, CKf_, CKb_;
     DeviceArray<double> precond_;
+    std::vector<uint32_t> order_;  // device particle k is prepared particle order_[k]
     DeviceArray<double2>* pending_ck_ = nullptr;
     double min_bin_ = 0, max_bin_ = 0, bin_height_ = 1;
     double smoothing_ = 0, penalty_ = 1, reference_penalty_ = 1, initial_overflow_ = 0, last_overflow_ = 0, last_step_ = 0;
@@ -732,6 +786,14 @@ int32_t mp_gp_session_kernel_timing(mp_gp_session* h, char* names, size_t names_
         translate(std::current_exception());
         return -1;
     }
+}
+
+mp_gp_status mp_gp_session_particle_order(const mp_gp_session* h, int64_t* order) {
+    return guarded([&] {
+        if (!h || !order) throw std::invalid_argument("null session or output");
+        const auto& o = h->s->order();
+        std::copy(o.begin(), o.end(), order);
+    });
 }
 
 mp_gp_backend* mp_gp_session_backend(mp_gp_session* h) { return h ? &h->view : nullptr; }

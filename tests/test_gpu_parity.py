@@ -232,3 +232,28 @@ def test_session_steps_and_history():
     assert np.all(c[mov, 0] - d.width[mov] / 2 >= core[0] - 1e-9)
     assert np.all(c[mov, 0] + d.width[mov] / 2 <= core[2] + 1e-9)
     s.close()
+
+
+def test_session_renumbering_is_invisible_outside():
+    """The session renumbers particles on the device along a Z-order curve; positions come back in
+    the prepared order, and its backend view evaluates the renumbered particles consistently with a
+    backend built from the prepared layout."""
+    d, o = _setup(design.plain_design(), 0.9, 64)
+    pp = hb.PreparedDesign(d, o)
+    say("[gpu] session renumbering plain-64")
+    s = hb.PlacementSession(pp, o)
+    order = s.particle_order()
+    n_mov = int(pp.info.movable_count)
+    assert sorted(order.tolist()) == list(range(len(order)))
+    assert set(order[:n_mov].tolist()) == set(range(n_mov))
+    xy = s.positions()
+    assert np.array_equal(xy, pp.initial_positions())
+    ref = hb.HipPlacementBackend.create(pp.layout())
+    a = ref.evaluate(xy, 3.0, True)
+    b = s.backend().evaluate(xy[order], 3.0, True)
+    assert rel(b.hpwl, a.hpwl) < 1e-12
+    assert rel(b.electrostatic_energy, a.electrostatic_energy) < 1e-6
+    assert b.optimizer_density == a.optimizer_density
+    assert nrel(b.wire_gradient, a.wire_gradient[order]) < 1e-6
+    ref.close()
+    s.close()
