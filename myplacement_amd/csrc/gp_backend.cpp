@@ -240,6 +240,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
              "allocate host-mapped scalars");
     std::memset(host_scal_, 0, sizeof(HostScalars));
     hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&dev_scal_), host_scal_, 0), "map scalar block");
+    dev_done_ = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(dev_scal_) + offsetof(HostScalars, done));
 
     std::vector<int> layers(L.particle_layer, L.particle_layer + P_);
     for (int v : layers)
@@ -260,18 +261,45 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     const auto sx = basis(nxp_, nxp_, nx_, 3, W);
     const auto iy = basis(nyp_, nyp_, ny_, 2, H);
     const auto sy = basis(nyp_, nyp_, ny_, 3, H);
-    // inverse x factors side by side: [k][i] = ix[i][k] (DCT-III), [k][nxp + i] = sx[i][k] (DST-III)
-    std::vector<float> bx2(static_cast<std::size_t>(nxp_) * 2 * nxp_, 0.f);
-    for (int k = 0; k < nxp_; ++k)
-        for (int i = 0; i < nxp_; ++i) {
-            bx2[static_cast<std::size_t>(k) * 2 * nxp_ + i] = ix[static_cast<std::size_t>(i) * nxp_ + k];
-            bx2[static_cast<std::size_t>(k) * 2 * nxp_ + nxp_ + i] = sx[static_cast<std::size_t>(i) * nxp_ + k];
+    folded_ = foldedDct(nx_, ny_);
+    if (folded_) {
+        // half-length factors (launchDct): parity p of the frequency index in the slowest dimension
+        const std::size_t hx = nx_ / 2, hy = ny_ / 2, n = nx_;
+        std::vector<float> fxh(2 * hx * hx), fyh(2 * hy * hy), bxh(2 * hx * n), iyh(2 * hy * hy), syh(2 * hy * hy);
+        for (std::size_t p = 0; p < 2; ++p) {
+            for (std::size_t t = 0; t < hx; ++t) {
+                for (std::size_t c = 0; c < hx; ++c) fxh[(p * hx + t) * hx + c] = fx[t * nxp_ + 2 * c + p];
+                for (std::size_t c = 0; c < n; ++c)
+                    bxh[(p * hx + t) * n + c] = c < hx ? ix[c * nxp_ + 2 * t + p] : sx[(c - hx) * nxp_ + 2 * t + p];
+            }
+            for (std::size_t r = 0; r < hy; ++r)
+                for (std::size_t t = 0; t < hy; ++t) {
+                    fyh[(p * hy + r) * hy + t] = fy[(2 * r + p) * nyp_ + t];
+                    iyh[(p * hy + r) * hy + t] = iy[r * nyp_ + 2 * t + p];
+                    syh[(p * hy + r) * hy + t] = sy[r * nyp_ + 2 * t + p];
+                }
         }
-    bxf_.upload(fx.data(), fx.size(), stream, "upload forward x factor");
-    ayf_.upload(fy.data(), fy.size(), stream, "upload forward y factor");
-    bxi_.upload(bx2.data(), bx2.size(), stream, "upload inverse x factors");
-    ayi_.upload(iy.data(), iy.size(), stream, "upload inverse y factor");
-    syi_.upload(sy.data(), sy.size(), stream, "upload inverse y sine factor");
+        bxf_.upload(fxh.data(), fxh.size(), stream, "upload forward x factor");
+        ayf_.upload(fyh.data(), fyh.size(), stream, "upload forward y factor");
+        bxi_.upload(bxh.data(), bxh.size(), stream, "upload inverse x factors");
+        ayi_.upload(iyh.data(), iyh.size(), stream, "upload inverse y factor");
+        syi_.upload(syh.data(), syh.size(), stream, "upload inverse y sine factor");
+        hipCheck(hipStreamSynchronize(stream), "upload DCT factors");  // the host vectors go out of scope
+    } else {
+        // inverse x factors side by side: [k][i] = ix[i][k] (DCT-III), [k][nxp + i] = sx[i][k] (DST-III)
+        std::vector<float> bx2(static_cast<std::size_t>(nxp_) * 2 * nxp_, 0.f);
+        for (int k = 0; k < nxp_; ++k)
+            for (int i = 0; i < nxp_; ++i) {
+                bx2[static_cast<std::size_t>(k) * 2 * nxp_ + i] = ix[static_cast<std::size_t>(i) * nxp_ + k];
+                bx2[static_cast<std::size_t>(k) * 2 * nxp_ + nxp_ + i] = sx[static_cast<std::size_t>(i) * nxp_ + k];
+            }
+        bxf_.upload(fx.data(), fx.size(), stream, "upload forward x factor");
+        ayf_.upload(fy.data(), fy.size(), stream, "upload forward y factor");
+        bxi_.upload(bx2.data(), bx2.size(), stream, "upload inverse x factors");
+        ayi_.upload(iy.data(), iy.size(), stream, "upload inverse y factor");
+        syi_.upload(sy.data(), sy.size(), stream, "upload inverse y sine factor");
+        hipCheck(hipStreamSynchronize(stream), "upload DCT factors");
+    }
     tall_.upload(tall.data(), tall.size(), stream, "upload tall-particle flags");
     tall_ids_.upload(tall_ids.data(), tall_ids.size(), stream, "upload tall-particle ids");
     // the padding of the deviation grids stays zero: the band kernel writes only the nx x ny part
@@ -448,37 +476,80 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
         launchGemm(g, batch, M, N, K, lda, ldb, ldc, epi, sp, dct_precision_, s);
         timedEnd(kTDct, 2.0 * M * static_cast<double>(N) * K * batch);
     };
-    // forward: T = rho * Bx ; chat = Ay * T -> phihat (+ energy)
-    GemmBatch f1{}, f2{};
-    for (int e = 0; e < count; ++e) {
-        f1.A[e] = slots_[e].rho.get();
-        f1.B[e] = bxf_.get();
-        f1.C[e] = slots_[e].tmp.get();
-        f2.A[e] = ayf_.get();
-        f2.B[e] = slots_[e].tmp.get();
-        f2.C[e] = slots_[e].phi.get();
+    // half-length products: `rows` x `cols` outputs over a contraction of half the transform side
+    auto dct = [&](int mode, const DctBatch& d, int batch, int lda, int ldb, int ldc, int step) {
+        timedBegin(kTDct);
+        launchDct(mode, d, batch, ny_, nx_, lda, ldb, ldc, step, sp, s);
+        const double half = (mode == kFoldCols || mode == kMirrorCols) ? 0.5 * nx_ : 0.5 * ny_;
+        timedEnd(kTDct, 2.0 * ny_ * static_cast<double>(nx_) * half * batch * (mode == kMirrorCols ? 2 : 1));
+    };
+    if (folded_) {
+        // forward: T = fold_x(rho) ; chat = fold_y(T) -> phihat (+ energy)
+        DctBatch f1{}, f2{};
+        for (int e = 0; e < count; ++e) {
+            f1.A[e] = slots_[e].rho.get();
+            f1.B[e] = bxf_.get();
+            f1.C[e] = slots_[e].tmp.get();
+            f2.A[e] = ayf_.get();
+            f2.B[e] = slots_[e].tmp.get();
+            f2.C[e] = slots_[e].phi.get();
+        }
+        dct(kFoldCols, f1, count, nxp_, 0, nxp_, 1);
+        dct(kFoldRows, f2, count, 0, nxp_, nxp_, 1);
+        for (int e = 0; e < count; ++e)
+            addReduction(kRedEnergy, specs[e].bank, (nx_ / 32) * (ny_ / 64), 0.5 * region_.bw * region_.bh);
+        if (!want_grad) return;
+        // [T1 | T2] = mirror_x(phihat) ; Ex = mirror_y(T2, cosine) ; Ey = mirror_y(T1, sine)
+        DctBatch i1{}, i2{};
+        for (int e = 0; e < count; ++e) {
+            const Slot& sl = slots_[e];
+            i1.A[e] = sl.phi.get();
+            i1.B[e] = bxi_.get();
+            i1.C[e] = sl.t12.get();
+            i2.A[2 * e] = ayi_.get();
+            i2.B[2 * e] = sl.t12.get() + nxp_;
+            i2.C[2 * e] = sl.field.get();
+            i2.sigma[2 * e] = 1.f;
+            i2.A[2 * e + 1] = syi_.get();
+            i2.B[2 * e + 1] = sl.t12.get();
+            i2.C[2 * e + 1] = sl.field.get() + 1;
+            i2.sigma[2 * e + 1] = -1.f;
+        }
+        dct(kMirrorCols, i1, count, nxp_, 0, 2 * nxp_, 1);
+        dct(kMirrorRows, i2, 2 * count, 0, 2 * nxp_, nxp_, 2);
+    } else {
+        // forward: T = rho * Bx ; chat = Ay * T -> phihat (+ energy)
+        GemmBatch f1{}, f2{};
+        for (int e = 0; e < count; ++e) {
+            f1.A[e] = slots_[e].rho.get();
+            f1.B[e] = bxf_.get();
+            f1.C[e] = slots_[e].tmp.get();
+            f2.A[e] = ayf_.get();
+            f2.B[e] = slots_[e].tmp.get();
+            f2.C[e] = slots_[e].phi.get();
+        }
+        gemm(f1, count, nyp_, nxp_, nxp_, nxp_, nxp_, nxp_, kEpiStore);
+        gemm(f2, count, nyp_, nxp_, nyp_, nyp_, nxp_, nxp_, kEpiSpectral);
+        for (int e = 0; e < count; ++e)
+            addReduction(kRedEnergy, specs[e].bank, (nxp_ / 32) * (nyp_ / 32), 0.5 * region_.bw * region_.bh);
+        if (!want_grad) return;
+        // [T1 | T2] = phihat * [Cx_inv | Sx_inv] ; Ex = Ay_inv * T2 ; Ey = Sy_inv * T1
+        GemmBatch i1{}, i2{};
+        for (int e = 0; e < count; ++e) {
+            const Slot& sl = slots_[e];
+            i1.A[e] = sl.phi.get();
+            i1.B[e] = bxi_.get();
+            i1.C[e] = sl.t12.get();
+            i2.A[2 * e] = ayi_.get();
+            i2.B[2 * e] = sl.t12.get() + nxp_;
+            i2.C[2 * e] = sl.field.get();
+            i2.A[2 * e + 1] = syi_.get();
+            i2.B[2 * e + 1] = sl.t12.get();
+            i2.C[2 * e + 1] = sl.field.get() + 1;
+        }
+        gemm(i1, count, nyp_, 2 * nxp_, nxp_, nxp_, 2 * nxp_, 2 * nxp_, kEpiStore);
+        gemm(i2, 2 * count, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiInterleave);
     }
-    gemm(f1, count, nyp_, nxp_, nxp_, nxp_, nxp_, nxp_, kEpiStore);
-    gemm(f2, count, nyp_, nxp_, nyp_, nyp_, nxp_, nxp_, kEpiSpectral);
-    for (int e = 0; e < count; ++e)
-        addReduction(kRedEnergy, specs[e].bank, (nxp_ / 32) * (nyp_ / 32), 0.5 * region_.bw * region_.bh);
-    if (!want_grad) return;
-    // [T1 | T2] = phihat * [Cx_inv | Sx_inv] ; Ex = Ay_inv * T2 ; Ey = Sy_inv * T1
-    GemmBatch i1{}, i2{};
-    for (int e = 0; e < count; ++e) {
-        const Slot& sl = slots_[e];
-        i1.A[e] = sl.phi.get();
-        i1.B[e] = bxi_.get();
-        i1.C[e] = sl.t12.get();
-        i2.A[2 * e] = ayi_.get();
-        i2.B[2 * e] = sl.t12.get() + nxp_;
-        i2.C[2 * e] = sl.field.get();
-        i2.A[2 * e + 1] = syi_.get();
-        i2.B[2 * e + 1] = sl.t12.get();
-        i2.C[2 * e + 1] = sl.field.get() + 1;
-    }
-    gemm(i1, count, nyp_, 2 * nxp_, nxp_, nxp_, 2 * nxp_, 2 * nxp_, kEpiStore);
-    gemm(i2, 2 * count, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiInterleave);
     have_fields_ = true;
 
     GatherArgs ga{};
@@ -523,9 +594,31 @@ void Backend::addReduction(int kind, int bank, int blocks, double p0, double p1,
 }
 
 const HostScalars& Backend::fetchScalars() {
+    const int jobs = pending_.n;
+    if (jobs == 0) {
+        hipCheck(hipStreamSynchronize(stream_), "synchronize placement evaluation");
+        return *host_scal_;
+    }
+    pending_.seq = ++seq_;
+    pending_.flag = dev_done_;
     launchFinalize(pending_, stream_);
     pending_.n = 0;
-    hipCheck(hipStreamSynchronize(stream_), "synchronize placement evaluation");
+    // Poll the completion words: the results are on the host as soon as the last one lands, some
+    // microseconds before the stream reports the kernel retired.  The stream is queried now and
+    // then so that a failed launch surfaces as an error instead of a hang.
+    auto written = [&] {
+        for (int b = 0; b < jobs; ++b)
+            if (__atomic_load_n(&host_scal_->done[b], __ATOMIC_ACQUIRE) != seq_) return false;
+        return true;
+    };
+    for (long spins = 0; !written(); ++spins) {
+        if ((spins & 1023) != 1023) continue;
+        const hipError_t st = hipStreamQuery(stream_);
+        if (st == hipErrorNotReady) continue;
+        hipCheck(st, "synchronize placement evaluation");
+        if (!written()) throw std::runtime_error("reduction finalize retired without writing its completion words");
+        break;
+    }
     return *host_scal_;
 }
 
@@ -567,6 +660,8 @@ void Backend::evaluateHost(const double* xy, double gamma, bool want_grad, mp_gp
                  "download density gradients");
     }
     const HostScalars& h = fetchScalars();
+    // the gradient copies go to pageable host memory: wait for the stream to finish them
+    if (want_grad && P_ > 0) hipCheck(hipStreamSynchronize(s), "download gradients");
     toScalars(h, out);
     if (!std::isfinite(out->smooth_wirelength) || !std::isfinite(out->hpwl) || !std::isfinite(out->electrostatic_energy))
         throw std::runtime_error("HIP global-placement evaluation produced a non-finite metric.");
@@ -577,10 +672,15 @@ void Backend::debugGrids(double* source, double* potential, double* field_x, dou
     if (!have_fields_) throw std::runtime_error("no gradient evaluation has produced field grids yet");
     const std::size_t Gp = static_cast<std::size_t>(nxp_) * nyp_;
     const Slot& sl = slots_[0];
-    if (potential) {
+    if (potential) {  // phi = Ay_inv * T1
         SpectralArgs sp{};
-        GemmBatch g{{ayi_.get()}, {sl.t12.get()}, {pot_.get()}};
-        launchGemm(g, 1, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiStore, sp, dct_precision_, stream_);
+        if (folded_) {
+            DctBatch d{{ayi_.get()}, {sl.t12.get()}, {pot_.get()}, {1.f}};
+            launchDct(kMirrorRows, d, 1, ny_, nx_, 0, 2 * nxp_, nxp_, 1, sp, stream_);
+        } else {
+            GemmBatch g{{ayi_.get()}, {sl.t12.get()}, {pot_.get()}};
+            launchGemm(g, 1, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiStore, sp, dct_precision_, stream_);
+        }
     }
     std::vector<float> buf(2 * Gp);
     auto pull = [&](const DeviceArray<float>& a, double* dst, int stride, int offset) {

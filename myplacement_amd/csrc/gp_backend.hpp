@@ -73,6 +73,7 @@ private:
 
 struct HostScalars {
     double v[kScalarCount * kScalarBanks];
+    unsigned done[kMaxRedJobs];  // k_finalize's completion words, one per reduction job
     const double* bank(int b) const { return v + b * kScalarCount; }
 };
 
@@ -108,7 +109,8 @@ public:
     void enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bool want_grad);
     // Rebuilds the particle row order at `pos`.
     void sortParticles(const double2* pos);
-    // Reduces the pending partials into the scalar block and waits for the stream.
+    // Reduces the pending partials into the scalar block and waits until k_finalize has written
+    // it (by polling its completion words; the rest of the kernel's retirement is not waited for).
     const HostScalars& fetchScalars();
     void toScalars(const HostScalars& h, mp_gp_scalars* out, int bank = 0) const;
 
@@ -157,6 +159,7 @@ private:
     double t_ = 0.0;
     double dark_total_ = 0.0, placeable_total_ = 0.0;
     int dct_precision_ = 0;
+    bool folded_ = false;  // half-length DCT products (both grid sides multiples of 64)
     bool have_fields_ = false;
     DeviceBudget budget_;
     Slot slots_[kSlots];
@@ -182,6 +185,8 @@ private:
     DeviceArray<double2> wire_out_, density_out_;
     HostScalars* host_scal_ = nullptr;  // host-mapped, written by k_finalize
     double* dev_scal_ = nullptr;        // device address of host_scal_
+    unsigned* dev_done_ = nullptr;      // device address of host_scal_->done
+    unsigned seq_ = 0;                  // completion value of the last finalize
 
     // timing (events come from a pool that is reused once they have been read)
     static constexpr int kTimingClasses = 6;

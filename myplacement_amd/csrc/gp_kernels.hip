@@ -613,6 +613,206 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int K, int lda, 
 }
 
 // ---------------------------------------------------------------------------------------------
+// Half-length transforms (modes in gp_kernels.hpp).  A workgroup of eight wavefronts owns a 32 x 32
+// tile of the half-size output for both parities p (two accumulator sets); the wavefronts split
+// the contraction in 16-wide chunks, fold or de-interleave the operands as they load them, and
+// meet in LDS, where the epilogue combines the parities into the full-length outputs.
+constexpr int kDctThreads = 512;
+constexpr int kDctWaves = kDctThreads / 64;
+
+// Operands of one 16-wide chunk for this lane: a[p][row half][j] = A-side value at contraction
+// index t + j for rows r16 (+16), b[p][col half][j] likewise for columns r16 (+16).
+template <int MODE>
+__device__ inline void dctLoad(const float* __restrict__ A, const float* __restrict__ B, int n, int h, int lda,
+                               int ldb, int row0, int col0, int t, int r16, float (&a)[2][2][4], float (&b)[2][2][4]) {
+    if (MODE == kFoldCols) {
+#pragma unroll
+        for (int rh = 0; rh < 2; ++rh) {
+            const float* ar = A + static_cast<size_t>(row0 + 16 * rh + r16) * lda;
+            const float4 x = *reinterpret_cast<const float4*>(ar + t);
+            const float4 y = *reinterpret_cast<const float4*>(ar + n - 4 - t);  // mirror of t + j is y[3 - j]
+            const float xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.w, y.z, y.y, y.x};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                a[0][rh][j] = xs[j] + ys[j];
+                a[1][rh][j] = xs[j] - ys[j];
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    b[p][ch][j] = B[static_cast<size_t>(p) * h * h + static_cast<size_t>(t + j) * h + col0 + 16 * ch + r16];
+    } else if (MODE == kFoldRows) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int rh = 0; rh < 2; ++rh) {
+                const float4 x = *reinterpret_cast<const float4*>(A + static_cast<size_t>(p) * h * h +
+                                                                  static_cast<size_t>(row0 + 16 * rh + r16) * h + t);
+                a[p][rh][0] = x.x;
+                a[p][rh][1] = x.y;
+                a[p][rh][2] = x.z;
+                a[p][rh][3] = x.w;
+            }
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int col = col0 + 16 * ch + r16;
+                const float u = B[static_cast<size_t>(t + j) * ldb + col];
+                const float v = B[static_cast<size_t>(n - 1 - t - j) * ldb + col];
+                b[0][ch][j] = u + v;
+                b[1][ch][j] = u - v;
+            }
+    } else if (MODE == kMirrorCols) {
+#pragma unroll
+        for (int rh = 0; rh < 2; ++rh) {
+            const float* ar = A + static_cast<size_t>(row0 + 16 * rh + r16) * lda + 2 * t;
+            const float4 x0 = *reinterpret_cast<const float4*>(ar);
+            const float4 x1 = *reinterpret_cast<const float4*>(ar + 4);
+            a[0][rh][0] = x0.x;
+            a[0][rh][1] = x0.z;
+            a[0][rh][2] = x1.x;
+            a[0][rh][3] = x1.z;
+            a[1][rh][0] = x0.y;
+            a[1][rh][1] = x0.w;
+            a[1][rh][2] = x1.y;
+            a[1][rh][3] = x1.w;
+        }
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    b[p][ch][j] = B[static_cast<size_t>(p) * h * n + static_cast<size_t>(t + j) * n + col0 + 16 * ch + r16];
+    } else {  // kMirrorRows
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int rh = 0; rh < 2; ++rh) {
+                const float4 x = *reinterpret_cast<const float4*>(A + static_cast<size_t>(p) * h * h +
+                                                                  static_cast<size_t>(row0 + 16 * rh + r16) * h + t);
+                a[p][rh][0] = x.x;
+                a[p][rh][1] = x.y;
+                a[p][rh][2] = x.z;
+                a[p][rh][3] = x.w;
+            }
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    b[p][ch][j] = B[static_cast<size_t>(2 * (t + j) + p) * ldb + col0 + 16 * ch + r16];
+    }
+}
+
+__device__ inline void dctMma(const float (&a)[2][2][4], const float (&b)[2][2][4], floatx4 (&acc)[2][2][2]) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int rh = 0; rh < 2; ++rh)
+#pragma unroll
+                for (int ch = 0; ch < 2; ++ch)
+                    acc[p][rh][ch] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[p][rh][j], b[p][ch][j], acc[p][rh][ch], 0, 0, 0);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kDctThreads) void k_dct(DctBatch d, int n, int lda, int ldb, int ldc, int step,
+                                                     SpectralArgs sp) {
+    __shared__ float red[2][kDctWaves][32][33];
+    __shared__ double lds[kDctWaves + 1];
+    const int z = blockIdx.z;
+    const float* __restrict__ A = d.A[z];
+    const float* __restrict__ B = d.B[z];
+    float* __restrict__ C = d.C[z];
+    const int h = n >> 1;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r16 = lane & 15, grp = lane >> 4;
+    const int row0 = blockIdx.y * 32, col0 = blockIdx.x * 32;
+    floatx4 acc[2][2][2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int rh = 0; rh < 2; ++rh)
+#pragma unroll
+            for (int ch = 0; ch < 2; ++ch) acc[p][rh][ch] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // two chunks per pass, both loaded before the products run (h is a multiple of 32)
+    constexpr int stride = 16 * kDctWaves;
+    for (int k0 = wave * 16; k0 < h; k0 += 2 * stride) {
+        float a0[2][2][4], b0[2][2][4], a1[2][2][4], b1[2][2][4];
+        const bool second = k0 + stride < h;
+        dctLoad<MODE>(A, B, n, h, lda, ldb, row0, col0, k0 + 4 * grp, r16, a0, b0);
+        if (second) dctLoad<MODE>(A, B, n, h, lda, ldb, row0, col0, k0 + stride + 4 * grp, r16, a1, b1);
+        dctMma(a0, b0, acc);
+        if (second) dctMma(a1, b1, acc);
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int rh = 0; rh < 2; ++rh)
+#pragma unroll
+            for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) red[p][wave][16 * rh + 4 * grp + v][16 * ch + r16] = acc[p][rh][ch][v];
+    __syncthreads();
+    double energy = 0.0;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int e = tid + kDctThreads * q;
+        const int row = e >> 5, col = e & 31;
+        float ev = 0.f, od = 0.f;
+#pragma unroll
+        for (int w = 0; w < kDctWaves; ++w) {
+            ev += red[0][w][row][col];
+            od += red[1][w][row][col];
+        }
+        if (MODE == kFoldCols) {
+            *reinterpret_cast<float2*>(C + static_cast<size_t>(row0 + row) * ldc + 2 * (col0 + col)) = make_float2(ev, od);
+        } else if (MODE == kFoldRows) {
+            const int k = col0 + col;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int l = 2 * (row0 + row) + p;
+                float out = 0.f;
+                if (l < sp.ny && k < sp.nx) {
+                    const double wx = kPi * static_cast<double>(k) / sp.W;
+                    const double wy = kPi * static_cast<double>(l) / sp.H;
+                    const double den = wx * wx + wy * wy;
+                    if (den > kEps) {
+                        const double chat = static_cast<double>(p ? od : ev);
+                        const double phi = chat * sp.normalization / den;
+                        out = static_cast<float>(phi);
+                        const double weight = (k == 0 || l == 0) ? 0.5 : 1.0;
+                        energy += weight * chat * phi;
+                    }
+                }
+                C[static_cast<size_t>(l) * ldc + k] = out;
+            }
+        } else if (MODE == kMirrorCols) {
+            const bool sine = col0 >= h;
+            const int c = col0 + col - (sine ? h : 0);
+            float* out = C + static_cast<size_t>(row0 + row) * ldc + (sine ? n : 0);
+            out[c] = ev + od;
+            out[n - 1 - c] = sine ? od - ev : ev - od;
+        } else {  // kMirrorRows
+            const int r = row0 + row, k = col0 + col;
+            C[static_cast<size_t>(step) * (static_cast<size_t>(r) * ldc + k)] = ev + od;
+            C[static_cast<size_t>(step) * (static_cast<size_t>(n - 1 - r) * ldc + k)] = d.sigma[z] * (ev - od);
+        }
+    }
+    if (MODE == kFoldRows) {
+        double v[1] = {energy};
+        publishPartials<1, kDctWaves>(v, 0U, blockIdx.y * gridDim.x + blockIdx.x, sp.partials[z], lds);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Field gather with the Neumann mirror parity (ElectrostaticField.cpp:289-320).
 __device__ inline int reflectIndex(int index, int size, bool odd, bool& negate) {
     while (index < 0 || index >= size) {
@@ -754,19 +954,42 @@ __global__ __launch_bounds__(kThreads) void k_zero(double* p, size_t count) {
 // One workgroup per pending reduction: sums (or maxima) of the partial rows in index order, then
 // the reference's post-processing of each scalar (DensityMap::metrics Metrics.cpp:101-119, energy
 // clamp .cu:1008, alignment sign AdaptiveGlobalPlacer.cpp:610-617).
-__device__ const int kRedWidths[kRedKinds] = {2, 9, 1, 7, 2};
+// Widths of the partial rows by kind: kRedWire 2, kRedMetrics 9, kRedEnergy 1, kRedGather 7,
+// kRedPropose 2 (redWidth on the host).
+// Accumulates the partial rows tid, tid + T, tid + 2T, ... (T threads) of a K-wide reduction in
+// that order.  Rows are read kFinalizeBatch at a time with every load issued before the first
+// add: the partials come from another XCD's kernel, so each load waits on the fabric.
+constexpr int kFinalizeBatch = 4;
+template <int K>
+__device__ void accumulateRows(const double* __restrict__ p, int rows, unsigned max_mask, double (&acc)[9]) {
+    for (int b0 = threadIdx.x; b0 < rows; b0 += kFinalizeBatch * kFinalizeThreads) {
+        double x[kFinalizeBatch][K];
+#pragma unroll
+        for (int u = 0; u < kFinalizeBatch; ++u) {
+            const int b = b0 + u * kFinalizeThreads;
+#pragma unroll
+            for (int k = 0; k < K; ++k) x[u][k] = b < rows ? p[static_cast<size_t>(b) * K + k] : 0.0;
+        }
+        // rows past the end add 0 and take max with 0: every accumulator starts at 0
+#pragma unroll
+        for (int u = 0; u < kFinalizeBatch; ++u)
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc[k] = ((max_mask >> k) & 1U) ? fmax(acc[k], x[u][k]) : acc[k] + x[u][k];
+    }
+}
 
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(RedList l) {
     __shared__ double lds[(kFinalizeThreads / 64 + 1) * 9];
     const RedJob j = l.job[blockIdx.x];
-    const int K = kRedWidths[j.kind];
     const unsigned max_mask = j.kind == kRedMetrics ? ((1U << 7) | (1U << 8)) : (j.kind == kRedPropose ? 2U : 0U);
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < j.blocks; b += kFinalizeThreads)
-        for (int k = 0; k < K; ++k) {
-            const double x = j.partials[static_cast<size_t>(b) * K + k];
-            acc[k] = ((max_mask >> k) & 1U) ? fmax(acc[k], x) : acc[k] + x;
-        }
+    switch (j.kind) {
+        case kRedWire: accumulateRows<2>(j.partials, j.blocks, max_mask, acc); break;
+        case kRedMetrics: accumulateRows<9>(j.partials, j.blocks, max_mask, acc); break;
+        case kRedEnergy: accumulateRows<1>(j.partials, j.blocks, max_mask, acc); break;
+        case kRedGather: accumulateRows<7>(j.partials, j.blocks, max_mask, acc); break;
+        default: accumulateRows<2>(j.partials, j.blocks, max_mask, acc); break;
+    }
     blockReduce<9, kFinalizeThreads / 64>(acc, max_mask, lds);
     if (threadIdx.x != 0) return;
     double* o = j.out;
@@ -804,6 +1027,12 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(RedList l) {
             o[kPredicted] = acc[0];
             o[kMaxDisp] = acc[1];
             break;
+    }
+    // completion word for a host that polls the scalar block instead of waiting for the stream:
+    // written after the results, which the system-scope fence orders before it
+    if (l.flag) {
+        __threadfence_system();
+        l.flag[blockIdx.x] = l.seq;
     }
 }
 
@@ -868,6 +1097,34 @@ void launchGemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int
     else
         hipLaunchKernelGGL(k_gemm<kEpiStore>, grid, dim3(kThreads), 0, s, g, K, lda, ldb, ldc, sp);
     check(hipGetLastError(), "launch DCT product kernel");
+}
+
+bool foldedDct(int nx, int ny) { return nx > 0 && ny > 0 && nx % 64 == 0 && ny % 64 == 0; }
+
+void launchDct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
+               const SpectralArgs& sp, hipStream_t s) {
+    if (mode == kMirrorRows && step != 1 && step != 2) throw std::invalid_argument("bad DCT output element stride");
+    if (rows <= 0 || cols <= 0 || rows % 64 || cols % 64)
+        throw std::invalid_argument("half-length DCT products need both grid sides to be multiples of 64");
+    if (batch < 1 || batch > kMaxGemmBatch) throw std::invalid_argument("bad DCT product batch");
+    const dim3 block(kDctThreads);
+    switch (mode) {
+        case kFoldCols:
+            hipLaunchKernelGGL(k_dct<kFoldCols>, dim3(cols / 64, rows / 32, batch), block, 0, s, d, cols, lda, ldb, ldc, 1, sp);
+            break;
+        case kFoldRows:
+            hipLaunchKernelGGL(k_dct<kFoldRows>, dim3(cols / 32, rows / 64, batch), block, 0, s, d, rows, lda, ldb, ldc, 1, sp);
+            break;
+        case kMirrorCols:
+            hipLaunchKernelGGL(k_dct<kMirrorCols>, dim3(cols / 32, rows / 32, batch), block, 0, s, d, cols, lda, ldb, ldc, 1, sp);
+            break;
+        case kMirrorRows:
+            hipLaunchKernelGGL(k_dct<kMirrorRows>, dim3(cols / 32, rows / 64, batch), block, 0, s, d, rows, lda, ldb, ldc, step, sp);
+            break;
+        default:
+            throw std::invalid_argument("unknown DCT product mode");
+    }
+    check(hipGetLastError(), "launch half-length DCT product kernel");
 }
 
 int particleBlocks(int n) { return std::max(1, blocksFor(n)); }

@@ -44,6 +44,8 @@ constexpr int kMaxRedJobs = 12;
 struct RedList {
     RedJob job[kMaxRedJobs];
     int n;
+    unsigned seq;              // completion value: workgroup b stores it to flag[b] after its results
+    volatile unsigned* flag;   // host-mapped completion words (null: none)
 };
 int redWidth(int kind);
 void launchFinalize(const RedList& l, hipStream_t s);
@@ -132,6 +134,34 @@ struct SpectralArgs {
 // C[MxN] = A[MxK] * B[KxN], all row-major, M, N multiples of 32, K multiple of 16.
 void launchGemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
                 const SpectralArgs& sp, int precision, hipStream_t s);
+
+// Half-length transforms.  Every factor of the path has the mirror symmetry
+// F[n-1-i][k] = s_k F[i][k] (s_k = (-1)^k for the cosines, (-1)^(k+1) for the sines), so a
+// length-n transform splits into two n/2 x n/2 products, one per parity of k, over the folded
+// input x[i] +- x[n-1-i] (forward) or producing x[i] = E + O, x[n-1-i] = sigma (E - O)
+// (inverse); half the multiply-adds of the dense product.  Operands (h = n / 2, row-major):
+//   kFoldCols   out[m][2c+p] = sum_t (A[m][t] + (-1)^p A[m][n-1-t]) B_p[t][c]
+//               A: M x n data, B: factor halves [2][h][h], out: M x n
+//   kFoldRows   out[2r+p][k] = sum_t A_p[r][t] (B[t][k] + (-1)^p B[n-1-t][k]), spectral epilogue
+//               A: factor halves [2][h][h], B: n x N data, out: n x N
+//   kMirrorCols E/O[m][c] = sum_t A[m][2t+p] B_p[t][c]; columns c < h are the cosine half
+//               (out[m][c], out[m][n-1-c], sigma +1), c >= h the sine half (written at column
+//               offset n, sigma -1); A: M x n data, B: factor halves [2][h][2h], out: M x 2n
+//   kMirrorRows E/O[r][k] = sum_t A_p[r][t] B[2t+p][k]; out rows r and n-1-r, sigma[z],
+//               stored at out + step (row ldc + k) (step 2: one of two interleaved grids);
+//               A: factor halves [2][h][h], B: n x N
+enum DctMode : int { kFoldCols = 0, kFoldRows = 1, kMirrorCols = 2, kMirrorRows = 3 };
+struct DctBatch {
+    const float* A[kMaxGemmBatch];
+    const float* B[kMaxGemmBatch];
+    float* C[kMaxGemmBatch];
+    float sigma[kMaxGemmBatch];  // kMirrorRows: sign of the mirrored half per product
+};
+// True when an n_x x n_y grid takes the half-length products (both sides multiples of 64).
+bool foldedDct(int nx, int ny);
+// `rows` x `cols` of the output grid (for kMirrorCols the transform side, cols = n).
+void launchDct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
+               const SpectralArgs& sp, hipStream_t s);
 
 // ---- field gather + gradient assembly ---------------------------------------------------------
 struct GatherArgs {

@@ -94,18 +94,24 @@ def test_evaluate_matches_oracle(ob, name, make):
     be.close()
 
 
-def test_fields_match_oracle(ob):
-    d, o = _setup(design.plain_design(), 0.9, 64)
+@pytest.mark.parametrize("bx,by", [(64, 64), (128, 64), (48, 48)])
+def test_fields_match_oracle(ob, bx, by):
+    """Source, potential and field grids: half-length products (both sides multiples of 64,
+    square and not) and the dense products (48 x 48)."""
+    d = design.plain_design()
+    d.center = hb.random_placement(d, 2026)
+    o = abi.default_options(target_density=0.9, bins_x=bx, bins_y=by)
     orc = ob.OracleSession(d, o)
     L = orc.layout()
-    say("[gpu] field grids plain-64")
+    say(f"[gpu] field grids plain {bx}x{by}")
     be = hb.HipPlacementBackend.create(L)
     xy = orc.arrays()["positions"]
     gamma = 0.75 * 360 / 64
-    orc.evaluate(xy, gamma, True)
-    be.evaluate(xy, gamma, True)
+    s_ref = orc.evaluate(xy, gamma, True)
+    s = be.evaluate(xy, gamma, True)
+    assert rel(s.electrostatic_energy, s_ref["electrostatic_energy"]) < 2e-4
     g_ref = orc.grids()
-    g = be.debug_grids(64 * 64)
+    g = be.debug_grids(bx * by)
     assert nrel(g["source"], g_ref["source"]) < 1e-6
     for k in ("potential", "field_x", "field_y"):
         assert nrel(g[k], g_ref[k]) < 5e-4, (k, nrel(g[k], g_ref[k]))
