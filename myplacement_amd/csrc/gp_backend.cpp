@@ -108,18 +108,22 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     if (N_ > 0 && (L.net_offsets == nullptr || L.net_offsets[0] != 0 || L.net_offsets[N_] != I_))
         throw InvalidArgument("net offsets do not cover the pin array");
     // Internal net order for the wirelength kernel: degree classes 2..8, 9..32, >32 (nets with fewer
-    // than two pins contribute nothing and are dropped, .cu:251-253, :324-326), within a class by the
-    // lowest particle index among the net's pins, so the nets of one wavefront gather nearby
-    // particles; pins follow.
+    // than two pins contribute nothing and are dropped, .cu:251-253, :324-326).  Small nets by degree,
+    // then by the lowest particle index among their pins (the nets of one wavefront gather nearby
+    // particles and share one degree); medium nets by lowest particle; large nets largest first
+    // (their workgroups start first and take longest).  Pins follow the net order, except that each
+    // small-net degree group stores its pins slot-major: pin j of the group's i-th net at
+    // group base + j * group size + i, so one wavefront's loads of slot j are contiguous.
     std::vector<int> order;
     std::vector<int> lowest(static_cast<std::size_t>(N_), std::numeric_limits<int>::max());
     for (int n = 0; n < N_; ++n)
         for (int q = L.net_offsets[n]; q < L.net_offsets[n + 1]; ++q)
             if (L.pin_particles[q] >= 0) lowest[n] = std::min(lowest[n], static_cast<int>(L.pin_particles[q]));
+    auto degree = [&](int n) { return L.net_offsets[n + 1] - L.net_offsets[n]; };
     for (int cls = 0; cls < 3; ++cls) {
         const std::size_t first = order.size();
         for (int n = 0; n < N_; ++n) {
-            const int deg = L.net_offsets[n + 1] - L.net_offsets[n];
+            const int deg = degree(n);
             if (deg < 0) throw InvalidArgument("net offsets are not monotone");
             if (deg < 2) continue;
             const int c = deg <= kSmallDegree ? 0 : (deg <= kMediumDegree ? 1 : 2);
@@ -127,24 +131,46 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
             order.push_back(n);
             (cls == 0 ? n_small_ : (cls == 1 ? n_medium_ : n_large_)) += 1;
         }
-        std::stable_sort(order.begin() + first, order.end(), [&](int a, int b) { return lowest[a] < lowest[b]; });
+        std::stable_sort(order.begin() + first, order.end(), [&](int a, int b) {
+            if (cls == 0 && degree(a) != degree(b)) return degree(a) < degree(b);
+            if (cls == 2 && degree(a) != degree(b)) return degree(a) > degree(b);
+            return lowest[a] < lowest[b];
+        });
     }
     std::vector<int> inet_off(order.size() + 1, 0);
     std::vector<double> inet_w(order.size());
-    std::vector<int> ipin_part;
-    std::vector<double2> ipin_off;
-    ipin_part.reserve(I_);
-    ipin_off.reserve(I_);
+    for (std::size_t k = 0; k < order.size(); ++k) {
+        inet_w[k] = L.net_weights[order[k]];
+        inet_off[k + 1] = inet_off[k] + degree(order[k]);
+    }
+    for (int d = 0; d <= kSmallDegree + 1; ++d) small_net_[d] = small_pin_[d] = 0;
+    {
+        int k = 0;
+        for (int d = 2; d <= kSmallDegree; ++d) {
+            small_net_[d] = k;
+            while (k < n_small_ && degree(order[k]) == d) ++k;
+        }
+        small_net_[kSmallDegree + 1] = n_small_;
+        for (int d = 2; d <= kSmallDegree + 1; ++d) small_pin_[d] = inet_off[small_net_[d]];
+    }
+    std::vector<int> ipin_part(static_cast<std::size_t>(inet_off.back()));
+    std::vector<double2> ipin_off(ipin_part.size());
     for (std::size_t k = 0; k < order.size(); ++k) {
         const int n = order[k];
-        inet_w[k] = L.net_weights[n];
-        for (int p = L.net_offsets[n]; p < L.net_offsets[n + 1]; ++p) {
+        const int deg = degree(n);
+        for (int j = 0; j < deg; ++j) {
+            const int p = L.net_offsets[n] + j;
             const int part = L.pin_particles[p];
             if (part >= P_) throw InvalidArgument("pin refers to an unknown particle");
-            ipin_part.push_back(part);
-            ipin_off.push_back(make_double2(L.pin_offsets_or_static[2 * p], L.pin_offsets_or_static[2 * p + 1]));
+            std::size_t q = static_cast<std::size_t>(inet_off[k]) + j;
+            if (static_cast<int>(k) < n_small_) {  // slot-major inside the degree group
+                const int count = small_net_[deg + 1] - small_net_[deg];
+                q = static_cast<std::size_t>(small_pin_[deg]) + static_cast<std::size_t>(j) * count +
+                    (static_cast<int>(k) - small_net_[deg]);
+            }
+            ipin_part[q] = part;
+            ipin_off[q] = make_double2(L.pin_offsets_or_static[2 * p], L.pin_offsets_or_static[2 * p + 1]);
         }
-        inet_off[k + 1] = static_cast<int>(ipin_part.size());
     }
     const std::size_t Ii = ipin_part.size();
 
@@ -433,6 +459,10 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
     wa.n_small = n_small_;
     wa.n_medium = n_medium_;
     wa.n_large = n_large_;
+    for (int d = 0; d <= kSmallDegree + 1; ++d) {
+        wa.small_net[d] = small_net_[d];
+        wa.small_pin[d] = small_pin_[d];
+    }
     wa.gamma = gamma;
     wa.want_grad = want_grad ? 1 : 0;
     for (int e = 0; e < count; ++e) {

@@ -169,6 +169,7 @@ private:
     DeviceArray<int> layer_;
     DeviceArray<int> net_off_, pin_part_, ppin_off_, ppin_idx_;
     int n_small_ = 0, n_medium_ = 0, n_large_ = 0;
+    int small_net_[kSmallDegree + 2] = {}, small_pin_[kSmallDegree + 2] = {};  // small-net degree groups
     DeviceArray<double> net_w_;
     DeviceArray<double2> pin_off_;
     DeviceArray<double> static_charge_;

@@ -163,44 +163,87 @@ __device__ inline float groupSum(float v) {
 }
 
 // One net on a group of W lanes (W = 32 or 64); returns (hpwl, smooth) in lane 0 of the group.
-template <int W>
+// Lane `sub` owns pins b + sub + W i.  Nets of up to W * PER pins keep their pins in registers
+// from one load pass (all loads of a lane in flight together); larger nets stream them three times.
+template <int W, int PER>
 __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_grad, int net, int sub, double inv_gamma,
                           float gamma_f, double& hp, double& wl) {
     const int b = a.net_off[net], e = a.net_off[net + 1];
-    double mx = -INFINITY, nx = INFINITY, my = -INFINITY, ny = INFINITY;
-    for (int p = b + sub; p < e; p += W) {
-        const double2 q = pinPosition(pos, a.pin_part, a.pin_off, p);
-        mx = fmax(mx, q.x); nx = fmin(nx, q.x);
-        my = fmax(my, q.y); ny = fmin(ny, q.y);
-    }
-    mx = groupMax<W>(mx); nx = groupMin<W>(nx); my = groupMax<W>(my); ny = groupMin<W>(ny);
-    float spx = 0.f, snx = 0.f, spy = 0.f, sny = 0.f;
-    for (int p = b + sub; p < e; p += W) {
-        const double2 q = pinPosition(pos, a.pin_part, a.pin_off, p);
-        spx += ex32(q.x - mx, inv_gamma);
-        snx += ex32(nx - q.x, inv_gamma);
-        spy += ex32(q.y - my, inv_gamma);
-        sny += ex32(ny - q.y, inv_gamma);
-    }
-    spx = groupSum<W>(spx); snx = groupSum<W>(snx); spy = groupSum<W>(spy); sny = groupSum<W>(sny);
     const double w = a.net_w[net];
+    double mx = -INFINITY, nx = INFINITY, my = -INFINITY, ny = INFINITY;
+    float spx = 0.f, snx = 0.f, spy = 0.f, sny = 0.f;
+    if (e - b <= W * PER) {
+        double qx[PER], qy[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int p = b + sub + W * i;
+            if (p < e) {
+                const double2 q = pinPosition(pos, a.pin_part, a.pin_off, p);
+                qx[i] = q.x;
+                qy[i] = q.y;
+                mx = fmax(mx, q.x); nx = fmin(nx, q.x);
+                my = fmax(my, q.y); ny = fmin(ny, q.y);
+            }
+        }
+        mx = groupMax<W>(mx); nx = groupMin<W>(nx); my = groupMax<W>(my); ny = groupMin<W>(ny);
+        float epx[PER], enx[PER], epy[PER], eny[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            if (b + sub + W * i < e) {
+                epx[i] = ex32(qx[i] - mx, inv_gamma);
+                enx[i] = ex32(nx - qx[i], inv_gamma);
+                epy[i] = ex32(qy[i] - my, inv_gamma);
+                eny[i] = ex32(ny - qy[i], inv_gamma);
+                spx += epx[i]; snx += enx[i]; spy += epy[i]; sny += eny[i];
+            }
+        }
+        spx = groupSum<W>(spx); snx = groupSum<W>(snx); spy = groupSum<W>(spy); sny = groupSum<W>(sny);
+        if (a.want_grad) {
+            const float wf = static_cast<float>(w);
+            const float ipx = wf / spx, inx = wf / snx, ipy = wf / spy, iny = wf / sny;
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int p = b + sub + W * i;
+                if (p < e) pin_grad[p] = make_float2(epx[i] * ipx - enx[i] * inx, epy[i] * ipy - eny[i] * iny);
+            }
+        }
+    } else {
+        for (int p = b + sub; p < e; p += W) {
+            const double2 q = pinPosition(pos, a.pin_part, a.pin_off, p);
+            mx = fmax(mx, q.x); nx = fmin(nx, q.x);
+            my = fmax(my, q.y); ny = fmin(ny, q.y);
+        }
+        mx = groupMax<W>(mx); nx = groupMin<W>(nx); my = groupMax<W>(my); ny = groupMin<W>(ny);
+        for (int p = b + sub; p < e; p += W) {
+            const double2 q = pinPosition(pos, a.pin_part, a.pin_off, p);
+            spx += ex32(q.x - mx, inv_gamma);
+            snx += ex32(nx - q.x, inv_gamma);
+            spy += ex32(q.y - my, inv_gamma);
+            sny += ex32(ny - q.y, inv_gamma);
+        }
+        spx = groupSum<W>(spx); snx = groupSum<W>(snx); spy = groupSum<W>(spy); sny = groupSum<W>(sny);
+        if (a.want_grad) {
+            const float wf = static_cast<float>(w);
+            const float ipx = wf / spx, inx = wf / snx, ipy = wf / spy, iny = wf / sny;
+            for (int p = b + sub; p < e; p += W) {
+                const double2 q = pinPosition(pos, a.pin_part, a.pin_off, p);
+                pin_grad[p] = make_float2(ex32(q.x - mx, inv_gamma) * ipx - ex32(nx - q.x, inv_gamma) * inx,
+                                          ex32(q.y - my, inv_gamma) * ipy - ex32(ny - q.y, inv_gamma) * iny);
+            }
+        }
+    }
     if (sub == 0) {
         const double lse = static_cast<double>(gamma_f * (__logf(spx) + __logf(snx) + __logf(spy) + __logf(sny)));
         wl = w * ((mx - nx) + (my - ny) + lse);
         hp = w * ((mx - nx) + (my - ny));
     }
-    if (a.want_grad) {
-        const float wf = static_cast<float>(w);
-        const float ipx = wf / spx, inx = wf / snx, ipy = wf / spy, iny = wf / sny;
-        for (int p = b + sub; p < e; p += W) {
-            const double2 q = pinPosition(pos, a.pin_part, a.pin_off, p);
-            pin_grad[p] = make_float2(ex32(q.x - mx, inv_gamma) * ipx - ex32(nx - q.x, inv_gamma) * inx,
-                                      ex32(q.y - my, inv_gamma) * ipy - ex32(ny - q.y, inv_gamma) * iny);
-        }
-    }
 }
 
-__global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int small_blocks, int medium_blocks) {
+// Workgroups in the order large nets (one wavefront each, largest first), medium nets (a
+// half-wavefront each), small nets (a thread each): the longest-running workgroups are dispatched
+// first.
+constexpr int kLargePins = 8;  // pins per lane held in registers for a large net (512 per net)
+__global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int large_blocks, int medium_blocks) {
     __shared__ double lds[(kWaves + 1) * 2];
     double wl = 0.0, hp = 0.0;
     const double inv_gamma = 1.0 / a.gamma;
@@ -209,17 +252,29 @@ __global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int small_blocks,
     const int block = eb.y;
     const double2* pos = a.pos[eb.x];
     float2* pin_grad = a.pin_grad[eb.x];
-    if (block < small_blocks) {
-        const int net = block * kThreads + threadIdx.x;
+    if (block < large_blocks) {
+        const int net = a.n_small + a.n_medium + block * kWaves + (threadIdx.x >> 6);
+        if (net < a.n_small + a.n_medium + a.n_large)
+            wireGroup<64, kLargePins>(a, pos, pin_grad, net, threadIdx.x & 63, inv_gamma, gamma_f, hp, wl);
+    } else if (block < large_blocks + medium_blocks) {
+        const int lane = threadIdx.x & 63;
+        const int net = a.n_small + (block - large_blocks) * (2 * kWaves) + 2 * (threadIdx.x >> 6) + (lane >> 5);
+        if (net < a.n_small + a.n_medium) wireGroup<32, 1>(a, pos, pin_grad, net, lane & 31, inv_gamma, gamma_f, hp, wl);
+    } else {
+        const int net = (block - large_blocks - medium_blocks) * kThreads + threadIdx.x;
         if (net < a.n_small) {
-            const int b = a.net_off[net];
-            const int deg = a.net_off[net + 1] - b;
+            // degree group of this net (groups are contiguous; empty ones repeat the next start)
+            int deg = 2;
+#pragma unroll
+            for (int d = 3; d <= kSmallDegree; ++d) deg = net >= a.small_net[d] ? d : deg;
+            const int size = a.small_net[deg + 1] - a.small_net[deg];
+            const int b = a.small_pin[deg] + (net - a.small_net[deg]);  // pin j at b + j * size
             double qx[kSmallDegree], qy[kSmallDegree];
             double mx = -INFINITY, nx = INFINITY, my = -INFINITY, ny = INFINITY;
 #pragma unroll
             for (int i = 0; i < kSmallDegree; ++i) {
                 if (i < deg) {
-                    const double2 q = pinPosition(pos, a.pin_part, a.pin_off, b + i);
+                    const double2 q = pinPosition(pos, a.pin_part, a.pin_off, b + i * size);
                     qx[i] = q.x;
                     qy[i] = q.y;
                     mx = fmax(mx, q.x); nx = fmin(nx, q.x);
@@ -248,16 +303,9 @@ __global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int small_blocks,
 #pragma unroll
                 for (int i = 0; i < kSmallDegree; ++i)
                     if (i < deg)
-                        pin_grad[b + i] = make_float2(epx[i] * ipx - enx[i] * inx, epy[i] * ipy - eny[i] * iny);
+                        pin_grad[b + i * size] = make_float2(epx[i] * ipx - enx[i] * inx, epy[i] * ipy - eny[i] * iny);
             }
         }
-    } else if (block < small_blocks + medium_blocks) {
-        const int lane = threadIdx.x & 63;
-        const int net = a.n_small + (block - small_blocks) * (2 * kWaves) + 2 * (threadIdx.x >> 6) + (lane >> 5);
-        if (net < a.n_small + a.n_medium) wireGroup<32>(a, pos, pin_grad, net, lane & 31, inv_gamma, gamma_f, hp, wl);
-    } else {
-        const int net = a.n_small + a.n_medium + (block - small_blocks - medium_blocks) * kWaves + (threadIdx.x >> 6);
-        if (net < a.n_small + a.n_medium + a.n_large) wireGroup<64>(a, pos, pin_grad, net, threadIdx.x & 63, inv_gamma, gamma_f, hp, wl);
     }
     double v[2] = {hp, wl};
     publishPartials<2>(v, 0U, block, a.partials[eb.x], lds);
@@ -1045,12 +1093,14 @@ int wireBlocks(int n_small, int n_medium, int n_large) {
 }
 
 void launchWire(const WireArgs& a, int count, hipStream_t s) {
-    const int small_blocks = blocksFor(a.n_small);
+    const int large_blocks = (a.n_large + kWaves - 1) / kWaves;
     const int medium_blocks = (a.n_medium + 2 * kWaves - 1) / (2 * kWaves);
     const int blocks = wireBlocks(a.n_small, a.n_medium, a.n_large);
     if (blocks == 0) return;
     if (count < 1 || count > kMaxEval) throw std::invalid_argument("bad evaluation batch");
-    hipLaunchKernelGGL(k_wire, dim3(blocks, count), dim3(kThreads), 0, s, a, small_blocks, medium_blocks);
+    if (a.small_net[2] != 0 || a.small_net[kSmallDegree + 1] != a.n_small)
+        throw std::invalid_argument("small-net degree groups do not cover the small nets");
+    hipLaunchKernelGGL(k_wire, dim3(blocks, count), dim3(kThreads), 0, s, a, large_blocks, medium_blocks);
     check(hipGetLastError(), "launch wirelength kernel");
 }
 

@@ -61,8 +61,12 @@ struct Region {
 };
 
 // ---- wirelength ------------------------------------------------------------------------------
-// Nets (degree >= 2 only) in the backend's internal order: n_small nets of degree 2..kSmallDegree,
-// then n_medium of degree ..kMediumDegree, then n_large; pins follow the same order.
+// Nets (degree >= 2 only) in the backend's internal order: n_small nets of degree 2..kSmallDegree
+// grouped by degree, then n_medium of degree ..kMediumDegree, then n_large; pins follow the same
+// order (CSR net_off), except inside a small-net degree group d, nets [small_net[d],
+// small_net[d + 1]), which stores pin j of its i-th net at small_pin[d] + j * size + i.
+constexpr int kSmallDegree = 8;
+constexpr int kMediumDegree = 32;
 struct WireArgs {
     const double2* pos[kMaxEval];
     const int* net_off;
@@ -70,13 +74,13 @@ struct WireArgs {
     const int* pin_part;
     const double2* pin_off;
     int n_small, n_medium, n_large;
+    int small_net[kSmallDegree + 2];  // [d]: first net of degree d (d = 2..kSmallDegree), [kSmallDegree + 1] = n_small
+    int small_pin[kSmallDegree + 2];  // [d]: first pin of that group
     double gamma;
     int want_grad;
     float2* pin_grad[kMaxEval];  // per-pin d(WL)/d(pin), internal pin order
     double* partials[kMaxEval];
 };
-constexpr int kSmallDegree = 8;
-constexpr int kMediumDegree = 32;
 int wireBlocks(int n_small, int n_medium, int n_large);
 void launchWire(const WireArgs& a, int count, hipStream_t s);
 
