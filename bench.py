@@ -11,6 +11,11 @@ placement (InitialPlacer quadraticPlacement, solved on the GPU before the sessio
 iterations (10 warm-up + 270 timed) with early stop disabled.  Inputs are resident in HBM before
 the timed region starts.
 
+--workload precision: 1.3 M cells, 20 macros (12 %), 1.4 M nets (~5.7 M pins), 3,000 rows x 3,000
+sites, 1024 x 1024 bins, target density 0.7, quadratic initial placement, DCT products in split
+BF16 (bf16x3).  --workload plain: 2,000 cells, 64 x 64 bins, random initial placement.
+`roofline_dct` reports the DCT products against the MFMA peak of their arithmetic.
+
 Prints one JSON line.  `roofline` reports k_wire, the kernel with the largest share of GPU time,
 against MI355X's HBM peak: its algorithmic bytes (per evaluation 12 per pin, 8 per net, 16 per
 particle with the gradient and 8 without -- one pass over each array at 4-byte indices and 8-byte
@@ -44,6 +49,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+METRICS = {
+    "large": "GP iters/s @210k cells 512\u00b2 bins; p50/p99 placement latency on served mix",
+    "precision": "GP iters/s @1.3M cells 1024\u00b2 bins, bf16x3 DCT products",
+    "plain": "GP iters/s @2k cells 64\u00b2 bins",
+}
 
 
 def log(*a):
@@ -58,6 +69,14 @@ def make_workload(name: str):
         o = abi.default_options(target_density=0.8, bins_x=512, bins_y=512)
         desc = {"workload": "synthetic 210k cells, 12 macros (15%), 220k nets (~0.9M pins), 1200x1200 sites, "
                             "512x512 bins, t=0.8, quadratic init", "cells": 210000, "bins": 512, "target_density": 0.8}
+        d.center, _ = hb.quadratic_placement(d)
+    elif name == "precision":
+        d = design.large_design(seed=2026, cells=1_300_000, nets=1_400_000, row_count=3000, sites=3000, macros=20,
+                                macro_fraction=0.12)
+        o = abi.default_options(target_density=0.7, bins_x=1024, bins_y=1024, dct_precision=abi.MP_DCT_BF16X3)
+        desc = {"workload": "synthetic 1.3M cells, 20 macros (12%), 1.4M nets (~5.7M pins), 3000x3000 sites, "
+                            "1024x1024 bins, t=0.7, quadratic init, split-BF16 (bf16x3) DCT products",
+                "cells": 1_300_000, "bins": 1024, "target_density": 0.7}
         d.center, _ = hb.quadratic_placement(d)
     elif name == "plain":
         d = design.plain_design(seed=2026)
@@ -154,6 +173,21 @@ def measured_traffic(kernel: str):
     return None
 
 
+def dct_roofline(t, precision: int):
+    """MFMA roofline of the DCT products over the all-class timing stretch: the multiply-adds the
+    half-length products execute (2 flops each, FP32-equivalent) over their event-timed launch
+    durations, against FP32 MFMA or, for split-BF16, dense BF16 MFMA (which runs three bf16
+    products per FP32-equivalent product)."""
+    if not t or not t["launches"] or t["total_us"] <= 0:
+        return None
+    achieved = t["work"] / (t["total_us"] * 1e-6) / 1e12
+    bf16 = precision == 1
+    peak = BF16_MFMA_PEAK_TFLOPS / 3.0 if bf16 else FP32_MFMA_PEAK_TFLOPS
+    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "arithmetic": "bf16x3: 3 bf16 products per FP32-equivalent product, peak = 2500/3" if bf16 else "fp32",
+            "kernel": "k_dct", "avg_launch_us": t["total_us"] / t["launches"], "launches": t["launches"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,7 +255,7 @@ def main() -> int:
     _, res = sess.finish()
 
     line = {
-        "metric": "GP iters/s @210k cells 512\u00b2 bins; p50/p99 placement latency on served mix",
+        "metric": METRICS[args.workload],
         "value": done / elapsed,
         "unit": "GP iterations/s",
         "n_gpus": 1,
@@ -231,13 +265,15 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 positions/reductions, f32 pin terms and DCT products",
+        "dtype": "f64 positions/reductions, f32 pin terms, "
+                 + ("bf16x3 DCT products" if o.dct_precision == 1 else "f32 DCT products"),
         "data": "synthetic",
         "config": dict(desc, evaluations_per_step=(
             sum(1 for _ in hist) and (res["gradient_evaluations"] + res["value_evaluations"]) / max(1, res["completed_iterations"]))),
         "roofline": roof,
         "kernel_timing_us": {k: round(v["total_us"] / max(1, v["launches"]), 3) for k, v in timing.items()
                              if v["launches"]},
+        "roofline_dct": dct_roofline(timing.get("k_gemm"), o.dct_precision),
         "setup_seconds": setup_s,
     }
     sess.close()

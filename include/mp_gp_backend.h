@@ -41,7 +41,9 @@ typedef enum mp_gp_status {
 /* Particle layers of the density scatter (.cu:811-822): standard cell, movable macro, filler. */
 enum { MP_LAYER_STANDARD = 0, MP_LAYER_MACRO = 1, MP_LAYER_FILLER = 2 };
 
-/* Arithmetic used for the DCT bin-grid products. */
+/* Arithmetic used for the DCT bin-grid products: FP32 MFMA, or split-BF16 (each operand as
+ * hi + lo bf16 pair, three bf16 MFMA products accumulated in FP32, relative error ~1e-5 per
+ * product); MP_DCT_BF16X3 needs both bin counts to be multiples of 64. */
 enum { MP_DCT_FP32 = 0, MP_DCT_BF16X3 = 1 };
 
 /* ---- netlist description (the PlacementDatabase fields the GP path reads) ---------------

@@ -91,7 +91,8 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     if (!(L.target_density > 0.0)) throw InvalidArgument("target density must be positive");
     if (L.dct_precision != MP_DCT_FP32 && L.dct_precision != MP_DCT_BF16X3)
         throw InvalidArgument("unknown DCT precision");
-    if (L.dct_precision == MP_DCT_BF16X3) throw std::runtime_error("the split-BF16 DCT products are not built yet");
+    if (L.dct_precision == MP_DCT_BF16X3 && !foldedDct(L.bins_x, L.bins_y))
+        throw InvalidArgument("split-BF16 DCT products need both bin counts to be multiples of 64");
     P_ = static_cast<int>(L.particle_count);
     N_ = static_cast<int>(L.net_count);
     I_ = static_cast<int>(L.pin_count);
@@ -509,7 +510,7 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
     // half-length products: `rows` x `cols` outputs over a contraction of half the transform side
     auto dct = [&](int mode, const DctBatch& d, int batch, int lda, int ldb, int ldc, int step) {
         timedBegin(kTDct);
-        launchDct(mode, d, batch, ny_, nx_, lda, ldb, ldc, step, sp, s);
+        launchDct(mode, d, batch, ny_, nx_, lda, ldb, ldc, step, dct_precision_, sp, s);
         const double half = (mode == kFoldCols || mode == kMirrorCols) ? 0.5 * nx_ : 0.5 * ny_;
         timedEnd(kTDct, 2.0 * ny_ * static_cast<double>(nx_) * half * batch * (mode == kMirrorCols ? 2 : 1));
     };
@@ -706,7 +707,7 @@ void Backend::debugGrids(double* source, double* potential, double* field_x, dou
         SpectralArgs sp{};
         if (folded_) {
             DctBatch d{{ayi_.get()}, {sl.t12.get()}, {pot_.get()}, {1.f}};
-            launchDct(kMirrorRows, d, 1, ny_, nx_, 0, 2 * nxp_, nxp_, 1, sp, stream_);
+            launchDct(kMirrorRows, d, 1, ny_, nx_, 0, 2 * nxp_, nxp_, 1, dct_precision_, sp, stream_);
         } else {
             GemmBatch g{{ayi_.get()}, {sl.t12.get()}, {pot_.get()}};
             launchGemm(g, 1, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiStore, sp, dct_precision_, stream_);

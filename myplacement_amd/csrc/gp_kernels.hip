@@ -663,27 +663,47 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int K, int lda, 
 // ---------------------------------------------------------------------------------------------
 // Half-length transforms (modes in gp_kernels.hpp).  A workgroup of eight wavefronts owns a 32 x 32
 // tile of the half-size output for both parities p (two accumulator sets); the wavefronts split
-// the contraction in 16-wide chunks, fold or de-interleave the operands as they load them, and
-// meet in LDS, where the epilogue combines the parities into the full-length outputs.
+// the contraction in chunks, fold or de-interleave the operands as they load them, and meet in
+// LDS, where the epilogue combines the parities into the full-length outputs.
+//
+// Precision: FP32 runs v_mfma_f32_16x16x4_f32 over chunks of 16 (KPL = 4 values per lane);
+// split-BF16 runs v_mfma_f32_16x16x32_bf16 over chunks of 32 (KPL = 8), each operand split as
+// x = hi + lo with hi = bf16(x), lo = bf16(x - hi), and accumulates lo*hi + hi*lo + hi*hi in FP32
+// (the lo*lo term, below 2^-16 of the product, is dropped).  The bf16 instruction takes A[i][k]
+// from lane i + 16 (k / 8), element k % 8, and B[k][j] from lane j + 16 (k / 8).
 constexpr int kDctThreads = 512;
 constexpr int kDctWaves = kDctThreads / 64;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-// Operands of one 16-wide chunk for this lane: a[p][row half][j] = A-side value at contraction
-// index t + j for rows r16 (+16), b[p][col half][j] likewise for columns r16 (+16).
-template <int MODE>
+template <int N>
+__device__ inline void loadFloats(const float* p, float (&v)[N]) {
+#pragma unroll
+    for (int q = 0; q < N / 4; ++q) {
+        const float4 x = *reinterpret_cast<const float4*>(p + 4 * q);
+        v[4 * q] = x.x;
+        v[4 * q + 1] = x.y;
+        v[4 * q + 2] = x.z;
+        v[4 * q + 3] = x.w;
+    }
+}
+
+// Operands of one chunk for this lane: a[p][row half][j] = A-side value at contraction index t + j
+// for rows r16 (+16), b[p][col half][j] likewise for columns r16 (+16), j < KPL.
+template <int MODE, int KPL>
 __device__ inline void dctLoad(const float* __restrict__ A, const float* __restrict__ B, int n, int h, int lda,
-                               int ldb, int row0, int col0, int t, int r16, float (&a)[2][2][4], float (&b)[2][2][4]) {
+                               int ldb, int row0, int col0, int t, int r16, float (&a)[2][2][KPL],
+                               float (&b)[2][2][KPL]) {
     if (MODE == kFoldCols) {
 #pragma unroll
         for (int rh = 0; rh < 2; ++rh) {
             const float* ar = A + static_cast<size_t>(row0 + 16 * rh + r16) * lda;
-            const float4 x = *reinterpret_cast<const float4*>(ar + t);
-            const float4 y = *reinterpret_cast<const float4*>(ar + n - 4 - t);  // mirror of t + j is y[3 - j]
-            const float xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.w, y.z, y.y, y.x};
+            float x[KPL], y[KPL];
+            loadFloats<KPL>(ar + t, x);
+            loadFloats<KPL>(ar + n - KPL - t, y);  // mirror of t + j is y[KPL - 1 - j]
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                a[0][rh][j] = xs[j] + ys[j];
-                a[1][rh][j] = xs[j] - ys[j];
+            for (int j = 0; j < KPL; ++j) {
+                a[0][rh][j] = x[j] + y[KPL - 1 - j];
+                a[1][rh][j] = x[j] - y[KPL - 1 - j];
             }
         }
 #pragma unroll
@@ -691,24 +711,18 @@ __device__ inline void dctLoad(const float* __restrict__ A, const float* __restr
 #pragma unroll
             for (int ch = 0; ch < 2; ++ch)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < KPL; ++j)
                     b[p][ch][j] = B[static_cast<size_t>(p) * h * h + static_cast<size_t>(t + j) * h + col0 + 16 * ch + r16];
     } else if (MODE == kFoldRows) {
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
-            for (int rh = 0; rh < 2; ++rh) {
-                const float4 x = *reinterpret_cast<const float4*>(A + static_cast<size_t>(p) * h * h +
-                                                                  static_cast<size_t>(row0 + 16 * rh + r16) * h + t);
-                a[p][rh][0] = x.x;
-                a[p][rh][1] = x.y;
-                a[p][rh][2] = x.z;
-                a[p][rh][3] = x.w;
-            }
+            for (int rh = 0; rh < 2; ++rh)
+                loadFloats<KPL>(A + static_cast<size_t>(p) * h * h + static_cast<size_t>(row0 + 16 * rh + r16) * h + t, a[p][rh]);
 #pragma unroll
         for (int ch = 0; ch < 2; ++ch)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < KPL; ++j) {
                 const int col = col0 + 16 * ch + r16;
                 const float u = B[static_cast<size_t>(t + j) * ldb + col];
                 const float v = B[static_cast<size_t>(n - 1 - t - j) * ldb + col];
@@ -718,43 +732,33 @@ __device__ inline void dctLoad(const float* __restrict__ A, const float* __restr
     } else if (MODE == kMirrorCols) {
 #pragma unroll
         for (int rh = 0; rh < 2; ++rh) {
-            const float* ar = A + static_cast<size_t>(row0 + 16 * rh + r16) * lda + 2 * t;
-            const float4 x0 = *reinterpret_cast<const float4*>(ar);
-            const float4 x1 = *reinterpret_cast<const float4*>(ar + 4);
-            a[0][rh][0] = x0.x;
-            a[0][rh][1] = x0.z;
-            a[0][rh][2] = x1.x;
-            a[0][rh][3] = x1.z;
-            a[1][rh][0] = x0.y;
-            a[1][rh][1] = x0.w;
-            a[1][rh][2] = x1.y;
-            a[1][rh][3] = x1.w;
+            float x[2 * KPL];
+            loadFloats<2 * KPL>(A + static_cast<size_t>(row0 + 16 * rh + r16) * lda + 2 * t, x);
+#pragma unroll
+            for (int j = 0; j < KPL; ++j) {
+                a[0][rh][j] = x[2 * j];
+                a[1][rh][j] = x[2 * j + 1];
+            }
         }
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
             for (int ch = 0; ch < 2; ++ch)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < KPL; ++j)
                     b[p][ch][j] = B[static_cast<size_t>(p) * h * n + static_cast<size_t>(t + j) * n + col0 + 16 * ch + r16];
     } else {  // kMirrorRows
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
-            for (int rh = 0; rh < 2; ++rh) {
-                const float4 x = *reinterpret_cast<const float4*>(A + static_cast<size_t>(p) * h * h +
-                                                                  static_cast<size_t>(row0 + 16 * rh + r16) * h + t);
-                a[p][rh][0] = x.x;
-                a[p][rh][1] = x.y;
-                a[p][rh][2] = x.z;
-                a[p][rh][3] = x.w;
-            }
+            for (int rh = 0; rh < 2; ++rh)
+                loadFloats<KPL>(A + static_cast<size_t>(p) * h * h + static_cast<size_t>(row0 + 16 * rh + r16) * h + t, a[p][rh]);
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
             for (int ch = 0; ch < 2; ++ch)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < KPL; ++j)
                     b[p][ch][j] = B[static_cast<size_t>(2 * (t + j) + p) * ldb + col0 + 16 * ch + r16];
     }
 }
@@ -771,7 +775,37 @@ __device__ inline void dctMma(const float (&a)[2][2][4], const float (&b)[2][2][
                     acc[p][rh][ch] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[p][rh][j], b[p][ch][j], acc[p][rh][ch], 0, 0, 0);
 }
 
-template <int MODE>
+__device__ inline void splitBf16(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const __bf16 h = static_cast<__bf16>(x[j]);
+        hi[j] = h;
+        lo[j] = static_cast<__bf16>(x[j] - static_cast<float>(h));
+    }
+}
+
+__device__ inline void dctMma(const float (&a)[2][2][8], const float (&b)[2][2][8], floatx4 (&acc)[2][2][2]) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            splitBf16(a[p][q], ah[q], al[q]);
+            splitBf16(b[p][q], bh[q], bl[q]);
+        }
+#pragma unroll
+        for (int rh = 0; rh < 2; ++rh)
+#pragma unroll
+            for (int ch = 0; ch < 2; ++ch) {
+                floatx4 c = acc[p][rh][ch];
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rh], bh[ch], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rh], bl[ch], c, 0, 0, 0);
+                acc[p][rh][ch] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rh], bh[ch], c, 0, 0, 0);
+            }
+    }
+}
+
+template <int MODE, int KPL>
 __global__ __launch_bounds__(kDctThreads) void k_dct(DctBatch d, int n, int lda, int ldb, int ldc, int step,
                                                      SpectralArgs sp) {
     __shared__ float red[2][kDctWaves][32][33];
@@ -790,13 +824,15 @@ __global__ __launch_bounds__(kDctThreads) void k_dct(DctBatch d, int n, int lda,
         for (int rh = 0; rh < 2; ++rh)
 #pragma unroll
             for (int ch = 0; ch < 2; ++ch) acc[p][rh][ch] = floatx4{0.f, 0.f, 0.f, 0.f};
-    // two chunks per pass, both loaded before the products run (h is a multiple of 32)
-    constexpr int stride = 16 * kDctWaves;
-    for (int k0 = wave * 16; k0 < h; k0 += 2 * stride) {
-        float a0[2][2][4], b0[2][2][4], a1[2][2][4], b1[2][2][4];
+    // chunks of 4 KPL contraction indices, two per pass, both loaded before the products run
+    // (h is a multiple of 32)
+    constexpr int chunk = 4 * KPL;
+    constexpr int stride = chunk * kDctWaves;
+    for (int k0 = wave * chunk; k0 < h; k0 += 2 * stride) {
+        float a0[2][2][KPL], b0[2][2][KPL], a1[2][2][KPL], b1[2][2][KPL];
         const bool second = k0 + stride < h;
-        dctLoad<MODE>(A, B, n, h, lda, ldb, row0, col0, k0 + 4 * grp, r16, a0, b0);
-        if (second) dctLoad<MODE>(A, B, n, h, lda, ldb, row0, col0, k0 + stride + 4 * grp, r16, a1, b1);
+        dctLoad<MODE, KPL>(A, B, n, h, lda, ldb, row0, col0, k0 + KPL * grp, r16, a0, b0);
+        if (second) dctLoad<MODE, KPL>(A, B, n, h, lda, ldb, row0, col0, k0 + stride + KPL * grp, r16, a1, b1);
         dctMma(a0, b0, acc);
         if (second) dctMma(a1, b1, acc);
     }
@@ -1163,29 +1199,44 @@ void launchGemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int
 
 bool foldedDct(int nx, int ny) { return nx > 0 && ny > 0 && nx % 64 == 0 && ny % 64 == 0; }
 
-void launchDct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
-               const SpectralArgs& sp, hipStream_t s) {
-    if (mode == kMirrorRows && step != 1 && step != 2) throw std::invalid_argument("bad DCT output element stride");
-    if (rows <= 0 || cols <= 0 || rows % 64 || cols % 64)
-        throw std::invalid_argument("half-length DCT products need both grid sides to be multiples of 64");
-    if (batch < 1 || batch > kMaxGemmBatch) throw std::invalid_argument("bad DCT product batch");
+template <int KPL>
+void launchDctPrec(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
+                   const SpectralArgs& sp, hipStream_t s) {
     const dim3 block(kDctThreads);
     switch (mode) {
         case kFoldCols:
-            hipLaunchKernelGGL(k_dct<kFoldCols>, dim3(cols / 64, rows / 32, batch), block, 0, s, d, cols, lda, ldb, ldc, 1, sp);
+            hipLaunchKernelGGL((k_dct<kFoldCols, KPL>), dim3(cols / 64, rows / 32, batch), block, 0, s, d, cols, lda, ldb,
+                               ldc, 1, sp);
             break;
         case kFoldRows:
-            hipLaunchKernelGGL(k_dct<kFoldRows>, dim3(cols / 32, rows / 64, batch), block, 0, s, d, rows, lda, ldb, ldc, 1, sp);
+            hipLaunchKernelGGL((k_dct<kFoldRows, KPL>), dim3(cols / 32, rows / 64, batch), block, 0, s, d, rows, lda, ldb,
+                               ldc, 1, sp);
             break;
         case kMirrorCols:
-            hipLaunchKernelGGL(k_dct<kMirrorCols>, dim3(cols / 32, rows / 32, batch), block, 0, s, d, cols, lda, ldb, ldc, 1, sp);
+            hipLaunchKernelGGL((k_dct<kMirrorCols, KPL>), dim3(cols / 32, rows / 32, batch), block, 0, s, d, cols, lda,
+                               ldb, ldc, 1, sp);
             break;
         case kMirrorRows:
-            hipLaunchKernelGGL(k_dct<kMirrorRows>, dim3(cols / 32, rows / 64, batch), block, 0, s, d, rows, lda, ldb, ldc, step, sp);
+            hipLaunchKernelGGL((k_dct<kMirrorRows, KPL>), dim3(cols / 32, rows / 64, batch), block, 0, s, d, rows, lda,
+                               ldb, ldc, step, sp);
             break;
         default:
             throw std::invalid_argument("unknown DCT product mode");
     }
+}
+
+void launchDct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
+               int precision, const SpectralArgs& sp, hipStream_t s) {
+    if (rows <= 0 || cols <= 0 || rows % 64 || cols % 64)
+        throw std::invalid_argument("half-length DCT products need both grid sides to be multiples of 64");
+    if (batch < 1 || batch > kMaxGemmBatch) throw std::invalid_argument("bad DCT product batch");
+    if (mode == kMirrorRows && step != 1 && step != 2) throw std::invalid_argument("bad DCT output element stride");
+    if (precision == 0)
+        launchDctPrec<4>(mode, d, batch, rows, cols, lda, ldb, ldc, step, sp, s);
+    else if (precision == 1)
+        launchDctPrec<8>(mode, d, batch, rows, cols, lda, ldb, ldc, step, sp, s);
+    else
+        throw std::invalid_argument("unknown DCT product precision");
     check(hipGetLastError(), "launch half-length DCT product kernel");
 }
 

@@ -163,9 +163,10 @@ struct DctBatch {
 };
 // True when an n_x x n_y grid takes the half-length products (both sides multiples of 64).
 bool foldedDct(int nx, int ny);
-// `rows` x `cols` of the output grid (for kMirrorCols the transform side, cols = n).
+// `rows` x `cols` of the output grid (for kMirrorCols the transform side, cols = n); precision 0:
+// FP32 products, 1: split-BF16 (bf16x3) products with FP32 accumulation.
 void launchDct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
-               const SpectralArgs& sp, hipStream_t s);
+               int precision, const SpectralArgs& sp, hipStream_t s);
 
 // ---- field gather + gradient assembly ---------------------------------------------------------
 struct GatherArgs {
