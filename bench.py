@@ -249,7 +249,7 @@ def main() -> int:
         avg_us = live["total_us"] / live["launches"]
         achieved = live["work"] / (live["total_us"] * 1e-6) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_wire"), "kernel": "k_wire",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_wire") if args.workload == "large" else None, "kernel": "k_wire",
                 "avg_launch_us": avg_us, "launches": live["launches"],
                 "algorithmic_bytes_per_launch": live["work"] / live["launches"]}
     _, res = sess.finish()
