@@ -348,6 +348,45 @@ mp_gp_status mp_gp_server_wait(mp_gp_server* server, int64_t job, double* center
 /* Finishes the running jobs, drops the waiting ones and frees the service. */
 void mp_gp_server_destroy(mp_gp_server* server);
 
+/* ---- detailed placement: four-cell window reordering on the device -----------------------
+ * Replaces tryRunCudaDetailedPlacement (src/placement/CudaDetailedPlacementBackend.hpp:14-17,
+ * cuda/CudaDetailedPlacementBackend.cu:523-711) with DetailedPlacementOptions /
+ * DetailedPlacementResult (include/myplacement/placement/DetailedPlacer.hpp:25-57) as C structs.
+ * Runs on a legal placement: windows of four abutting standard cells are reordered in place. */
+enum { MP_DP_NONE = 0, MP_DP_ADJACENT_SWAP = 1, MP_DP_WINDOW_REORDER = 2 };
+
+typedef struct mp_dp_options {
+    int32_t method;               /* MP_DP_WINDOW_REORDER (the only method the device pass runs) */
+    int32_t passes;               /* 2: staggered window alignments */
+    int32_t window_size;          /* 4 (the only size the device pass runs) */
+    uint64_t maximum_net_degree;  /* 64: windows touching a larger net are left as they are */
+    double improvement_epsilon;   /* 1e-9 */
+    int32_t device;               /* HIP device ordinal (0) */
+    uint64_t maximum_device_bytes;/* 2 GiB */
+} mp_dp_options;
+
+typedef struct mp_dp_result {
+    double hpwl_before, hpwl_after;      /* calculateHpwl of the whole design */
+    double elapsed_seconds;
+    uint64_t evaluated_windows;
+    uint64_t evaluated_permutations;
+    uint64_t accepted_operations;
+    int32_t completed_passes;
+    int32_t device_used;                 /* -1 when no device pass ran (method none, no rows) */
+    uint64_t reserved_device_bytes;      /* peak */
+} mp_dp_result;
+
+void mp_dp_default_options(mp_dp_options* options);
+/* Starts from design->center / design->orientation; on success writes the placement to centers
+ * [2M] and orientation [M] (may be NULL).  row_orientation [R]: orientation code of each row that
+ * placed cells take (NULL: North, the SiteRow default; 8 = Unknown keeps the cell's own).
+ * MP_GP_INVALID_ARGUMENT for options DetailedPlacer::run rejects (DetailedPlacer.cpp:50-71);
+ * MP_GP_UNAVAILABLE (with mp_gp_last_error) for another method or window size, or no usable
+ * device -- the caller then runs the CPU reference placer, as the reference does.  The output
+ * arrays are untouched unless the call succeeds. */
+mp_gp_status mp_dp_run(const mp_design* design, const int32_t* row_orientation, const mp_dp_options* options,
+                       double* centers, int32_t* orientation, mp_dp_result* result);
+
 #ifdef __cplusplus
 }
 #endif

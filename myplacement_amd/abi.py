@@ -216,6 +216,48 @@ class MpQpResult(C.Structure):
         return d
 
 
+MP_DP_NONE, MP_DP_ADJACENT_SWAP, MP_DP_WINDOW_REORDER = 0, 1, 2
+
+
+class MpDpOptions(C.Structure):
+    """mp_dp_options: DetailedPlacementOptions (DetailedPlacer.hpp:25-45)."""
+    _fields_ = [
+        ("method", C.c_int32),
+        ("passes", C.c_int32),
+        ("window_size", C.c_int32),
+        ("maximum_net_degree", C.c_uint64),
+        ("improvement_epsilon", C.c_double),
+        ("device", C.c_int32),
+        ("maximum_device_bytes", C.c_uint64),
+    ]
+
+
+def default_dp_options(**overrides) -> MpDpOptions:
+    o = MpDpOptions(method=MP_DP_WINDOW_REORDER, passes=2, window_size=4, maximum_net_degree=64,
+                    improvement_epsilon=1e-9, device=0, maximum_device_bytes=2 << 30)
+    for key, value in overrides.items():
+        setattr(o, key, value)
+    return o
+
+
+class MpDpResult(C.Structure):
+    """mp_dp_result: DetailedPlacementResult (DetailedPlacer.hpp:47-57)."""
+    _fields_ = [
+        ("hpwl_before", C.c_double),
+        ("hpwl_after", C.c_double),
+        ("elapsed_seconds", C.c_double),
+        ("evaluated_windows", C.c_uint64),
+        ("evaluated_permutations", C.c_uint64),
+        ("accepted_operations", C.c_uint64),
+        ("completed_passes", C.c_int32),
+        ("device_used", C.c_int32),
+        ("reserved_device_bytes", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class MpServerOptions(C.Structure):
     _fields_ = [
         ("device", C.c_int32),

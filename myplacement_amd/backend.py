@@ -66,6 +66,9 @@ def library():
                                             C.POINTER(C.c_int64)]
         lib.mp_gp_server_wait.argtypes = [V, C.c_int64, V, C.POINTER(abi.MpJobReport)]
         lib.mp_gp_server_destroy.argtypes = [V]
+        lib.mp_dp_default_options.argtypes = [C.POINTER(abi.MpDpOptions)]
+        lib.mp_dp_run.argtypes = [C.POINTER(abi.MpDesign), V, C.POINTER(abi.MpDpOptions), V, V,
+                                  C.POINTER(abi.MpDpResult)]
         lib.mp_gp_quadratic_placement.argtypes = [C.POINTER(abi.MpDesign), C.POINTER(abi.MpQpOptions), V,
                                                   C.POINTER(abi.MpQpResult)]
         lib.mp_gp_session_create.argtypes = [V, C.POINTER(abi.MpOptions), V, C.POINTER(V)]
@@ -397,6 +400,24 @@ def global_place(design, options: abi.MpOptions, center=None, history_capacity: 
     _raise(library().mp_gp_global_place(C.byref(d), C.byref(options), _p(c), C.byref(r), hist, history_capacity))
     n = min(r.completed_iterations, history_capacity)
     return c, r.as_dict(), [hist[i].as_dict() for i in range(n)]
+
+
+def detailed_place(design, options: abi.MpDpOptions | None = None, center=None, orientation=None,
+                   row_orientation=None):
+    """Window-reordering detailed placement on the device (tryRunCudaDetailedPlacement,
+    CudaDetailedPlacementBackend.hpp:14-17) from a legal placement; returns (centres,
+    orientations, result dict).  BackendUnavailable for a method or window size the device pass
+    does not run, as the reference's pass declines them."""
+    o = abi.default_dp_options() if options is None else options
+    c = np.ascontiguousarray(design.center if center is None else center, dtype=np.float64).reshape(-1, 2).copy()
+    ori = np.ascontiguousarray(design.orientation if orientation is None else orientation, dtype=np.int32).copy()
+    d = design.to_c(c, ori)
+    ro = None if row_orientation is None else np.ascontiguousarray(row_orientation, dtype=np.int32)
+    out_c = c.copy()
+    out_o = ori.copy()
+    r = abi.MpDpResult()
+    _raise(library().mp_dp_run(C.byref(d), _p(ro), C.byref(o), _p(out_c), _p(out_o), C.byref(r)))
+    return out_c, out_o, r.as_dict()
 
 
 def estimate_device_bytes(design, options: abi.MpOptions) -> int:

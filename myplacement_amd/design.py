@@ -74,14 +74,16 @@ class Design:
         x_end = r[:, 2] + r[:, 4] * r[:, 3]
         return (float(r[:, 2].min()), float(r[:, 0].min()), float(x_end.max()), float((r[:, 0] + r[:, 1]).max()))
 
-    def to_c(self, center: np.ndarray | None = None) -> abi.MpDesign:
-        """A C view of this design; the arrays stay owned by `self`."""
+    def to_c(self, center: np.ndarray | None = None, orientation: np.ndarray | None = None) -> abi.MpDesign:
+        """A C view of this design (with other centres / orientations if given); the arrays stay
+        owned by `self`."""
         c = self.center if center is None else np.ascontiguousarray(center, dtype=np.float64).reshape(-1, 2)
-        self._keep = [c]
+        o = self.orientation if orientation is None else np.ascontiguousarray(orientation, dtype=np.int32)
+        self._keep = [c, o]
         return abi.MpDesign(
             module_count=self.module_count,
             width=abi.dptr(self.width), height=abi.dptr(self.height), center=abi.dptr(c),
-            orientation=abi.i32ptr(self.orientation), is_terminal=abi.u8ptr(self.is_terminal),
+            orientation=abi.i32ptr(o), is_terminal=abi.u8ptr(self.is_terminal),
             is_fixed=abi.u8ptr(self.is_fixed), net_count=self.net_count,
             net_weight=abi.dptr(self.net_weight), net_pin_start=abi.i64ptr(self.net_pin_start),
             pin_count=self.pin_count, pin_module=abi.i32ptr(self.pin_module),

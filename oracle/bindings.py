@@ -66,6 +66,8 @@ def oracle():
                                         C.POINTER(C.c_int32), C.POINTER(abi.MpDensityMetrics),
                                         C.POINTER(abi.MpDensityMetrics), C.c_void_p]
         lib.orc_set_method.argtypes = [C.c_int]
+        lib.orc_detailed_place.argtypes = [C.POINTER(abi.MpDesign), C.c_void_p, C.POINTER(abi.MpDpOptions), _dp,
+                                           C.c_void_p, C.POINTER(abi.MpDpResult)]
         _oracle = lib
     return _oracle
 
@@ -90,6 +92,11 @@ def reference():
         lib.ref_db_destroy.argtypes = [C.c_void_p]
         lib.ref_db_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         lib.ref_db_export.argtypes = [C.c_void_p] + [C.c_void_p] * 12
+        lib.ref_legalize.argtypes = [C.POINTER(abi.MpDesign), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _dp,
+                                     C.c_void_p, _dp]
+        lib.ref_legality.argtypes = [C.POINTER(abi.MpDesign), C.c_double, _dp]
+        lib.ref_detailed_place.argtypes = [C.POINTER(abi.MpDesign), C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_double,
+                                           _dp, C.c_void_p, _dp]
         _ref = lib
     return _ref
 
@@ -267,3 +274,79 @@ def global_place(design, options: abi.MpOptions, center=None, history_capacity: 
                                      history_capacity, C.byref(n)), oracle())
     rows = [hist[i].as_dict() for i in range(min(n.value, history_capacity))]
     return c, res.as_dict(), rows
+
+
+# ---------------------------------------------------------------------------------------------
+# after global placement: the reference's own legalizer and CPU detailed placer (oracle/_ref),
+# and the restatement of the device window-reordering pass
+
+LEGALITY_FIELDS = ("legal", "placed_standard_cells", "unplaced_standard_cells", "overlap_pairs",
+                   "out_of_core_modules", "off_row_modules", "abacus_reverse_pass_selected",
+                   "standard_cell_total_displacement", "standard_cell_total_squared_displacement",
+                   "standard_cell_weighted_squared_displacement", "standard_cell_maximum_displacement",
+                   "elapsed_seconds")
+
+
+def _placed(design, center, orientation):
+    c = np.ascontiguousarray(design.center if center is None else center, dtype=np.float64).reshape(-1, 2).copy()
+    o = np.ascontiguousarray(design.orientation if orientation is None else orientation, dtype=np.int32).copy()
+    return c, o
+
+
+def detailed_place(design, options: abi.MpDpOptions, center=None, orientation=None, row_orientation=None):
+    """The reference's device window-reordering pass restated on the CPU (gp_oracle.cpp
+    detailedPlace); returns (centres, orientations, result dict)."""
+    c, o = _placed(design, center, orientation)
+    d = design.to_c(c, o)
+    ro = None if row_orientation is None else np.ascontiguousarray(row_orientation, dtype=np.int32)
+    out_c, out_o = c.copy(), o.copy()
+    r = abi.MpDpResult()
+    _check(oracle().orc_detailed_place(C.byref(d), _p(ro), C.byref(options), abi.dptr(out_c), _p(out_o), C.byref(r)),
+           oracle())
+    return out_c, out_o, r.as_dict()
+
+
+def ref_legalize(design, center=None, orientation=None, strategy: int = 0, candidate_rows: int = 8,
+                 macro_search_radius: int = 80, bidirectional: bool = True, epsilon: float = 1e-6):
+    """Legalizer::legalize of the reference (Legalizer.cpp:707-738) with LegalizationOptions'
+    defaults; returns (centres, orientations, report dict)."""
+    lib = reference()
+    if lib is None:
+        raise OracleError("oracle/_ref was not built (the reference tree is absent)")
+    c, o = _placed(design, center, orientation)
+    d = design.to_c(c, o)
+    out_c, out_o = c.copy(), o.copy()
+    rep = np.zeros(12)
+    _check(lib.ref_legalize(C.byref(d), strategy, candidate_rows, macro_search_radius, int(bidirectional), epsilon,
+                            abi.dptr(out_c), _p(out_o), abi.dptr(rep)), lib)
+    return out_c, out_o, dict(zip(LEGALITY_FIELDS, rep.tolist()))
+
+
+def ref_legality(design, center=None, orientation=None, epsilon: float = 1e-6) -> dict:
+    """Legalizer::check of the reference (Legalizer.cpp:740-805)."""
+    lib = reference()
+    if lib is None:
+        raise OracleError("oracle/_ref was not built (the reference tree is absent)")
+    c, o = _placed(design, center, orientation)
+    d = design.to_c(c, o)
+    rep = np.zeros(4)
+    _check(lib.ref_legality(C.byref(d), epsilon, abi.dptr(rep)), lib)
+    return dict(zip(("legal", "overlap_pairs", "out_of_core_modules", "off_row_modules"), rep.tolist()))
+
+
+def ref_detailed_place(design, center=None, orientation=None, method: int = abi.MP_DP_WINDOW_REORDER, passes: int = 2,
+                       window_size: int = 4, maximum_net_degree: int = 64, improvement_epsilon: float = 1e-9):
+    """DetailedPlacer::run of the reference on the CPU (DetailedPlacer.cpp:289-335); returns
+    (centres, orientations, result dict)."""
+    lib = reference()
+    if lib is None:
+        raise OracleError("oracle/_ref was not built (the reference tree is absent)")
+    c, o = _placed(design, center, orientation)
+    d = design.to_c(c, o)
+    out_c, out_o = c.copy(), o.copy()
+    res = np.zeros(7)
+    _check(lib.ref_detailed_place(C.byref(d), method, passes, window_size, maximum_net_degree, improvement_epsilon,
+                                  abi.dptr(out_c), _p(out_o), abi.dptr(res)), lib)
+    keys = ("hpwl_before", "hpwl_after", "evaluated_windows", "evaluated_permutations", "accepted_operations",
+            "completed_passes", "elapsed_seconds")
+    return out_c, out_o, dict(zip(keys, res.tolist()))

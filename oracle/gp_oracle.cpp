@@ -1421,6 +1421,229 @@ QuadraticRun quadraticPlacement(Db& db, const mp_qp_options& o) {
     return run;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Detailed placement, window reordering as the reference's device pass runs it
+// (src/placement/cuda/CudaDetailedPlacementBackend.cu:523-711): per pass, row segments and
+// four-cell windows at the current placement; every eligible window's 24 orderings scored
+// against the placement at the start of the pass (the device kernel :144-220, restated here on
+// the CPU); then, in window order, the best ordering re-scored on the current placement and
+// applied when it improves by more than epsilon; then the other windows searched exhaustively
+// (optimizeCpuWindow :400-424, the same search as DetailedPlacer.cpp:186-236).
+struct DpRow {
+    double bottom, height, x_start, spacing;
+    int orientation;
+};
+
+struct DpWork {
+    Db db;
+    std::vector<DpRow> rows;
+    std::vector<std::size_t> by_bottom;               // sortedRows, DetailedPlacer.cpp:80-87
+    std::vector<std::vector<std::size_t>> pins_of;    // Module::pins
+    std::vector<std::size_t> net_of;                  // Pin::net
+
+    static double alignUp(double value, double origin, double step) {  // DetailedPlacer.cpp:73-76
+        if (step <= kEpsilon) return value;
+        return origin + std::ceil((value - origin - kEpsilon) / step) * step;
+    }
+    double rowWidth(std::size_t m, std::size_t r) const { return alignUp(db.w[m], 0.0, rows[r].spacing); }
+    double llx(std::size_t m) const { return db.c[m].x - db.w[m] * 0.5; }
+    double lly(std::size_t m) const { return db.c[m].y - db.h[m] * 0.5; }
+
+    std::size_t nearestRow(double y) const {  // DetailedPlacer.cpp:89-104
+        auto it = std::lower_bound(by_bottom.begin(), by_bottom.end(), y, [&](std::size_t r, double v) {
+            return rows[r].bottom + rows[r].height * 0.5 < v;
+        });
+        if (it == by_bottom.begin()) return *it;
+        if (it == by_bottom.end()) return by_bottom.back();
+        const std::size_t right = *it, left = *(it - 1);
+        const double dl = std::abs(rows[left].bottom + rows[left].height * 0.5 - y);
+        const double dr = std::abs(rows[right].bottom + rows[right].height * 0.5 - y);
+        return dl <= dr ? left : right;
+    }
+
+    // buildRowSegments, DetailedPlacer.cpp:106-145
+    std::vector<std::pair<std::size_t, std::vector<std::size_t>>> segments(double eps) const {
+        std::vector<std::pair<std::size_t, std::vector<std::size_t>>> out;
+        if (rows.empty()) return out;
+        std::vector<std::vector<std::size_t>> cells(rows.size());
+        for (std::size_t m = 0; m < db.modules(); ++m) {
+            if (db.fixed[m] || db.macro[m]) continue;
+            const std::size_t r = nearestRow(db.c[m].y);
+            if (std::abs(lly(m) - rows[r].bottom) > eps) continue;
+            cells[r].push_back(m);
+        }
+        for (std::size_t r = 0; r < cells.size(); ++r) {
+            std::vector<std::size_t>& v = cells[r];
+            if (v.size() < 2U) continue;
+            std::sort(v.begin(), v.end(), [&](std::size_t a, std::size_t b) { return llx(a) < llx(b); });
+            std::vector<std::size_t> cur;
+            for (const std::size_t m : v) {
+                if (!cur.empty() && std::abs(llx(m) - (llx(cur.back()) + rowWidth(cur.back(), r))) > eps) {
+                    if (cur.size() >= 2U) out.emplace_back(r, cur);
+                    cur.clear();
+                }
+                cur.push_back(m);
+            }
+            if (cur.size() >= 2U) out.emplace_back(r, cur);
+        }
+        return out;
+    }
+
+    std::vector<std::size_t> netsOf(const std::vector<std::size_t>& window) const {
+        std::vector<std::size_t> nets;
+        for (const std::size_t m : window)
+            for (const std::size_t p : pins_of[m]) nets.push_back(net_of[p]);
+        std::sort(nets.begin(), nets.end());
+        nets.erase(std::unique(nets.begin(), nets.end()), nets.end());
+        return nets;
+    }
+    std::size_t degree(std::size_t n) const { return static_cast<std::size_t>(db.net_start[n + 1] - db.net_start[n]); }
+
+    // Window HPWL with the window's cells at `at` (centres) and pin offsets oriented by `orient`.
+    double windowHpwl(const std::vector<std::size_t>& window, const std::vector<std::size_t>& nets,
+                      const std::vector<V2>& at, const std::vector<V2>& centres, const std::vector<int>& orient) const {
+        double total = 0.0;
+        for (const std::size_t n : nets) {
+            double lox = std::numeric_limits<double>::infinity(), hix = -lox, loy = lox, hiy = -lox;
+            for (int64_t p = db.net_start[n]; p < db.net_start[n + 1]; ++p) {
+                const std::size_t m = static_cast<std::size_t>(db.pin_module[static_cast<std::size_t>(p)]);
+                V2 c = centres[m];
+                for (std::size_t k = 0; k < window.size(); ++k)
+                    if (window[k] == m) {
+                        c = at[k];
+                        break;
+                    }
+                const V2 q = c + transformOffset(db.pin_offset[static_cast<std::size_t>(p)], orient[m]);
+                lox = std::min(lox, q.x);
+                hix = std::max(hix, q.x);
+                loy = std::min(loy, q.y);
+                hiy = std::max(hiy, q.y);
+            }
+            total += db.net_weight[n] * ((hix - lox) + (hiy - loy));
+        }
+        return total;
+    }
+
+    // centersForOrder from the window's first lower-left x in `centres`
+    std::vector<V2> packed(const std::vector<std::size_t>& window, std::size_t r, const std::vector<std::size_t>& order,
+                           const std::vector<V2>& centres) const {
+        std::vector<V2> at(window.size());
+        double cursor = centres[window.front()].x - db.w[window.front()] * 0.5;
+        for (const std::size_t local : order) {
+            const std::size_t m = window[local];
+            at[local] = {cursor + db.w[m] * 0.5, rows[r].bottom + db.h[m] * 0.5};
+            cursor += rowWidth(m, r);
+        }
+        return at;
+    }
+
+    void apply(const std::vector<std::size_t>& window, std::size_t r, const std::vector<std::size_t>& order) {
+        const std::vector<V2> at = packed(window, r, order, db.c);
+        for (std::size_t k = 0; k < window.size(); ++k) {
+            db.c[window[k]] = at[k];
+            if (rows[r].orientation != 8) db.orient[window[k]] = rows[r].orientation;
+        }
+    }
+};
+
+mp_dp_result detailedPlace(DpWork& w, const mp_dp_options& o) {
+    mp_dp_result r{};
+    r.device_used = -1;
+    r.hpwl_before = designHpwl(w.db);
+    if (o.method == MP_DP_NONE || w.rows.empty()) {
+        r.hpwl_after = r.hpwl_before;
+        return r;
+    }
+    if (o.method != MP_DP_WINDOW_REORDER || o.window_size != 4)
+        throw std::runtime_error("the device pass runs four-cell window reordering only");
+    r.device_used = o.device;
+    for (int pass = 0; pass < o.passes; ++pass) {
+        const auto segs = w.segments(1e-6);
+        const std::size_t phase = static_cast<std::size_t>((pass % 2) * 2);
+        struct Win {
+            std::size_t row;
+            std::vector<std::size_t> cells, nets;
+        };
+        std::vector<Win> dev, cpu;
+        for (const auto& seg : segs) {
+            for (std::size_t start = phase; start + 1U < seg.second.size(); start += 4) {
+                const std::size_t count = std::min<std::size_t>(4, seg.second.size() - start);
+                if (count < 2U) break;
+                Win win{seg.first, std::vector<std::size_t>(seg.second.begin() + static_cast<std::ptrdiff_t>(start),
+                                                             seg.second.begin() + static_cast<std::ptrdiff_t>(start + count)),
+                        {}};
+                win.nets = w.netsOf(win.cells);
+                bool eligible = win.cells.size() == 4U && !win.nets.empty();
+                for (const std::size_t n : win.nets)
+                    if (w.degree(n) > o.maximum_net_degree) eligible = false;
+                (eligible ? dev : cpu).push_back(std::move(win));
+            }
+        }
+        // scores of every ordering at the placement of the start of the pass
+        const std::vector<V2> start_c = w.db.c;
+        const std::vector<int> start_o = w.db.orient;
+        std::vector<int> best(dev.size(), 0);
+        for (std::size_t k = 0; k < dev.size(); ++k) {
+            std::vector<std::size_t> order{0, 1, 2, 3};
+            double best_value = std::numeric_limits<double>::infinity();
+            int index = 0;
+            do {
+                const double v = w.windowHpwl(dev[k].cells, dev[k].nets, w.packed(dev[k].cells, dev[k].row, order, start_c),
+                                              start_c, start_o);
+                if (v < best_value) {  // strictly lower: ties keep the lower index
+                    best_value = v;
+                    best[k] = index;
+                }
+                ++index;
+            } while (std::next_permutation(order.begin(), order.end()));
+        }
+        for (std::size_t k = 0; k < dev.size(); ++k) {
+            ++r.evaluated_windows;
+            r.evaluated_permutations += 24;
+            if (best[k] <= 0) continue;
+            std::vector<std::size_t> order{0, 1, 2, 3};
+            for (int s = 0; s < best[k]; ++s) std::next_permutation(order.begin(), order.end());
+            std::vector<V2> here(4);
+            for (std::size_t j = 0; j < 4; ++j) here[j] = w.db.c[dev[k].cells[j]];
+            const double before = w.windowHpwl(dev[k].cells, dev[k].nets, here, w.db.c, w.db.orient);
+            const double after =
+                w.windowHpwl(dev[k].cells, dev[k].nets, w.packed(dev[k].cells, dev[k].row, order, w.db.c), w.db.c, w.db.orient);
+            if (after < before - o.improvement_epsilon) {
+                w.apply(dev[k].cells, dev[k].row, order);
+                ++r.accepted_operations;
+            }
+        }
+        for (const Win& win : cpu) {
+            ++r.evaluated_windows;
+            bool skip = win.nets.empty();
+            for (const std::size_t n : win.nets)
+                if (w.degree(n) > o.maximum_net_degree) skip = true;
+            if (skip) continue;
+            std::vector<V2> here(win.cells.size());
+            for (std::size_t j = 0; j < win.cells.size(); ++j) here[j] = w.db.c[win.cells[j]];
+            double best_value = w.windowHpwl(win.cells, win.nets, here, w.db.c, w.db.orient);
+            std::vector<std::size_t> order(win.cells.size()), best_order;
+            std::iota(order.begin(), order.end(), std::size_t{0});
+            do {
+                const double v = w.windowHpwl(win.cells, win.nets, w.packed(win.cells, win.row, order, w.db.c), w.db.c,
+                                              w.db.orient);
+                ++r.evaluated_permutations;
+                if (v < best_value - o.improvement_epsilon) {
+                    best_value = v;
+                    best_order = order;
+                }
+            } while (std::next_permutation(order.begin(), order.end()));
+            if (!best_order.empty()) {
+                w.apply(win.cells, win.row, best_order);
+                ++r.accepted_operations;
+            }
+        }
+        r.completed_passes = pass + 1;
+    }
+    r.hpwl_after = designHpwl(w.db);
+    return r;
+}
+
 }  // namespace oracle
 
 using namespace oracle;
@@ -1558,6 +1781,43 @@ int orc_quadratic_placement(const mp_design* design, const mp_qp_options* option
             centers[2 * m + 1] = db.c[m].y;
         }
         if (result) *result = r;
+    });
+}
+
+// Window-reordering detailed placement as the reference's device pass runs it (see detailedPlace).
+int orc_detailed_place(const mp_design* design, const int32_t* row_orientation, const mp_dp_options* options,
+                       double* centers, int32_t* orientation, mp_dp_result* result) {
+    return guarded([&] {
+        if (!options || !centers || !result) throw std::invalid_argument("null options, centres or result");
+        const mp_dp_options& o = *options;
+        if (o.method < MP_DP_NONE || o.method > MP_DP_WINDOW_REORDER || !std::isfinite(o.improvement_epsilon) ||
+            o.improvement_epsilon < 0.0 ||
+            (o.method != MP_DP_NONE && (o.passes <= 0 || o.maximum_net_degree < 2U)))  // DetailedPlacer.cpp:50-71
+            throw std::invalid_argument("Detailed placement options are outside their valid range.");
+        DpWork w;
+        w.db = loadDesign(design);
+        for (int64_t r = 0; r < design->row_count; ++r) {
+            const double* row = design->rows + 5 * r;
+            w.rows.push_back({row[0], row[1], row[2], row[3], row_orientation ? row_orientation[r] : 0});
+        }
+        w.by_bottom.resize(w.rows.size());
+        std::iota(w.by_bottom.begin(), w.by_bottom.end(), std::size_t{0});
+        std::sort(w.by_bottom.begin(), w.by_bottom.end(),
+                  [&](std::size_t a, std::size_t b) { return w.rows[a].bottom < w.rows[b].bottom; });
+        w.pins_of.assign(w.db.modules(), {});
+        w.net_of.assign(w.db.pin_module.size(), 0);
+        for (std::size_t n = 0; n + 1 < w.db.net_start.size(); ++n)
+            for (int64_t p = w.db.net_start[n]; p < w.db.net_start[n + 1]; ++p) w.net_of[static_cast<std::size_t>(p)] = n;
+        for (std::size_t p = 0; p < w.db.pin_module.size(); ++p) w.pins_of[static_cast<std::size_t>(w.db.pin_module[p])].push_back(p);
+        const auto t0 = std::chrono::steady_clock::now();
+        mp_dp_result r = detailedPlace(w, o);
+        r.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (std::size_t m = 0; m < w.db.modules(); ++m) {
+            centers[2 * m] = w.db.c[m].x;
+            centers[2 * m + 1] = w.db.c[m].y;
+            if (orientation) orientation[m] = w.db.orient[m];
+        }
+        *result = r;
     });
 }
 

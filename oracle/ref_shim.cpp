@@ -3,7 +3,9 @@
 // TEST INFRASTRUCTURE.  Compiled together with these unmodified reference sources, read in place
 // from the reference tree by oracle/Makefile (output: oracle/_ref/libmpref.so):
 //   src/model/PlacementDatabase.cpp, src/metrics/Metrics.cpp,
-//   src/placement/GlobalPlacementSupport.cpp, src/io/BookshelfParser.cpp
+//   src/placement/GlobalPlacementSupport.cpp, src/io/BookshelfParser.cpp,
+//   src/placement/Legalizer.cpp, src/placement/DetailedPlacer.cpp and the reference's own
+//   CPU-only stub src/placement/CudaDetailedPlacementBackendStub.cpp
 // The rest of the reference GP path (ElectrostaticField.cpp needs FFTW3, InitialPlacer.cpp needs
 // Eigen, AdaptiveGlobalPlacer.cpp links both) is not buildable in this image.
 
@@ -12,6 +14,8 @@
 #include "myplacement/io/BookshelfParser.hpp"
 #include "myplacement/metrics/Metrics.hpp"
 #include "myplacement/model/PlacementDatabase.hpp"
+#include "myplacement/placement/DetailedPlacer.hpp"
+#include "myplacement/placement/Legalizer.hpp"
 #include "GlobalPlacementSupport.hpp"
 
 #include <cstring>
@@ -277,6 +281,74 @@ int ref_db_export(const ref_db* h, double* width, double* height, double* center
         core[1] = db.core_region.ll.y;
         core[2] = db.core_region.ur.x;
         core[3] = db.core_region.ur.y;
+    });
+}
+
+// Legalizer::legalize on the design (Legalizer.cpp:707-738).  report[12]: legal, placed,
+// unplaced, overlap pairs, out-of-core, off-row, reverse pass selected, total / squared / weighted
+// squared / maximum displacement, elapsed seconds.
+int ref_legalize(const mp_design* design, int strategy, int candidate_rows, int macro_search_radius,
+                 int bidirectional, double epsilon, double* centers, int32_t* orientation, double* report) {
+    return guarded([&] {
+        PlacementDatabase db = buildDatabase(design);
+        LegalizationOptions o;
+        o.strategy = strategy == 0 ? LegalizationStrategy::Abacus : LegalizationStrategy::GreedyIsotonic;
+        o.candidate_rows = candidate_rows;
+        o.macro_search_radius = macro_search_radius;
+        o.abacus_bidirectional = bidirectional != 0;
+        o.epsilon = epsilon;
+        const LegalityReport r = Legalizer().legalize(db, o);
+        const double v[12] = {r.legal ? 1.0 : 0.0, static_cast<double>(r.placed_standard_cells),
+                              static_cast<double>(r.unplaced_standard_cells), static_cast<double>(r.overlap_pairs),
+                              static_cast<double>(r.out_of_core_modules), static_cast<double>(r.off_row_modules),
+                              r.abacus_reverse_pass_selected ? 1.0 : 0.0, r.standard_cell_total_displacement,
+                              r.standard_cell_total_squared_displacement, r.standard_cell_weighted_squared_displacement,
+                              r.standard_cell_maximum_displacement, r.elapsed_seconds};
+        std::memcpy(report, v, sizeof v);
+        for (std::size_t m = 0; m < db.modules.size(); ++m) {
+            centers[2 * m] = db.modules[m].center.x;
+            centers[2 * m + 1] = db.modules[m].center.y;
+            if (orientation) orientation[m] = orientationIndex(db.modules[m].orientation);
+        }
+    });
+}
+
+// Legalizer::check (Legalizer.cpp:740-805); report[4]: legal, overlap pairs, out-of-core, off-row.
+int ref_legality(const mp_design* design, double epsilon, double* report) {
+    return guarded([&] {
+        const PlacementDatabase db = buildDatabase(design);
+        const LegalityReport r = Legalizer().check(db, epsilon);
+        report[0] = r.legal ? 1.0 : 0.0;
+        report[1] = static_cast<double>(r.overlap_pairs);
+        report[2] = static_cast<double>(r.out_of_core_modules);
+        report[3] = static_cast<double>(r.off_row_modules);
+    });
+}
+
+// DetailedPlacer::run on the CPU (DetailedPlacer.cpp:289-335).  result[7]: hpwl before / after,
+// evaluated windows, evaluated permutations, accepted operations, completed passes, elapsed.
+int ref_detailed_place(const mp_design* design, int method, int passes, int window_size, uint64_t maximum_net_degree,
+                       double improvement_epsilon, double* centers, int32_t* orientation, double* result) {
+    return guarded([&] {
+        PlacementDatabase db = buildDatabase(design);
+        DetailedPlacementOptions o;
+        o.method = method == 0 ? DetailedPlacementMethod::None
+                               : (method == 1 ? DetailedPlacementMethod::AdjacentSwap : DetailedPlacementMethod::WindowReorder);
+        o.passes = passes;
+        o.window_size = window_size;
+        o.maximum_net_degree = static_cast<std::size_t>(maximum_net_degree);
+        o.improvement_epsilon = improvement_epsilon;
+        o.compute_backend = DetailedPlacementBackend::Cpu;
+        const DetailedPlacementResult r = DetailedPlacer().run(db, o);
+        const double v[7] = {r.hpwl_before, r.hpwl_after, static_cast<double>(r.evaluated_windows),
+                             static_cast<double>(r.evaluated_permutations), static_cast<double>(r.accepted_operations),
+                             static_cast<double>(r.completed_passes), r.elapsed_seconds};
+        std::memcpy(result, v, sizeof v);
+        for (std::size_t m = 0; m < db.modules.size(); ++m) {
+            centers[2 * m] = db.modules[m].center.x;
+            centers[2 * m + 1] = db.modules[m].center.y;
+            if (orientation) orientation[m] = orientationIndex(db.modules[m].orientation);
+        }
     });
 }
 

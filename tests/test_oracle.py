@@ -24,6 +24,11 @@ Parity status of each oracle function (oracle/gp_oracle.cpp), and what pins it:
                                                                                       -> pinned
   adaptive controller                no numeric pin exists in the reference; invariants of
                                      PlacementFlowTests.cpp:153-216 only              -> parity unpinned
+  window-reordering detailed pass    the reference's fixture (DetailedPlacementTests.cpp:55-70);
+                                     bit-equal to the reference's own CPU placer (built from
+                                     DetailedPlacer.cpp) when no two windows of a pass share a net;
+                                     the same windows and orderings visited on a legalized
+                                     placement                                        -> pinned
 """
 
 import ctypes as C
@@ -512,3 +517,67 @@ def test_quadratic_invariants_like_the_reference():
     assert rp["hpwl_after"] < h_centre
     with pytest.raises(ValueError):
         ob.quadratic_placement(p, abi.default_qp_options(minimum_distance=0.0))
+
+
+# ---- detailed placement: the device window-reordering pass ----------------------------------
+
+def test_window_reordering_fixture_like_the_reference():
+    """tests/DetailedPlacementTests.cpp:55-70: one pass, one window, 24 orderings, one accepted
+    reorder, HPWL 20 -> 10; the reference's own CPU placer gives the same."""
+    from dp_designs import reordering_fixture
+    d = reordering_fixture()
+    _, _, r = ob.detailed_place(d, abi.default_dp_options(passes=1))
+    assert r["accepted_operations"] == 1 and r["evaluated_permutations"] == 24 and r["evaluated_windows"] == 1
+    assert r["hpwl_before"] == 20.0 and abs(r["hpwl_after"] - 10.0) < 1e-12
+    if ob.reference() is not None:
+        _, _, rr = ob.ref_detailed_place(d, passes=1)
+        assert rr["hpwl_after"] == r["hpwl_after"] and rr["accepted_operations"] == 1
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_window_reordering_matches_reference_placer_when_windows_share_no_net(ref, seed):
+    """With every net inside one phase-0 window plus a fixed terminal, scoring the windows at the
+    start of the pass (the device pass) and one after another (the reference's CPU placer,
+    DetailedPlacer.cpp:289-335, built from its own source) must give the same placement bit for
+    bit: segments, windows, orderings, the re-scoring and the apply rule all coincide.  (The
+    terminal anchors each net: without it, orderings that only translate a net score equal up to
+    rounding, and the device pass, which takes the lowest score, and the CPU placer, which keeps the
+    first ordering better by more than epsilon, may pick different ones of equal HPWL.)"""
+    from dp_designs import abutted_design
+    d = abutted_design(seed=seed, private=True, terminal_share=1.0)
+    c, o, r = ob.detailed_place(d, abi.default_dp_options(passes=1))
+    rc, ro, rr = ob.ref_detailed_place(d, passes=1)
+    assert r["accepted_operations"] > 0
+    assert np.array_equal(c, rc) and np.array_equal(o, ro)
+    for k in ("evaluated_windows", "evaluated_permutations", "accepted_operations"):
+        assert r[k] == rr[k], k
+    assert r["hpwl_after"] == rr["hpwl_after"]
+
+
+def test_window_reordering_on_a_legalized_placement(ref):
+    """Oracle GP -> the reference's Abacus legalizer -> the device pass: still legal, HPWL not
+    worse, the same windows and orderings visited as the reference's CPU placer (whose sequential
+    scoring may accept a few different reorders), and a final HPWL within 0.1 % of it."""
+    d = design.plain_design(seed=2026)
+    d.center = ob.random_placement(d, 2026)
+    c, _, _ = ob.global_place(d, abi.default_options(target_density=0.9, bins_x=64, bins_y=64, iterations=120))
+    lc, lo, rep = ob.ref_legalize(d, c)
+    assert rep["legal"] == 1.0
+    oc, oo, r = ob.detailed_place(d, abi.default_dp_options(), lc, lo)
+    _, _, rr = ob.ref_detailed_place(d, lc, lo)
+    assert ob.ref_legality(d, oc, oo)["legal"] == 1.0
+    assert r["hpwl_after"] <= r["hpwl_before"] and r["accepted_operations"] > 0
+    assert r["evaluated_windows"] == rr["evaluated_windows"]
+    assert r["evaluated_permutations"] == rr["evaluated_permutations"]
+    assert abs(r["hpwl_after"] / rr["hpwl_after"] - 1.0) < 1e-3
+    h, _ = ob.design_metrics(d, oc, 8, 8, 0.9)
+    assert abs(h - r["hpwl_after"]) < 1e-9 * h
+
+
+def test_window_reordering_invalid_options_rejected():
+    from dp_designs import reordering_fixture
+    d = reordering_fixture()
+    for bad in (dict(passes=0), dict(maximum_net_degree=1), dict(improvement_epsilon=-1.0),
+                dict(improvement_epsilon=float("nan")), dict(method=7)):
+        with pytest.raises(ValueError):
+            ob.detailed_place(d, abi.default_dp_options(**bad))
