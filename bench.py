@@ -14,6 +14,7 @@ the timed region starts.
 --workload precision: 1.3 M cells, 20 macros (12 %), 1.4 M nets (~5.7 M pins), 3,000 rows x 3,000
 sites, 1024 x 1024 bins, target density 0.7, quadratic initial placement, DCT products in split
 BF16 (bf16x3).  --workload plain: 2,000 cells, 64 x 64 bins, random initial placement.
+--workload batch: 256 independent blocks (see batch_sample) through the placement service.
 `roofline_dct` reports the DCT products against the MFMA peak of their arithmetic.
 
 Prints one JSON line.  `roofline` reports k_wire, the kernel with the largest share of GPU time,
@@ -54,6 +55,7 @@ METRICS = {
     "large": "GP iters/s @210k cells 512\u00b2 bins; p50/p99 placement latency on served mix",
     "precision": "GP iters/s @1.3M cells 1024\u00b2 bins, bf16x3 DCT products",
     "plain": "GP iters/s @2k cells 64\u00b2 bins",
+    "batch": "GP iters/s aggregate over 256 independent blocks (500-50k cells) on one GPU",
 }
 
 
@@ -161,6 +163,46 @@ def serving_sample(jobs: int, rate: float, slots: int, seed: int = 2026) -> dict
     }
 
 
+def batch_sample(jobs: int, slot_counts, iterations: int, seed: int = 2026) -> dict:
+    """The batching configuration's workload: `jobs` independent blocks, each from its own seed,
+    cells log-uniform in 500-50k (designs.served_design), bin grids by designs.served_bins (32 or 64
+    at these sizes), target density 0.8, random initial placement, a fixed `iterations` adaptive
+    iterations each (early stop off).  All jobs are submitted at once to the placement service with
+    each of `slot_counts` concurrent slots (1 = one job at a time); the aggregate rate is the total
+    of the jobs' GP iterations over the makespan."""
+    from myplacement_amd import abi, design
+    from myplacement_amd import backend as hb
+    rng = np.random.default_rng(seed)
+    work = []
+    for k in range(jobs):
+        cells = int(round(math.exp(rng.uniform(math.log(500), math.log(50_000)))))
+        side = design.served_bins(cells)
+        d = design.served_design(seed + 1 + k, cells)
+        d.center = hb.random_placement(d, seed + 1 + k)
+        o = abi.default_options(target_density=0.8, bins_x=side, bins_y=side, iterations=iterations,
+                                feasible_refinement_iterations=2 ** 31 - 1, seed=seed + 1 + k)
+        work.append((d, o, cells, side))
+    log(f"batch: {jobs} blocks generated, {sum(w[2] for w in work)} cells in total")
+    runs = {}
+    for slots in slot_counts:
+        server = hb.PlacementServer(max_active_jobs=slots, memory_budget_bytes=200 << 30)
+        t0 = time.perf_counter()
+        ids = [server.submit(d, o) for d, o, _, _ in work]
+        reports = [server.wait(j, w[0].module_count)[1] for j, w in zip(ids, work)]
+        wall = time.perf_counter() - t0
+        server.close()
+        its = sum(r["result"]["completed_iterations"] for r in reports)
+        runs[str(slots)] = {"makespan_s": wall, "gp_iterations": its, "gp_iterations_per_s": its / wall,
+                            "jobs_per_hour": 3600.0 * jobs / wall}
+        log(f"batch: {slots} slots, makespan {wall:.2f}s, {its / wall:.0f} GP it/s")
+    sides = [w[3] for w in work]
+    return {"jobs": jobs, "iterations_per_job": iterations, "runs_by_slots": runs,
+            "cells_range": [int(min(w[2] for w in work)), int(max(w[2] for w in work))],
+            "grids": {str(g): sides.count(g) for g in sorted(set(sides))},
+            "sample": f"{jobs} blocks, cells log-uniform 500-50k, bins 32/64 by >=4 cells/bin, t=0.8, random init, "
+                      f"{iterations} iterations each, all submitted at once"}
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed counter summary (profiles/), or None."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_hbm_traffic.md")
@@ -203,6 +245,9 @@ def main() -> int:
                     help="no HIP-event timing (for hardware-counter runs, where the profiler rejects it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=20)
+    ap.add_argument("--batch-jobs", type=int, default=256)
+    ap.add_argument("--batch-slots", default="1,8,16")
+    ap.add_argument("--batch-iterations", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU split is a design note, "
@@ -218,6 +263,16 @@ def main() -> int:
         log(f"copy calibration: {args.copy_calibration} MiB read, {args.copy_calibration} MiB written")
         return 0
     from myplacement_amd import backend as hb
+    if args.workload == "batch":
+        b = batch_sample(args.batch_jobs, [int(x) for x in args.batch_slots.split(",")], args.batch_iterations)
+        best = max(b["runs_by_slots"].values(), key=lambda r: r["gp_iterations_per_s"])
+        line = {"metric": METRICS["batch"], "value": best["gp_iterations_per_s"], "unit": "GP iterations/s",
+                "n_gpus": 1, "steps": b["jobs"] * b["iterations_per_job"], "warmup": 0,
+                "ms_per_step": 1000.0 / best["gp_iterations_per_s"], "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f64 positions/reductions, f32 pin terms, f32 DCT products",
+                "data": "synthetic", "config": {"workload": b["sample"]}, "batch": b}
+        print(json.dumps(line), flush=True)
+        return 0
     d, o, desc = make_workload(args.workload)
     t0 = time.perf_counter()
     prep = hb.PreparedDesign(d, o)
