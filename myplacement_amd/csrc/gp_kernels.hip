@@ -31,7 +31,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
-constexpr int kFinalizeThreads = 1024;
+constexpr int kFinalizeThreads = 256;
 constexpr int kGemmGroup = 4;
 constexpr double kEps = 1e-9;
 constexpr double kPi = 3.141592653589793238462643383279502884;
@@ -508,7 +508,10 @@ __global__ __launch_bounds__(kDepositThreads) void k_ddeposit(DensityArgs a, int
 }
 
 // Bin metrics (Metrics.cpp:101-119 per bin) and the FP32 deviation grid for the DCT from the
-// fixed-point layers, which it clears for the next evaluation.
+// fixed-point layers, which it clears for the next evaluation.  Each thread takes kBinsPerThread
+// consecutive bins (loads of all of them in flight before the arithmetic), so one block reduction
+// covers kBinsPerThread * kThreads bins.
+constexpr int kBinsPerThread = 4;
 __global__ __launch_bounds__(kThreads) void k_dbins(DensityArgs a) {
     __shared__ double lds[(kWaves + 1) * 9];
     const int2 eb = batchBlock();
@@ -517,30 +520,42 @@ __global__ __launch_bounds__(kThreads) void k_dbins(DensityArgs a) {
     const size_t G = static_cast<size_t>(nx) * r.ny;
     unsigned long long* grid = a.grid[eb.x];
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const size_t i = static_cast<size_t>(eb.y) * kThreads + threadIdx.x;
-    if (i < G) {
+    const size_t i0 = (static_cast<size_t>(eb.y) * kThreads + threadIdx.x) * kBinsPerThread;
+    unsigned long long gm[kBinsPerThread], gf[kBinsPerThread];
+    double sc[kBinsPerThread];
+#pragma unroll
+    for (int q = 0; q < kBinsPerThread; ++q) {
+        const size_t i = i0 + q;
+        gm[q] = i < G ? grid[i] : 0ULL;
+        gf[q] = i < G ? grid[G + i] : 0ULL;
+        sc[q] = i < G ? a.static_charge[i] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kBinsPerThread; ++q) {
+        const size_t i = i0 + q;
+        if (i >= G) break;
         const int row = static_cast<int>(i / nx), col = static_cast<int>(i - static_cast<size_t>(row) * nx);
         const double left = r.llx + static_cast<double>(col) * r.bw;
         const double bottom = r.lly + static_cast<double>(row) * r.bh;
         const double area = fmax(0.0, (left + r.bw) - left) * fmax(0.0, (bottom + r.bh) - bottom);
-        const double mov = static_cast<double>(grid[i]) * a.inv_fix;
-        const double fil = static_cast<double>(grid[G + i]) * a.inv_fix;
+        const double mov = static_cast<double>(gm[q]) * a.inv_fix;
+        const double fil = static_cast<double>(gf[q]) * a.inv_fix;
         grid[i] = 0ULL;
         grid[G + i] = 0ULL;
-        const double q_wo = mov + a.static_charge[i];
+        const double q_wo = mov + sc[q];
         const double q_w = q_wo + a.t * fil;
         const double cap = a.t * area;
         const double inv_area = 1.0 / fmax(area, kEps);
         const double d_w = q_w * inv_area, d_wo = q_wo * inv_area;
-        v[0] = fmax(0.0, q_w - cap);
-        v[1] = fmax(0.0, q_wo - cap);
-        v[2] = q_w;
-        v[3] = q_wo;
-        v[4] = mov;
-        v[5] = d_w;
-        v[6] = d_wo;
-        v[7] = d_w;
-        v[8] = d_wo;
+        v[0] += fmax(0.0, q_w - cap);
+        v[1] += fmax(0.0, q_wo - cap);
+        v[2] += q_w;
+        v[3] += q_wo;
+        v[4] += mov;
+        v[5] += d_w;
+        v[6] += d_wo;
+        v[7] = fmax(v[7], d_w);
+        v[8] = fmax(v[8], d_wo);
         a.rho[eb.x][static_cast<size_t>(row) * a.ldrho + col] = static_cast<float>(d_w - a.t);
     }
     publishPartials<9>(v, (1U << 7) | (1U << 8), eb.y, a.partials[eb.x], lds);
@@ -1166,7 +1181,8 @@ void launchDensitySort(const DensityArgs& a, hipStream_t s) {
 }
 
 int densityBinBlocks(const Region& r) {
-    return static_cast<int>((static_cast<long long>(r.nx) * r.ny + kThreads - 1) / kThreads);
+    const long long per_block = static_cast<long long>(kThreads) * kBinsPerThread;
+    return static_cast<int>((static_cast<long long>(r.nx) * r.ny + per_block - 1) / per_block);
 }
 
 void launchDensity(const DensityArgs& a, int count, hipStream_t s) {
