@@ -316,14 +316,31 @@ void mp_gp_session_destroy(mp_gp_session* session);
 mp_gp_status mp_gp_global_place(const mp_design* design, const mp_gp_options* options, double* centers,
                                  mp_gp_result* result, mp_gp_history_row* history, int64_t history_capacity);
 
+/* mp_gp_global_place for `count` independent designs at once (no counterpart in the reference,
+ * which places one design per process).  The runs advance in lockstep rounds: each round runs
+ * every job's recorded launches of one stage (sorting, wirelength, density, each DCT product,
+ * gather, the controller passes) as ONE batched launch over all jobs, and one reduction finalize
+ * for all, then every job's controller decides its next step.  Each job's result is bit-equal to
+ * its own mp_gp_global_place run.  All jobs run on options[0].device and hold their device state
+ * together.  centers[i]: [2 M_i] in/out; results, statuses: [count] (may be NULL).  Returns the
+ * first failing job's status (MP_GP_OK when all succeed); statuses[i] has each job's own. */
+mp_gp_status mp_gp_global_place_batch(int32_t count, const mp_design* designs, const mp_gp_options* options,
+                                       double* const* centers, mp_gp_result* results, mp_gp_status* statuses);
+
 /* ---- placement service (no counterpart in the reference, which places one design per process) --
- * Jobs of mixed sizes are admitted against a device-memory budget into `max_active_jobs` slots;
- * each runs mp_gp_global_place to completion on its own stream, so admitted jobs overlap on the
- * device.  A job's result does not depend on what runs beside it.                               */
+ * Jobs of mixed sizes are admitted against a device-memory budget into `max_active_jobs` slots.
+ * MP_GP_SERVE_STREAMS: each runs mp_gp_global_place to completion on its own stream and worker
+ * thread, so admitted jobs overlap on the device.  MP_GP_SERVE_BATCHED: one driver runs every
+ * admitted job in the lockstep rounds of mp_gp_global_place_batch, and admits waiting jobs between
+ * rounds (continuous batching).  A job's result does not depend on what runs beside it.         */
+typedef enum mp_gp_serve_mode { MP_GP_SERVE_STREAMS = 0, MP_GP_SERVE_BATCHED = 1 } mp_gp_serve_mode;
+
 typedef struct mp_gp_server_options {
     int32_t device;                /* HIP device ordinal (0) */
     int32_t max_active_jobs;       /* 8: jobs running at once */
     uint64_t memory_budget_bytes;  /* 200 GiB: admission budget for the jobs' device footprints */
+    int32_t mode;                  /* mp_gp_serve_mode (MP_GP_SERVE_STREAMS) */
+    int32_t reserved;
 } mp_gp_server_options;
 
 typedef struct mp_gp_job_report {

@@ -258,11 +258,16 @@ class MpDpResult(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+MP_GP_SERVE_STREAMS, MP_GP_SERVE_BATCHED = 0, 1
+
+
 class MpServerOptions(C.Structure):
     _fields_ = [
         ("device", C.c_int32),
         ("max_active_jobs", C.c_int32),
         ("memory_budget_bytes", C.c_uint64),
+        ("mode", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

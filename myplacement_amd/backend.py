@@ -86,6 +86,8 @@ def library():
         lib.mp_gp_session_destroy.argtypes = [V]
         lib.mp_gp_global_place.argtypes = [C.POINTER(abi.MpDesign), C.POINTER(abi.MpOptions), V,
                                            C.POINTER(abi.MpResult), V, C.c_int64]
+        lib.mp_gp_global_place_batch.argtypes = [C.c_int32, C.POINTER(abi.MpDesign), C.POINTER(abi.MpOptions),
+                                                 C.POINTER(C.c_void_p), C.POINTER(abi.MpResult), C.POINTER(C.c_int32)]
         _lib = lib
     return _lib
 
@@ -402,6 +404,28 @@ def global_place(design, options: abi.MpOptions, center=None, history_capacity: 
     return c, r.as_dict(), [hist[i].as_dict() for i in range(n)]
 
 
+def global_place_batch(designs, options, centers=None):
+    """mp_gp_global_place_batch: the adaptive GP of several independent designs in lockstep rounds,
+    each stage one batched launch over all of them.  options: one MpOptions or one per design;
+    centers: starting centres per design (default: each design's own).  Returns a list of
+    (centres, result dict), one per design; raises the first failing job's error."""
+    n = len(designs)
+    opts = list(options) if isinstance(options, (list, tuple)) else [options] * n
+    if len(opts) != n:
+        raise ValueError("one options structure per design")
+    cs = [np.ascontiguousarray(d.center if centers is None or centers[i] is None else centers[i],
+                               dtype=np.float64).copy() for i, d in enumerate(designs)]
+    ds = (abi.MpDesign * max(n, 1))()
+    for i, d in enumerate(designs):
+        ds[i] = d.to_c(cs[i])
+    os_ = (abi.MpOptions * max(n, 1))(*opts)
+    ptrs = (C.c_void_p * max(n, 1))(*[c.ctypes.data for c in cs])
+    rs = (abi.MpResult * max(n, 1))()
+    st = (C.c_int32 * max(n, 1))()
+    _raise(library().mp_gp_global_place_batch(n, ds, os_, ptrs, rs, st))
+    return [(cs[i], rs[i].as_dict()) for i in range(n)]
+
+
 def detailed_place(design, options: abi.MpDpOptions | None = None, center=None, orientation=None,
                    row_orientation=None):
     """Window-reordering detailed placement on the device (tryRunCudaDetailedPlacement,
@@ -428,15 +452,21 @@ def estimate_device_bytes(design, options: abi.MpOptions) -> int:
 
 class PlacementServer:
     """The placement service of mp_gp_server_*: jobs of mixed sizes admitted against a device
-    memory budget into `max_active_jobs` slots, each run to completion on its own stream.  `wait`
-    releases the GIL while it blocks, so several Python threads may wait at once."""
+    memory budget into `max_active_jobs` slots; mode "streams" runs each on its own stream, mode
+    "batched" runs all admitted jobs in lockstep rounds of batched launches.  `wait` releases the
+    GIL while it blocks, so several Python threads may wait at once."""
 
-    def __init__(self, max_active_jobs: int = 8, memory_budget_bytes: int = 200 << 30, device: int = 0):
+    def __init__(self, max_active_jobs: int = 8, memory_budget_bytes: int = 200 << 30, device: int = 0,
+                 mode: str = "streams"):
         o = abi.MpServerOptions()
         library().mp_gp_default_server_options(C.byref(o))
         o.max_active_jobs = max_active_jobs
         o.memory_budget_bytes = memory_budget_bytes
         o.device = device
+        modes = {"streams": abi.MP_GP_SERVE_STREAMS, "batched": abi.MP_GP_SERVE_BATCHED}
+        if mode not in modes:
+            raise ValueError(f"unknown service mode {mode!r}")
+        o.mode = modes[mode]
         h = C.c_void_p()
         _raise(library().mp_gp_server_create(C.byref(o), C.byref(h)))
         self._h = h

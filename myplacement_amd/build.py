@@ -33,7 +33,7 @@ def hipcc() -> str:
 
 def hip_flags() -> list:
     return [
-        f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
+        f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-munsafe-fp-atomics",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-Wall", "-Wno-unused-result", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}",
     ]

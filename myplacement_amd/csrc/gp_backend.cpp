@@ -204,6 +204,8 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     // ---- device ----
     setDevice();
     hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "create HIP stream");
+    own_launches_ = std::make_unique<StreamLaunches>(stream_);
+    launches_ = own_launches_.get();
     hipStream_t stream = stream_;
     pos_.allocate(P_, budget_, "particle positions");
     size_.allocate(P_, budget_, "particle sizes");
@@ -440,7 +442,7 @@ void Backend::sortParticles(const double2* pos) {
     DensityArgs da = densityArgs();
     da.pos[0] = pos;
     timedBegin(kTSort);
-    launchDensitySort(da, stream_);
+    launches_->sort(da);
     // algorithmic bytes: positions and sizes read twice, keys written and read, order and sizes written
     timedEnd(kTSort, 64.0 * P_);
 }
@@ -449,7 +451,6 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
     if (count < 1 || count > kSlots) throw std::logic_error("one or two evaluations per batch");
     for (int e = 0; e < count; ++e)
         if (!specs[e].session && count != 1) throw std::logic_error("backend-mode evaluations are not batched");
-    hipStream_t s = stream_;
     if (specs[0].reorder) sortParticles(specs[0].pos);
 
     WireArgs wa{};
@@ -473,10 +474,10 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
     }
     timedBegin(kTWire);
     if (n_small_ + n_medium_ + n_large_ > 0) {
-        launchWire(wa, count, s);
+        launches_->wire(wa, count);
         for (int e = 0; e < count; ++e) addReduction(kRedWire, specs[e].bank, wireBlocks(n_small_, n_medium_, n_large_));
     } else {
-        for (int e = 0; e < count; ++e) launchZero(dev_scal_ + specs[e].bank * kScalarCount + kHpwl, 2, s);
+        for (int e = 0; e < count; ++e) launches_->zero(dev_scal_ + specs[e].bank * kScalarCount + kHpwl, 2);
     }
     // algorithmic bytes per evaluation, by the convention of one pass over each array at the path's
     // nominal widths (4-byte indices and scalars, 8-byte positions): 12 per pin, 8 per net, 16 per
@@ -491,7 +492,7 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
         da.partials[e] = partials(kRedMetrics, specs[e].bank);
     }
     timedBegin(kTBand);
-    launchDensity(da, count, s);
+    launches_->density(da, count);
     // algorithmic bytes per evaluation (same convention): scatter 20 per particle and 12 per bin,
     // metrics and deviation 24 per bin
     timedEnd(kTBand, count * (20.0 * P_ + 36.0 * static_cast<double>(nx_) * ny_));
@@ -504,13 +505,13 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
     for (int e = 0; e < count; ++e) sp.partials[e] = partials(kRedEnergy, specs[e].bank);
     auto gemm = [&](const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epi) {
         timedBegin(kTDct);
-        launchGemm(g, batch, M, N, K, lda, ldb, ldc, epi, sp, dct_precision_, s);
+        launches_->gemm(g, batch, M, N, K, lda, ldb, ldc, epi, sp, dct_precision_);
         timedEnd(kTDct, 2.0 * M * static_cast<double>(N) * K * batch);
     };
     // half-length products: `rows` x `cols` outputs over a contraction of half the transform side
     auto dct = [&](int mode, const DctBatch& d, int batch, int lda, int ldb, int ldc, int step) {
         timedBegin(kTDct);
-        launchDct(mode, d, batch, ny_, nx_, lda, ldb, ldc, step, dct_precision_, sp, s);
+        launches_->dct(mode, d, batch, ny_, nx_, lda, ldb, ldc, step, dct_precision_, sp);
         const double half = (mode == kFoldCols || mode == kMirrorCols) ? 0.5 * nx_ : 0.5 * ny_;
         timedEnd(kTDct, 2.0 * ny_ * static_cast<double>(nx_) * half * batch * (mode == kMirrorCols ? 2 : 1));
     };
@@ -613,10 +614,100 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
         }
     }
     timedBegin(kTGather);
-    launchGather(ga, count, s);
+    launches_->gather(ga, count);
     for (int e = 0; e < count; ++e) addReduction(kRedGather, specs[e].bank, particleBlocks(P_));
     // algorithmic bytes per evaluation (same convention): 20 per particle and 8 per bin
     timedEnd(kTGather, count * (20.0 * P_ + 8.0 * static_cast<double>(nx_) * ny_));
+}
+
+void StreamLaunches::sort(const DensityArgs& a) { launchDensitySort(a, s_); }
+void StreamLaunches::wire(const WireArgs& a, int count) { launchWire(a, count, s_); }
+void StreamLaunches::zero(double* p, int n) { launchZero(p, static_cast<size_t>(n), s_); }
+void StreamLaunches::density(const DensityArgs& a, int count) { launchDensity(a, count, s_); }
+void StreamLaunches::gemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
+                          const SpectralArgs& sp, int precision) {
+    launchGemm(g, batch, M, N, K, lda, ldb, ldc, epilogue, sp, precision, s_);
+}
+void StreamLaunches::dct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc,
+                         int step, int precision, const SpectralArgs& sp) {
+    launchDct(mode, d, batch, rows, cols, lda, ldb, ldc, step, precision, sp, s_);
+}
+void StreamLaunches::gather(const GatherArgs& a, int count) { launchGather(a, count, s_); }
+void StreamLaunches::lookahead(const LookaheadArgs& l) { launchLookahead(l, s_); }
+void StreamLaunches::propose(const ProposeArgs& p) { launchPropose(p, s_); }
+void StreamLaunches::copy(double2* dst, const double2* src, int n) {
+    if (n > 0)
+        hipCheck(hipMemcpyAsync(dst, src, static_cast<size_t>(n) * sizeof(double2), hipMemcpyDeviceToDevice, s_),
+                 "copy positions");
+}
+
+template <typename E>
+void RecordLaunches::add(int stage, int kernel, const E& e, int blocks, std::size_t lds) {
+    static_assert(sizeof(E) <= kMaxEntryBytes && alignof(E) <= 16, "batch entry does not fit a recorded launch");
+    if (blocks <= 0) return;
+    BatchOp& op = ops.emplace_back();
+    op.stage = stage;
+    op.kernel = kernel;
+    op.blocks = blocks;
+    op.bytes = static_cast<int>(sizeof(E));
+    op.lds = lds;
+    std::memcpy(op.entry, &e, sizeof(E));
+}
+void RecordLaunches::sort(const DensityArgs& a) {
+    if (a.n == 0) return;
+    const std::size_t ny = static_cast<std::size_t>(a.r.ny);
+    const SortEntry h = sortEntry(a, 0), c = sortEntry(a, 1), p = sortEntry(a, 2);
+    add(kStageSortHist, kBatchSortHist, h, h.gx, ny * sizeof(int));
+    add(kStageSortScan, kBatchSortScan, c, c.gx);
+    add(kStageSortPlace, kBatchSortPlace, p, p.gx, (2 * ny + 1) * sizeof(int));
+}
+void RecordLaunches::wire(const WireArgs& a, int count) {
+    if (count < 1 || count > kMaxEval) throw std::invalid_argument("bad evaluation batch");
+    const WireEntry e = wireEntry(a, count);
+    add(kStageWire, kBatchWire, e, e.gx * e.gy);
+}
+void RecordLaunches::zero(double* p, int n) {
+    const ZeroEntry e = zeroEntry(p, n);
+    add(kStageZero, kBatchZero, e, e.gx);
+}
+void RecordLaunches::density(const DensityArgs& a, int count) {
+    if (count < 1 || count > kMaxEval) throw std::invalid_argument("bad evaluation batch");
+    const DensityEntry d = depositEntry(a, count), b = binsEntry(a, count);
+    add(kStageDeposit, kBatchDeposit, d, d.gx * d.gy);
+    add(kStageBins, kBatchBins, b, b.gx * b.gy);
+    spectral_ = 0;
+}
+void RecordLaunches::gemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
+                          const SpectralArgs& sp, int precision) {
+    if (precision != 0) throw std::invalid_argument("only the FP32 DCT products are built");
+    if (spectral_ >= 4) throw std::logic_error("more than four spectral products in one evaluation");
+    const GemmEntry e = gemmEntry(g, batch, M, N, K, lda, ldb, ldc, epilogue, sp);
+    const int k = epilogue == kEpiSpectral ? kBatchGemmSpectral
+                  : epilogue == kEpiInterleave ? kBatchGemmInterleave : kBatchGemmStore;
+    add(kStageSpectral + spectral_++, k, e, e.gx * e.gy * e.gz);
+}
+void RecordLaunches::dct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc,
+                         int step, int precision, const SpectralArgs& sp) {
+    if (spectral_ >= 4) throw std::logic_error("more than four spectral products in one evaluation");
+    const DctEntry e = dctEntry(mode, d, batch, rows, cols, lda, ldb, ldc, step, precision, sp);
+    add(kStageSpectral + spectral_++, kBatchDct + 2 * e.mode + (e.kpl == 8 ? 1 : 0), e, e.gx * e.gy * e.gz);
+}
+void RecordLaunches::gather(const GatherArgs& a, int count) {
+    if (count < 1 || count > kMaxEval) throw std::invalid_argument("bad evaluation batch");
+    const GatherEntry e = gatherEntry(a, count);
+    add(kStageGather, kBatchGather, e, e.gx * e.gy);
+}
+void RecordLaunches::lookahead(const LookaheadArgs& l) {
+    const LookaheadEntry e = lookaheadEntry(l);
+    add(kStageLookahead, kBatchLookahead, e, e.gx);
+}
+void RecordLaunches::propose(const ProposeArgs& p) {
+    const ProposeEntry e = proposeEntry(p);
+    add(kStagePropose, kBatchPropose, e, e.gx);
+}
+void RecordLaunches::copy(double2* dst, const double2* src, int n) {
+    const CopyEntry e = copyEntry(dst, src, n);
+    add(kStageCopy, kBatchCopy, e, e.gx);
 }
 
 void Backend::addReduction(int kind, int bank, int blocks, double p0, double p1, double p2) {

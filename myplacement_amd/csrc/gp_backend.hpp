@@ -12,6 +12,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <stdexcept>
 #include <memory>
@@ -69,6 +70,92 @@ public:
 private:
     T* p_ = nullptr;
     std::size_t n_ = 0;
+};
+
+// Where a backend's kernels go: launched at once on its stream (StreamLaunches), or recorded for
+// launches batched over many jobs.
+class Launches {
+public:
+    virtual ~Launches() = default;
+    virtual void sort(const DensityArgs& a) = 0;
+    virtual void wire(const WireArgs& a, int count) = 0;
+    virtual void zero(double* p, int n) = 0;
+    virtual void density(const DensityArgs& a, int count) = 0;
+    virtual void gemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
+                      const SpectralArgs& sp, int precision) = 0;
+    virtual void dct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
+                     int precision, const SpectralArgs& sp) = 0;
+    virtual void gather(const GatherArgs& a, int count) = 0;
+    virtual void lookahead(const LookaheadArgs& l) = 0;
+    virtual void propose(const ProposeArgs& p) = 0;
+    virtual void copy(double2* dst, const double2* src, int n) = 0;
+};
+
+class StreamLaunches final : public Launches {
+public:
+    explicit StreamLaunches(hipStream_t s) : s_(s) {}
+    void sort(const DensityArgs& a) override;
+    void wire(const WireArgs& a, int count) override;
+    void zero(double* p, int n) override;
+    void density(const DensityArgs& a, int count) override;
+    void gemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
+              const SpectralArgs& sp, int precision) override;
+    void dct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
+             int precision, const SpectralArgs& sp) override;
+    void gather(const GatherArgs& a, int count) override;
+    void lookahead(const LookaheadArgs& l) override;
+    void propose(const ProposeArgs& p) override;
+    void copy(double2* dst, const double2* src, int n) override;
+
+private:
+    hipStream_t s_;
+};
+
+// Stages of a job's launches in the order one evaluation chain issues them (sorting, wirelength,
+// density, up to four spectral products, gather; the controller passes before).  A batch driver
+// runs every job's launches of one stage in one batched launch; a job whose recorded launches go
+// back to an earlier stage continues in a second pass, so each job's own order is kept.
+enum BatchStage : int {
+    kStageCopy = 0, kStageLookahead, kStagePropose, kStageSortHist, kStageSortScan, kStageSortPlace, kStageWire,
+    kStageZero, kStageDeposit, kStageBins, kStageSpectral,  // + product index 0..3
+    kStageGather = kStageSpectral + 4, kStageCount
+};
+constexpr std::size_t kMaxEntryBytes =
+    std::max({sizeof(WireEntry), sizeof(SortEntry), sizeof(DensityEntry), sizeof(GemmEntry), sizeof(DctEntry),
+              sizeof(GatherEntry), sizeof(LookaheadEntry), sizeof(ProposeEntry), sizeof(CopyEntry), sizeof(ZeroEntry)});
+
+// One recorded launch: the job's entry of a batched kernel.
+struct BatchOp {
+    int stage;        // BatchStage
+    int kernel;       // BatchKernel
+    int blocks;       // workgroups of the job's own launch
+    int bytes;        // sizeof the entry type (the table's stride for this kernel)
+    std::size_t lds;  // dynamic LDS the job's launch needs
+    alignas(16) unsigned char entry[kMaxEntryBytes];
+};
+
+// Records a job's launches as batch entries instead of launching them.
+class RecordLaunches final : public Launches {
+public:
+    void sort(const DensityArgs& a) override;
+    void wire(const WireArgs& a, int count) override;
+    void zero(double* p, int n) override;
+    void density(const DensityArgs& a, int count) override;
+    void gemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
+              const SpectralArgs& sp, int precision) override;
+    void dct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
+             int precision, const SpectralArgs& sp) override;
+    void gather(const GatherArgs& a, int count) override;
+    void lookahead(const LookaheadArgs& l) override;
+    void propose(const ProposeArgs& p) override;
+    void copy(double2* dst, const double2* src, int n) override;
+
+    std::vector<BatchOp> ops;
+
+private:
+    template <typename E>
+    void add(int stage, int kernel, const E& e, int blocks, std::size_t lds = 0);
+    int spectral_ = 0;  // product index within the current evaluation chain
 };
 
 struct HostScalars {
@@ -131,6 +218,18 @@ public:
     // Queues the final reduction of `blocks` partial rows; done by the next fetchScalars().
     void addReduction(int kind, int bank, int blocks, double p0 = 0, double p1 = 0, double p2 = 0);
 
+    // The kernels' destination: the backend's stream by default; a batch driver substitutes a
+    // recorder (and then finalizes the pending reductions itself, see takePending).
+    Launches& launches() { return *launches_; }
+    void setLaunches(Launches* l) { launches_ = l ? l : own_launches_.get(); }
+    // Hands over the queued reductions (for a batched finalize) and clears the queue.
+    RedList takePending() {
+        RedList l = pending_;
+        pending_.n = 0;
+        return l;
+    }
+    const HostScalars& hostScalars() const { return *host_scal_; }
+
     // Optional per-kernel-class timing with HIP events on the backend's stream; bit i of the mask
     // times class i of timingNames() (0: off).
     void enableTiming(unsigned class_mask);
@@ -153,6 +252,8 @@ private:
 
     int device_ = 0;
     hipStream_t stream_ = nullptr;
+    std::unique_ptr<StreamLaunches> own_launches_;
+    Launches* launches_ = nullptr;
     int P_ = 0, N_ = 0, I_ = 0;
     int nx_ = 0, ny_ = 0, nxp_ = 0, nyp_ = 0;
     Region region_{};

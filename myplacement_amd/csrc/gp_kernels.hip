@@ -40,13 +40,40 @@ void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// (evaluation, block) of a launch batched over blockIdx.y.  With two evaluations the even
+// Block coordinates and grid extent a kernel body works with: those of its own launch, or, in a
+// launch batched over independent jobs, those of the workgroup within its job's entry.
+struct Blk {
+    int x, y, z, gx, gy, gz;
+};
+__device__ inline Blk nativeBlk() {
+    return Blk{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z),
+               static_cast<int>(gridDim.x), static_cast<int>(gridDim.y), static_cast<int>(gridDim.z)};
+}
+
+// (evaluation, block) of a launch batched over the y extent.  With two evaluations the even
 // workgroups serve the first and the odd ones the second: workgroups go to the eight XCDs round
 // robin, so each XCD's L2 then holds the gathered data of one evaluation only.
-__device__ inline int2 batchBlock() {
-    if (gridDim.y == 1) return make_int2(0, blockIdx.x);
-    const int lin = blockIdx.x + blockIdx.y * gridDim.x;
-    return make_int2(lin % static_cast<int>(gridDim.y), lin / static_cast<int>(gridDim.y));
+__device__ inline int2 batchBlock(const Blk& b) {
+    if (b.gy == 1) return make_int2(0, b.x);
+    const int lin = b.x + b.y * b.gx;
+    return make_int2(lin % b.gy, lin / b.gy);
+}
+
+// Entry of a launch batched over jobs: entry e owns the flattened blocks [first[e], first[e + 1]),
+// first[0] = 0.  Every thread tests entries against this workgroup's index; the count of entries
+// starting at or before it is the entry's index + 1.
+template <int THREADS>
+__device__ inline int batchEntry(const int* __restrict__ first, int entries) {
+    int count = 0;
+    for (int k0 = 0; k0 < entries; k0 += THREADS) {
+        const int k = k0 + static_cast<int>(threadIdx.x);
+        count += __syncthreads_count(k < entries && first[k] <= static_cast<int>(blockIdx.x));
+    }
+    return count - 1;
+}
+// Local block coordinates within an entry of extent gx x gy x gz.
+__device__ inline Blk entryBlk(int local, int gx, int gy, int gz) {
+    return Blk{local % gx, (local / gx) % gy, local / (gx * gy), gx, gy, gz};
 }
 
 __device__ inline double waveSum(double v) {
@@ -243,12 +270,12 @@ __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_gra
 // half-wavefront each), small nets (a thread each): the longest-running workgroups are dispatched
 // first.
 constexpr int kLargePins = 8;  // pins per lane held in registers for a large net (512 per net)
-__global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int large_blocks, int medium_blocks) {
+__device__ __forceinline__ void wireBody(const WireArgs& a, int large_blocks, int medium_blocks, const Blk& bk) {
     __shared__ double lds[(kWaves + 1) * 2];
     double wl = 0.0, hp = 0.0;
     const double inv_gamma = 1.0 / a.gamma;
     const float gamma_f = static_cast<float>(a.gamma);
-    const int2 eb = batchBlock();
+    const int2 eb = batchBlock(bk);
     const int block = eb.y;
     const double2* pos = a.pos[eb.x];
     float2* pin_grad = a.pin_grad[eb.x];
@@ -310,6 +337,9 @@ __global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int large_blocks,
     double v[2] = {hp, wl};
     publishPartials<2>(v, 0U, block, a.partials[eb.x], lds);
 }
+__global__ __launch_bounds__(kThreads) void k_wire(WireArgs a, int large_blocks, int medium_blocks) {
+    wireBody(a, large_blocks, medium_blocks, nativeBlk());
+}
 
 // ---------------------------------------------------------------------------------------------
 // Density.  The exact rectangle / bin overlaps (FP64 geometry, bin index math of Metrics.cpp:78-85)
@@ -341,12 +371,12 @@ __device__ inline unsigned long long toFixed(double area, double fix) {
 // key: first covered row (clamped into the grid), or ny for the tall particles the bands visit
 // directly.  Particles whose clipped area is empty keep their row key: they may become non-empty
 // while the order is reused, and the bands test emptiness at the positions they deposit.
-__global__ __launch_bounds__(kSortThreads) void k_dhist(DensityArgs a) {
+__device__ __forceinline__ void dhistBody(const DensityArgs& a, const Blk& bk) {
     extern __shared__ int hist[];
     const int ny = a.r.ny;
     for (int k = threadIdx.x; k < ny; k += kSortThreads) hist[k] = 0;
     __syncthreads();
-    const int p = blockIdx.x * kSortThreads + threadIdx.x;
+    const int p = bk.x * kSortThreads + threadIdx.x;
     if (p < a.n) {
         double l, rr, bo, to;
         int c0, c1, r0, r1;
@@ -357,13 +387,14 @@ __global__ __launch_bounds__(kSortThreads) void k_dhist(DensityArgs a) {
         if (key < ny) atomicAdd(&hist[key], 1);
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < ny; k += kSortThreads) a.hist[static_cast<size_t>(blockIdx.x) * ny + k] = hist[k];
+    for (int k = threadIdx.x; k < ny; k += kSortThreads) a.hist[static_cast<size_t>(bk.x) * ny + k] = hist[k];
 }
+__global__ __launch_bounds__(kSortThreads) void k_dhist(DensityArgs a) { dhistBody(a, nativeBlk()); }
 
 // one workgroup per key (bin row): exclusive prefix of that key's count over the histogram rows
-__global__ __launch_bounds__(kThreads) void k_dscan(DensityArgs a, int rows) {
+__device__ __forceinline__ void dscanBody(const DensityArgs& a, int rows, const Blk& bk) {
     __shared__ int lds[kWaves];
-    const int ny = a.r.ny, k = blockIdx.x;
+    const int ny = a.r.ny, k = bk.x;
     int carry = 0;
     for (int h0 = 0; h0 < rows; h0 += kThreads) {
         const int h = h0 + threadIdx.x;
@@ -376,9 +407,10 @@ __global__ __launch_bounds__(kThreads) void k_dscan(DensityArgs a, int rows) {
     }
     if (threadIdx.x == 0) a.keytotal[k] = carry;
 }
+__global__ __launch_bounds__(kThreads) void k_dscan(DensityArgs a, int rows) { dscanBody(a, rows, nativeBlk()); }
 
 // row starts from the key totals (every workgroup, in LDS), then each particle's slot in row order
-__global__ __launch_bounds__(kSortThreads) void k_dplace(DensityArgs a) {
+__device__ __forceinline__ void dplaceBody(const DensityArgs& a, const Blk& bk) {
     extern __shared__ int sh[];
     int* start = sh;             // [ny + 1]
     int* rank = sh + a.r.ny + 1; // [ny]
@@ -398,25 +430,26 @@ __global__ __launch_bounds__(kSortThreads) void k_dplace(DensityArgs a) {
     }
     if (threadIdx.x == 0) start[ny] = carry;
     __syncthreads();
-    if (blockIdx.x == 0)
+    if (bk.x == 0)
         for (int k = threadIdx.x; k <= ny; k += kSortThreads) a.keystart[k] = start[k];
-    const int p = blockIdx.x * kSortThreads + threadIdx.x;
+    const int p = bk.x * kSortThreads + threadIdx.x;
     if (p >= a.n) return;
     const int key = a.key[p];
     if (key >= ny) return;
-    const int slot = start[key] + a.colprefix[static_cast<size_t>(blockIdx.x) * ny + key] + atomicAdd(&rank[key], 1);
+    const int slot = start[key] + a.colprefix[static_cast<size_t>(bk.x) * ny + key] + atomicAdd(&rank[key], 1);
     a.perm[slot] = static_cast<unsigned>(p) | (static_cast<unsigned>(a.layer[p]) << kPermLayerShift);
     a.sorted_size[slot] = a.size[p];
 }
+__global__ __launch_bounds__(kSortThreads) void k_dplace(DensityArgs a) { dplaceBody(a, nativeBlk()); }
 // Deposit: one workgroup per kDepositThreads consecutive particles of the row order (then the tall
 // particles).  The workgroup finds the bin rows its particles cover at the current positions and
 // walks them in slabs that fit its LDS window: accumulate the slab in LDS, then add its non-zero
 // cells to the global fixed-point layers.  Integer addition keeps the result independent of the
 // order in which workgroups and lanes arrive.
-__global__ __launch_bounds__(kDepositThreads) void k_ddeposit(DensityArgs a, int sorted_blocks) {
+__device__ __forceinline__ void ddepositBody(const DensityArgs& a, int sorted_blocks, const Blk& bk) {
     __shared__ unsigned long long win[2 * kWindowCells];
     __shared__ int lds[2 * (kDepositThreads / 64)];
-    const int2 eb = batchBlock();
+    const int2 eb = batchBlock(bk);
     const Region& r = a.r;
     const int nx = r.nx;
     const size_t G = static_cast<size_t>(nx) * r.ny;
@@ -506,15 +539,18 @@ __global__ __launch_bounds__(kDepositThreads) void k_ddeposit(DensityArgs a, int
         __syncthreads();
     }
 }
+__global__ __launch_bounds__(kDepositThreads) void k_ddeposit(DensityArgs a, int sorted_blocks) {
+    ddepositBody(a, sorted_blocks, nativeBlk());
+}
 
 // Bin metrics (Metrics.cpp:101-119 per bin) and the FP32 deviation grid for the DCT from the
 // fixed-point layers, which it clears for the next evaluation.  Each thread takes kBinsPerThread
 // consecutive bins (loads of all of them in flight before the arithmetic), so one block reduction
 // covers kBinsPerThread * kThreads bins.
 constexpr int kBinsPerThread = 4;
-__global__ __launch_bounds__(kThreads) void k_dbins(DensityArgs a) {
+__device__ __forceinline__ void dbinsBody(const DensityArgs& a, const Blk& bk) {
     __shared__ double lds[(kWaves + 1) * 9];
-    const int2 eb = batchBlock();
+    const int2 eb = batchBlock(bk);
     const Region& r = a.r;
     const int nx = r.nx;
     const size_t G = static_cast<size_t>(nx) * r.ny;
@@ -560,6 +596,7 @@ __global__ __launch_bounds__(kThreads) void k_dbins(DensityArgs a) {
     }
     publishPartials<9>(v, (1U << 7) | (1U << 8), eb.y, a.partials[eb.x], lds);
 }
+__global__ __launch_bounds__(kThreads) void k_dbins(DensityArgs a) { dbinsBody(a, nativeBlk()); }
 
 // ---------------------------------------------------------------------------------------------
 // C = A * B on v_mfma_f32_16x16x4_f32.  Workgroup tile 32x32, four wavefronts split K in
@@ -568,14 +605,15 @@ __global__ __launch_bounds__(kThreads) void k_dbins(DensityArgs a) {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 template <int EPI>
-__global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int K, int lda, int ldb, int ldc, SpectralArgs sp) {
+__device__ __forceinline__ void gemmBody(const GemmBatch& g, int K, int lda, int ldb, int ldc, const SpectralArgs& sp,
+                                         const Blk& bk) {
     __shared__ float red[kWaves][32][33];
     __shared__ double lds[kWaves + 1];
-    const int z = blockIdx.z;
+    const int z = bk.z;
     const float* __restrict__ A = g.A[z];
     const float* __restrict__ B = g.B[z];
     float* __restrict__ C = g.C[z];
-    const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+    const int m0 = bk.y * 32, n0 = bk.x * 32;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r16 = lane & 15, grp = lane >> 4;
     floatx4 acc00 = {0.f, 0.f, 0.f, 0.f}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
     const float* a_row0 = A + static_cast<size_t>(m0 + r16) * lda + 4 * grp;
@@ -671,8 +709,12 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int K, int lda, 
     }
     if (EPI == kEpiSpectral) {
         double v[1] = {energy};
-        publishPartials<1>(v, 0U, blockIdx.y * gridDim.x + blockIdx.x, sp.partials[blockIdx.z], lds);
+        publishPartials<1>(v, 0U, bk.y * bk.gx + bk.x, sp.partials[z], lds);
     }
+}
+template <int EPI>
+__global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int K, int lda, int ldb, int ldc, SpectralArgs sp) {
+    gemmBody<EPI>(g, K, lda, ldb, ldc, sp, nativeBlk());
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -821,17 +863,17 @@ __device__ inline void dctMma(const float (&a)[2][2][8], const float (&b)[2][2][
 }
 
 template <int MODE, int KPL>
-__global__ __launch_bounds__(kDctThreads) void k_dct(DctBatch d, int n, int lda, int ldb, int ldc, int step,
-                                                     SpectralArgs sp) {
+__device__ __forceinline__ void dctBody(const DctBatch& d, int n, int lda, int ldb, int ldc, int step,
+                                        const SpectralArgs& sp, const Blk& bk) {
     __shared__ float red[2][kDctWaves][32][33];
     __shared__ double lds[kDctWaves + 1];
-    const int z = blockIdx.z;
+    const int z = bk.z;
     const float* __restrict__ A = d.A[z];
     const float* __restrict__ B = d.B[z];
     float* __restrict__ C = d.C[z];
     const int h = n >> 1;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r16 = lane & 15, grp = lane >> 4;
-    const int row0 = blockIdx.y * 32, col0 = blockIdx.x * 32;
+    const int row0 = bk.y * 32, col0 = bk.x * 32;
     floatx4 acc[2][2][2];
 #pragma unroll
     for (int p = 0; p < 2; ++p)
@@ -907,8 +949,13 @@ __global__ __launch_bounds__(kDctThreads) void k_dct(DctBatch d, int n, int lda,
     }
     if (MODE == kFoldRows) {
         double v[1] = {energy};
-        publishPartials<1, kDctWaves>(v, 0U, blockIdx.y * gridDim.x + blockIdx.x, sp.partials[z], lds);
+        publishPartials<1, kDctWaves>(v, 0U, bk.y * bk.gx + bk.x, sp.partials[z], lds);
     }
+}
+template <int MODE, int KPL>
+__global__ __launch_bounds__(kDctThreads) void k_dct(DctBatch d, int n, int lda, int ldb, int ldc, int step,
+                                                     SpectralArgs sp) {
+    dctBody<MODE, KPL>(d, n, lda, ldb, ldc, step, sp, nativeBlk());
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -947,9 +994,9 @@ __device__ inline double2 sampleField(const float2* __restrict__ f, int ld, doub
     return make_double2(lx * (1.0 - ty) + ux * ty, ly * (1.0 - ty) + uy * ty);
 }
 
-__global__ __launch_bounds__(kThreads) void k_gather(GatherArgs a) {
+__device__ __forceinline__ void gatherBody(const GatherArgs& a, const Blk& bk) {
     __shared__ double lds[(kWaves + 1) * 7];
-    const int2 eb = batchBlock();
+    const int2 eb = batchBlock(bk);
     const int e = eb.x;
     const int p = eb.y * kThreads + threadIdx.x;
     // wire L1, density L1, |raw|^2, |pre|^2, V.pre, |pos - px|^2, |pre - pg|^2
@@ -1006,6 +1053,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(GatherArgs a) {
     }
     publishPartials<7>(v, 0U, eb.y, a.partials[e], lds);
 }
+__global__ __launch_bounds__(kThreads) void k_gather(GatherArgs a) { gatherBody(a, nativeBlk()); }
 
 // ---------------------------------------------------------------------------------------------
 // Controller vector passes.
@@ -1018,21 +1066,27 @@ __device__ inline double2 clampCenter(double2 c, double2 sz, const Region& r) { 
 }
 
 // B = clamp(X + beta * (X - X_prev)): the look-ahead of :776-780 with the velocity of :861-863
-__global__ __launch_bounds__(kThreads) void k_lookahead(const double2* x, const double2* x_prev, double beta,
-                                                        double2* out, ClampArgs c, int n) {
-    const int p = blockIdx.x * kThreads + threadIdx.x;
-    if (p >= n) return;
-    const double2 a = x[p], o = x_prev[p];
-    out[p] = clampCenter(make_double2(a.x + (a.x - o.x) * beta, a.y + (a.y - o.y) * beta), c.size[p], c.r);
+__device__ __forceinline__ void lookaheadBody(const LookaheadArgs& l, const Blk& bk) {
+    const int p = bk.x * kThreads + threadIdx.x;
+    if (p >= l.n) return;
+    const double2 a = l.x[p], o = l.x_prev[p];
+    l.out[p] = clampCenter(make_double2(a.x + (a.x - o.x) * l.beta, a.y + (a.y - o.y) * l.beta), l.c.size[p], l.c.r);
 }
+__global__ __launch_bounds__(kThreads) void k_lookahead(LookaheadArgs l) { lookaheadBody(l, nativeBlk()); }
 
-__global__ __launch_bounds__(kThreads) void k_propose(const double2* b, const double2* g, const double2* raw, double step,
-                                                      double max_disp, double2* y, double2* b_copy, double2* g_copy,
-                                                      ClampArgs c, int n, double* partials) {
+__device__ __forceinline__ void proposeBody(const ProposeArgs& pa, const Blk& bk) {
     __shared__ double lds[(kWaves + 1) * 2];
-    const int p = blockIdx.x * kThreads + threadIdx.x;
+    const double2* b = pa.b;
+    const double2* g = pa.g;
+    const double2* raw = pa.raw;
+    const double step = pa.step, max_disp = pa.max_disp;
+    double2* y = pa.y;
+    double2* b_copy = pa.b_copy;
+    double2* g_copy = pa.g_copy;
+    const ClampArgs& c = pa.c;
+    const int p = bk.x * kThreads + threadIdx.x;
     double v[2] = {0.0, 0.0};
-    if (p < n) {
+    if (p < pa.n) {
         const double2 base = b[p], gr = g[p];
         double dx = gr.x * (-step), dy = gr.y * (-step);
         const double dn = sqrt(dx * dx + dy * dy);
@@ -1052,8 +1106,9 @@ __global__ __launch_bounds__(kThreads) void k_propose(const double2* b, const do
         v[0] = r.x * ax + r.y * ay;
         v[1] = sqrt(ax * ax + ay * ay);
     }
-    publishPartials<2>(v, 2U, blockIdx.x, partials, lds);
+    publishPartials<2>(v, 2U, bk.x, pa.partials, lds);
 }
+__global__ __launch_bounds__(kThreads) void k_propose(ProposeArgs pa) { proposeBody(pa, nativeBlk()); }
 
 __global__ __launch_bounds__(kThreads) void k_zero(double* p, size_t count) {
     for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < count;
@@ -1089,9 +1144,8 @@ __device__ void accumulateRows(const double* __restrict__ p, int rows, unsigned 
     }
 }
 
-__global__ __launch_bounds__(kFinalizeThreads) void k_finalize(RedList l) {
+__device__ __forceinline__ void finalizeBody(const RedJob& j, volatile unsigned* flag, unsigned seq) {
     __shared__ double lds[(kFinalizeThreads / 64 + 1) * 9];
-    const RedJob j = l.job[blockIdx.x];
     const unsigned max_mask = j.kind == kRedMetrics ? ((1U << 7) | (1U << 8)) : (j.kind == kRedPropose ? 2U : 0U);
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     switch (j.kind) {
@@ -1141,13 +1195,94 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(RedList l) {
     }
     // completion word for a host that polls the scalar block instead of waiting for the stream:
     // written after the results, which the system-scope fence orders before it
-    if (l.flag) {
+    if (flag) {
         __threadfence_system();
-        l.flag[blockIdx.x] = l.seq;
+        *flag = seq;
     }
+}
+__global__ __launch_bounds__(kFinalizeThreads) void k_finalize(RedList l) {
+    finalizeBody(l.job[blockIdx.x], l.flag ? l.flag + blockIdx.x : nullptr, l.seq);
 }
 
 int blocksFor(long long n) { return static_cast<int>((n + kThreads - 1) / kThreads); }
+
+// ---------------------------------------------------------------------------------------------
+// The same stages batched over independent jobs: one launch runs the stage for every entry, each
+// workgroup finding its entry and its block coordinates within it.
+__global__ __launch_bounds__(kThreads) void k_wire_batch(const WireEntry* e, const int* first, int n) {
+    const int j = batchEntry<kThreads>(first, n);
+    const WireEntry& E = e[j];
+    wireBody(E.a, E.large_blocks, E.medium_blocks, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, 1));
+}
+__global__ __launch_bounds__(kSortThreads) void k_dhist_batch(const SortEntry* e, const int* first, int n) {
+    const int j = batchEntry<kSortThreads>(first, n);
+    const SortEntry& E = e[j];
+    dhistBody(E.a, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, 1, 1));
+}
+__global__ __launch_bounds__(kThreads) void k_dscan_batch(const SortEntry* e, const int* first, int n) {
+    const int j = batchEntry<kThreads>(first, n);
+    const SortEntry& E = e[j];
+    dscanBody(E.a, E.rows, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, 1, 1));
+}
+__global__ __launch_bounds__(kSortThreads) void k_dplace_batch(const SortEntry* e, const int* first, int n) {
+    const int j = batchEntry<kSortThreads>(first, n);
+    const SortEntry& E = e[j];
+    dplaceBody(E.a, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, 1, 1));
+}
+__global__ __launch_bounds__(kDepositThreads) void k_ddeposit_batch(const DensityEntry* e, const int* first, int n) {
+    const int j = batchEntry<kDepositThreads>(first, n);
+    const DensityEntry& E = e[j];
+    ddepositBody(E.a, E.sorted_blocks, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, 1));
+}
+__global__ __launch_bounds__(kThreads) void k_dbins_batch(const DensityEntry* e, const int* first, int n) {
+    const int j = batchEntry<kThreads>(first, n);
+    const DensityEntry& E = e[j];
+    dbinsBody(E.a, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, 1));
+}
+template <int EPI>
+__global__ __launch_bounds__(kThreads) void k_gemm_batch(const GemmEntry* e, const int* first, int n) {
+    const int j = batchEntry<kThreads>(first, n);
+    const GemmEntry& E = e[j];
+    gemmBody<EPI>(E.g, E.K, E.lda, E.ldb, E.ldc, E.sp, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, E.gz));
+}
+template <int MODE, int KPL>
+__global__ __launch_bounds__(kDctThreads) void k_dct_batch(const DctEntry* e, const int* first, int n) {
+    const int j = batchEntry<kDctThreads>(first, n);
+    const DctEntry& E = e[j];
+    dctBody<MODE, KPL>(E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp,
+                       entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, E.gz));
+}
+__global__ __launch_bounds__(kThreads) void k_gather_batch(const GatherEntry* e, const int* first, int n) {
+    const int j = batchEntry<kThreads>(first, n);
+    const GatherEntry& E = e[j];
+    gatherBody(E.a, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, 1));
+}
+__global__ __launch_bounds__(kThreads) void k_lookahead_batch(const LookaheadEntry* e, const int* first, int n) {
+    const int j = batchEntry<kThreads>(first, n);
+    const LookaheadEntry& E = e[j];
+    lookaheadBody(E.l, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, 1, 1));
+}
+__global__ __launch_bounds__(kThreads) void k_propose_batch(const ProposeEntry* e, const int* first, int n) {
+    const int j = batchEntry<kThreads>(first, n);
+    const ProposeEntry& E = e[j];
+    proposeBody(E.p, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, 1, 1));
+}
+__global__ __launch_bounds__(kThreads) void k_copy_batch(const CopyEntry* e, const int* first, int n) {
+    const int j = batchEntry<kThreads>(first, n);
+    const CopyEntry& E = e[j];
+    const int p = (static_cast<int>(blockIdx.x) - first[j]) * kThreads + static_cast<int>(threadIdx.x);
+    if (p < E.n) E.dst[p] = E.src[p];
+}
+__global__ __launch_bounds__(kThreads) void k_zero_batch(const ZeroEntry* e, const int* first, int n) {
+    const int j = batchEntry<kThreads>(first, n);
+    const ZeroEntry& E = e[j];
+    const int p = (static_cast<int>(blockIdx.x) - first[j]) * kThreads + static_cast<int>(threadIdx.x);
+    if (p < E.n) E.p[p] = 0.0;
+}
+__global__ __launch_bounds__(kFinalizeThreads) void k_finalize_batch(const RedJob* jobs, volatile unsigned* flags,
+                                                                    unsigned seq) {
+    finalizeBody(jobs[blockIdx.x], flags + blockIdx.x, seq);
+}
 
 }  // namespace
 
@@ -1264,17 +1399,14 @@ void launchGather(const GatherArgs& a, int count, hipStream_t s) {
     check(hipGetLastError(), "launch field gather kernel");
 }
 
-void launchLookahead(const double2* x, const double2* x_prev, double beta, double2* out, const ClampArgs& c, int n,
-                     hipStream_t s) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_lookahead, dim3(blocksFor(n)), dim3(kThreads), 0, s, x, x_prev, beta, out, c, n);
+void launchLookahead(const LookaheadArgs& l, hipStream_t s) {
+    if (l.n == 0) return;
+    hipLaunchKernelGGL(k_lookahead, dim3(blocksFor(l.n)), dim3(kThreads), 0, s, l);
     check(hipGetLastError(), "launch look-ahead kernel");
 }
 
-void launchPropose(const double2* b, const double2* g, const double2* raw, double step, double max_disp, double2* y,
-                   double2* b_copy, double2* g_copy, const ClampArgs& c, int n, double* partials, hipStream_t s) {
-    hipLaunchKernelGGL(k_propose, dim3(particleBlocks(n)), dim3(kThreads), 0, s, b, g, raw, step, max_disp, y, b_copy,
-                       g_copy, c, n, partials);
+void launchPropose(const ProposeArgs& p, hipStream_t s) {
+    hipLaunchKernelGGL(k_propose, dim3(particleBlocks(p.n)), dim3(kThreads), 0, s, p);
     check(hipGetLastError(), "launch propose kernel");
 }
 
@@ -1294,6 +1426,136 @@ void launchZero(double* p, size_t count, hipStream_t s) {
     const int blocks = static_cast<int>(std::min<size_t>(1024, (count + kThreads - 1) / kThreads));
     hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(kThreads), 0, s, p, count);
     check(hipGetLastError(), "launch zero kernel");
+}
+
+// ---- entries and launches batched over jobs ---------------------------------------------------------
+WireEntry wireEntry(const WireArgs& a, int count) {
+    WireEntry e{a, (a.n_large + kWaves - 1) / kWaves, (a.n_medium + 2 * kWaves - 1) / (2 * kWaves),
+                wireBlocks(a.n_small, a.n_medium, a.n_large), count};
+    if (a.small_net[2] != 0 || a.small_net[kSmallDegree + 1] != a.n_small)
+        throw std::invalid_argument("small-net degree groups do not cover the small nets");
+    return e;
+}
+SortEntry sortEntry(const DensityArgs& a, int stage) {
+    const int hb = densityHistBlocks(a.n);
+    return SortEntry{a, hb, stage == 1 ? a.r.ny : hb};
+}
+DensityEntry depositEntry(const DensityArgs& a, int count) {
+    const int sorted_blocks = (a.n + kDepositThreads - 1) / kDepositThreads;
+    const int tall_blocks = (a.n_tall + kDepositThreads - 1) / kDepositThreads;
+    return DensityEntry{a, sorted_blocks, sorted_blocks + tall_blocks, count};
+}
+DensityEntry binsEntry(const DensityArgs& a, int count) { return DensityEntry{a, 0, densityBinBlocks(a.r), count}; }
+GemmEntry gemmEntry(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
+                    const SpectralArgs& sp) {
+    if (M % 32 || N % 32 || K % 16) throw std::invalid_argument("DCT product dimensions are not tile multiples");
+    if (batch < 1 || batch > kMaxGemmBatch) throw std::invalid_argument("bad DCT product batch");
+    return GemmEntry{g, sp, K, lda, ldb, ldc, epilogue, N / 32, M / 32, batch};
+}
+DctEntry dctEntry(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
+                  int precision, const SpectralArgs& sp) {
+    if (rows <= 0 || cols <= 0 || rows % 64 || cols % 64)
+        throw std::invalid_argument("half-length DCT products need both grid sides to be multiples of 64");
+    if (batch < 1 || batch > kMaxGemmBatch) throw std::invalid_argument("bad DCT product batch");
+    if (precision != 0 && precision != 1) throw std::invalid_argument("unknown DCT product precision");
+    DctEntry e{d, sp, mode, precision == 0 ? 4 : 8, 0, lda, ldb, ldc, step, 0, 0, batch};
+    switch (mode) {
+        case kFoldCols: e.n = cols; e.gx = cols / 64; e.gy = rows / 32; e.step = 1; break;
+        case kFoldRows: e.n = rows; e.gx = cols / 32; e.gy = rows / 64; e.step = 1; break;
+        case kMirrorCols: e.n = cols; e.gx = cols / 32; e.gy = rows / 32; e.step = 1; break;
+        case kMirrorRows:
+            if (step != 1 && step != 2) throw std::invalid_argument("bad DCT output element stride");
+            e.n = rows; e.gx = cols / 32; e.gy = rows / 64;
+            break;
+        default: throw std::invalid_argument("unknown DCT product mode");
+    }
+    return e;
+}
+GatherEntry gatherEntry(const GatherArgs& a, int count) { return GatherEntry{a, particleBlocks(a.n), count}; }
+LookaheadEntry lookaheadEntry(const LookaheadArgs& l) { return LookaheadEntry{l, blocksFor(l.n)}; }
+ProposeEntry proposeEntry(const ProposeArgs& p) { return ProposeEntry{p, particleBlocks(p.n)}; }
+CopyEntry copyEntry(double2* dst, const double2* src, int n) { return CopyEntry{dst, src, n, blocksFor(n)}; }
+ZeroEntry zeroEntry(double* p, int n) { return ZeroEntry{p, n, blocksFor(n)}; }
+
+void launchBatched(BatchKernel k, const void* entries, const int* first, int n, int total_blocks, std::size_t dyn_lds,
+                   hipStream_t s) {
+    if (n <= 0 || total_blocks <= 0) return;
+    const dim3 grid(total_blocks);
+    switch (k) {
+        case kBatchWire:
+            hipLaunchKernelGGL(k_wire_batch, grid, dim3(kThreads), 0, s, static_cast<const WireEntry*>(entries), first, n);
+            break;
+        case kBatchSortHist:
+            hipLaunchKernelGGL(k_dhist_batch, grid, dim3(kSortThreads), dyn_lds, s, static_cast<const SortEntry*>(entries),
+                               first, n);
+            break;
+        case kBatchSortScan:
+            hipLaunchKernelGGL(k_dscan_batch, grid, dim3(kThreads), 0, s, static_cast<const SortEntry*>(entries), first, n);
+            break;
+        case kBatchSortPlace:
+            hipLaunchKernelGGL(k_dplace_batch, grid, dim3(kSortThreads), dyn_lds, s,
+                               static_cast<const SortEntry*>(entries), first, n);
+            break;
+        case kBatchDeposit:
+            hipLaunchKernelGGL(k_ddeposit_batch, grid, dim3(kDepositThreads), 0, s,
+                               static_cast<const DensityEntry*>(entries), first, n);
+            break;
+        case kBatchBins:
+            hipLaunchKernelGGL(k_dbins_batch, grid, dim3(kThreads), 0, s, static_cast<const DensityEntry*>(entries), first, n);
+            break;
+        case kBatchGemmStore:
+            hipLaunchKernelGGL(k_gemm_batch<kEpiStore>, grid, dim3(kThreads), 0, s,
+                               static_cast<const GemmEntry*>(entries), first, n);
+            break;
+        case kBatchGemmSpectral:
+            hipLaunchKernelGGL(k_gemm_batch<kEpiSpectral>, grid, dim3(kThreads), 0, s,
+                               static_cast<const GemmEntry*>(entries), first, n);
+            break;
+        case kBatchGemmInterleave:
+            hipLaunchKernelGGL(k_gemm_batch<kEpiInterleave>, grid, dim3(kThreads), 0, s,
+                               static_cast<const GemmEntry*>(entries), first, n);
+            break;
+        case kBatchGather:
+            hipLaunchKernelGGL(k_gather_batch, grid, dim3(kThreads), 0, s, static_cast<const GatherEntry*>(entries), first, n);
+            break;
+        case kBatchLookahead:
+            hipLaunchKernelGGL(k_lookahead_batch, grid, dim3(kThreads), 0, s, static_cast<const LookaheadEntry*>(entries),
+                               first, n);
+            break;
+        case kBatchPropose:
+            hipLaunchKernelGGL(k_propose_batch, grid, dim3(kThreads), 0, s, static_cast<const ProposeEntry*>(entries),
+                               first, n);
+            break;
+        case kBatchCopy:
+            hipLaunchKernelGGL(k_copy_batch, grid, dim3(kThreads), 0, s, static_cast<const CopyEntry*>(entries), first, n);
+            break;
+        case kBatchZero:
+            hipLaunchKernelGGL(k_zero_batch, grid, dim3(kThreads), 0, s, static_cast<const ZeroEntry*>(entries), first, n);
+            break;
+        default: {
+            // half-length products: kBatchDct + 2 * mode + (kpl == 8)
+            const int v = static_cast<int>(k) - kBatchDct;
+            if (v < 0 || v >= 8) throw std::invalid_argument("unknown batched kernel");
+            const DctEntry* de = static_cast<const DctEntry*>(entries);
+            switch (v) {
+                case 0: hipLaunchKernelGGL((k_dct_batch<kFoldCols, 4>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
+                case 1: hipLaunchKernelGGL((k_dct_batch<kFoldCols, 8>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
+                case 2: hipLaunchKernelGGL((k_dct_batch<kFoldRows, 4>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
+                case 3: hipLaunchKernelGGL((k_dct_batch<kFoldRows, 8>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
+                case 4: hipLaunchKernelGGL((k_dct_batch<kMirrorCols, 4>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
+                case 5: hipLaunchKernelGGL((k_dct_batch<kMirrorCols, 8>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
+                case 6: hipLaunchKernelGGL((k_dct_batch<kMirrorRows, 4>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
+                default: hipLaunchKernelGGL((k_dct_batch<kMirrorRows, 8>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
+            }
+        }
+    }
+    check(hipGetLastError(), "launch batched kernel");
+}
+
+void launchFinalizeBatch(const RedJob* jobs, int n, volatile unsigned* flags, unsigned seq, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_finalize_batch, dim3(n), dim3(kFinalizeThreads), 0, s, jobs, flags, seq);
+    check(hipGetLastError(), "launch batched reduction finalize kernel");
 }
 
 }  // namespace mpgp

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 namespace mpgp {
@@ -203,13 +204,104 @@ struct ClampArgs {
     Region r;
 };
 // out = clamp(x + beta * (x - x_prev))
-void launchLookahead(const double2* x, const double2* x_prev, double beta, double2* out, const ClampArgs& c, int n,
-                     hipStream_t s);
+// out = clamp(x + beta (x - x_prev))
+struct LookaheadArgs {
+    const double2* x;
+    const double2* x_prev;
+    double beta;
+    double2* out;
+    ClampArgs c;
+    int n;
+};
+void launchLookahead(const LookaheadArgs& l, hipStream_t s);
 // y = clamp(b + cap(-step * g, max_disp)); partials of sum raw.(b - y) and max |b - y|; optional
 // staged copies b_copy = b, g_copy = g
-void launchPropose(const double2* b, const double2* g, const double2* raw, double step, double max_disp,
-                   double2* y, double2* b_copy, double2* g_copy, const ClampArgs& c, int n, double* partials,
+struct ProposeArgs {
+    const double2* b;
+    const double2* g;
+    const double2* raw;
+    double step, max_disp;
+    double2* y;
+    double2* b_copy;
+    double2* g_copy;
+    ClampArgs c;
+    int n;
+    double* partials;
+};
+void launchPropose(const ProposeArgs& p, hipStream_t s);
+// ---- launches batched over independent jobs -----------------------------------------------------
+// One launch runs a stage for many jobs.  Entry e (an array in device memory) holds the arguments
+// of one job's single launch and its grid extent gx x gy x gz; it owns the flattened workgroups
+// [first[e], first[e + 1]), first[0] = 0.
+struct WireEntry {
+    WireArgs a;
+    int large_blocks, medium_blocks, gx, gy;
+};
+struct SortEntry {  // histogram and placement: gx = histogram blocks; scan: gx = bin rows
+    DensityArgs a;
+    int rows;       // histogram blocks (the scan's row count)
+    int gx;
+};
+struct DensityEntry {  // deposit (sorted_blocks set) and bin metrics
+    DensityArgs a;
+    int sorted_blocks, gx, gy;
+};
+struct GemmEntry {
+    GemmBatch g;
+    SpectralArgs sp;
+    int K, lda, ldb, ldc, epi, gx, gy, gz;
+};
+struct DctEntry {
+    DctBatch d;
+    SpectralArgs sp;
+    int mode, kpl, n, lda, ldb, ldc, step, gx, gy, gz;
+};
+struct GatherEntry {
+    GatherArgs a;
+    int gx, gy;
+};
+struct LookaheadEntry {
+    LookaheadArgs l;
+    int gx;
+};
+struct ProposeEntry {
+    ProposeArgs p;
+    int gx;
+};
+struct CopyEntry {
+    double2* dst;
+    const double2* src;
+    int n, gx;
+};
+struct ZeroEntry {
+    double* p;
+    int n, gx;
+};
+enum BatchKernel : int {
+    kBatchWire = 0, kBatchSortHist, kBatchSortScan, kBatchSortPlace, kBatchDeposit, kBatchBins, kBatchGemmStore,
+    kBatchGemmSpectral, kBatchGemmInterleave, kBatchGather, kBatchLookahead, kBatchPropose, kBatchCopy, kBatchZero,
+    kBatchDct  // + 2 * mode + (8 values per lane ? 1 : 0)
+};
+// Entry builders (the grid extent of the job's own launch; they validate as the launches do).
+WireEntry wireEntry(const WireArgs& a, int count);
+SortEntry sortEntry(const DensityArgs& a, int stage);  // 0 histogram, 1 scan, 2 placement
+DensityEntry depositEntry(const DensityArgs& a, int count);
+DensityEntry binsEntry(const DensityArgs& a, int count);
+GemmEntry gemmEntry(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
+                    const SpectralArgs& sp);
+DctEntry dctEntry(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
+                  int precision, const SpectralArgs& sp);
+GatherEntry gatherEntry(const GatherArgs& a, int count);
+LookaheadEntry lookaheadEntry(const LookaheadArgs& l);
+ProposeEntry proposeEntry(const ProposeArgs& p);
+CopyEntry copyEntry(double2* dst, const double2* src, int n);
+ZeroEntry zeroEntry(double* p, int n);
+// `entries` and `first` in device memory; total_blocks = first[n].
+void launchBatched(BatchKernel k, const void* entries, const int* first, int n, int total_blocks, std::size_t dyn_lds,
                    hipStream_t s);
+// k_finalize over n reduction jobs; flags[b] receives seq after job b's results.
+void launchFinalizeBatch(const RedJob* jobs, int n, volatile unsigned* flags, unsigned seq, hipStream_t s);
+
 // fill with zero (used instead of memset so the sequence can be captured)
 void launchZero(double* p, size_t count, hipStream_t s);
 

@@ -3,13 +3,16 @@
 // Jobs of mixed sizes are submitted with their netlist and options; a job is admitted when one of
 // `max_active_jobs` slots is free and its estimated device footprint fits the memory budget next to
 // the jobs already running (first come first served, a later job that fits may pass a head job that
-// does not).  Each admitted job runs to completion on its own stream, driven by its own worker
-// thread, so the jobs' kernels overlap on the device; its result is the same as when it runs alone
-// (every evaluation is deterministic).  The reference places one design per process and has no
-// counterpart; the run itself is mp_gp_global_place (runAdaptiveGlobalPlacement,
+// does not).  In stream mode each admitted job runs to completion on its own stream, driven by its
+// own worker thread, so the jobs' kernels overlap on the device.  In batched mode one driver thread
+// runs all admitted jobs in lockstep rounds (one batched launch per stage over every job, see
+// gp_batch.hpp) and admits waiting jobs between rounds.  Either way a job's result is the same as
+// when it runs alone (every evaluation is deterministic).  The reference places one design per
+// process and has no counterpart; the run itself is mp_gp_global_place (runAdaptiveGlobalPlacement,
 // AdaptiveGlobalPlacer.cpp:652-953).
 
 #include "mp_gp_backend.h"
+#include "gp_batch.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -77,7 +80,11 @@ public:
     explicit Server(const mp_gp_server_options& o) : o_(o), t0_(std::chrono::steady_clock::now()) {
         if (o_.max_active_jobs < 1 || o_.memory_budget_bytes == 0)
             throw std::invalid_argument("the placement service needs at least one slot and a memory budget");
-        for (int k = 0; k < o_.max_active_jobs; ++k) workers_.emplace_back([this] { work(); });
+        if (o_.mode == MP_GP_SERVE_BATCHED) workers_.emplace_back([this] { drive(); });
+        else if (o_.mode == MP_GP_SERVE_STREAMS)
+            for (int k = 0; k < o_.max_active_jobs; ++k) workers_.emplace_back([this] { work(); });
+        else
+            throw std::invalid_argument("unknown placement service mode");
     }
     ~Server() {
         {
@@ -114,6 +121,7 @@ public:
             throw std::invalid_argument("the job's device footprint exceeds the service's whole memory budget");
         j->options.maximum_device_bytes = std::min<uint64_t>(o.maximum_device_bytes, j->estimate + (256ULL << 20));
         std::lock_guard<std::mutex> g(m_);
+        if (!broken_.empty()) throw std::runtime_error("the placement service is down: " + broken_);
         j->id = next_id_++;
         j->submit = now();
         const int64_t id = j->id;
@@ -192,8 +200,68 @@ private:
         }
     }
 
+    void finished(Job* j, mp_gp_status st, const char* err, const mp_gp_result& r) {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            j->result = r;
+            j->status = st;
+            j->error = err ? err : "";
+            j->finish = now();
+            j->done = true;
+            used_ -= j->estimate;
+            --active_;
+        }
+        cv_.notify_all();
+    }
+
+    // batched mode: one thread admits jobs between rounds and runs the rounds
+    void drive() {
+        try {
+            BatchDriver driver(o_.device);
+            for (;;) {
+                std::vector<Job*> start;
+                {
+                    std::unique_lock<std::mutex> g(m_);
+                    auto take = [&] {
+                        Job* j = nullptr;
+                        while (!stop_ && active_ < o_.max_active_jobs && (j = admit()) != nullptr) {
+                            j->start = now();
+                            ++active_;
+                            start.push_back(j);
+                        }
+                    };
+                    take();
+                    if (start.empty() && driver.running() == 0) {
+                        cv_.wait(g, [&] {
+                            take();
+                            return stop_ || !start.empty();
+                        });
+                        if (start.empty()) return;  // stopping with nothing running
+                    }
+                }
+                for (Job* j : start)
+                    driver.add(j->design, j->options, j->center.data(),
+                               [this, j](mp_gp_status st, const char* err, const mp_gp_result& r) { finished(j, st, err, r); });
+                if (driver.running() > 0) driver.round();
+            }
+        } catch (const std::exception& e) {
+            // the driver could not be set up: fail what waits and what is admitted from now on
+            std::lock_guard<std::mutex> g(m_);
+            broken_ = e.what();
+            for (Job* j : pending_) {
+                j->status = MP_GP_RUNTIME_ERROR;
+                j->error = broken_;
+                j->done = true;
+            }
+            pending_.clear();
+            cv_.notify_all();
+        }
+    }
+
     mp_gp_server_options o_;
     std::chrono::steady_clock::time_point t0_;
+    int active_ = 0;           // batched mode: admitted jobs not finished
+    std::string broken_;       // batched mode: why the driver stopped
     std::mutex m_;
     std::condition_variable cv_;
     std::deque<Job*> pending_;
@@ -220,6 +288,7 @@ void mp_gp_default_server_options(mp_gp_server_options* o) {
     o->device = 0;
     o->max_active_jobs = 8;
     o->memory_budget_bytes = 200ULL << 30;
+    o->mode = MP_GP_SERVE_STREAMS;
 }
 
 uint64_t mp_gp_estimate_device_bytes(const mp_design* design, const mp_gp_options* options) {

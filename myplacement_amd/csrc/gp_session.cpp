@@ -13,11 +13,15 @@
 // below 1e-9 (AdaptiveGlobalPlacer.cpp:823-826) that evaluation is discarded unread.
 
 #include "gp_backend.hpp"
+#include "gp_batch.hpp"
 #include "gp_host.hpp"
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <coroutine>
+#include <cstring>
+#include <exception>
 #include <limits>
 #include <memory>
 #include <numeric>
@@ -50,6 +54,44 @@ struct Checkpoint {
     bool found = false;
 };
 
+// A step of the controller that runs until it needs the device's scalars.  Started eagerly; when
+// the session launches straight onto its stream the wait is served in place and the step runs to
+// the end inside the call.  Under a batch driver the step suspends at each wait, after recording
+// its launches, and the driver resumes it once the batched launches have delivered the scalars.
+class Step {
+public:
+    struct promise_type {
+        std::exception_ptr error;
+        Step get_return_object() { return Step(std::coroutine_handle<promise_type>::from_promise(*this)); }
+        std::suspend_never initial_suspend() noexcept { return {}; }
+        std::suspend_always final_suspend() noexcept { return {}; }
+        void return_void() {}
+        void unhandled_exception() { error = std::current_exception(); }
+    };
+    Step() = default;
+    explicit Step(std::coroutine_handle<promise_type> h) : h_(h) {}
+    Step(Step&& o) noexcept : h_(std::exchange(o.h_, {})) {}
+    Step& operator=(Step&& o) noexcept {
+        if (this != &o) {
+            if (h_) h_.destroy();
+            h_ = std::exchange(o.h_, {});
+        }
+        return *this;
+    }
+    ~Step() {
+        if (h_) h_.destroy();
+    }
+    explicit operator bool() const { return static_cast<bool>(h_); }
+    bool done() const { return h_.done(); }
+    // rethrows what the finished step threw
+    void result() const {
+        if (h_.promise().error) std::rethrow_exception(h_.promise().error);
+    }
+
+private:
+    std::coroutine_handle<promise_type> h_;
+};
+
 void validateOptions(const mp_gp_options& o) {  // AdaptiveGlobalPlacer.cpp:413-454 (non-RUDY part)
     if (o.iterations <= 0 || o.bins_x < 2 || o.bins_y < 2 || o.target_density <= 0.0 || o.target_density > 1.0 ||
         o.maximum_bin_overflow < 0.0 || o.initial_smoothing <= 0.0 || o.final_smoothing <= 0.0 ||
@@ -72,7 +114,9 @@ void validateOptions(const mp_gp_options& o) {  // AdaptiveGlobalPlacer.cpp:413-
 
 class Session {
 public:
-    Session(std::shared_ptr<const Prepared> prep, const mp_gp_options& o, const double* centers)
+    // batched: the session's launches are recorded for a batch driver, which also starts and
+    // resumes its steps (initialize, then iterate until stopped)
+    Session(std::shared_ptr<const Prepared> prep, const mp_gp_options& o, const double* centers, bool batched = false)
         : prep_(std::move(prep)), o_(o) {
         validateOptions(o_);
         const Prepared& p = *prep_;
@@ -143,8 +187,13 @@ public:
         max_bin_ = std::max(p.bw, p.bh);
         clamp_ = ClampArgs{be_->sizes(), be_->region()};
         bin_height_ = be_->region().bh;
+        if (batched) {
+            rec_ = std::make_unique<RecordLaunches>();
+            be_->setLaunches(rec_.get());
+            return;
+        }
         const auto t0 = std::chrono::steady_clock::now();
-        initialize();
+        runInPlace(initialize());
         setup_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
 
@@ -153,10 +202,11 @@ public:
     const std::vector<mp_gp_history_row>& history() const { return history_; }
 
     int step(int iterations) {
+        if (rec_) throw std::logic_error("a batched session is stepped by its batch driver");
         const auto t0 = std::chrono::steady_clock::now();
         int done = 0;
         while (done < iterations && !stopped_) {
-            iterate();
+            runInPlace(iterate());
             ++done;
         }
         loop_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -174,6 +224,47 @@ public:
             xy[2 * order_[k]] = pos[k].x;
             xy[2 * order_[k] + 1] = pos[k].y;
         }
+    }
+
+    // ---- batch driver interface ----------------------------------------------------------------
+    RecordLaunches* recorder() { return rec_.get(); }
+    // the suspended step's continuation (null when none waits)
+    std::coroutine_handle<> takeWaiting() { return std::exchange(waiting_, {}); }
+    bool waiting() const { return static_cast<bool>(waiting_); }
+    // hands the scalars the batched launches delivered to the suspended step
+    void deliver(const HostScalars* h) { scalars_ = h; }
+    Step iteration() { return iterate(); }
+    void addTime(double setup, double loop) {
+        setup_seconds_ += setup;
+        loop_seconds_ += loop;
+    }
+    // back to launching onto the session's own stream (after the driver's last round for it)
+    void detach() {
+        be_->setLaunches(nullptr);
+        rec_.reset();
+    }
+
+    // :672-733
+    Step initialize() {
+        smoothing_ = o_.initial_smoothing * max_bin_;
+        penalty_ = 1.0;
+        reference_penalty_ = 1.0;
+        enqueueGradEval();
+        EvalRec init = record(co_await ScalarWait{this}, true, kBankCurrent);
+        initial_overflow_ = std::max(init.s.optimizer_density.normalized_overflow, kEps);
+        smoothing_ = smoothingFromOverflow(init.s.optimizer_density.normalized_overflow);
+        const double raw_penalty = init.density_l1 > kEps ? init.wire_l1 / init.density_l1 : 1.0;
+        penalty_ = clampd(raw_penalty, o_.minimum_density_penalty, o_.maximum_density_penalty);
+        reference_penalty_ = penalty_;
+        enqueueGradEval();
+        init = record(co_await ScalarWait{this}, true, kBankCurrent);
+        last_overflow_ = init.s.optimizer_density.normalized_overflow;
+        const double rms = init.pre_norm / std::sqrt(static_cast<double>(std::max(1, P_)));
+        last_step_ = o_.initial_movement_in_bins * min_bin_ / std::max(rms, kEps);
+        bool dummy = false;
+        considerCheckpoint(init, 0, dummy);
+        feasible_refinement_ = init.s.design_density.normalized_overflow <= o_.maximum_bin_overflow ? 1 : 0;
+        current_ = init;
     }
 
     void finish(double* centers, mp_gp_result* r) {
@@ -220,6 +311,24 @@ public:
     const std::vector<uint32_t>& order() const { return order_; }
 
 private:
+    // Awaiting the device's scalars: served in place (fetchScalars) when the session launches onto
+    // its own stream; otherwise the step suspends until the batch driver delivers them.
+    struct ScalarWait {
+        Session* s;
+        bool await_ready() {
+            if (s->rec_) return false;
+            s->scalars_ = &s->be_->fetchScalars();
+            return true;
+        }
+        void await_suspend(std::coroutine_handle<> h) { s->waiting_ = h; }
+        const HostScalars& await_resume() const { return *s->scalars_; }
+    };
+    // runs a step that launches onto the session's own stream: it never suspends
+    static void runInPlace(const Step& st) {
+        if (!st.done()) throw std::logic_error("a session step suspended outside a batch driver");
+        st.result();
+    }
+
     // ---- evaluation helpers -----------------------------------------------------------------
     double strength() const { return std::sqrt(1.0 + std::max(0.0, penalty_ / std::max(reference_penalty_, kEps))); }
 
@@ -254,20 +363,18 @@ private:
         }
         return e;
     }
-    EvalRec gradEval() {
+    // gradient evaluation at X into bank 0 (read back by the caller's wait)
+    void enqueueGradEval() {
         const Backend::SessionOut so = sessionOut(Rc_.get(), Gc_.get(), nullptr, nullptr);
         const Backend::EvalSpec spec{X_.get(), &so, kBankCurrent, true};
         be_->enqueueEvaluate(&spec, 1, smoothing_, true);
         dX_ = 0.0;
-        return record(be_->fetchScalars(), true, kBankCurrent);
     }
     // A new best position is copied by the next gradient evaluation at X (its gather writes the
     // copy); a copy still pending when that evaluation will not come is made here.
     void flushCheckpoint() {
         if (!pending_ck_) return;
-        hipCheck(hipMemcpyAsync(pending_ck_->get(), X_.get(), P_ * sizeof(double2), hipMemcpyDeviceToDevice,
-                                be_->stream()),
-                 "copy checkpoint");
+        be_->launches().copy(pending_ck_->get(), X_.get(), P_);
         pending_ck_ = nullptr;
     }
 
@@ -344,27 +451,6 @@ private:
         return r;
     }
 
-    // :672-733
-    void initialize() {
-        smoothing_ = o_.initial_smoothing * max_bin_;
-        penalty_ = 1.0;
-        reference_penalty_ = 1.0;
-        EvalRec init = gradEval();
-        initial_overflow_ = std::max(init.s.optimizer_density.normalized_overflow, kEps);
-        smoothing_ = smoothingFromOverflow(init.s.optimizer_density.normalized_overflow);
-        const double raw_penalty = init.density_l1 > kEps ? init.wire_l1 / init.density_l1 : 1.0;
-        penalty_ = clampd(raw_penalty, o_.minimum_density_penalty, o_.maximum_density_penalty);
-        reference_penalty_ = penalty_;
-        init = gradEval();
-        last_overflow_ = init.s.optimizer_density.normalized_overflow;
-        const double rms = init.pre_norm / std::sqrt(static_cast<double>(std::max(1, P_)));
-        last_step_ = o_.initial_movement_in_bins * min_bin_ / std::max(rms, kEps);
-        bool dummy = false;
-        considerCheckpoint(init, 0, dummy);
-        feasible_refinement_ = init.s.design_density.normalized_overflow <= o_.maximum_bin_overflow ? 1 : 0;
-        current_ = init;
-    }
-
     // One pass of the loop body :735-928.  Everything the first decisions need is enqueued before
     // one read-back: the gradient evaluation at X (slot 0) with the momentum alignment and the
     // curvature sums fused into its gather, and -- when momentum may be used -- the look-ahead
@@ -377,8 +463,7 @@ private:
     // beta * mX_ (mX_ bounds |X - X_prev|), |Y - base| <= max_disp, and the order is rebuilt when
     // the bound passes the reuse limit.  The bound only steers speed: results do not depend on the
     // order.
-    void iterate() {
-        hipStream_t s = be_->stream();
+    Step iterate() {
         const int it = result_.completed_iterations;
         const bool try_momentum = worse_streak_ < 2 && momentum_age_ > 0;
         const double beta = std::min(o_.maximum_momentum, static_cast<double>(momentum_age_) / (momentum_age_ + 3));
@@ -400,7 +485,7 @@ private:
                                       {B_.get(), &so_look, kBankLook, fresh}};
         if (try_momentum) {
             be_->timedBegin(kTController);
-            launchLookahead(X_.get(), XP_.get(), beta, B_.get(), clamp_, P_, s);
+            be_->launches().lookahead(LookaheadArgs{X_.get(), XP_.get(), beta, B_.get(), clamp_, P_});
             be_->timedEnd(kTController);
         }
         if (it > 0) {
@@ -414,7 +499,7 @@ private:
         }
         const bool need_read = it > 0;
         HostScalars first{};
-        if (need_read) first = be_->fetchScalars();
+        if (need_read) first = co_await ScalarWait{this};
         if (it > 0) current_ = record(first, true, kBankCurrent);
         const EvalRec& cur = current_;
 
@@ -475,14 +560,14 @@ private:
             const double max_disp = o_.maximum_movement_in_bins * min_bin_ * (0.35 + 0.65 * ratio);
             for (int attempt = 0; attempt <= o_.maximum_backtracks; ++attempt) {
                 be_->timedBegin(kTController);
-                launchPropose(base, base_pre, base_raw, step, max_disp, Y_.get(), PXn_.get(), PGn_.get(), clamp_, P_,
-                              be_->partials(kRedPropose, kBankCandidate), s);
+                be_->launches().propose(ProposeArgs{base, base_pre, base_raw, step, max_disp, Y_.get(), PXn_.get(),
+                                                    PGn_.get(), clamp_, P_, be_->partials(kRedPropose, kBankCandidate)});
                 be_->addReduction(kRedPropose, kBankCandidate, particleBlocks(P_));
                 be_->timedEnd(kTController);
                 const Backend::SessionOut so_cand = sessionOut(nullptr, nullptr, nullptr, nullptr);
                 const Backend::EvalSpec cand_spec{Y_.get(), &so_cand, kBankCandidate, fresh};
                 be_->enqueueEvaluate(&cand_spec, 1, smoothing_, false);
-                const HostScalars& h = be_->fetchScalars();
+                const HostScalars& h = co_await ScalarWait{this};
                 const double predicted = h.bank(kBankCandidate)[kPredicted];
                 const double disp = h.bank(kBankCandidate)[kMaxDisp];
                 if (predicted <= kEps) break;
@@ -578,7 +663,307 @@ private:
     mp_gp_result result_{};
     std::vector<mp_gp_history_row> history_;
     double setup_seconds_ = 0.0, loop_seconds_ = 0.0;
+    std::unique_ptr<RecordLaunches> rec_;  // batched: the launches go to the batch driver
+    std::coroutine_handle<> waiting_;      // batched: the step suspended on a wait
+    const HostScalars* scalars_ = nullptr; // the scalars a wait resumes with
 };
+
+// ---- batch driver ----------------------------------------------------------------------------------
+struct BatchDriver::Impl {
+    struct Job {
+        std::unique_ptr<Session> s;
+        Step step;
+        bool initialized = false, finished = false;
+        std::exception_ptr error;
+        double* centers = nullptr;
+        double hpwl_before = 0.0, overflow_before = 0.0;
+        std::chrono::steady_clock::time_point t0, t_loop;
+        Callback done;
+    };
+    // one batched launch of the round: a run of entries of one kernel in the table
+    struct Launch {
+        int kernel, n, blocks;
+        std::size_t lds, entries, first;
+    };
+
+    int device;
+    hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<Job>> jobs;
+    unsigned char* host_table = nullptr;  // pinned staging of the round's table
+    unsigned char* dev_table = nullptr;
+    std::size_t table_bytes = 0;
+    unsigned* host_flags = nullptr;  // host-mapped completion words, one per reduction job
+    unsigned* dev_flags = nullptr;
+    int flag_count = 0;
+    unsigned seq = 0;
+
+    explicit Impl(int d) : device(d) {
+        hipCheck(hipSetDevice(device), "select HIP device");
+        hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "create batch stream");
+    }
+    ~Impl() {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        jobs.clear();
+        if (host_table) (void)hipHostFree(host_table);
+        if (dev_table) (void)hipFree(dev_table);
+        if (host_flags) (void)hipHostFree(host_flags);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    static double since(std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+    }
+
+    // Runs the job's steps until one waits for the device (its launches are then recorded) or
+    // the run has stopped.
+    static void advance(Job& j) {
+        try {
+            if (j.s->waiting()) j.s->takeWaiting().resume();
+            for (;;) {
+                if (j.s->waiting()) return;
+                if (j.step) {  // the step ran to its end
+                    j.step.result();
+                    j.step = Step();
+                    if (!j.initialized) {
+                        j.initialized = true;
+                        j.s->addTime(since(j.t0), 0.0);
+                        j.t_loop = std::chrono::steady_clock::now();
+                    }
+                }
+                if (!j.initialized) j.step = j.s->initialize();
+                else if (!j.s->stopped()) j.step = j.s->iteration();
+                else {
+                    j.finished = true;
+                    return;
+                }
+            }
+        } catch (...) {
+            j.error = std::current_exception();
+            j.finished = true;
+        }
+    }
+
+    void complete(Job& j) {
+        mp_gp_result r{};
+        mp_gp_status st = MP_GP_OK;
+        std::string msg;
+        if (!j.error) {
+            try {
+                j.s->addTime(0.0, since(j.t_loop));
+                j.s->detach();
+                j.s->finish(j.centers, &r);
+                r.hpwl_before = j.hpwl_before;
+                r.overflow_before = j.overflow_before;
+                r.elapsed_seconds = since(j.t0);
+            } catch (...) {
+                j.error = std::current_exception();
+            }
+        }
+        if (j.error) {
+            st = translate(j.error);
+            msg = g_last_error;
+            r = mp_gp_result{};
+        }
+        j.done(st, msg.c_str(), r);
+    }
+
+    void failAll(const std::exception_ptr& e) {
+        (void)hipStreamSynchronize(stream);
+        const mp_gp_status st = translate(e);
+        const std::string msg = g_last_error;
+        std::vector<std::unique_ptr<Job>> gone;
+        gone.swap(jobs);
+        for (auto& j : gone) j->done(st, msg.c_str(), mp_gp_result{});
+    }
+
+    void reserve(std::size_t bytes, int flags) {
+        if (bytes > table_bytes) {
+            if (host_table) hipCheck(hipHostFree(host_table), "free batch table");
+            if (dev_table) hipCheck(hipFree(dev_table), "free batch table");
+            host_table = nullptr;
+            dev_table = nullptr;
+            table_bytes = 0;
+            const std::size_t cap = std::max<std::size_t>(bytes + bytes / 2, std::size_t{1} << 20);
+            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&host_table), cap, hipHostMallocDefault), "allocate batch table");
+            hipCheck(hipMalloc(reinterpret_cast<void**>(&dev_table), cap), "allocate batch table");
+            table_bytes = cap;
+        }
+        if (flags > flag_count) {
+            if (host_flags) hipCheck(hipHostFree(host_flags), "free completion words");
+            host_flags = nullptr;
+            flag_count = 0;
+            const int cap = std::max(flags + flags / 2, 1024);
+            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&host_flags), cap * sizeof(unsigned),
+                                   hipHostMallocMapped | hipHostMallocCoherent),
+                     "allocate completion words");
+            std::memset(host_flags, 0, cap * sizeof(unsigned));
+            hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&dev_flags), host_flags, 0), "map completion words");
+            flag_count = cap;
+        }
+    }
+
+    void round() {
+        hipCheck(hipSetDevice(device), "select HIP device");
+        // Retire finished jobs.  Every launch they recorded ran in an earlier round, which has
+        // completed (its completion words were all seen, or the stream was synchronized).
+        for (std::size_t k = 0; k < jobs.size();) {
+            Job& j = *jobs[k];
+            if (j.finished && (j.error || j.s->recorder()->ops.empty())) {
+                std::unique_ptr<Job> gone = std::move(jobs[k]);
+                jobs.erase(jobs.begin() + static_cast<std::ptrdiff_t>(k));
+                complete(*gone);
+            } else {
+                ++k;
+            }
+        }
+        if (jobs.empty()) return;
+        try {
+            runRound();
+        } catch (...) {
+            failAll(std::current_exception());
+        }
+    }
+
+    void runRound() {
+        // every recorded launch, keyed (pass, stage, kernel, job, position); a job's launches that go
+        // back to an earlier stage start its next pass
+        struct Ref {
+            int pass, stage, kernel, job, op;
+        };
+        std::vector<Ref> refs;
+        std::vector<RedJob> reds;
+        for (int ji = 0; ji < static_cast<int>(jobs.size()); ++ji) {
+            Job& j = *jobs[ji];
+            if (j.error) continue;
+            const std::vector<BatchOp>& ops = j.s->recorder()->ops;
+            int pass = 0, prev = -1;
+            for (int k = 0; k < static_cast<int>(ops.size()); ++k) {
+                if (ops[k].stage < prev) ++pass;
+                prev = ops[k].stage;
+                refs.push_back(Ref{pass, ops[k].stage, ops[k].kernel, ji, k});
+            }
+            const RedList l = j.s->backend().takePending();
+            reds.insert(reds.end(), l.job, l.job + l.n);
+        }
+        std::sort(refs.begin(), refs.end(), [](const Ref& a, const Ref& b) {
+            if (a.pass != b.pass) return a.pass < b.pass;
+            if (a.stage != b.stage) return a.stage < b.stage;
+            if (a.kernel != b.kernel) return a.kernel < b.kernel;
+            if (a.job != b.job) return a.job < b.job;
+            return a.op < b.op;
+        });
+        auto align = [](std::size_t o, std::size_t a) { return (o + a - 1) / a * a; };
+        std::vector<Launch> launches;
+        std::size_t off = 0;
+        for (std::size_t a = 0; a < refs.size();) {
+            std::size_t b = a;
+            while (b < refs.size() && refs[b].pass == refs[a].pass && refs[b].stage == refs[a].stage &&
+                   refs[b].kernel == refs[a].kernel)
+                ++b;
+            const BatchOp& op0 = jobs[refs[a].job]->s->recorder()->ops[refs[a].op];
+            Launch L{refs[a].kernel, static_cast<int>(b - a), 0, 0, 0, 0};
+            off = align(off, 256);
+            L.entries = off;
+            off += static_cast<std::size_t>(L.n) * op0.bytes;
+            off = align(off, 16);
+            L.first = off;
+            off += (static_cast<std::size_t>(L.n) + 1) * sizeof(int);
+            launches.push_back(L);
+            a = b;
+        }
+        off = align(off, 256);
+        const std::size_t red_off = off;
+        off += reds.size() * sizeof(RedJob);
+        reserve(off, static_cast<int>(reds.size()));
+        // fill the staging table
+        std::size_t r = 0;
+        for (Launch& L : launches) {
+            int* first = reinterpret_cast<int*>(host_table + L.first);
+            long long total = 0;
+            first[0] = 0;
+            for (int e = 0; e < L.n; ++e, ++r) {
+                const BatchOp& op = jobs[refs[r].job]->s->recorder()->ops[refs[r].op];
+                std::memcpy(host_table + L.entries + static_cast<std::size_t>(e) * op.bytes, op.entry, op.bytes);
+                total += op.blocks;
+                if (total > (1LL << 30)) throw std::runtime_error("batched launch exceeds the grid limit");
+                first[e + 1] = static_cast<int>(total);
+                L.lds = std::max(L.lds, op.lds);
+            }
+            L.blocks = static_cast<int>(total);
+        }
+        if (!reds.empty()) std::memcpy(host_table + red_off, reds.data(), reds.size() * sizeof(RedJob));
+        if (off > 0) hipCheck(hipMemcpyAsync(dev_table, host_table, off, hipMemcpyHostToDevice, stream), "upload batch table");
+        for (const Launch& L : launches)
+            launchBatched(static_cast<BatchKernel>(L.kernel), dev_table + L.entries,
+                          reinterpret_cast<const int*>(dev_table + L.first), L.n, L.blocks, L.lds, stream);
+        for (auto& j : jobs) j->s->recorder()->ops.clear();
+        const int nred = static_cast<int>(reds.size());
+        if (nred > 0) {
+            ++seq;
+            if (seq == 0) seq = 1;
+            launchFinalizeBatch(reinterpret_cast<const RedJob*>(dev_table + red_off), nred, dev_flags, seq, stream);
+            auto written = [&] {
+                for (int b = 0; b < nred; ++b)
+                    if (__atomic_load_n(&host_flags[b], __ATOMIC_ACQUIRE) != seq) return false;
+                return true;
+            };
+            for (long spins = 0; !written(); ++spins) {
+                if ((spins & 1023) != 1023) continue;
+                const hipError_t st = hipStreamQuery(stream);
+                if (st == hipErrorNotReady) continue;
+                hipCheck(st, "synchronize batched evaluation");
+                if (!written()) throw std::runtime_error("batched finalize retired without writing its completion words");
+                break;
+            }
+        } else {
+            hipCheck(hipStreamSynchronize(stream), "synchronize batched launches");
+        }
+        for (auto& jp : jobs) {
+            Job& j = *jp;
+            if (j.finished) continue;
+            if (j.s->waiting()) j.s->deliver(&j.s->backend().hostScalars());
+            advance(j);
+        }
+    }
+};
+
+BatchDriver::BatchDriver(int device) : impl_(std::make_unique<Impl>(device)) {}
+BatchDriver::~BatchDriver() = default;
+int BatchDriver::running() const { return static_cast<int>(impl_->jobs.size()); }
+void BatchDriver::round() { impl_->round(); }
+
+void BatchDriver::add(const mp_design& design, const mp_gp_options& options, double* centers, Callback done) {
+    auto j = std::make_unique<Impl::Job>();
+    j->t0 = std::chrono::steady_clock::now();
+    j->centers = centers;
+    j->done = std::move(done);
+    try {
+        if (!centers) throw std::invalid_argument("null centres");
+        if (options.device != impl_->device) throw std::invalid_argument("a batch runs on one device");
+        mp_design d = design;
+        d.center = centers;
+        Netlist probe(d);
+        if (probe.movable.empty()) {  // nothing to place
+            j->done(MP_GP_OK, "", mp_gp_result{});
+            return;
+        }
+        validateOptions(options);
+        j->hpwl_before = probe.hpwl(probe.cx.data(), probe.cy.data());
+        j->overflow_before =
+            probe.designOverflow(probe.cx.data(), probe.cy.data(), options.bins_x, options.bins_y, options.target_density);
+        auto prep = std::make_shared<const Prepared>(d, options);
+        hipCheck(hipSetDevice(impl_->device), "select HIP device");
+        j->s = std::make_unique<Session>(prep, options, nullptr, true);
+    } catch (...) {
+        const mp_gp_status st = translate(std::current_exception());
+        const std::string msg = g_last_error;
+        j->done(st, msg.c_str(), mp_gp_result{});
+        return;
+    }
+    Impl::advance(*j);
+    impl_->jobs.push_back(std::move(j));
+}
 
 }  // namespace mpgp
 
@@ -831,6 +1216,36 @@ mp_gp_status mp_gp_global_place(const mp_design* design, const mp_gp_options* op
             std::copy(hist.begin(), hist.begin() + n, history);
         }
     });
+}
+
+mp_gp_status mp_gp_global_place_batch(int32_t count, const mp_design* designs, const mp_gp_options* options,
+                                       double* const* centers, mp_gp_result* results, mp_gp_status* statuses) {
+    std::vector<mp_gp_status> st(count > 0 ? static_cast<std::size_t>(count) : 0, MP_GP_OK);
+    std::vector<std::string> err(st.size());
+    const mp_gp_status first = guarded([&] {
+        if (count < 0 || (count > 0 && (!designs || !options || !centers)))
+            throw std::invalid_argument("bad job count or null designs, options or centres");
+        if (count == 0) return;
+        BatchDriver driver(options[0].device);
+        for (int32_t i = 0; i < count; ++i) {
+            mp_gp_result* r = results ? results + i : nullptr;
+            driver.add(designs[i], options[i], centers[i], [&st, &err, r, i](mp_gp_status s, const char* e, const mp_gp_result& res) {
+                st[i] = s;
+                err[i] = e ? e : "";
+                if (r) *r = res;
+            });
+        }
+        while (driver.running() > 0) driver.round();
+    });
+    if (statuses)
+        for (std::size_t i = 0; i < st.size(); ++i) statuses[i] = st[i];
+    if (first != MP_GP_OK) return first;
+    for (std::size_t i = 0; i < st.size(); ++i)
+        if (st[i] != MP_GP_OK) {
+            g_last_error = "job " + std::to_string(i) + ": " + err[i];
+            return st[i];
+        }
+    return MP_GP_OK;
 }
 
 }  // extern "C"
