@@ -104,12 +104,13 @@ def cpu_baseline(name: str, iterations: int) -> dict:
             "loop_seconds": r["loop_seconds"]}
 
 
-def serving_sample(jobs: int, rate: float, slots: int, seed: int = 2026) -> dict:
+def serving_sample(jobs: int, rate: float, slots: int, seed: int = 2026, mode: str = "streams") -> dict:
     """A bounded sample of the served mixed-size stream: `jobs` requests with Poisson arrivals at
     `rate` per second, sizes log-uniform in 1k-200k cells (designs.served_design), bin grids by
     designs.served_bins, target density uniform in 0.7-0.9, random initial placement, each run until
     design overflow <= 0.10 (checked from iteration 21 on) or 280 iterations.  Jobs are generated
-    before the stream starts; latency is submit -> result."""
+    before the stream starts; latency is submit -> result.  mode: the service's mode ("streams": a
+    stream per job; "batched": lockstep rounds of batched launches, jobs joining between rounds)."""
     import threading
     from myplacement_amd import abi, design
     from myplacement_amd import backend as hb
@@ -125,7 +126,7 @@ def serving_sample(jobs: int, rate: float, slots: int, seed: int = 2026) -> dict
                                 feasible_refinement_iterations=1, seed=seed + 1 + k)
         work.append((d, o, cells, side))
     gaps = rng.exponential(1.0 / rate, size=jobs)
-    server = hb.PlacementServer(max_active_jobs=slots, memory_budget_bytes=200 << 30)
+    server = hb.PlacementServer(max_active_jobs=slots, memory_budget_bytes=200 << 30, mode=mode)
     reports = [None] * jobs
     threads = []
 
@@ -152,7 +153,7 @@ def serving_sample(jobs: int, rate: float, slots: int, seed: int = 2026) -> dict
     first = min(r["submit_seconds"] for r in reports)
     last = max(r["finish_seconds"] for r in reports)
     return {
-        "jobs": jobs, "arrival_rate_per_s": rate, "slots": slots,
+        "jobs": jobs, "arrival_rate_per_s": rate, "slots": slots, "mode": mode,
         "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
         "mean_latency_s": float(lat.mean()), "jobs_per_hour": 3600.0 * jobs / (last - first),
         "converged_fraction": float(np.mean(conv)),
@@ -243,6 +244,8 @@ def main() -> int:
     ap.add_argument("--serve-jobs", type=int, default=40, help="served-stream sample size (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=8.0)
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-mode", default="streams", choices=("streams", "batched"))
+    ap.add_argument("--serve-only", action="store_true", help="only the served-stream sample")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="no HIP-event timing (for hardware-counter runs, where the profiler rejects it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -266,6 +269,10 @@ def main() -> int:
         log(f"copy calibration: {args.copy_calibration} MiB read, {args.copy_calibration} MiB written")
         return 0
     from myplacement_amd import backend as hb
+    if args.serve_only:
+        sv = serving_sample(args.serve_jobs, args.serve_rate, args.serve_slots, mode=args.serve_mode)
+        print(json.dumps({"serving": sv}), flush=True)
+        return 0
     if args.workload == "batch":
         spec = [(m, int(k)) for m, k in (x.split(":") for x in args.batch_runs.split(","))]
         b = batch_sample(args.batch_jobs, spec, args.batch_iterations)
@@ -339,7 +346,7 @@ def main() -> int:
     sess.close()
     if args.serve_jobs > 0:
         log(f"served stream sample: {args.serve_jobs} jobs ...")
-        line["serving"] = serving_sample(args.serve_jobs, args.serve_rate, args.serve_slots)
+        line["serving"] = serving_sample(args.serve_jobs, args.serve_rate, args.serve_slots, mode=args.serve_mode)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         cb = cpu_baseline(args.workload, args.cpu_iterations)

@@ -19,12 +19,18 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <coroutine>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <exception>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <numeric>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -234,6 +240,7 @@ public:
     // hands the scalars the batched launches delivered to the suspended step
     void deliver(const HostScalars* h) { scalars_ = h; }
     Step iteration() { return iterate(); }
+    int particles() const { return P_; }
     void addTime(double setup, double loop) {
         setup_seconds_ += setup;
         loop_seconds_ += loop;
@@ -669,8 +676,13 @@ private:
 };
 
 // ---- batch driver ----------------------------------------------------------------------------------
+// Jobs are prepared (netlist preparation, device allocation and upload) and finished (download,
+// result metrics, release) by a pool of host threads, so that this host work overlaps the rounds
+// of the running jobs; the driver thread only runs rounds.
 struct BatchDriver::Impl {
     struct Job {
+        mp_design design{};
+        mp_gp_options options{};
         std::unique_ptr<Session> s;
         Step step;
         bool initialized = false, finished = false;
@@ -686,28 +698,81 @@ struct BatchDriver::Impl {
         std::size_t lds, entries, first;
     };
 
+    // A cohort of running jobs with its own table and completion words.  Two cohorts take turns on
+    // the stream: while the device runs one cohort's round, the host resumes the other cohort's jobs
+    // and builds and issues its next round.
+    struct Cohort {
+        std::vector<std::unique_ptr<Job>> jobs;
+        std::vector<std::unique_ptr<Job>> joining;  // prepared, joining at the cohort's next turn
+        long long particles = 0;                    // of its jobs and joining jobs
+        unsigned char* host_table = nullptr;  // pinned staging of the round's table
+        unsigned char* dev_table = nullptr;
+        std::size_t table_bytes = 0;
+        unsigned* host_flags = nullptr;  // host-mapped completion words, one per reduction job
+        unsigned* dev_flags = nullptr;
+        int flag_count = 0;
+        unsigned seq = 0;
+        bool issued = false;  // a round is issued and not yet collected
+        int reductions = 0;   // of the issued round
+    };
+
     int device;
     hipStream_t stream = nullptr;
-    std::vector<std::unique_ptr<Job>> jobs;
-    unsigned char* host_table = nullptr;  // pinned staging of the round's table
-    unsigned char* dev_table = nullptr;
-    std::size_t table_bytes = 0;
-    unsigned* host_flags = nullptr;  // host-mapped completion words, one per reduction job
-    unsigned* dev_flags = nullptr;
-    int flag_count = 0;
-    unsigned seq = 0;
+    Cohort cohorts[2];
+    int turn = 0;
+    // MP_GP_BATCH_STATS set: a summary of the rounds on stderr when the driver ends
+    bool stats = std::getenv("MP_GP_BATCH_STATS") != nullptr;
+    long long st_rounds = 0, st_job_rounds = 0, st_launches = 0, st_reductions = 0, st_table_bytes = 0;
+    double st_build = 0.0, st_issue = 0.0, st_wait = 0.0, st_advance = 0.0;
+    long long st_prepared = 0, st_finished = 0;
+    double st_prepare = 0.0, st_finish = 0.0, st_last_ready = 0.0, st_last_done = 0.0;  // guarded by mu
+    const std::chrono::steady_clock::time_point t_created = std::chrono::steady_clock::now();
+
+    // host work pool
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::unique_ptr<Job>> to_prepare, ready, to_finish;
+    int in_pool = 0;  // jobs queued for or inside preparation / finishing
+    bool stop = false;
+    std::vector<std::thread> pool;
 
     explicit Impl(int d) : device(d) {
         hipCheck(hipSetDevice(device), "select HIP device");
         hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "create batch stream");
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int threads = static_cast<int>(std::min(16U, std::max(2U, hw)));
+        for (int k = 0; k < threads; ++k) pool.emplace_back([this] { work(); });
     }
     ~Impl() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        for (auto& t : pool) t.join();
+        if (stats && st_rounds > 0)
+            std::fprintf(stderr,
+                         "batch driver: %lld cohort rounds, %.1f jobs/round, %.1f launches/round, %.1f reductions/round, "
+                         "%.0f table bytes/round; per round: build %.1f us, issue %.1f us, wait %.1f us, "
+                         "resume %.1f us; %lld jobs prepared (%.2f ms each, last ready at %.3f s), %lld finished "
+                         "(%.2f ms each, last at %.3f s); %.3f s in rounds\n",
+                         st_rounds, double(st_job_rounds) / st_rounds, double(st_launches) / st_rounds,
+                         double(st_reductions) / st_rounds, double(st_table_bytes) / st_rounds,
+                         1e6 * st_build / st_rounds, 1e6 * st_issue / st_rounds, 1e6 * st_wait / st_rounds,
+                         1e6 * st_advance / st_rounds, st_prepared, 1e3 * st_prepare / std::max(1LL, st_prepared),
+                         st_last_ready, st_finished, 1e3 * st_finish / std::max(1LL, st_finished), st_last_done,
+                         st_build + st_issue + st_wait + st_advance);
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
-        jobs.clear();
-        if (host_table) (void)hipHostFree(host_table);
-        if (dev_table) (void)hipFree(dev_table);
-        if (host_flags) (void)hipHostFree(host_flags);
+        ready.clear();
+        to_prepare.clear();
+        for (Cohort& c : cohorts) {
+            c.jobs.clear();
+            c.joining.clear();
+            if (c.host_table) (void)hipHostFree(c.host_table);
+            if (c.dev_table) (void)hipFree(c.dev_table);
+            if (c.host_flags) (void)hipHostFree(c.host_flags);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -744,6 +809,86 @@ struct BatchDriver::Impl {
         }
     }
 
+    void work() {
+        bool selected = false;
+        for (;;) {
+            std::unique_ptr<Job> j;
+            bool finishing = false;
+            {
+                std::unique_lock<std::mutex> g(mu);
+                cv.wait(g, [&] { return stop || !to_finish.empty() || !to_prepare.empty(); });
+                if (!to_finish.empty()) {
+                    j = std::move(to_finish.front());
+                    to_finish.pop_front();
+                    finishing = true;
+                } else if (!to_prepare.empty()) {
+                    j = std::move(to_prepare.front());
+                    to_prepare.pop_front();
+                } else {
+                    return;  // stopping with nothing queued
+                }
+            }
+            if (!selected) selected = hipSetDevice(device) == hipSuccess;
+            bool to_ready = false;
+            const auto t0 = std::chrono::steady_clock::now();
+            if (finishing) {
+                complete(*j);
+            } else {
+                prepare(*j);
+                if (j->s) to_ready = true;
+            }
+            if (!to_ready) j.reset();  // done: release its device state here, off the driver thread
+            const double took = since(t0), at = since(t_created);
+            std::lock_guard<std::mutex> g(mu);
+            if (finishing) {
+                ++st_finished;
+                st_finish += took;
+                st_last_done = at;
+            } else {
+                ++st_prepared;
+                st_prepare += took;
+                st_last_ready = at;
+            }
+            if (to_ready) ready.push_back(std::move(j));
+            --in_pool;
+            cv.notify_all();
+        }
+    }
+
+    // the host side of starting a job; a failure completes it at once
+    void prepare(Job& j) {
+        try {
+            if (!j.centers) throw std::invalid_argument("null centres");
+            if (j.options.device != device) throw std::invalid_argument("a batch runs on one device");
+            hipCheck(hipSetDevice(device), "select HIP device");
+            mp_design d = j.design;
+            d.center = j.centers;
+            Netlist probe(d);
+            if (probe.movable.empty()) {  // nothing to place
+                j.done(MP_GP_OK, "", mp_gp_result{});
+                return;
+            }
+            validateOptions(j.options);
+            j.hpwl_before = probe.hpwl(probe.cx.data(), probe.cy.data());
+            j.overflow_before = probe.designOverflow(probe.cx.data(), probe.cy.data(), j.options.bins_x,
+                                                     j.options.bins_y, j.options.target_density);
+            auto prep = std::make_shared<const Prepared>(d, j.options);
+            j.s = std::make_unique<Session>(prep, j.options, nullptr, true);
+        } catch (...) {
+            j.s.reset();
+            const mp_gp_status st = translate(std::current_exception());
+            const std::string msg = g_last_error;
+            j.done(st, msg.c_str(), mp_gp_result{});
+        }
+    }
+
+    int running() {
+        std::lock_guard<std::mutex> g(mu);
+        std::size_t n = ready.size();
+        for (const Cohort& c : cohorts) n += c.jobs.size() + c.joining.size();
+        return static_cast<int>(n) + in_pool;
+    }
+
     void complete(Job& j) {
         mp_gp_result r{};
         mp_gp_status st = MP_GP_OK;
@@ -768,64 +913,121 @@ struct BatchDriver::Impl {
         j.done(st, msg.c_str(), r);
     }
 
-    void failAll(const std::exception_ptr& e) {
+    // queue finished jobs for the pool (their callbacks and release)
+    void handOver(std::vector<std::unique_ptr<Job>>& done) {
+        if (done.empty()) return;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            for (auto& j : done) {
+                to_finish.push_back(std::move(j));
+                ++in_pool;
+            }
+        }
+        done.clear();
+        cv.notify_all();
+    }
+    // a round of the cohort failed: every job of it fails with that error
+    void failCohort(Cohort& c, const std::exception_ptr& e) {
         (void)hipStreamSynchronize(stream);
-        const mp_gp_status st = translate(e);
-        const std::string msg = g_last_error;
+        c.issued = false;
         std::vector<std::unique_ptr<Job>> gone;
-        gone.swap(jobs);
-        for (auto& j : gone) j->done(st, msg.c_str(), mp_gp_result{});
+        gone.swap(c.jobs);
+        c.particles = 0;
+        for (auto& j : c.joining) c.particles += j->s->particles();
+        for (auto& j : gone) {
+            j->error = e;
+            j->finished = true;
+        }
+        handOver(gone);
     }
 
-    void reserve(std::size_t bytes, int flags) {
-        if (bytes > table_bytes) {
-            if (host_table) hipCheck(hipHostFree(host_table), "free batch table");
-            if (dev_table) hipCheck(hipFree(dev_table), "free batch table");
-            host_table = nullptr;
-            dev_table = nullptr;
-            table_bytes = 0;
+    static void reserve(Cohort& c, std::size_t bytes, int flags) {
+        if (bytes > c.table_bytes) {
+            if (c.host_table) hipCheck(hipHostFree(c.host_table), "free batch table");
+            if (c.dev_table) hipCheck(hipFree(c.dev_table), "free batch table");
+            c.host_table = nullptr;
+            c.dev_table = nullptr;
+            c.table_bytes = 0;
             const std::size_t cap = std::max<std::size_t>(bytes + bytes / 2, std::size_t{1} << 20);
-            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&host_table), cap, hipHostMallocDefault), "allocate batch table");
-            hipCheck(hipMalloc(reinterpret_cast<void**>(&dev_table), cap), "allocate batch table");
-            table_bytes = cap;
+            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&c.host_table), cap, hipHostMallocDefault),
+                     "allocate batch table");
+            hipCheck(hipMalloc(reinterpret_cast<void**>(&c.dev_table), cap), "allocate batch table");
+            c.table_bytes = cap;
         }
-        if (flags > flag_count) {
-            if (host_flags) hipCheck(hipHostFree(host_flags), "free completion words");
-            host_flags = nullptr;
-            flag_count = 0;
+        if (flags > c.flag_count) {
+            if (c.host_flags) hipCheck(hipHostFree(c.host_flags), "free completion words");
+            c.host_flags = nullptr;
+            c.flag_count = 0;
             const int cap = std::max(flags + flags / 2, 1024);
-            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&host_flags), cap * sizeof(unsigned),
+            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&c.host_flags), cap * sizeof(unsigned),
                                    hipHostMallocMapped | hipHostMallocCoherent),
                      "allocate completion words");
-            std::memset(host_flags, 0, cap * sizeof(unsigned));
-            hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&dev_flags), host_flags, 0), "map completion words");
-            flag_count = cap;
+            std::memset(c.host_flags, 0, cap * sizeof(unsigned));
+            hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&c.dev_flags), c.host_flags, 0),
+                     "map completion words");
+            c.flag_count = cap;
         }
     }
 
+    // One turn: collect the cohort's issued round (wait, resume its jobs), let prepared jobs join it,
+    // retire its finished jobs, and issue its next round without waiting for it.
     void round() {
         hipCheck(hipSetDevice(device), "select HIP device");
-        // Retire finished jobs.  Every launch they recorded ran in an earlier round, which has
-        // completed (its completion words were all seen, or the stream was synchronized).
-        for (std::size_t k = 0; k < jobs.size();) {
-            Job& j = *jobs[k];
+        Cohort& c = cohorts[turn];
+        Cohort& other = cohorts[turn ^ 1];
+        turn ^= 1;
+        // prepared jobs go to the cohort with fewer particles, joining it at its turn
+        {
+            std::lock_guard<std::mutex> g(mu);
+            for (; !ready.empty(); ready.pop_front()) {
+                Cohort& to = c.particles <= other.particles ? c : other;
+                to.particles += ready.front()->s->particles();
+                to.joining.push_back(std::move(ready.front()));
+            }
+        }
+        if (c.issued) {
+            try {
+                collect(c);
+            } catch (...) {
+                failCohort(c, std::current_exception());
+            }
+        }
+        for (auto& j : c.joining) {
+            advance(*j);  // its first step records its first launches
+            c.jobs.push_back(std::move(j));
+        }
+        c.joining.clear();
+        // Retire finished jobs.  Every launch they recorded ran in a round that has been collected.
+        std::vector<std::unique_ptr<Job>> done;
+        for (std::size_t k = 0; k < c.jobs.size();) {
+            Job& j = *c.jobs[k];
             if (j.finished && (j.error || j.s->recorder()->ops.empty())) {
-                std::unique_ptr<Job> gone = std::move(jobs[k]);
-                jobs.erase(jobs.begin() + static_cast<std::ptrdiff_t>(k));
-                complete(*gone);
+                c.particles -= j.s->particles();
+                done.push_back(std::move(c.jobs[k]));
+                c.jobs.erase(c.jobs.begin() + static_cast<std::ptrdiff_t>(k));
             } else {
                 ++k;
             }
         }
-        if (jobs.empty()) return;
-        try {
-            runRound();
-        } catch (...) {
-            failAll(std::current_exception());
+        handOver(done);
+        if (!c.jobs.empty()) {
+            try {
+                issue(c);
+            } catch (...) {
+                failCohort(c, std::current_exception());
+            }
+            return;
+        }
+        if (other.jobs.empty() && other.joining.empty()) {
+            // nothing runs: wait until the pool has a job ready or has nothing left
+            std::unique_lock<std::mutex> g(mu);
+            cv.wait(g, [&] { return !ready.empty() || in_pool == 0; });
         }
     }
 
-    void runRound() {
+    // builds the cohort's table from its jobs' recorded launches and issues the round
+    void issue(Cohort& c) {
+        const auto t_build = std::chrono::steady_clock::now();
         // every recorded launch, keyed (pass, stage, kernel, job, position); a job's launches that go
         // back to an earlier stage start its next pass
         struct Ref {
@@ -833,8 +1035,8 @@ struct BatchDriver::Impl {
         };
         std::vector<Ref> refs;
         std::vector<RedJob> reds;
-        for (int ji = 0; ji < static_cast<int>(jobs.size()); ++ji) {
-            Job& j = *jobs[ji];
+        for (int ji = 0; ji < static_cast<int>(c.jobs.size()); ++ji) {
+            Job& j = *c.jobs[ji];
             if (j.error) continue;
             const std::vector<BatchOp>& ops = j.s->recorder()->ops;
             int pass = 0, prev = -1;
@@ -861,7 +1063,7 @@ struct BatchDriver::Impl {
             while (b < refs.size() && refs[b].pass == refs[a].pass && refs[b].stage == refs[a].stage &&
                    refs[b].kernel == refs[a].kernel)
                 ++b;
-            const BatchOp& op0 = jobs[refs[a].job]->s->recorder()->ops[refs[a].op];
+            const BatchOp& op0 = c.jobs[refs[a].job]->s->recorder()->ops[refs[a].op];
             Launch L{refs[a].kernel, static_cast<int>(b - a), 0, 0, 0, 0};
             off = align(off, 256);
             L.entries = off;
@@ -875,16 +1077,16 @@ struct BatchDriver::Impl {
         off = align(off, 256);
         const std::size_t red_off = off;
         off += reds.size() * sizeof(RedJob);
-        reserve(off, static_cast<int>(reds.size()));
-        // fill the staging table
+        reserve(c, off, static_cast<int>(reds.size()));
+        // fill the staging table (the cohort's previous round has been collected: its copy is done)
         std::size_t r = 0;
         for (Launch& L : launches) {
-            int* first = reinterpret_cast<int*>(host_table + L.first);
+            int* first = reinterpret_cast<int*>(c.host_table + L.first);
             long long total = 0;
             first[0] = 0;
             for (int e = 0; e < L.n; ++e, ++r) {
-                const BatchOp& op = jobs[refs[r].job]->s->recorder()->ops[refs[r].op];
-                std::memcpy(host_table + L.entries + static_cast<std::size_t>(e) * op.bytes, op.entry, op.bytes);
+                const BatchOp& op = c.jobs[refs[r].job]->s->recorder()->ops[refs[r].op];
+                std::memcpy(c.host_table + L.entries + static_cast<std::size_t>(e) * op.bytes, op.entry, op.bytes);
                 total += op.blocks;
                 if (total > (1LL << 30)) throw std::runtime_error("batched launch exceeds the grid limit");
                 first[e + 1] = static_cast<int>(total);
@@ -892,20 +1094,42 @@ struct BatchDriver::Impl {
             }
             L.blocks = static_cast<int>(total);
         }
-        if (!reds.empty()) std::memcpy(host_table + red_off, reds.data(), reds.size() * sizeof(RedJob));
-        if (off > 0) hipCheck(hipMemcpyAsync(dev_table, host_table, off, hipMemcpyHostToDevice, stream), "upload batch table");
+        if (!reds.empty()) std::memcpy(c.host_table + red_off, reds.data(), reds.size() * sizeof(RedJob));
+        const auto t_issue = std::chrono::steady_clock::now();
+        for (auto& j : c.jobs) j->s->recorder()->ops.clear();
+        c.issued = true;
+        c.reductions = static_cast<int>(reds.size());
+        if (off > 0)
+            hipCheck(hipMemcpyAsync(c.dev_table, c.host_table, off, hipMemcpyHostToDevice, stream), "upload batch table");
         for (const Launch& L : launches)
-            launchBatched(static_cast<BatchKernel>(L.kernel), dev_table + L.entries,
-                          reinterpret_cast<const int*>(dev_table + L.first), L.n, L.blocks, L.lds, stream);
-        for (auto& j : jobs) j->s->recorder()->ops.clear();
-        const int nred = static_cast<int>(reds.size());
+            launchBatched(static_cast<BatchKernel>(L.kernel), c.dev_table + L.entries,
+                          reinterpret_cast<const int*>(c.dev_table + L.first), L.n, L.blocks, L.lds, stream);
+        if (c.reductions > 0) {
+            if (++c.seq == 0) c.seq = 1;
+            launchFinalizeBatch(reinterpret_cast<const RedJob*>(c.dev_table + red_off), c.reductions, c.dev_flags,
+                                c.seq, stream);
+        }
+        if (stats) {
+            using D = std::chrono::duration<double>;
+            ++st_rounds;
+            st_job_rounds += static_cast<long long>(c.jobs.size());
+            st_launches += static_cast<long long>(launches.size());
+            st_reductions += c.reductions;
+            st_table_bytes += static_cast<long long>(off);
+            st_build += D(t_issue - t_build).count();
+            st_issue += D(std::chrono::steady_clock::now() - t_issue).count();
+        }
+    }
+
+    // waits for the cohort's issued round and resumes its jobs with their scalars
+    void collect(Cohort& c) {
+        const auto t_wait = std::chrono::steady_clock::now();
+        c.issued = false;
+        const int nred = c.reductions;
         if (nred > 0) {
-            ++seq;
-            if (seq == 0) seq = 1;
-            launchFinalizeBatch(reinterpret_cast<const RedJob*>(dev_table + red_off), nred, dev_flags, seq, stream);
             auto written = [&] {
                 for (int b = 0; b < nred; ++b)
-                    if (__atomic_load_n(&host_flags[b], __ATOMIC_ACQUIRE) != seq) return false;
+                    if (__atomic_load_n(&c.host_flags[b], __ATOMIC_ACQUIRE) != c.seq) return false;
                 return true;
             };
             for (long spins = 0; !written(); ++spins) {
@@ -919,51 +1143,41 @@ struct BatchDriver::Impl {
         } else {
             hipCheck(hipStreamSynchronize(stream), "synchronize batched launches");
         }
-        for (auto& jp : jobs) {
+        const auto t_advance = std::chrono::steady_clock::now();
+        for (auto& jp : c.jobs) {
             Job& j = *jp;
             if (j.finished) continue;
             if (j.s->waiting()) j.s->deliver(&j.s->backend().hostScalars());
             advance(j);
+        }
+        if (stats) {
+            using D = std::chrono::duration<double>;
+            st_wait += D(t_advance - t_wait).count();
+            st_advance += D(std::chrono::steady_clock::now() - t_advance).count();
         }
     }
 };
 
 BatchDriver::BatchDriver(int device) : impl_(std::make_unique<Impl>(device)) {}
 BatchDriver::~BatchDriver() = default;
-int BatchDriver::running() const { return static_cast<int>(impl_->jobs.size()); }
+int BatchDriver::running() const { return impl_->running(); }
 void BatchDriver::round() { impl_->round(); }
 
 void BatchDriver::add(const mp_design& design, const mp_gp_options& options, double* centers, Callback done) {
     auto j = std::make_unique<Impl::Job>();
     j->t0 = std::chrono::steady_clock::now();
+    j->design = design;
+    j->options = options;
     j->centers = centers;
     j->done = std::move(done);
-    try {
-        if (!centers) throw std::invalid_argument("null centres");
-        if (options.device != impl_->device) throw std::invalid_argument("a batch runs on one device");
-        mp_design d = design;
-        d.center = centers;
-        Netlist probe(d);
-        if (probe.movable.empty()) {  // nothing to place
-            j->done(MP_GP_OK, "", mp_gp_result{});
-            return;
-        }
-        validateOptions(options);
-        j->hpwl_before = probe.hpwl(probe.cx.data(), probe.cy.data());
-        j->overflow_before =
-            probe.designOverflow(probe.cx.data(), probe.cy.data(), options.bins_x, options.bins_y, options.target_density);
-        auto prep = std::make_shared<const Prepared>(d, options);
-        hipCheck(hipSetDevice(impl_->device), "select HIP device");
-        j->s = std::make_unique<Session>(prep, options, nullptr, true);
-    } catch (...) {
-        const mp_gp_status st = translate(std::current_exception());
-        const std::string msg = g_last_error;
-        j->done(st, msg.c_str(), mp_gp_result{});
-        return;
+    {
+        std::lock_guard<std::mutex> g(impl_->mu);
+        impl_->to_prepare.push_back(std::move(j));
+        ++impl_->in_pool;
     }
-    Impl::advance(*j);
-    impl_->jobs.push_back(std::move(j));
+    impl_->cv.notify_all();
 }
+
 
 }  // namespace mpgp
 
