@@ -76,7 +76,7 @@ void DeviceBudget::reserve(std::size_t bytes, const char* what) {
     reserved_ += bytes;
 }
 
-Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
+Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const BorrowedResources* borrowed)
     : device_(L.device), budget_(allocation_limit) {
     if (L.particle_count < 0 || L.net_count < 0 || L.pin_count < 0)
         throw InvalidArgument("negative counts in the placement layout");
@@ -203,7 +203,13 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
 
     // ---- device ----
     setDevice();
-    hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "create HIP stream");
+    if (borrowed && borrowed->stream) {
+        stream_ = borrowed->stream;
+        owns_stream_ = false;
+    } else {
+        hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "create HIP stream");
+    }
+    if (borrowed) budget_.setArena(borrowed->arena);
     own_launches_ = std::make_unique<StreamLaunches>(stream_);
     launches_ = own_launches_.get();
     hipStream_t stream = stream_;
@@ -264,11 +270,18 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit)
     wire_out_.allocate(P_, budget_, "wire gradient");
     density_out_.allocate(P_, budget_, "density gradient");
     // coherent host memory: k_finalize's stores go straight to the host, no copy after it
-    hipCheck(hipHostMalloc(reinterpret_cast<void**>(&host_scal_), sizeof(HostScalars),
-                           hipHostMallocMapped | hipHostMallocCoherent),
-             "allocate host-mapped scalars");
-    std::memset(host_scal_, 0, sizeof(HostScalars));
-    hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&dev_scal_), host_scal_, 0), "map scalar block");
+    if (borrowed && borrowed->scalars) {
+        host_scal_ = borrowed->scalars;
+        dev_scal_ = reinterpret_cast<double*>(borrowed->scalars_device);
+        owns_scalars_ = false;
+        std::memset(host_scal_, 0, sizeof(HostScalars));
+    } else {
+        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&host_scal_), sizeof(HostScalars),
+                               hipHostMallocMapped | hipHostMallocCoherent),
+                 "allocate host-mapped scalars");
+        std::memset(host_scal_, 0, sizeof(HostScalars));
+        hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&dev_scal_), host_scal_, 0), "map scalar block");
+    }
     dev_done_ = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(dev_scal_) + offsetof(HostScalars, done));
 
     std::vector<int> layers(L.particle_layer, L.particle_layer + P_);
@@ -345,9 +358,9 @@ Backend::~Backend() {
         (void)hipSetDevice(device_);
         (void)hipStreamSynchronize(stream_);
         for (auto e : ev_pool_) (void)hipEventDestroy(e);
-        (void)hipStreamDestroy(stream_);
+        if (owns_stream_) (void)hipStreamDestroy(stream_);
     }
-    if (host_scal_) (void)hipHostFree(host_scal_);
+    if (host_scal_ && owns_scalars_) (void)hipHostFree(host_scal_);
 }
 
 void Backend::setDevice() const { hipCheck(hipSetDevice(device_), "select HIP device"); }

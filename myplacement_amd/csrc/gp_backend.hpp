@@ -29,16 +29,33 @@ public:
 
 void hipCheck(hipError_t e, const char* what);
 
-// Allocation accounting against a byte limit (DeviceMemoryBudget, .cu:60-79).
+// Device memory that a job's arrays are carved from (256-byte aligned) instead of being allocated
+// one by one; the owner of the block keeps it alive past every array carved from it.
+struct DeviceArena {
+    char* base = nullptr;
+    std::size_t size = 0, used = 0;
+    void* carve(std::size_t bytes) {
+        const std::size_t at = (used + 255) / 256 * 256;
+        if (!base || at > size || bytes > size - at) return nullptr;
+        used = at + bytes;
+        return base + at;
+    }
+};
+
+// Allocation accounting against a byte limit (DeviceMemoryBudget, .cu:60-79); with an arena the
+// arrays are carved from it while it has room.
 class DeviceBudget {
 public:
     explicit DeviceBudget(std::size_t limit) : limit_(limit) {}
     void reserve(std::size_t bytes, const char* what);
     std::size_t reserved() const { return reserved_; }
+    void setArena(DeviceArena* a) { arena_ = a; }
+    void* carve(std::size_t bytes) { return arena_ ? arena_->carve(bytes) : nullptr; }
 
 private:
     std::size_t limit_;
     std::size_t reserved_ = 0;
+    DeviceArena* arena_ = nullptr;
 };
 
 template <typename T>
@@ -46,14 +63,20 @@ class DeviceArray {
 public:
     DeviceArray() = default;
     ~DeviceArray() {
-        if (p_) (void)hipFree(p_);
+        if (p_ && owned_) (void)hipFree(p_);
     }
     DeviceArray(const DeviceArray&) = delete;
     DeviceArray& operator=(const DeviceArray&) = delete;
     void allocate(std::size_t n, DeviceBudget& budget, const char* what) {
         if (n == 0) n = 1;  // keep a valid pointer for empty sets
         budget.reserve(n * sizeof(T), what);
-        hipCheck(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)), what);
+        if (void* q = budget.carve(n * sizeof(T))) {
+            p_ = static_cast<T*>(q);
+            owned_ = false;
+        } else {
+            hipCheck(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)), what);
+            owned_ = true;
+        }
         n_ = n;
     }
     void upload(const T* host, std::size_t n, hipStream_t s, const char* what) {
@@ -65,11 +88,13 @@ public:
     void swapWith(DeviceArray& o) {
         std::swap(p_, o.p_);
         std::swap(n_, o.n_);
+        std::swap(owned_, o.owned_);
     }
 
 private:
     T* p_ = nullptr;
     std::size_t n_ = 0;
+    bool owned_ = false;
 };
 
 // Where a backend's kernels go: launched at once on its stream (StreamLaunches), or recorded for
@@ -164,9 +189,19 @@ struct HostScalars {
     const double* bank(int b) const { return v + b * kScalarCount; }
 };
 
+// What a backend borrows instead of creating its own (the jobs of a batch driver): the stream of
+// its uploads and downloads, device memory its arrays are carved from, and a host-mapped scalar
+// block (host and device addresses).  Each may be null.
+struct BorrowedResources {
+    hipStream_t stream = nullptr;
+    DeviceArena* arena = nullptr;
+    HostScalars* scalars = nullptr;
+    HostScalars* scalars_device = nullptr;
+};
+
 class Backend {
 public:
-    Backend(const mp_gp_static& layout, std::size_t allocation_limit);
+    Backend(const mp_gp_static& layout, std::size_t allocation_limit, const BorrowedResources* borrowed = nullptr);
     ~Backend();
     Backend(const Backend&) = delete;
     Backend& operator=(const Backend&) = delete;
@@ -285,6 +320,7 @@ private:
     DeviceArray<double> partials_[kRedKinds * kScalarBanks];
     RedList pending_{};
     DeviceArray<double2> wire_out_, density_out_;
+    bool owns_stream_ = true, owns_scalars_ = true;
     HostScalars* host_scal_ = nullptr;  // host-mapped, written by k_finalize
     double* dev_scal_ = nullptr;        // device address of host_scal_
     unsigned* dev_done_ = nullptr;      // device address of host_scal_->done

@@ -18,6 +18,9 @@
 
 namespace mpgp {
 
+// Upper bound of the device bytes one global placement run of the design reserves.
+uint64_t estimateBytes(const mp_design& design, const mp_gp_options& options);
+
 class BatchDriver {
 public:
     // status, error message (empty on success), result

@@ -33,8 +33,8 @@ extern thread_local std::string g_last_error;
 mp_gp_status translate(const std::exception_ptr& e);
 
 namespace {
-
 int roundUp32(long long n) { return static_cast<int>((n + 31) / 32 * 32); }
+}  // namespace
 
 // Upper bound of the device bytes one run reserves (the backend's and the session's allocations,
 // gp_backend.cpp / gp_session.cpp), from the design's counts.
@@ -55,6 +55,8 @@ uint64_t estimateBytes(const mp_design& d, const mp_gp_options& o) {
     bytes += 64LL << 20;                                               // partials, scalars, alignment slack
     return static_cast<uint64_t>(bytes);
 }
+
+namespace {
 
 struct Job {
     int64_t id = 0;

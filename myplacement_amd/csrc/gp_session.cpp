@@ -27,10 +27,12 @@
 #include <deque>
 #include <exception>
 #include <limits>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <numeric>
 #include <thread>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -122,7 +124,8 @@ class Session {
 public:
     // batched: the session's launches are recorded for a batch driver, which also starts and
     // resumes its steps (initialize, then iterate until stopped)
-    Session(std::shared_ptr<const Prepared> prep, const mp_gp_options& o, const double* centers, bool batched = false)
+    Session(std::shared_ptr<const Prepared> prep, const mp_gp_options& o, const double* centers, bool batched = false,
+            const BorrowedResources* borrowed = nullptr)
         : prep_(std::move(prep)), o_(o) {
         validateOptions(o_);
         const Prepared& p = *prep_;
@@ -179,7 +182,7 @@ public:
         L.particle_charge_area = pq.data();
         L.particle_layer = player.data();
         L.pin_particles = pin_particles.data();
-        be_ = std::make_unique<Backend>(L, limit);
+        be_ = std::make_unique<Backend>(L, limit, borrowed);
         P_ = be_->particles();
         DeviceBudget& b = be_->budget();
         for (DeviceArray<double2>* a : {&X_, &XP_, &B_, &Y_, &PX_, &PG_, &PXn_, &PGn_, &Rc_, &Gc_, &Rl_, &Gl_, &CKf_, &CKb_})
@@ -683,6 +686,10 @@ struct BatchDriver::Impl {
     struct Job {
         mp_design design{};
         mp_gp_options options{};
+        // borrowed by the session's backend; released after the session
+        DeviceArena arena;
+        HostScalars* scalars = nullptr;
+        HostScalars* scalars_device = nullptr;
         std::unique_ptr<Session> s;
         Step step;
         bool initialized = false, finished = false;
@@ -723,7 +730,7 @@ struct BatchDriver::Impl {
     // MP_GP_BATCH_STATS set: a summary of the rounds on stderr when the driver ends
     bool stats = std::getenv("MP_GP_BATCH_STATS") != nullptr;
     long long st_rounds = 0, st_job_rounds = 0, st_launches = 0, st_reductions = 0, st_table_bytes = 0;
-    double st_build = 0.0, st_issue = 0.0, st_wait = 0.0, st_advance = 0.0;
+    double st_build = 0.0, st_issue = 0.0, st_wait = 0.0, st_advance = 0.0, st_upload = 0.0;
     long long st_prepared = 0, st_finished = 0;
     double st_prepare = 0.0, st_finish = 0.0, st_last_ready = 0.0, st_last_done = 0.0;  // guarded by mu
     const std::chrono::steady_clock::time_point t_created = std::chrono::steady_clock::now();
@@ -735,6 +742,15 @@ struct BatchDriver::Impl {
     int in_pool = 0;  // jobs queued for or inside preparation / finishing
     bool stop = false;
     std::vector<std::thread> pool;
+    std::vector<hipStream_t> pool_streams;  // one per pool thread, lent to the jobs it prepares
+    // Device blocks the jobs' arrays are carved from, and host-mapped scalar blocks: kept and reused
+    // for later jobs, so that starting and ending a job allocates and frees nothing on the device
+    // (a free waits for the whole device).  Guarded by mu.
+    std::multimap<std::size_t, char*> free_slabs;
+    std::vector<char*> slabs;
+    std::vector<HostScalars*> free_scalars;
+    std::vector<HostScalars*> scalar_chunks;
+    std::map<HostScalars*, HostScalars*> scalar_device;  // chunk host address -> device address
 
     explicit Impl(int d) : device(d) {
         hipCheck(hipSetDevice(device), "select HIP device");
@@ -754,12 +770,12 @@ struct BatchDriver::Impl {
             std::fprintf(stderr,
                          "batch driver: %lld cohort rounds, %.1f jobs/round, %.1f launches/round, %.1f reductions/round, "
                          "%.0f table bytes/round; per round: build %.1f us, issue %.1f us, wait %.1f us, "
-                         "resume %.1f us; %lld jobs prepared (%.2f ms each, last ready at %.3f s), %lld finished "
+                         "resume %.1f us (upload call %.1f us); %lld jobs prepared (%.2f ms each, last ready at %.3f s), %lld finished "
                          "(%.2f ms each, last at %.3f s); %.3f s in rounds\n",
                          st_rounds, double(st_job_rounds) / st_rounds, double(st_launches) / st_rounds,
                          double(st_reductions) / st_rounds, double(st_table_bytes) / st_rounds,
                          1e6 * st_build / st_rounds, 1e6 * st_issue / st_rounds, 1e6 * st_wait / st_rounds,
-                         1e6 * st_advance / st_rounds, st_prepared, 1e3 * st_prepare / std::max(1LL, st_prepared),
+                         1e6 * st_advance / st_rounds, 1e6 * st_upload / st_rounds, st_prepared, 1e3 * st_prepare / std::max(1LL, st_prepared),
                          st_last_ready, st_finished, 1e3 * st_finish / std::max(1LL, st_finished), st_last_done,
                          st_build + st_issue + st_wait + st_advance);
         (void)hipSetDevice(device);
@@ -773,7 +789,59 @@ struct BatchDriver::Impl {
             if (c.dev_table) (void)hipFree(c.dev_table);
             if (c.host_flags) (void)hipHostFree(c.host_flags);
         }
+        for (char* p : slabs) (void)hipFree(p);
+        for (HostScalars* p : scalar_chunks) (void)hipHostFree(p);
+        for (hipStream_t t : pool_streams) (void)hipStreamDestroy(t);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // a device block of at least `bytes` (a kept one up to twice that size, else a new one)
+    DeviceArena takeSlab(std::size_t bytes) {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            auto it = free_slabs.lower_bound(bytes);
+            if (it != free_slabs.end() && it->first <= 2 * bytes) {
+                DeviceArena a{it->second, it->first, 0};
+                free_slabs.erase(it);
+                return a;
+            }
+        }
+        const std::size_t size = (bytes + (std::size_t{2} << 20) - 1) / (std::size_t{2} << 20) * (std::size_t{2} << 20);
+        char* p = nullptr;
+        hipCheck(hipMalloc(reinterpret_cast<void**>(&p), size), "allocate job memory");
+        std::lock_guard<std::mutex> g(mu);
+        slabs.push_back(p);
+        return DeviceArena{p, size, 0};
+    }
+    // a host-mapped scalar block: host and device addresses
+    std::pair<HostScalars*, HostScalars*> takeScalars() {
+        std::lock_guard<std::mutex> g(mu);
+        if (free_scalars.empty()) {
+            constexpr int kChunk = 256;
+            HostScalars* h = nullptr;
+            HostScalars* d = nullptr;
+            hipCheck(hipHostMalloc(reinterpret_cast<void**>(&h), kChunk * sizeof(HostScalars),
+                                   hipHostMallocMapped | hipHostMallocCoherent),
+                     "allocate host-mapped scalars");
+            scalar_chunks.push_back(h);
+            hipCheck(hipHostGetDevicePointer(reinterpret_cast<void**>(&d), h, 0), "map scalar blocks");
+            for (int k = kChunk - 1; k >= 0; --k) {
+                free_scalars.push_back(h + k);
+                scalar_device[h + k] = d + k;
+            }
+        }
+        HostScalars* h = free_scalars.back();
+        free_scalars.pop_back();
+        return {h, scalar_device[h]};
+    }
+    // ends the job's session and keeps what it borrowed for later jobs
+    void release(Job& j) {
+        j.s.reset();
+        std::lock_guard<std::mutex> g(mu);
+        if (j.arena.base) free_slabs.emplace(j.arena.size, j.arena.base);
+        if (j.scalars) free_scalars.push_back(j.scalars);
+        j.arena = DeviceArena{};
+        j.scalars = nullptr;
     }
 
     static double since(std::chrono::steady_clock::time_point t) {
@@ -810,7 +878,14 @@ struct BatchDriver::Impl {
     }
 
     void work() {
-        bool selected = false;
+        bool selected = hipSetDevice(device) == hipSuccess;
+        hipStream_t lent = nullptr;
+        if (selected && hipStreamCreateWithFlags(&lent, hipStreamNonBlocking) == hipSuccess) {
+            std::lock_guard<std::mutex> g(mu);
+            pool_streams.push_back(lent);
+        } else {
+            lent = nullptr;  // the jobs then make their own
+        }
         for (;;) {
             std::unique_ptr<Job> j;
             bool finishing = false;
@@ -834,10 +909,13 @@ struct BatchDriver::Impl {
             if (finishing) {
                 complete(*j);
             } else {
-                prepare(*j);
+                prepare(*j, lent);
                 if (j->s) to_ready = true;
             }
-            if (!to_ready) j.reset();  // done: release its device state here, off the driver thread
+            if (!to_ready) {  // done: end it here, off the driver thread
+                release(*j);
+                j.reset();
+            }
             const double took = since(t0), at = since(t_created);
             std::lock_guard<std::mutex> g(mu);
             if (finishing) {
@@ -856,7 +934,7 @@ struct BatchDriver::Impl {
     }
 
     // the host side of starting a job; a failure completes it at once
-    void prepare(Job& j) {
+    void prepare(Job& j, hipStream_t lent) {
         try {
             if (!j.centers) throw std::invalid_argument("null centres");
             if (j.options.device != device) throw std::invalid_argument("a batch runs on one device");
@@ -873,9 +951,12 @@ struct BatchDriver::Impl {
             j.overflow_before = probe.designOverflow(probe.cx.data(), probe.cy.data(), j.options.bins_x,
                                                      j.options.bins_y, j.options.target_density);
             auto prep = std::make_shared<const Prepared>(d, j.options);
-            j.s = std::make_unique<Session>(prep, j.options, nullptr, true);
+            j.arena = takeSlab(estimateBytes(d, j.options));
+            std::tie(j.scalars, j.scalars_device) = takeScalars();
+            const BorrowedResources borrowed{lent, &j.arena, j.scalars, j.scalars_device};
+            j.s = std::make_unique<Session>(prep, j.options, nullptr, true, &borrowed);
         } catch (...) {
-            j.s.reset();
+            release(j);
             const mp_gp_status st = translate(std::current_exception());
             const std::string msg = g_last_error;
             j.done(st, msg.c_str(), mp_gp_result{});
@@ -1101,6 +1182,7 @@ struct BatchDriver::Impl {
         c.reductions = static_cast<int>(reds.size());
         if (off > 0)
             hipCheck(hipMemcpyAsync(c.dev_table, c.host_table, off, hipMemcpyHostToDevice, stream), "upload batch table");
+        const auto t_launch = std::chrono::steady_clock::now();
         for (const Launch& L : launches)
             launchBatched(static_cast<BatchKernel>(L.kernel), c.dev_table + L.entries,
                           reinterpret_cast<const int*>(c.dev_table + L.first), L.n, L.blocks, L.lds, stream);
@@ -1118,6 +1200,7 @@ struct BatchDriver::Impl {
             st_table_bytes += static_cast<long long>(off);
             st_build += D(t_issue - t_build).count();
             st_issue += D(std::chrono::steady_clock::now() - t_issue).count();
+            st_upload += D(t_launch - t_issue).count();
         }
     }
 
