@@ -152,7 +152,17 @@ def serving_sample(jobs: int, rate: float, slots: int, seed: int = 2026, mode: s
     conv = [r["result"]["overflow_after"] <= 0.10 + 1e-12 for r in reports]
     first = min(r["submit_seconds"] for r in reports)
     last = max(r["finish_seconds"] for r in reports)
+    # capacity: the same requests all submitted at once, to the plain backend one job at a time and
+    # to the batched service
+    capacity = {}
+    for name, m, k in (("plain_one_at_a_time", "streams", 1), ("batched", "batched", jobs)):
+        wall, reps = run_all_at_once([(d, o) for d, o, _, _ in work], m, k)
+        capacity[name] = {"mode": m, "slots": k, "makespan_s": wall, "jobs_per_hour": 3600.0 * jobs / wall,
+                          "gp_iterations_per_s": sum(r["result"]["completed_iterations"] for r in reps) / wall}
+        log(f"serving capacity: {name}, makespan {wall:.2f}s")
+    capacity["ratio"] = capacity["batched"]["jobs_per_hour"] / capacity["plain_one_at_a_time"]["jobs_per_hour"]
     return {
+        "capacity_all_at_once": capacity,
         "jobs": jobs, "arrival_rate_per_s": rate, "slots": slots, "mode": mode,
         "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
         "mean_latency_s": float(lat.mean()), "jobs_per_hour": 3600.0 * jobs / (last - first),
@@ -162,6 +172,19 @@ def serving_sample(jobs: int, rate: float, slots: int, seed: int = 2026, mode: s
         "sample": f"{jobs} requests, Poisson {rate}/s, 1k-200k cells log-uniform, t~U(0.7,0.9), "
                   "bins 32-256 by >=4 cells/bin, stop at overflow<=0.10 or 280 iterations",
     }
+
+
+def run_all_at_once(work, mode: str, slots: int):
+    """Submits every (design, options) of `work` at once to a placement service; returns the
+    makespan and the job reports."""
+    from myplacement_amd import backend as hb
+    server = hb.PlacementServer(max_active_jobs=slots, memory_budget_bytes=200 << 30, mode=mode)
+    t0 = time.perf_counter()
+    ids = [server.submit(d, o) for d, o in work]
+    reports = [server.wait(j, w[0].module_count)[1] for j, w in zip(ids, work)]
+    wall = time.perf_counter() - t0
+    server.close()
+    return wall, reports
 
 
 def batch_sample(jobs: int, runs_spec, iterations: int, seed: int = 2026) -> dict:
@@ -188,12 +211,7 @@ def batch_sample(jobs: int, runs_spec, iterations: int, seed: int = 2026) -> dic
     log(f"batch: {jobs} blocks generated, {sum(w[2] for w in work)} cells in total")
     runs = {}
     for mode, slots in runs_spec:
-        server = hb.PlacementServer(max_active_jobs=slots, memory_budget_bytes=200 << 30, mode=mode)
-        t0 = time.perf_counter()
-        ids = [server.submit(d, o) for d, o, _, _ in work]
-        reports = [server.wait(j, w[0].module_count)[1] for j, w in zip(ids, work)]
-        wall = time.perf_counter() - t0
-        server.close()
+        wall, reports = run_all_at_once([(d, o) for d, o, _, _ in work], mode, slots)
         its = sum(r["result"]["completed_iterations"] for r in reports)
         runs[f"{mode}/{slots}"] = {"mode": mode, "slots": slots, "makespan_s": wall, "gp_iterations": its,
                                    "gp_iterations_per_s": its / wall, "jobs_per_hour": 3600.0 * jobs / wall}
@@ -250,8 +268,8 @@ def main() -> int:
                     help="no HIP-event timing (for hardware-counter runs, where the profiler rejects it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=20)
-    ap.add_argument("--batch-jobs", type=int, default=256)
-    ap.add_argument("--batch-runs", default="streams:16,batched:16,batched:64,batched:256",
+    ap.add_argument("--batch-jobs", type=int, default=256, help="batching-form sample size (0: skip)")
+    ap.add_argument("--batch-runs", default="streams:1,streams:16,batched:64,batched:256",
                     help="service mode:slots pairs for the batch workload")
     ap.add_argument("--batch-iterations", type=int, default=100)
     args = ap.parse_args()
@@ -347,6 +365,14 @@ def main() -> int:
     if args.serve_jobs > 0:
         log(f"served stream sample: {args.serve_jobs} jobs ...")
         line["serving"] = serving_sample(args.serve_jobs, args.serve_rate, args.serve_slots, mode=args.serve_mode)
+    if args.batch_jobs > 0:
+        log(f"batching sample: {args.batch_jobs} blocks ...")
+        spec = [(m, int(k)) for m, k in (x.split(":") for x in args.batch_runs.split(","))]
+        b = batch_sample(args.batch_jobs, spec, args.batch_iterations)
+        best = max(b["runs"].values(), key=lambda r: r["gp_iterations_per_s"])
+        b["metric"] = METRICS["batch"]
+        b["best"] = f"{best['mode']}/{best['slots']}"
+        line["batching"] = b
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         cb = cpu_baseline(args.workload, args.cpu_iterations)
