@@ -262,7 +262,7 @@ def main() -> int:
     ap.add_argument("--serve-jobs", type=int, default=40, help="served-stream sample size (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=8.0)
     ap.add_argument("--serve-slots", type=int, default=8)
-    ap.add_argument("--serve-mode", default="streams", choices=("streams", "batched"))
+    ap.add_argument("--serve-mode", default="batched", choices=("streams", "batched"))
     ap.add_argument("--serve-only", action="store_true", help="only the served-stream sample")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="no HIP-event timing (for hardware-counter runs, where the profiler rejects it)")

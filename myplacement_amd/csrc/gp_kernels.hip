@@ -1552,6 +1552,97 @@ void launchBatched(BatchKernel k, const void* entries, const int* first, int n, 
     check(hipGetLastError(), "launch batched kernel");
 }
 
+void launchEntry(BatchKernel k, const void* entry, std::size_t dyn_lds, hipStream_t s) {
+    switch (k) {
+        case kBatchWire: {
+            const WireEntry& E = *static_cast<const WireEntry*>(entry);
+            hipLaunchKernelGGL(k_wire, dim3(E.gx, E.gy), dim3(kThreads), 0, s, E.a, E.large_blocks, E.medium_blocks);
+            break;
+        }
+        case kBatchSortHist: {
+            const SortEntry& E = *static_cast<const SortEntry*>(entry);
+            hipLaunchKernelGGL(k_dhist, dim3(E.gx), dim3(kSortThreads), dyn_lds, s, E.a);
+            break;
+        }
+        case kBatchSortScan: {
+            const SortEntry& E = *static_cast<const SortEntry*>(entry);
+            hipLaunchKernelGGL(k_dscan, dim3(E.gx), dim3(kThreads), 0, s, E.a, E.rows);
+            break;
+        }
+        case kBatchSortPlace: {
+            const SortEntry& E = *static_cast<const SortEntry*>(entry);
+            hipLaunchKernelGGL(k_dplace, dim3(E.gx), dim3(kSortThreads), dyn_lds, s, E.a);
+            break;
+        }
+        case kBatchDeposit: {
+            const DensityEntry& E = *static_cast<const DensityEntry*>(entry);
+            hipLaunchKernelGGL(k_ddeposit, dim3(E.gx, E.gy), dim3(kDepositThreads), 0, s, E.a, E.sorted_blocks);
+            break;
+        }
+        case kBatchBins: {
+            const DensityEntry& E = *static_cast<const DensityEntry*>(entry);
+            hipLaunchKernelGGL(k_dbins, dim3(E.gx, E.gy), dim3(kThreads), 0, s, E.a);
+            break;
+        }
+        case kBatchGemmStore:
+        case kBatchGemmSpectral:
+        case kBatchGemmInterleave: {
+            const GemmEntry& E = *static_cast<const GemmEntry*>(entry);
+            const dim3 grid(E.gx, E.gy, E.gz);
+            if (k == kBatchGemmSpectral)
+                hipLaunchKernelGGL(k_gemm<kEpiSpectral>, grid, dim3(kThreads), 0, s, E.g, E.K, E.lda, E.ldb, E.ldc, E.sp);
+            else if (k == kBatchGemmInterleave)
+                hipLaunchKernelGGL(k_gemm<kEpiInterleave>, grid, dim3(kThreads), 0, s, E.g, E.K, E.lda, E.ldb, E.ldc, E.sp);
+            else
+                hipLaunchKernelGGL(k_gemm<kEpiStore>, grid, dim3(kThreads), 0, s, E.g, E.K, E.lda, E.ldb, E.ldc, E.sp);
+            break;
+        }
+        case kBatchGather: {
+            const GatherEntry& E = *static_cast<const GatherEntry*>(entry);
+            hipLaunchKernelGGL(k_gather, dim3(E.gx, E.gy), dim3(kThreads), 0, s, E.a);
+            break;
+        }
+        case kBatchLookahead: {
+            const LookaheadEntry& E = *static_cast<const LookaheadEntry*>(entry);
+            hipLaunchKernelGGL(k_lookahead, dim3(E.gx), dim3(kThreads), 0, s, E.l);
+            break;
+        }
+        case kBatchPropose: {
+            const ProposeEntry& E = *static_cast<const ProposeEntry*>(entry);
+            hipLaunchKernelGGL(k_propose, dim3(E.gx), dim3(kThreads), 0, s, E.p);
+            break;
+        }
+        case kBatchCopy: {
+            const CopyEntry& E = *static_cast<const CopyEntry*>(entry);
+            check(hipMemcpyAsync(E.dst, E.src, static_cast<size_t>(E.n) * sizeof(double2), hipMemcpyDeviceToDevice, s),
+                  "copy positions");
+            return;
+        }
+        case kBatchZero: {
+            const ZeroEntry& E = *static_cast<const ZeroEntry*>(entry);
+            hipLaunchKernelGGL(k_zero, dim3(E.gx), dim3(kThreads), 0, s, E.p, static_cast<size_t>(E.n));
+            break;
+        }
+        default: {
+            const int v = static_cast<int>(k) - kBatchDct;
+            if (v < 0 || v >= 8) throw std::invalid_argument("unknown batched kernel");
+            const DctEntry& E = *static_cast<const DctEntry*>(entry);
+            const dim3 grid(E.gx, E.gy, E.gz), block(kDctThreads);
+            switch (v) {
+                case 0: hipLaunchKernelGGL((k_dct<kFoldCols, 4>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
+                case 1: hipLaunchKernelGGL((k_dct<kFoldCols, 8>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
+                case 2: hipLaunchKernelGGL((k_dct<kFoldRows, 4>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
+                case 3: hipLaunchKernelGGL((k_dct<kFoldRows, 8>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
+                case 4: hipLaunchKernelGGL((k_dct<kMirrorCols, 4>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
+                case 5: hipLaunchKernelGGL((k_dct<kMirrorCols, 8>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
+                case 6: hipLaunchKernelGGL((k_dct<kMirrorRows, 4>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
+                default: hipLaunchKernelGGL((k_dct<kMirrorRows, 8>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
+            }
+        }
+    }
+    check(hipGetLastError(), "launch kernel from its entry");
+}
+
 void launchFinalizeBatch(const RedJob* jobs, int n, volatile unsigned* flags, unsigned seq, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_finalize_batch, dim3(n), dim3(kFinalizeThreads), 0, s, jobs, flags, seq);

@@ -299,6 +299,8 @@ ZeroEntry zeroEntry(double* p, int n);
 // `entries` and `first` in device memory; total_blocks = first[n].
 void launchBatched(BatchKernel k, const void* entries, const int* first, int n, int total_blocks, std::size_t dyn_lds,
                    hipStream_t s);
+// The job's own (unbatched) launch described by an entry held in host memory.
+void launchEntry(BatchKernel k, const void* entry, std::size_t dyn_lds, hipStream_t s);
 // k_finalize over n reduction jobs; flags[b] receives seq after job b's results.
 void launchFinalizeBatch(const RedJob* jobs, int n, volatile unsigned* flags, unsigned seq, hipStream_t s);
 

@@ -699,10 +699,12 @@ struct BatchDriver::Impl {
         std::chrono::steady_clock::time_point t0, t_loop;
         Callback done;
     };
-    // one batched launch of the round: a run of entries of one kernel in the table
+    // one launch of the round: a run of entries of one kernel in the table, or, for a single
+    // entry, the job's own launch from its recorded entry
     struct Launch {
         int kernel, n, blocks;
         std::size_t lds, entries, first;
+        const BatchOp* single;
     };
 
     // A cohort of running jobs with its own table and completion words.  Two cohorts take turns on
@@ -1145,23 +1147,35 @@ struct BatchDriver::Impl {
                    refs[b].kernel == refs[a].kernel)
                 ++b;
             const BatchOp& op0 = c.jobs[refs[a].job]->s->recorder()->ops[refs[a].op];
-            Launch L{refs[a].kernel, static_cast<int>(b - a), 0, 0, 0, 0};
-            off = align(off, 256);
-            L.entries = off;
-            off += static_cast<std::size_t>(L.n) * op0.bytes;
-            off = align(off, 16);
-            L.first = off;
-            off += (static_cast<std::size_t>(L.n) + 1) * sizeof(int);
+            Launch L{refs[a].kernel, static_cast<int>(b - a), 0, 0, 0, 0, nullptr};
+            if (L.n == 1) {
+                L.single = &op0;
+            } else {
+                off = align(off, 256);
+                L.entries = off;
+                off += static_cast<std::size_t>(L.n) * op0.bytes;
+                off = align(off, 16);
+                L.first = off;
+                off += (static_cast<std::size_t>(L.n) + 1) * sizeof(int);
+            }
             launches.push_back(L);
             a = b;
         }
+        // up to kMaxRedJobs reductions go to the finalize as a kernel argument
+        const bool red_in_table = reds.size() > static_cast<std::size_t>(kMaxRedJobs);
         off = align(off, 256);
         const std::size_t red_off = off;
-        off += reds.size() * sizeof(RedJob);
+        if (red_in_table) off += reds.size() * sizeof(RedJob);
         reserve(c, off, static_cast<int>(reds.size()));
         // fill the staging table (the cohort's previous round has been collected: its copy is done)
         std::size_t r = 0;
         for (Launch& L : launches) {
+            if (L.single) {
+                L.blocks = L.single->blocks;
+                L.lds = L.single->lds;
+                ++r;
+                continue;
+            }
             int* first = reinterpret_cast<int*>(c.host_table + L.first);
             long long total = 0;
             first[0] = 0;
@@ -1175,21 +1189,34 @@ struct BatchDriver::Impl {
             }
             L.blocks = static_cast<int>(total);
         }
-        if (!reds.empty()) std::memcpy(c.host_table + red_off, reds.data(), reds.size() * sizeof(RedJob));
+        if (red_in_table) std::memcpy(c.host_table + red_off, reds.data(), reds.size() * sizeof(RedJob));
         const auto t_issue = std::chrono::steady_clock::now();
-        for (auto& j : c.jobs) j->s->recorder()->ops.clear();
         c.issued = true;
         c.reductions = static_cast<int>(reds.size());
         if (off > 0)
             hipCheck(hipMemcpyAsync(c.dev_table, c.host_table, off, hipMemcpyHostToDevice, stream), "upload batch table");
         const auto t_launch = std::chrono::steady_clock::now();
-        for (const Launch& L : launches)
-            launchBatched(static_cast<BatchKernel>(L.kernel), c.dev_table + L.entries,
-                          reinterpret_cast<const int*>(c.dev_table + L.first), L.n, L.blocks, L.lds, stream);
+        for (const Launch& L : launches) {
+            if (L.single)
+                launchEntry(static_cast<BatchKernel>(L.kernel), L.single->entry, L.lds, stream);
+            else
+                launchBatched(static_cast<BatchKernel>(L.kernel), c.dev_table + L.entries,
+                              reinterpret_cast<const int*>(c.dev_table + L.first), L.n, L.blocks, L.lds, stream);
+        }
+        for (auto& j : c.jobs) j->s->recorder()->ops.clear();
         if (c.reductions > 0) {
             if (++c.seq == 0) c.seq = 1;
-            launchFinalizeBatch(reinterpret_cast<const RedJob*>(c.dev_table + red_off), c.reductions, c.dev_flags,
-                                c.seq, stream);
+            if (red_in_table) {
+                launchFinalizeBatch(reinterpret_cast<const RedJob*>(c.dev_table + red_off), c.reductions, c.dev_flags,
+                                    c.seq, stream);
+            } else {
+                RedList l{};
+                std::copy(reds.begin(), reds.end(), l.job);
+                l.n = c.reductions;
+                l.seq = c.seq;
+                l.flag = c.dev_flags;
+                launchFinalize(l, stream);
+            }
         }
         if (stats) {
             using D = std::chrono::duration<double>;
