@@ -153,3 +153,19 @@ def test_device_footprint_estimate_grows_with_the_design():
     small = hb.estimate_device_bytes(design.plain_design(cells=500, nets=600), o)
     large = hb.estimate_device_bytes(design.plain_design(cells=4000, nets=4600), o)
     assert 0 < small < large
+
+
+def test_batch_and_service_arguments_without_a_device():
+    """The batched entry points check their arguments before touching a device: an empty batch is
+    a no-op, a negative count and mismatched option lists are rejected, and the service's mode
+    defaults to one stream per job."""
+    assert hb.global_place_batch([], []) == []
+    with pytest.raises(ValueError):
+        hb.global_place_batch([design.plain_design(cells=50, nets=60)], [])
+    st = (C.c_int32 * 1)()
+    assert hb.library().mp_gp_global_place_batch(-1, None, None, None, None, st) == abi.MP_GP_INVALID_ARGUMENT
+    o = abi.MpServerOptions()
+    hb.library().mp_gp_default_server_options(C.byref(o))
+    assert (o.mode, o.max_active_jobs, o.memory_budget_bytes) == (abi.MP_GP_SERVE_STREAMS, 8, 200 << 30)
+    with pytest.raises(ValueError):
+        hb.PlacementServer(mode="lockstep")
