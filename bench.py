@@ -156,10 +156,10 @@ def serving_sample(jobs: int, rate: float, slots: int, seed: int = 2026, mode: s
     # to the batched service
     capacity = {}
     for name, m, k in (("plain_one_at_a_time", "streams", 1), ("batched", "batched", jobs)):
-        wall, reps = run_all_at_once([(d, o) for d, o, _, _ in work], m, k)
-        capacity[name] = {"mode": m, "slots": k, "makespan_s": wall, "jobs_per_hour": 3600.0 * jobs / wall,
-                          "gp_iterations_per_s": sum(r["result"]["completed_iterations"] for r in reps) / wall}
-        log(f"serving capacity: {name}, makespan {wall:.2f}s")
+        span, reps = run_all_at_once([(d, o) for d, o, _, _ in work], m, k)
+        capacity[name] = {"mode": m, "slots": k, "makespan_s": span, "jobs_per_hour": 3600.0 * jobs / span,
+                          "gp_iterations_per_s": sum(r["result"]["completed_iterations"] for r in reps) / span}
+        log(f"serving capacity: {name}, makespan {span:.2f}s")
     capacity["ratio"] = capacity["batched"]["jobs_per_hour"] / capacity["plain_one_at_a_time"]["jobs_per_hour"]
     return {
         "capacity_all_at_once": capacity,

@@ -838,6 +838,7 @@ struct BatchDriver::Impl {
     }
     // ends the job's session and keeps what it borrowed for later jobs
     void release(Job& j) {
+        j.step = Step();
         j.s.reset();
         std::lock_guard<std::mutex> g(mu);
         if (j.arena.base) free_slabs.emplace(j.arena.size, j.arena.base);
@@ -1055,6 +1056,20 @@ struct BatchDriver::Impl {
     // One turn: collect the cohort's issued round (wait, resume its jobs), let prepared jobs join it,
     // retire its finished jobs, and issue its next round without waiting for it.
     void round() {
+        try {
+            turnOnce();
+        } catch (...) {
+            // the device or the host failed outside any one cohort's round: every running job fails
+            const std::exception_ptr e = std::current_exception();
+            for (Cohort& c : cohorts) {
+                for (auto& j : c.joining) c.jobs.push_back(std::move(j));
+                c.joining.clear();
+                failCohort(c, e);
+            }
+        }
+    }
+
+    void turnOnce() {
         hipCheck(hipSetDevice(device), "select HIP device");
         Cohort& c = cohorts[turn];
         Cohort& other = cohorts[turn ^ 1];

@@ -164,9 +164,6 @@ def test_adaptive_run_follows_oracle_then_matches_ensemble(ob):
     say("[gpu] adaptive run plain-64, 280 iterations, unperturbed")
     c_h, r_h, hist_h = hb.global_place(d, o)
     c_o, r_o, hist_o = ob.global_place(d, o)
-    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-    np.save(os.path.join(ROOT, "gpurun_out", "hist_hip.npy"), np.array([[r[k] for k in abi.HISTORY_COLUMNS] for r in hist_h]))
-    np.save(os.path.join(ROOT, "gpurun_out", "hist_oracle.npy"), np.array([[r[k] for k in abi.HISTORY_COLUMNS] for r in hist_o]))
     assert r_h["completed_iterations"] == 280 and len(hist_h) == 280
     assert rel(r_h["hpwl_before"], r_o["hpwl_before"]) < 1e-12
     assert r_h["accepted_iterations"] > 0 and r_h["restored_best_checkpoint"] == 1
