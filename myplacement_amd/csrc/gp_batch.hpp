@@ -26,7 +26,9 @@ public:
     // status, error message (empty on success), result
     using Callback = std::function<void(mp_gp_status, const char*, const mp_gp_result&)>;
 
-    explicit BatchDriver(int device);
+    // memory_cap: the device memory the driver keeps for its jobs (blocks of finished jobs are kept
+    // for later ones while the total stays below it); 0 = four fifths of the device
+    explicit BatchDriver(int device, std::uint64_t memory_cap = 0);
     ~BatchDriver();
     BatchDriver(const BatchDriver&) = delete;
     BatchDriver& operator=(const BatchDriver&) = delete;

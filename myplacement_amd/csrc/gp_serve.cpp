@@ -219,7 +219,7 @@ private:
     // batched mode: one thread admits jobs between rounds and runs the rounds
     void drive() {
         try {
-            BatchDriver driver(o_.device);
+            BatchDriver driver(o_.device, o_.memory_budget_bytes);
             for (;;) {
                 std::vector<Job*> start;
                 {

@@ -749,14 +749,19 @@ struct BatchDriver::Impl {
     // for later jobs, so that starting and ending a job allocates and frees nothing on the device
     // (a free waits for the whole device).  Guarded by mu.
     std::multimap<std::size_t, char*> free_slabs;
-    std::vector<char*> slabs;
+    std::map<char*, std::size_t> slabs;  // every block held, in use or kept
+    std::size_t slab_bytes = 0;          // their total
+    std::size_t slab_cap = 0;            // kept blocks are freed while a new one would pass this
     std::vector<HostScalars*> free_scalars;
     std::vector<HostScalars*> scalar_chunks;
     std::map<HostScalars*, HostScalars*> scalar_device;  // chunk host address -> device address
 
-    explicit Impl(int d) : device(d) {
+    Impl(int d, std::uint64_t memory_cap) : device(d) {
         hipCheck(hipSetDevice(device), "select HIP device");
         hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "create batch stream");
+        std::size_t free_bytes = 0, total_bytes = 0;
+        hipCheck(hipMemGetInfo(&free_bytes, &total_bytes), "query device memory");
+        slab_cap = memory_cap ? static_cast<std::size_t>(memory_cap) : total_bytes / 5 * 4;
         const unsigned hw = std::thread::hardware_concurrency();
         const int threads = static_cast<int>(std::min(16U, std::max(2U, hw)));
         for (int k = 0; k < threads; ++k) pool.emplace_back([this] { work(); });
@@ -791,28 +796,50 @@ struct BatchDriver::Impl {
             if (c.dev_table) (void)hipFree(c.dev_table);
             if (c.host_flags) (void)hipHostFree(c.host_flags);
         }
-        for (char* p : slabs) (void)hipFree(p);
+        for (auto& kv : slabs) (void)hipFree(kv.first);
         for (HostScalars* p : scalar_chunks) (void)hipHostFree(p);
         for (hipStream_t t : pool_streams) (void)hipStreamDestroy(t);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    // a device block of at least `bytes` (a kept one up to twice that size, else a new one)
+    // Block sizes come in classes 2 MiB x ceil(2^(k/4)), so that later jobs of similar size reuse them.
+    static std::size_t slabClass(std::size_t bytes) {
+        const std::size_t unit = std::size_t{2} << 20;
+        double c = 1.0;
+        while (static_cast<double>(unit) * std::ceil(c) < static_cast<double>(bytes)) c *= 1.189207115002721;  // 2^(1/4)
+        return unit * static_cast<std::size_t>(std::ceil(c));
+    }
+    // a device block of at least `bytes`: a kept one up to twice that size, else a new one (kept
+    // blocks, largest first, are freed while the total would pass the cap)
     DeviceArena takeSlab(std::size_t bytes) {
+        const std::size_t size = slabClass(bytes);
+        std::vector<char*> drop;
         {
             std::lock_guard<std::mutex> g(mu);
-            auto it = free_slabs.lower_bound(bytes);
-            if (it != free_slabs.end() && it->first <= 2 * bytes) {
+            auto it = free_slabs.lower_bound(size);
+            if (it != free_slabs.end() && it->first <= 2 * size) {
                 DeviceArena a{it->second, it->first, 0};
                 free_slabs.erase(it);
                 return a;
             }
+            while (!free_slabs.empty() && slab_bytes + size > slab_cap) {
+                auto last = std::prev(free_slabs.end());
+                drop.push_back(last->second);
+                slab_bytes -= last->first;
+                slabs.erase(last->second);
+                free_slabs.erase(last);
+            }
+            slab_bytes += size;  // counted now, so that concurrent takers see it
         }
-        const std::size_t size = (bytes + (std::size_t{2} << 20) - 1) / (std::size_t{2} << 20) * (std::size_t{2} << 20);
+        for (char* q : drop) (void)hipFree(q);
         char* p = nullptr;
-        hipCheck(hipMalloc(reinterpret_cast<void**>(&p), size), "allocate job memory");
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), size);
         std::lock_guard<std::mutex> g(mu);
-        slabs.push_back(p);
+        if (e != hipSuccess) {
+            slab_bytes -= size;
+            hipCheck(e, "allocate job memory");
+        }
+        slabs.emplace(p, size);
         return DeviceArena{p, size, 0};
     }
     // a host-mapped scalar block: host and device addresses
@@ -1283,7 +1310,7 @@ struct BatchDriver::Impl {
     }
 };
 
-BatchDriver::BatchDriver(int device) : impl_(std::make_unique<Impl>(device)) {}
+BatchDriver::BatchDriver(int device, std::uint64_t memory_cap) : impl_(std::make_unique<Impl>(device, memory_cap)) {}
 BatchDriver::~BatchDriver() = default;
 int BatchDriver::running() const { return impl_->running(); }
 void BatchDriver::round() { impl_->round(); }
