@@ -21,6 +21,8 @@ namespace mpgp {
 // Upper bound of the device bytes one global placement run of the design reserves.
 uint64_t estimateBytes(const mp_design& design, const mp_gp_options& options);
 
+// add, running and round are called from one thread (the driver's); the callbacks run on the
+// driver's own host threads.
 class BatchDriver {
 public:
     // status, error message (empty on success), result

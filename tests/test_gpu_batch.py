@@ -101,3 +101,5 @@ def test_service_batched_mode_is_bit_equal_to_single_runs():
     assert max(s for s, _ in spans) > min(f for _, f in spans) - 1e-3
     for (d, o), (c, rep) in zip(jobs, got):
         _same_as_alone(d, o, c, rep["result"])
+        # the job's arrays were carved from one block sized by the footprint estimate
+        assert rep["estimated_device_bytes"] >= rep["result"]["reserved_device_bytes"]
