@@ -18,6 +18,7 @@
 //     coefficient, which the solve drops (ElectrostaticField.cpp:228).
 
 #include "gp_backend.hpp"
+#include "gp_host.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -64,6 +65,20 @@ std::vector<float> basis(int rows, int cols, int n, int kind, double wave_scale)
 
 void hipCheck(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+thread_local UploadStaging* t_upload_staging = nullptr;
+
+void stagedUpload(void* dst, const void* src, std::size_t bytes, hipStream_t s, const char* what) {
+    UploadStaging* st = t_upload_staging;
+    const std::size_t at = st ? (st->used + 255) / 256 * 256 : 0;
+    if (st && st->base && at <= st->size && bytes <= st->size - at) {
+        std::memcpy(st->base + at, src, bytes);
+        st->used = at + bytes;
+        hipCheck(hipMemcpyAsync(dst, st->base + at, bytes, hipMemcpyHostToDevice, s), what);
+    } else {
+        hipCheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), what);
+    }
 }
 
 void DeviceBudget::reserve(std::size_t bytes, const char* what) {
@@ -115,29 +130,37 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
     // (their workgroups start first and take longest).  Pins follow the net order, except that each
     // small-net degree group stores its pins slot-major: pin j of the group's i-th net at
     // group base + j * group size + i, so one wavefront's loads of slot j are contiguous.
-    std::vector<int> order;
-    std::vector<int> lowest(static_cast<std::size_t>(N_), std::numeric_limits<int>::max());
-    for (int n = 0; n < N_; ++n)
-        for (int q = L.net_offsets[n]; q < L.net_offsets[n + 1]; ++q)
-            if (L.pin_particles[q] >= 0) lowest[n] = std::min(lowest[n], static_cast<int>(L.pin_particles[q]));
     auto degree = [&](int n) { return L.net_offsets[n + 1] - L.net_offsets[n]; };
-    for (int cls = 0; cls < 3; ++cls) {
-        const std::size_t first = order.size();
-        for (int n = 0; n < N_; ++n) {
-            const int deg = degree(n);
-            if (deg < 0) throw InvalidArgument("net offsets are not monotone");
-            if (deg < 2) continue;
-            const int c = deg <= kSmallDegree ? 0 : (deg <= kMediumDegree ? 1 : 2);
-            if (c != cls) continue;
-            order.push_back(n);
-            (cls == 0 ? n_small_ : (cls == 1 ? n_medium_ : n_large_)) += 1;
-        }
-        std::stable_sort(order.begin() + first, order.end(), [&](int a, int b) {
-            if (cls == 0 && degree(a) != degree(b)) return degree(a) < degree(b);
-            if (cls == 2 && degree(a) != degree(b)) return degree(a) > degree(b);
-            return lowest[a] < lowest[b];
-        });
+    std::vector<uint32_t> lowest(static_cast<std::size_t>(N_), static_cast<uint32_t>(std::numeric_limits<int>::max()));
+    std::vector<uint32_t> by_lowest;  // the nets of two pins or more, by lowest particle (ties: index)
+    by_lowest.reserve(static_cast<std::size_t>(N_));
+    for (int n = 0; n < N_; ++n) {
+        const int deg = degree(n);
+        if (deg < 0) throw InvalidArgument("net offsets are not monotone");
+        for (int q = L.net_offsets[n]; q < L.net_offsets[n + 1]; ++q)
+            if (L.pin_particles[q] >= 0) lowest[n] = std::min(lowest[n], static_cast<uint32_t>(L.pin_particles[q]));
+        if (deg >= 2) by_lowest.push_back(static_cast<uint32_t>(n));
     }
+    stableSortByKey(by_lowest.data(), by_lowest.size(), lowest.data(), 31);
+    // then stably by bucket: small nets by degree (2 .. kSmallDegree), medium nets, large nets
+    // (these last by degree descending)
+    constexpr int kBuckets = kSmallDegree + 1;  // 0 .. kSmallDegree - 2 small, then medium, large
+    auto bucket = [&](int n) {
+        const int deg = degree(n);
+        return deg <= kSmallDegree ? deg - 2 : (deg <= kMediumDegree ? kSmallDegree - 1 : kSmallDegree);
+    };
+    std::vector<int> start(kBuckets + 1, 0);
+    for (uint32_t n : by_lowest) ++start[bucket(static_cast<int>(n)) + 1];
+    std::partial_sum(start.begin(), start.end(), start.begin());
+    std::vector<int> order(by_lowest.size());
+    {
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (uint32_t n : by_lowest) order[fill[bucket(static_cast<int>(n))]++] = static_cast<int>(n);
+    }
+    n_small_ = start[kSmallDegree - 1];
+    n_medium_ = start[kSmallDegree] - start[kSmallDegree - 1];
+    n_large_ = start[kBuckets] - start[kSmallDegree];
+    std::stable_sort(order.begin() + start[kSmallDegree], order.end(), [&](int a, int b) { return degree(a) > degree(b); });
     std::vector<int> inet_off(order.size() + 1, 0);
     std::vector<double> inet_w(order.size());
     for (std::size_t k = 0; k < order.size(); ++k) {
@@ -156,23 +179,26 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
     }
     std::vector<int> ipin_part(static_cast<std::size_t>(inet_off.back()));
     std::vector<double2> ipin_off(ipin_part.size());
-    for (std::size_t k = 0; k < order.size(); ++k) {
-        const int n = order[k];
-        const int deg = degree(n);
-        for (int j = 0; j < deg; ++j) {
-            const int p = L.net_offsets[n] + j;
-            const int part = L.pin_particles[p];
-            if (part >= P_) throw InvalidArgument("pin refers to an unknown particle");
-            std::size_t q = static_cast<std::size_t>(inet_off[k]) + j;
-            if (static_cast<int>(k) < n_small_) {  // slot-major inside the degree group
-                const int count = small_net_[deg + 1] - small_net_[deg];
-                q = static_cast<std::size_t>(small_pin_[deg]) + static_cast<std::size_t>(j) * count +
-                    (static_cast<int>(k) - small_net_[deg]);
+    // a gather through the net order (a cache miss per pin on large designs): in parallel parts
+    parallelFor(order.size(), 32768, [&](std::size_t k0, std::size_t k1) {
+        for (std::size_t k = k0; k < k1; ++k) {
+            const int n = order[k];
+            const int deg = degree(n);
+            for (int j = 0; j < deg; ++j) {
+                const int p = L.net_offsets[n] + j;
+                const int part = L.pin_particles[p];
+                if (part >= P_) throw InvalidArgument("pin refers to an unknown particle");
+                std::size_t q = static_cast<std::size_t>(inet_off[k]) + j;
+                if (static_cast<int>(k) < n_small_) {  // slot-major inside the degree group
+                    const int count = small_net_[deg + 1] - small_net_[deg];
+                    q = static_cast<std::size_t>(small_pin_[deg]) + static_cast<std::size_t>(j) * count +
+                        (static_cast<int>(k) - small_net_[deg]);
+                }
+                ipin_part[q] = part;
+                ipin_off[q] = make_double2(L.pin_offsets_or_static[2 * p], L.pin_offsets_or_static[2 * p + 1]);
             }
-            ipin_part[q] = part;
-            ipin_off[q] = make_double2(L.pin_offsets_or_static[2 * p], L.pin_offsets_or_static[2 * p + 1]);
         }
-    }
+    });
     const std::size_t Ii = ipin_part.size();
 
     // particle -> pin CSR over the movable pins, pins in internal order

@@ -58,6 +58,17 @@ private:
     DeviceArena* arena_ = nullptr;
 };
 
+// Pinned host memory that uploads made on this thread are staged through (set by a batch driver's
+// preparing thread for large jobs): the copies then run as asynchronous DMA from pinned memory
+// instead of each blocking on the runtime's staging of pageable memory.  The owner resets it only
+// after the stream has been synchronized past every copy from it.
+struct UploadStaging {
+    char* base = nullptr;
+    std::size_t size = 0, used = 0;
+};
+extern thread_local UploadStaging* t_upload_staging;
+void stagedUpload(void* dst, const void* src, std::size_t bytes, hipStream_t s, const char* what);
+
 template <typename T>
 class DeviceArray {
 public:
@@ -81,7 +92,7 @@ public:
     }
     void upload(const T* host, std::size_t n, hipStream_t s, const char* what) {
         if (n > n_) throw std::logic_error("device upload exceeds buffer");
-        if (n) hipCheck(hipMemcpyAsync(p_, host, n * sizeof(T), hipMemcpyHostToDevice, s), what);
+        if (n) stagedUpload(p_, host, n * sizeof(T), s, what);
     }
     T* get() const { return p_; }
     std::size_t size() const { return n_; }

@@ -21,6 +21,7 @@
 #include <numeric>
 #include <random>
 #include <stdexcept>
+#include <utility>
 
 namespace mpgp {
 
@@ -139,7 +140,9 @@ Netlist::Netlist(const mp_design& d) {
     }
 }
 
-Prepared::Prepared(const mp_design& d, const mp_gp_options& o) : net(d) {
+Prepared::Prepared(const mp_design& d, const mp_gp_options& o) : Prepared(Netlist(d), o) {}
+
+Prepared::Prepared(Netlist&& n, const mp_gp_options& o) : net(std::move(n)) {
     t = o.target_density;
     nx = o.bins_x;
     ny = o.bins_y;
@@ -279,6 +282,23 @@ Prepared::Prepared(const mp_design& d, const mp_gp_options& o) : net(d) {
     };
     for (const int64_t id : net.movable) precond.push_back(scale(net.width[id] * net.height[id], net.module_pins[id]));
     for (std::size_t i = 0; i < F; ++i) precond.push_back(scale(filler_w * filler_h, 0));
+}
+
+void stableSortByKey(std::uint32_t* idx, std::size_t n, const std::uint32_t* key, int key_bits) {
+    constexpr int kBits = 11;
+    constexpr std::uint32_t kRadix = 1U << kBits;
+    std::vector<std::uint32_t> tmp(n);
+    std::vector<std::size_t> count(kRadix + 1);
+    std::uint32_t* src = idx;
+    std::uint32_t* dst = tmp.data();
+    for (int shift = 0; shift < key_bits; shift += kBits) {
+        std::fill(count.begin(), count.end(), std::size_t{0});
+        for (std::size_t i = 0; i < n; ++i) ++count[((key[src[i]] >> shift) & (kRadix - 1)) + 1];
+        std::partial_sum(count.begin(), count.end(), count.begin());
+        for (std::size_t i = 0; i < n; ++i) dst[count[(key[src[i]] >> shift) & (kRadix - 1)]++] = src[i];
+        std::swap(src, dst);
+    }
+    if (src != idx) std::copy(src, src + n, idx);
 }
 
 mp_gp_static Prepared::layout(int device, int dct_precision, std::uint64_t maximum_bytes) const {

@@ -3,9 +3,12 @@
 
 #include "mp_gp_backend.h"
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
+#include <exception>
 #include <limits>
+#include <thread>
 #include <vector>
 
 namespace mpgp {
@@ -34,6 +37,8 @@ struct Netlist {
 // Static density, fillers, flattened layout and preconditioner of one run.
 struct Prepared {
     Prepared(const mp_design& d, const mp_gp_options& o);
+    // from a netlist already read from the design (taken over)
+    Prepared(Netlist&& n, const mp_gp_options& o);
     Netlist net;
     double t = 0.0;
     int nx = 0, ny = 0;
@@ -51,6 +56,36 @@ struct Prepared {
 };
 
 void randomPlacement(const mp_design& d, std::uint64_t seed, double* centers);
+
+// Runs body(begin, end) over [0, n) in contiguous parts on up to eight host threads (the calling
+// thread takes the first part), or in one part below `grain` items; rethrows a part's exception.
+template <typename F>
+void parallelFor(std::size_t n, std::size_t grain, F&& body) {
+    const unsigned hw = std::max(1U, std::thread::hardware_concurrency());
+    const std::size_t parts = std::min<std::size_t>({8, hw, std::max<std::size_t>(1, n / std::max<std::size_t>(grain, 1))});
+    if (parts <= 1) {
+        body(std::size_t{0}, n);
+        return;
+    }
+    std::vector<std::exception_ptr> errors(parts);
+    std::vector<std::thread> threads;
+    auto run = [&](std::size_t k) {
+        try {
+            body(n * k / parts, n * (k + 1) / parts);
+        } catch (...) {
+            errors[k] = std::current_exception();
+        }
+    };
+    for (std::size_t k = 1; k < parts; ++k) threads.emplace_back(run, k);
+    run(0);
+    for (auto& t : threads) t.join();
+    for (auto& e : errors)
+        if (e) std::rethrow_exception(e);
+}
+
+// Sorts the n indices idx[] stably by key[idx[i]] (the low key_bits bits): an LSD radix sort, 11
+// bits per pass, the same order as std::stable_sort by key in linear time.
+void stableSortByKey(std::uint32_t* idx, std::size_t n, const std::uint32_t* key, int key_bits);
 
 // transformOffset, PlacementDatabase.cpp:55-68 (orientation codes N,S,E,W,FN,FS,FE,FW = 0..7)
 void orient(double& x, double& y, int o);

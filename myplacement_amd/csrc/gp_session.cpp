@@ -151,15 +151,14 @@ public:
             const double fy = (x0[i].y - p.net.core[1]) / std::max(p.net.core[3] - p.net.core[1], 1e-9);
             const uint32_t ix = static_cast<uint32_t>(clampd(fx, 0.0, 1.0) * 1023.0);
             const uint32_t iy = static_cast<uint32_t>(clampd(fy, 0.0, 1.0) * 1023.0);
-            uint64_t k = 0;
+            uint32_t k = 0;
             for (int b = 9; b >= 0; --b) k = (k << 2) | (((iy >> b) & 1U) << 1) | ((ix >> b) & 1U);
             return k;
         };
-        std::vector<uint64_t> keys(P);
+        std::vector<uint32_t> keys(P);
         for (std::size_t i = 0; i < P; ++i) keys[i] = zkey(i);
-        auto byKey = [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; };
-        std::stable_sort(order_.begin(), order_.begin() + Mv, byKey);
-        std::stable_sort(order_.begin() + Mv, order_.end(), byKey);
+        stableSortByKey(order_.data(), Mv, keys.data(), 20);
+        stableSortByKey(order_.data() + Mv, P - Mv, keys.data(), 20);
         std::vector<int32_t> rank(P);
         for (std::size_t k = 0; k < P; ++k) rank[order_[k]] = static_cast<int32_t>(k);
         std::vector<double> pw(P), ph(P), pq(P), precond(P);
@@ -745,6 +744,10 @@ struct BatchDriver::Impl {
     bool stop = false;
     std::vector<std::thread> pool;
     std::vector<hipStream_t> pool_streams;  // one per pool thread, lent to the jobs it prepares
+    // pinned upload staging of a pool thread, made when it first prepares a large job
+    static constexpr std::size_t kStagingBytes = std::size_t{64} << 20;
+    static constexpr int64_t kStagingPins = 200000;
+    std::vector<char*> staging_blocks;
     // Device blocks the jobs' arrays are carved from, and host-mapped scalar blocks: kept and reused
     // for later jobs, so that starting and ending a job allocates and frees nothing on the device
     // (a free waits for the whole device).  Guarded by mu.
@@ -799,6 +802,7 @@ struct BatchDriver::Impl {
         for (auto& kv : slabs) (void)hipFree(kv.first);
         for (HostScalars* p : scalar_chunks) (void)hipHostFree(p);
         for (hipStream_t t : pool_streams) (void)hipStreamDestroy(t);
+        for (char* p : staging_blocks) (void)hipHostFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -916,6 +920,7 @@ struct BatchDriver::Impl {
         } else {
             lent = nullptr;  // the jobs then make their own
         }
+        UploadStaging staging;
         for (;;) {
             std::unique_ptr<Job> j;
             bool finishing = false;
@@ -939,7 +944,17 @@ struct BatchDriver::Impl {
             if (finishing) {
                 complete(*j);
             } else {
+                if (!staging.base && selected && j->design.pin_count > kStagingPins &&
+                    hipHostMalloc(reinterpret_cast<void**>(&staging.base), kStagingBytes, hipHostMallocDefault) ==
+                        hipSuccess) {
+                    staging.size = kStagingBytes;
+                    std::lock_guard<std::mutex> g(mu);
+                    staging_blocks.push_back(staging.base);
+                }
+                staging.used = 0;
+                t_upload_staging = staging.base ? &staging : nullptr;  // every preparation ends synchronized
                 prepare(*j, lent);
+                t_upload_staging = nullptr;
                 if (j->s) to_ready = true;
             }
             if (!to_ready) {  // done: end it here, off the driver thread
@@ -980,12 +995,14 @@ struct BatchDriver::Impl {
             j.hpwl_before = probe.hpwl(probe.cx.data(), probe.cy.data());
             j.overflow_before = probe.designOverflow(probe.cx.data(), probe.cy.data(), j.options.bins_x,
                                                      j.options.bins_y, j.options.target_density);
-            auto prep = std::make_shared<const Prepared>(d, j.options);
+            auto prep = std::make_shared<const Prepared>(std::move(probe), j.options);
             j.arena = takeSlab(estimateBytes(d, j.options));
             std::tie(j.scalars, j.scalars_device) = takeScalars();
             const BorrowedResources borrowed{lent, &j.arena, j.scalars, j.scalars_device};
             j.s = std::make_unique<Session>(prep, j.options, nullptr, true, &borrowed);
         } catch (...) {
+            // copies a failed set-up left in flight finish before its block and staging are reused
+            if (lent) (void)hipStreamSynchronize(lent);
             release(j);
             const mp_gp_status st = translate(std::current_exception());
             const std::string msg = g_last_error;
@@ -1567,7 +1584,7 @@ mp_gp_status mp_gp_global_place(const mp_design* design, const mp_gp_options* op
         const double hpwl_before = probe.hpwl(probe.cx.data(), probe.cy.data());
         const double ovf_before = probe.designOverflow(probe.cx.data(), probe.cy.data(), options->bins_x, options->bins_y,
                                                        options->target_density);
-        auto prep = std::make_shared<const Prepared>(d, *options);
+        auto prep = std::make_shared<const Prepared>(std::move(probe), *options);
         Session s(prep, *options, nullptr);
         s.step(options->iterations);
         mp_gp_result r{};
