@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmp_gp_hip.so")
-SOURCES = ["gp_kernels.hip", "gp_backend.cpp", "gp_host.cpp", "gp_session.cpp", "gp_quadratic.hip", "gp_serve.cpp",
+SOURCES = ["gp_kernels.hip", "gp_backend.cpp", "gp_host.cpp", "gp_session.cpp", "gp_batch.cpp", "gp_quadratic.hip", "gp_serve.cpp",
            "gp_detailed.hip"]
 ARCH = os.environ.get("MP_GP_ARCH", "gfx950")
 REFERENCE = "/root/reference"

@@ -18,6 +18,10 @@
 
 namespace mpgp {
 
+// Loads the library's device code on the device now (best effort: failures are left for the first
+// job to report), so that a service's first job does not wait for it.
+void loadDeviceCodeOn(int device) noexcept;
+
 // Upper bound of the device bytes one global placement run of the design reserves.
 uint64_t estimateBytes(const mp_design& design, const mp_gp_options& options);
 

@@ -1438,6 +1438,14 @@ void launchFinalize(const RedList& l, hipStream_t s) {
     check(hipGetLastError(), "launch reduction finalize kernel");
 }
 
+void loadDeviceCode(hipStream_t s) {
+    // an empty launch (count 0: no thread touches memory) makes the runtime load this library's
+    // device code now instead of inside the first job's first launch
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(kThreads), 0, s, static_cast<double*>(nullptr), size_t{0});
+    check(hipGetLastError(), "load device code");
+    check(hipStreamSynchronize(s), "load device code");
+}
+
 void launchZero(double* p, size_t count, hipStream_t s) {
     if (count == 0) return;
     const int blocks = static_cast<int>(std::min<size_t>(1024, (count + kThreads - 1) / kThreads));

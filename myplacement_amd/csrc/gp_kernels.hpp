@@ -304,6 +304,8 @@ void launchEntry(BatchKernel k, const void* entry, std::size_t dyn_lds, hipStrea
 // k_finalize over n reduction jobs; flags[b] receives seq after job b's results.
 void launchFinalizeBatch(const RedJob* jobs, int n, volatile unsigned* flags, unsigned seq, hipStream_t s);
 
+// Loads the library's device code on the stream's device (a service does this before its first job).
+void loadDeviceCode(hipStream_t s);
 // fill with zero (used instead of memset so the sequence can be captured)
 void launchZero(double* p, size_t count, hipStream_t s);
 

@@ -82,11 +82,14 @@ public:
     explicit Server(const mp_gp_server_options& o) : o_(o), t0_(std::chrono::steady_clock::now()) {
         if (o_.max_active_jobs < 1 || o_.memory_budget_bytes == 0)
             throw std::invalid_argument("the placement service needs at least one slot and a memory budget");
-        if (o_.mode == MP_GP_SERVE_BATCHED) workers_.emplace_back([this] { drive(); });
-        else if (o_.mode == MP_GP_SERVE_STREAMS)
-            for (int k = 0; k < o_.max_active_jobs; ++k) workers_.emplace_back([this] { work(); });
-        else
+        if (o_.mode != MP_GP_SERVE_BATCHED && o_.mode != MP_GP_SERVE_STREAMS)
             throw std::invalid_argument("unknown placement service mode");
+        if (o_.mode == MP_GP_SERVE_BATCHED) {
+            workers_.emplace_back([this] { drive(); });  // its driver loads the device code
+        } else {
+            loadDeviceCodeOn(o_.device);
+            for (int k = 0; k < o_.max_active_jobs; ++k) workers_.emplace_back([this] { work(); });
+        }
     }
     ~Server() {
         {
