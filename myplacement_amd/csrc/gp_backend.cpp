@@ -134,13 +134,15 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
     std::vector<uint32_t> lowest(static_cast<std::size_t>(N_), static_cast<uint32_t>(std::numeric_limits<int>::max()));
     std::vector<uint32_t> by_lowest;  // the nets of two pins or more, by lowest particle (ties: index)
     by_lowest.reserve(static_cast<std::size_t>(N_));
-    for (int n = 0; n < N_; ++n) {
-        const int deg = degree(n);
-        if (deg < 0) throw InvalidArgument("net offsets are not monotone");
-        for (int q = L.net_offsets[n]; q < L.net_offsets[n + 1]; ++q)
-            if (L.pin_particles[q] >= 0) lowest[n] = std::min(lowest[n], static_cast<uint32_t>(L.pin_particles[q]));
-        if (deg >= 2) by_lowest.push_back(static_cast<uint32_t>(n));
-    }
+    for (int n = 0; n < N_; ++n)
+        if (degree(n) < 0) throw InvalidArgument("net offsets are not monotone");
+    parallelFor(static_cast<std::size_t>(N_), 32768, [&](std::size_t n0, std::size_t n1) {
+        for (std::size_t n = n0; n < n1; ++n)
+            for (int q = L.net_offsets[n]; q < L.net_offsets[n + 1]; ++q)
+                if (L.pin_particles[q] >= 0) lowest[n] = std::min(lowest[n], static_cast<uint32_t>(L.pin_particles[q]));
+    });
+    for (int n = 0; n < N_; ++n)
+        if (degree(n) >= 2) by_lowest.push_back(static_cast<uint32_t>(n));
     stableSortByKey(by_lowest.data(), by_lowest.size(), lowest.data(), 31);
     // then stably by bucket: small nets by degree (2 .. kSmallDegree), medium nets, large nets
     // (these last by degree descending)

@@ -255,8 +255,9 @@ Prepared::Prepared(Netlist&& n, const mp_gp_options& o) : net(std::move(n)) {
     net_offsets.assign(N + 1, 0);
     pin_particles.resize(net.pin_module.size());
     pin_xy.resize(2 * net.pin_module.size());
-    for (std::size_t n = 0; n < N; ++n) {
-        for (int64_t p = net.net_start[n]; p < net.net_start[n + 1]; ++p) {
+    for (std::size_t n = 0; n < N; ++n) net_offsets[n + 1] = static_cast<int>(net.net_start[n + 1]);
+    parallelFor(net.pin_module.size(), 131072, [&](std::size_t p0, std::size_t p1) {
+        for (std::size_t p = p0; p < p1; ++p) {
             const int m = net.pin_module[p];
             const int part = module_particle[m];
             double dx = net.pin_dx[p], dy = net.pin_dy[p];
@@ -269,8 +270,7 @@ Prepared::Prepared(Netlist&& n, const mp_gp_options& o) : net(std::move(n)) {
             pin_xy[2 * p] = dx;
             pin_xy[2 * p + 1] = dy;
         }
-        net_offsets[n + 1] = static_cast<int>(net.net_start[n + 1]);
-    }
+    });
 
     // preconditioner base (AdaptiveGlobalPlacer.cpp:358-375)
     double total = 0.0;
@@ -330,21 +330,28 @@ mp_gp_static Prepared::layout(int device, int dct_precision, std::uint64_t maxim
 }
 
 double Netlist::hpwl(const double* cxv, const double* cyv) const {  // Metrics.cpp:128-147
-    double total = 0.0;
-    for (std::size_t n = 0; n + 1 < net_start.size(); ++n) {
-        const int64_t b = net_start[n], e = net_start[n + 1];
-        if (e - b < 2) continue;
-        double lx = INFINITY, ux = -INFINITY, ly = INFINITY, uy = -INFINITY;
-        for (int64_t p = b; p < e; ++p) {
-            const int m = pin_module[p];
-            double dx = pin_dx[p], dy = pin_dy[p];
-            orient(dx, dy, orientation[m]);
-            const double x = cxv[m] + dx, y = cyv[m] + dy;
-            lx = std::min(lx, x); ux = std::max(ux, x);
-            ly = std::min(ly, y); uy = std::max(uy, y);
+    // the nets' weighted extents in parallel parts, then their sum in net order
+    const std::size_t N = net_start.empty() ? 0 : net_start.size() - 1;
+    std::vector<double> term(N, 0.0);
+    parallelFor(N, 32768, [&](std::size_t n0, std::size_t n1) {
+        for (std::size_t n = n0; n < n1; ++n) {
+            const int64_t b = net_start[n], e = net_start[n + 1];
+            if (e - b < 2) continue;
+            double lx = INFINITY, ux = -INFINITY, ly = INFINITY, uy = -INFINITY;
+            for (int64_t p = b; p < e; ++p) {
+                const int m = pin_module[p];
+                double dx = pin_dx[p], dy = pin_dy[p];
+                orient(dx, dy, orientation[m]);
+                const double x = cxv[m] + dx, y = cyv[m] + dy;
+                lx = std::min(lx, x); ux = std::max(ux, x);
+                ly = std::min(ly, y); uy = std::max(uy, y);
+            }
+            term[n] = net_weight[n] * ((ux - lx) + (uy - ly));
         }
-        total += net_weight[n] * ((ux - lx) + (uy - ly));
-    }
+    });
+    double total = 0.0;
+    for (std::size_t n = 0; n < N; ++n)
+        if (net_start[n + 1] - net_start[n] >= 2) total += term[n];
     return total;
 }
 

@@ -139,7 +139,9 @@ public:
             return k;
         };
         std::vector<uint32_t> keys(P);
-        for (std::size_t i = 0; i < P; ++i) keys[i] = zkey(i);
+        parallelFor(P, 65536, [&](std::size_t i0, std::size_t i1) {
+            for (std::size_t i = i0; i < i1; ++i) keys[i] = zkey(i);
+        });
         stableSortByKey(order_.data(), Mv, keys.data(), 20);
         stableSortByKey(order_.data() + Mv, P - Mv, keys.data(), 20);
         std::vector<int32_t> rank(P);
@@ -147,17 +149,21 @@ public:
         std::vector<double> pw(P), ph(P), pq(P), precond(P);
         std::vector<int32_t> player(P), pin_particles(p.pin_particles.size());
         std::vector<double2> x(P);
-        for (std::size_t k = 0; k < P; ++k) {
-            const uint32_t i = order_[k];
-            pw[k] = p.pw[i];
-            ph[k] = p.ph[i];
-            pq[k] = p.pq[i];
-            player[k] = p.player[i];
-            precond[k] = p.precond[i];
-            x[k] = x0[i];
-        }
-        for (std::size_t q = 0; q < pin_particles.size(); ++q)
-            pin_particles[q] = p.pin_particles[q] < 0 ? -1 : rank[p.pin_particles[q]];
+        parallelFor(P, 65536, [&](std::size_t k0, std::size_t k1) {
+            for (std::size_t k = k0; k < k1; ++k) {
+                const uint32_t i = order_[k];
+                pw[k] = p.pw[i];
+                ph[k] = p.ph[i];
+                pq[k] = p.pq[i];
+                player[k] = p.player[i];
+                precond[k] = p.precond[i];
+                x[k] = x0[i];
+            }
+        });
+        parallelFor(pin_particles.size(), 131072, [&](std::size_t q0, std::size_t q1) {
+            for (std::size_t q = q0; q < q1; ++q)
+                pin_particles[q] = p.pin_particles[q] < 0 ? -1 : rank[p.pin_particles[q]];
+        });
         mp_gp_static L = p.layout(o_.device, o_.dct_precision, o_.maximum_device_bytes);
         L.particle_width = pw.data();
         L.particle_height = ph.data();
