@@ -315,6 +315,8 @@ struct BatchDriver::Impl {
                 }
                 staging.used = 0;
                 t_upload_staging = staging.base ? &staging : nullptr;  // every preparation ends synchronized
+                // the pool already prepares jobs side by side: only a large one splits its own passes
+                t_parallel_limit = j->design.pin_count > kStagingPins ? 8 : 1;
                 prepare(*j, lent);
                 t_upload_staging = nullptr;
                 if (j->s) to_ready = true;

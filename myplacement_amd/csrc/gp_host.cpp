@@ -284,6 +284,8 @@ Prepared::Prepared(Netlist&& n, const mp_gp_options& o) : net(std::move(n)) {
     for (std::size_t i = 0; i < F; ++i) precond.push_back(scale(filler_w * filler_h, 0));
 }
 
+thread_local int t_parallel_limit = 8;
+
 void stableSortByKey(std::uint32_t* idx, std::size_t n, const std::uint32_t* key, int key_bits) {
     constexpr int kBits = 11;
     constexpr std::uint32_t kRadix = 1U << kBits;
@@ -333,7 +335,7 @@ double Netlist::hpwl(const double* cxv, const double* cyv) const {  // Metrics.c
     // the nets' weighted extents in parallel parts, then their sum in net order
     const std::size_t N = net_start.empty() ? 0 : net_start.size() - 1;
     std::vector<double> term(N, 0.0);
-    parallelFor(N, 32768, [&](std::size_t n0, std::size_t n1) {
+    parallelFor(N, 8192, [&](std::size_t n0, std::size_t n1) {
         for (std::size_t n = n0; n < n1; ++n) {
             const int64_t b = net_start[n], e = net_start[n + 1];
             if (e - b < 2) continue;

@@ -57,12 +57,19 @@ struct Prepared {
 
 void randomPlacement(const mp_design& d, std::uint64_t seed, double* centers);
 
-// Runs body(begin, end) over [0, n) in contiguous parts on up to eight host threads (the calling
-// thread takes the first part), or in one part below `grain` items; rethrows a part's exception.
+// Most host threads parallelFor may use from this thread (8; a caller that already runs many
+// preparations side by side lowers it for its small jobs).
+extern thread_local int t_parallel_limit;
+
+// Runs body(begin, end) over [0, n) in contiguous parts on up to t_parallel_limit host threads (the
+// calling thread takes the first part), or in one part below `grain` items; rethrows a part's
+// exception.
 template <typename F>
 void parallelFor(std::size_t n, std::size_t grain, F&& body) {
     const unsigned hw = std::max(1U, std::thread::hardware_concurrency());
-    const std::size_t parts = std::min<std::size_t>({8, hw, std::max<std::size_t>(1, n / std::max<std::size_t>(grain, 1))});
+    const std::size_t limit = static_cast<std::size_t>(std::max(1, t_parallel_limit));
+    const std::size_t parts =
+        std::min<std::size_t>({limit, hw, std::max<std::size_t>(1, n / std::max<std::size_t>(grain, 1))});
     if (parts <= 1) {
         body(std::size_t{0}, n);
         return;
