@@ -335,7 +335,7 @@ double Netlist::hpwl(const double* cxv, const double* cyv) const {  // Metrics.c
     // the nets' weighted extents in parallel parts, then their sum in net order
     const std::size_t N = net_start.empty() ? 0 : net_start.size() - 1;
     std::vector<double> term(N, 0.0);
-    parallelFor(N, 8192, [&](std::size_t n0, std::size_t n1) {
+    parallelFor(N, 32768, [&](std::size_t n0, std::size_t n1) {
         for (std::size_t n = n0; n < n1; ++n) {
             const int64_t b = net_start[n], e = net_start[n + 1];
             if (e - b < 2) continue;
