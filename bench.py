@@ -187,15 +187,25 @@ def run_all_at_once(work, mode: str, slots: int):
     return wall, reports
 
 
+def parse_batch_runs(text: str):
+    """"mode:slots[:precision],..." -> [(mode, slots, precision)]."""
+    runs = []
+    for item in text.split(","):
+        f = item.split(":")
+        runs.append((f[0], int(f[1]), f[2] if len(f) > 2 else "fp32"))
+    return runs
+
+
 def batch_sample(jobs: int, runs_spec, iterations: int, seed: int = 2026) -> dict:
     """The batching configuration's workload: `jobs` independent blocks, each from its own seed,
     cells log-uniform in 500-50k (designs.served_design), bin grids by designs.served_bins (32 or 64
     at these sizes), target density 0.8, random initial placement, a fixed `iterations` adaptive
     iterations each (early stop off).  All jobs are submitted at once to the placement service, for
-    each (mode, slots) of `runs_spec`: mode "streams" runs each admitted job on its own stream,
-    "batched" runs the admitted jobs in lockstep rounds with one batched launch per stage; slots
-    bounds the jobs running at once.  The aggregate rate is the total of the jobs' GP iterations
-    over the makespan."""
+    each (mode, slots, precision) of `runs_spec`: mode "streams" runs each admitted job on its own
+    stream, "batched" runs the admitted jobs in lockstep rounds with one batched launch per stage;
+    slots bounds the jobs running at once; precision "bf16x3" gives the jobs whose grid allows it
+    (both sides multiples of 64) split-BF16 DCT products, "fp32" none.  The aggregate rate is the
+    total of the jobs' GP iterations over the makespan."""
     from myplacement_amd import abi, design
     from myplacement_amd import backend as hb
     rng = np.random.default_rng(seed)
@@ -210,12 +220,19 @@ def batch_sample(jobs: int, runs_spec, iterations: int, seed: int = 2026) -> dic
         work.append((d, o, cells, side))
     log(f"batch: {jobs} blocks generated, {sum(w[2] for w in work)} cells in total")
     runs = {}
-    for mode, slots in runs_spec:
-        wall, reports = run_all_at_once([(d, o) for d, o, _, _ in work], mode, slots)
+    for mode, slots, precision in runs_spec:
+        jobs_here = []
+        for d, o, _, side in work:
+            if precision == "bf16x3" and side % 64 == 0:
+                o = abi.MpOptions.from_buffer_copy(o)
+                o.dct_precision = abi.MP_DCT_BF16X3
+            jobs_here.append((d, o))
+        wall, reports = run_all_at_once(jobs_here, mode, slots)
         its = sum(r["result"]["completed_iterations"] for r in reports)
-        runs[f"{mode}/{slots}"] = {"mode": mode, "slots": slots, "makespan_s": wall, "gp_iterations": its,
-                                   "gp_iterations_per_s": its / wall, "jobs_per_hour": 3600.0 * jobs / wall}
-        log(f"batch: {mode}, {slots} slots, makespan {wall:.2f}s, {its / wall:.0f} GP it/s")
+        key = f"{mode}/{slots}" + ("" if precision == "fp32" else f"/{precision}")
+        runs[key] = {"mode": mode, "slots": slots, "precision": precision, "makespan_s": wall, "gp_iterations": its,
+                     "gp_iterations_per_s": its / wall, "jobs_per_hour": 3600.0 * jobs / wall}
+        log(f"batch: {mode}, {slots} slots, {precision}, makespan {wall:.2f}s, {its / wall:.0f} GP it/s")
     sides = [w[3] for w in work]
     return {"jobs": jobs, "iterations_per_job": iterations, "runs": runs,
             "cells_range": [int(min(w[2] for w in work)), int(max(w[2] for w in work))],
@@ -269,8 +286,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--batch-jobs", type=int, default=256, help="batching-form sample size (0: skip)")
-    ap.add_argument("--batch-runs", default="streams:1,streams:16,batched:64,batched:256",
-                    help="service mode:slots pairs for the batch workload")
+    ap.add_argument("--batch-runs", default="streams:1,streams:16,batched:64,batched:256,batched:256:bf16x3",
+                    help="service mode:slots[:bf16x3] runs of the batch workload")
     ap.add_argument("--batch-iterations", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -292,14 +309,14 @@ def main() -> int:
         print(json.dumps({"serving": sv}), flush=True)
         return 0
     if args.workload == "batch":
-        spec = [(m, int(k)) for m, k in (x.split(":") for x in args.batch_runs.split(","))]
-        b = batch_sample(args.batch_jobs, spec, args.batch_iterations)
+        b = batch_sample(args.batch_jobs, parse_batch_runs(args.batch_runs), args.batch_iterations)
         best = max(b["runs"].values(), key=lambda r: r["gp_iterations_per_s"])
         line = {"metric": METRICS["batch"], "value": best["gp_iterations_per_s"], "unit": "GP iterations/s",
                 "n_gpus": 1, "steps": b["jobs"] * b["iterations_per_job"], "warmup": 0,
                 "ms_per_step": 1000.0 / best["gp_iterations_per_s"], "higher_is_better": True, "scaling": "weak",
                 "vs_baseline": None, "dtype": "f64 positions/reductions, f32 pin terms, f32 DCT products",
-                "data": "synthetic", "config": {"workload": b["sample"], "mode": best["mode"], "slots": best["slots"]},
+                "data": "synthetic", "config": {"workload": b["sample"], "mode": best["mode"], "slots": best["slots"],
+                           "dct": best["precision"]},
                 "batch": b}
         print(json.dumps(line), flush=True)
         return 0
@@ -367,11 +384,10 @@ def main() -> int:
         line["serving"] = serving_sample(args.serve_jobs, args.serve_rate, args.serve_slots, mode=args.serve_mode)
     if args.batch_jobs > 0:
         log(f"batching sample: {args.batch_jobs} blocks ...")
-        spec = [(m, int(k)) for m, k in (x.split(":") for x in args.batch_runs.split(","))]
-        b = batch_sample(args.batch_jobs, spec, args.batch_iterations)
+        b = batch_sample(args.batch_jobs, parse_batch_runs(args.batch_runs), args.batch_iterations)
         best = max(b["runs"].values(), key=lambda r: r["gp_iterations_per_s"])
         b["metric"] = METRICS["batch"]
-        b["best"] = f"{best['mode']}/{best['slots']}"
+        b["best"] = f"{best['mode']}/{best['slots']}/{best['precision']}"
         line["batching"] = b
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
