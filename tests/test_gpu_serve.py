@@ -56,11 +56,12 @@ def test_results_are_the_same_alone_and_in_the_mix():
         assert r_mix["estimated_device_bytes"] >= r_mix["result"]["reserved_device_bytes"]
 
 
-def test_admission_waits_for_memory():
+@pytest.mark.parametrize("mode", ["streams", "batched"])
+def test_admission_waits_for_memory(mode):
     a, b = _job(7, 3000), _job(8, 3000)
     est = max(hb.estimate_device_bytes(a[0], a[1]), hb.estimate_device_bytes(b[0], b[1]))
-    say("[gpu] admission under a budget that fits one job")
-    server = hb.PlacementServer(max_active_jobs=4, memory_budget_bytes=int(1.5 * est))
+    say(f"[gpu] admission under a budget that fits one job ({mode})")
+    server = hb.PlacementServer(max_active_jobs=4, memory_budget_bytes=int(1.5 * est), mode=mode)
     ja, jb = server.submit(*a), server.submit(*b)
     _, ra = server.wait(ja, a[0].module_count)
     _, rb = server.wait(jb, b[0].module_count)
