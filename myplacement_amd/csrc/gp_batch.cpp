@@ -13,6 +13,7 @@
 #include <cstring>
 #include <deque>
 #include <exception>
+#include <future>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -55,7 +56,7 @@ struct BatchDriver::Impl {
         bool initialized = false, finished = false;
         std::exception_ptr error;
         double* centers = nullptr;
-        double hpwl_before = 0.0, overflow_before = 0.0;
+        std::future<std::pair<double, double>> before;  // HPWL and design overflow at the start
         std::chrono::steady_clock::time_point t0, t_loop;
         Callback done;
     };
@@ -356,10 +357,8 @@ struct BatchDriver::Impl {
                 return;
             }
             validateOptions(j.options);
-            j.hpwl_before = probe.hpwl(probe.cx.data(), probe.cy.data());
-            j.overflow_before = probe.designOverflow(probe.cx.data(), probe.cy.data(), j.options.bins_x,
-                                                     j.options.bins_y, j.options.target_density);
             auto prep = std::make_shared<const Prepared>(std::move(probe), j.options);
+            j.before = beforeMetrics(prep, j.options);
             j.arena = takeSlab(estimateBytes(d, j.options));
             std::tie(j.scalars, j.scalars_device) = takeScalars();
             const BorrowedResources borrowed{lent, &j.arena, j.scalars, j.scalars_device};
@@ -390,8 +389,7 @@ struct BatchDriver::Impl {
                 j.s->addTime(0.0, since(j.t_loop));
                 j.s->detach();
                 j.s->finish(j.centers, &r);
-                r.hpwl_before = j.hpwl_before;
-                r.overflow_before = j.overflow_before;
+                std::tie(r.hpwl_before, r.overflow_before) = j.before.get();
                 r.elapsed_seconds = since(j.t0);
             } catch (...) {
                 j.error = std::current_exception();

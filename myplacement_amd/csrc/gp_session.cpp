@@ -19,6 +19,7 @@
 #include <cmath>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace mpgp {
@@ -41,6 +42,15 @@ void validateOptions(const mp_gp_options& o) {  // AdaptiveGlobalPlacer.cpp:413-
         throw std::invalid_argument("Density grid exceeds the safe one-million-bin workspace limit.");
 }
 
+
+std::future<std::pair<double, double>> beforeMetrics(std::shared_ptr<const Prepared> prep, const mp_gp_options& o) {
+    auto compute = [prep, nx = o.bins_x, ny = o.bins_y, t = o.target_density] {
+        const Netlist& n = prep->net;
+        return std::make_pair(n.hpwl(n.cx.data(), n.cy.data()), n.designOverflow(n.cx.data(), n.cy.data(), nx, ny, t));
+    };
+    constexpr std::size_t kLargePins = 200000;
+    return std::async(prep->net.pin_module.size() > kLargePins ? std::launch::async : std::launch::deferred, compute);
+}
 
 }  // namespace mpgp
 
@@ -275,16 +285,13 @@ mp_gp_status mp_gp_global_place(const mp_design* design, const mp_gp_options* op
             return;
         }
         validateOptions(*options);
-        const double hpwl_before = probe.hpwl(probe.cx.data(), probe.cy.data());
-        const double ovf_before = probe.designOverflow(probe.cx.data(), probe.cy.data(), options->bins_x, options->bins_y,
-                                                       options->target_density);
         auto prep = std::make_shared<const Prepared>(std::move(probe), *options);
+        auto before = beforeMetrics(prep, *options);
         Session s(prep, *options, nullptr);
         s.step(options->iterations);
         mp_gp_result r{};
         s.finish(centers, &r);
-        r.hpwl_before = hpwl_before;
-        r.overflow_before = ovf_before;
+        std::tie(r.hpwl_before, r.overflow_before) = before.get();
         r.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (result) *result = r;
         if (history) {

@@ -24,6 +24,7 @@
 #include <cmath>
 #include <coroutine>
 #include <exception>
+#include <future>
 #include <limits>
 #include <memory>
 #include <numeric>
@@ -38,6 +39,10 @@ mp_gp_status translate(const std::exception_ptr& e);
 std::size_t allocationLimit(int device, std::uint64_t maximum_bytes, std::string& reason);
 // AdaptiveGlobalPlacer.cpp:413-454 (non-RUDY part)
 void validateOptions(const mp_gp_options& o);
+// HPWL and design overflow of the starting placement, the "before" figures of the result
+// (GlobalPlacer.cpp:134-146): for a large design on a thread of its own beside the run (they read
+// only the prepared netlist), else at once.
+std::future<std::pair<double, double>> beforeMetrics(std::shared_ptr<const Prepared> prep, const mp_gp_options& o);
 
 
 namespace session_detail {
