@@ -181,10 +181,15 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
     }
     std::vector<int> ipin_part(static_cast<std::size_t>(inet_off.back()));
     std::vector<double2> ipin_off(ipin_part.size());
-    // a gather through the net order (a cache miss per pin on large designs): in parallel parts
-    parallelFor(order.size(), 32768, [&](std::size_t k0, std::size_t k1) {
-        for (std::size_t k = k0; k < k1; ++k) {
-            const int n = order[k];
+    // pins walked in their input order (sequential reads), each stored at its place in the net
+    // order (scattered writes, which do not stall the walk), in parallel parts
+    std::vector<int> net_rank(static_cast<std::size_t>(N_), -1);
+    for (std::size_t k = 0; k < order.size(); ++k) net_rank[order[k]] = static_cast<int>(k);
+    parallelFor(static_cast<std::size_t>(N_), 32768, [&](std::size_t n0, std::size_t n1) {
+        for (std::size_t nn = n0; nn < n1; ++nn) {
+            const int n = static_cast<int>(nn);
+            const std::size_t k = static_cast<std::size_t>(net_rank[n]);
+            if (net_rank[n] < 0) continue;  // fewer than two pins
             const int deg = degree(n);
             for (int j = 0; j < deg; ++j) {
                 const int p = L.net_offsets[n] + j;
