@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 #include <random>
 #include <stdexcept>
@@ -60,6 +61,23 @@ void orient(double& x, double& y, int o) {  // PlacementDatabase.cpp:55-68
 }
 
 namespace {
+
+// Ascending sort of positive finite doubles: their bit patterns order as the values do, so an LSD
+// radix sort on them (16 bits per pass) gives std::sort's sequence in linear time.
+void sortPositive(std::vector<double>& v) {
+    const std::size_t n = v.size();
+    std::vector<std::uint64_t> a(n), b(n);
+    std::memcpy(a.data(), v.data(), n * sizeof(double));
+    std::vector<std::size_t> count(65537);
+    for (int shift = 0; shift < 64; shift += 16) {
+        std::fill(count.begin(), count.end(), std::size_t{0});
+        for (std::size_t i = 0; i < n; ++i) ++count[((a[i] >> shift) & 0xffffU) + 1];
+        std::partial_sum(count.begin(), count.end(), count.begin());
+        for (std::size_t i = 0; i < n; ++i) b[count[(a[i] >> shift) & 0xffffU]++] = a[i];
+        a.swap(b);
+    }
+    std::memcpy(v.data(), a.data(), n * sizeof(double));
+}
 
 // Bin span of a clipped rectangle (Metrics.cpp:78-85).
 void binSpan(const Box& b, const Box& region, double bw, double bh, int nx, int ny, int& c0, int& c1, int& r0, int& r1) {
@@ -207,7 +225,7 @@ Prepared::Prepared(Netlist&& n, const mp_gp_options& o) : net(std::move(n)) {
     const double required = std::max(0.0, t * region.area() - static_charge - (std_area + t * macro_area));
     const double filler_area = required / std::max(t, kEps);
     if (!(filler_area <= kEps || areas.empty() || o.maximum_fillers == 0)) {
-        std::sort(areas.begin(), areas.end());
+        sortPositive(areas);
         const std::size_t lower = areas.size() / 10U;
         const std::size_t upper = std::max(lower + 1U, areas.size() - areas.size() / 10U);
         const double representative = std::accumulate(areas.begin() + static_cast<std::ptrdiff_t>(lower),
