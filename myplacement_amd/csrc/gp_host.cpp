@@ -151,6 +151,7 @@ Netlist::Netlist(const mp_design& d) {
         }
     }
     macro.assign(M, 0);
+    movable.reserve(M);
     for (std::size_t i = 0; i < M; ++i) {
         if (terminal[i]) fixed[i] = 1;
         macro[i] = !terminal[i] && nominal_row_height > kEps && std::abs(height[i] - nominal_row_height) > kEps;
@@ -255,6 +256,11 @@ Prepared::Prepared(Netlist&& n, const mp_gp_options& o) : net(std::move(n)) {
         net.pin_module.size() > static_cast<std::size_t>(std::numeric_limits<int>::max()))
         throw std::invalid_argument("HIP backend cannot index more than INT_MAX particles or pins");
     std::vector<int> module_particle(net.M, -1);
+    pw.reserve(P);
+    ph.reserve(P);
+    pq.reserve(P);
+    player.reserve(P);
+    precond.reserve(P);
     for (std::size_t i = 0; i < Mv; ++i) {
         const int64_t m = net.movable[i];
         module_particle[m] = static_cast<int>(i);
