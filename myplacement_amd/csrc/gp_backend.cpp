@@ -761,6 +761,17 @@ void Backend::addReduction(int kind, int bank, int blocks, double p0, double p1,
     pending_.job[pending_.n++] = RedJob{partials(kind, bank), blocks, kind, dev_scal_ + bank * kScalarCount, p0, p1, p2};
 }
 
+void Backend::finalizePending() {
+    if (pending_.n == 0) return;
+    RedList l = pending_;
+    l.seq = 0;
+    l.flag = nullptr;
+    if (launches_ != own_launches_.get())
+        throw std::logic_error("finalizing early is for launches on the backend's own stream");
+    launchFinalize(l, stream_);
+    pending_.n = 0;
+}
+
 const HostScalars& Backend::fetchScalars() {
     const int jobs = pending_.n;
     if (jobs == 0) {

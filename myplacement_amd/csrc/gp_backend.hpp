@@ -245,6 +245,11 @@ public:
     // Reduces the pending partials into the scalar block and waits until k_finalize has written
     // it (by polling its completion words; the rest of the kernel's retirement is not waited for).
     const HostScalars& fetchScalars();
+    // Finalizes the queued reductions into the scalar block without waiting for them (a later
+    // kernel on the stream reads them: deviceBank).
+    void finalizePending();
+    // device address of a scalar bank
+    double* deviceBank(int bank) const { return dev_scal_ + bank * kScalarCount; }
     void toScalars(const HostScalars& h, mp_gp_scalars* out, int bank = 0) const;
 
     // Reference-shaped blocking evaluate (CudaPlacementBackend::evaluate).

@@ -1093,10 +1093,12 @@ __global__ __launch_bounds__(kThreads) void k_lookahead(LookaheadArgs l) { looka
 
 __device__ __forceinline__ void proposeBody(const ProposeArgs& pa, const Blk& bk) {
     __shared__ double lds[(kWaves + 1) * 2];
-    const double2* b = pa.b;
-    const double2* g = pa.g;
-    const double2* raw = pa.raw;
-    const double step = pa.step, max_disp = pa.max_disp;
+    const bool look = pa.decided && pa.decided[0] != 0.0;
+    const double2* b = look ? pa.b_look : pa.b;
+    const double2* g = look ? pa.g_look : pa.g;
+    const double2* raw = look ? pa.raw_look : pa.raw;
+    const double step = pa.decided ? pa.decided[1] : pa.step;
+    const double max_disp = pa.decided ? pa.decided[2] : pa.max_disp;
     double2* y = pa.y;
     double2* b_copy = pa.b_copy;
     double2* g_copy = pa.g_copy;
@@ -1126,6 +1128,48 @@ __device__ __forceinline__ void proposeBody(const ProposeArgs& pa, const Blk& bk
     publishPartials<2>(v, 2U, bk.x, pa.partials, lds);
 }
 __global__ __launch_bounds__(kThreads) void k_propose(ProposeArgs pa) { proposeBody(pa, nativeBlk()); }
+
+// One thread: the host law's first decision of an iteration, operation for operation (no
+// contraction, correctly rounded square root and division), from the finalized scalars.
+__global__ void k_decide(DecideArgs a) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double cur_objective = a.cur[kSmooth] + a.penalty * a.cur[kEnergy];
+    bool restarted = a.restarted != 0, use_look = false;
+    const double* base = a.cur;
+    if (a.try_momentum) {
+        if (a.cur[kAlignment] <= 0.0) {
+            restarted = true;
+        } else {
+            const double look_objective = a.look[kSmooth] + a.penalty * a.look[kEnergy];
+            if (look_objective > cur_objective * (1.0 + a.growth_threshold)) {
+                restarted = true;
+            } else {
+                use_look = true;
+                base = a.look;
+            }
+        }
+    }
+    if (restarted) {
+        use_look = false;
+        base = a.cur;
+    }
+    double step = fmax(a.last_step, kEps);
+    if (a.has_previous) {
+        const double dp = base[kCurvDp], dg = base[kCurvDg];
+        if (!(dp <= kEps || dg <= kEps)) {
+            const double curvature = __dsqrt_rn(__ddiv_rn(dg, dp));
+            step = fmax(step * 0.25, fmin(__ddiv_rn(1.0, fmax(curvature, kEps)), step * 4.0));
+        }
+    }
+    const double ratio = fmax(0.0, fmin(__ddiv_rn(base[kOptDensity + 1], fmax(a.initial_overflow, kEps)), 1.0));
+    const double max_disp = a.move_cap * (0.35 + 0.65 * ratio);
+    const double out[3] = {use_look ? 1.0 : 0.0, step, max_disp};
+    for (int k = 0; k < 3; ++k) {
+        a.decided[k] = out[k];
+        a.mirror[k] = out[k];
+    }
+}
 
 __global__ __launch_bounds__(kThreads) void k_zero(double* p, size_t count) {
     for (size_t i = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x; i < count;
@@ -1420,6 +1464,11 @@ void launchLookahead(const LookaheadArgs& l, hipStream_t s) {
     if (l.n == 0) return;
     hipLaunchKernelGGL(k_lookahead, dim3(blocksFor(l.n)), dim3(kThreads), 0, s, l);
     check(hipGetLastError(), "launch look-ahead kernel");
+}
+
+void launchDecide(const DecideArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, s, a);
+    check(hipGetLastError(), "launch step decision kernel");
 }
 
 void launchPropose(const ProposeArgs& p, hipStream_t s) {

@@ -25,6 +25,9 @@ enum ScalarSlot : int {
     kAlignment = 25,  // -sum (pos - prev_pos) . pre of a gather with prev_pos
     kCurvDp = 26,     // sum |pos - px|^2 of a gather with px / pg
     kCurvDg = 27,     // sum |pre - pg|^2
+    kDecidedLook = 28,  // k_decide's choice (1: look-ahead base), step and displacement cap
+    kDecidedStep = 29,
+    kDecidedDisp = 30,
     kScalarCount = 32
 };
 // The scalar block holds kScalarBanks banks of kScalarCount slots, so several evaluations can be
@@ -214,8 +217,22 @@ struct LookaheadArgs {
     int n;
 };
 void launchLookahead(const LookaheadArgs& l, hipStream_t s);
+// The first step of an iteration, decided on the device from the scalars of the current and
+// look-ahead evaluations exactly as the host law decides it (AdaptiveGlobalPlacer.cpp:796-820 and
+// estimateStep :470-492), so that the proposal and its evaluation follow without a read-back.
+// decided[0] = 1 for the look-ahead base, decided[1] the step, decided[2] the displacement cap.
+struct DecideArgs {
+    const double* cur;   // scalar banks (device addresses) of the current and look-ahead evaluations
+    const double* look;
+    double penalty, growth_threshold, last_step, initial_overflow, move_cap;
+    int try_momentum, restarted, has_previous;
+    double* decided;     // device memory, read by the propose kernel
+    double* mirror;      // scalar-bank slots kDecidedLook.. (host-mapped), for the host's check
+};
+void launchDecide(const DecideArgs& a, hipStream_t s);
 // y = clamp(b + cap(-step * g, max_disp)); partials of sum raw.(b - y) and max |b - y|; optional
-// staged copies b_copy = b, g_copy = g
+// staged copies b_copy = b, g_copy = g.  With `decided` set, the base (b or b_look, with its
+// gradients), the step and the cap come from k_decide's output instead.
 struct ProposeArgs {
     const double2* b;
     const double2* g;
@@ -227,6 +244,10 @@ struct ProposeArgs {
     ClampArgs c;
     int n;
     double* partials;
+    const double* decided = nullptr;
+    const double2* b_look = nullptr;
+    const double2* g_look = nullptr;
+    const double2* raw_look = nullptr;
 };
 void launchPropose(const ProposeArgs& p, hipStream_t s);
 // ---- launches batched over independent jobs -----------------------------------------------------

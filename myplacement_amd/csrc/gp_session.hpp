@@ -181,6 +181,7 @@ public:
         for (DeviceArray<double2>* a : {&X_, &XP_, &B_, &Y_, &PX_, &PG_, &PXn_, &PGn_, &Rc_, &Gc_, &Rl_, &Gl_, &CKf_, &CKb_})
             a->allocate(P_, b, "controller vector");
         precond_.allocate(P_, b, "preconditioner");
+        decided_.allocate(4, b, "step decision");
         hipStream_t s = be_->stream();
         X_.upload(x.data(), P_, s, "upload positions");
         precond_.upload(precond.data(), P_, s, "upload preconditioner");
@@ -500,6 +501,32 @@ private:
                 if (try_momentum) be_->enqueueEvaluate(&specs[1], 1, smoothing_, true);
             }
         }
+        // On the session's own stream the first step is decided on the device (k_decide, the law
+        // below restated) and proposed and evaluated before the one read-back of the iteration;
+        // the host then replays the decision from the same scalars and checks it.
+        const bool early = it > 0 && !rec_;
+        const double move_cap = o_.maximum_movement_in_bins * min_bin_;
+        if (early) {
+            be_->finalizePending();
+            DecideArgs da{be_->deviceBank(kBankCurrent), be_->deviceBank(kBankLook), penalty_,
+                          o_.hpwl_growth_restart_threshold, last_step_, initial_overflow_, move_cap,
+                          try_momentum ? 1 : 0, worse_streak_ >= 2 ? 1 : 0, has_previous_ ? 1 : 0, decided_.get(),
+                          be_->deviceBank(kBankCandidate) + kDecidedLook};
+            be_->timedBegin(kTController);
+            launchDecide(da, be_->stream());
+            ProposeArgs pa{X_.get(), Gc_.get(), Rc_.get(), 0.0, 0.0, Y_.get(), PXn_.get(), PGn_.get(), clamp_, P_,
+                           be_->partials(kRedPropose, kBankCandidate)};
+            pa.decided = decided_.get();
+            pa.b_look = B_.get();
+            pa.g_look = Gl_.get();
+            pa.raw_look = Rl_.get();
+            be_->launches().propose(pa);
+            be_->addReduction(kRedPropose, kBankCandidate, particleBlocks(P_));
+            be_->timedEnd(kTController);
+            const Backend::SessionOut so_cand = sessionOut(nullptr, nullptr, nullptr, nullptr);
+            const Backend::EvalSpec cand_spec{Y_.get(), &so_cand, kBankCandidate, fresh};
+            be_->enqueueEvaluate(&cand_spec, 1, smoothing_, false);
+        }
         const bool need_read = it > 0;
         HostScalars first{};
         if (need_read) first = co_await ScalarWait{this};
@@ -560,17 +587,27 @@ private:
                 }
             }
             const double ratio = clampd(base_eval->s.optimizer_density.normalized_overflow / std::max(initial_overflow_, kEps), 0.0, 1.0);
-            const double max_disp = o_.maximum_movement_in_bins * min_bin_ * (0.35 + 0.65 * ratio);
+            const double max_disp = move_cap * (0.35 + 0.65 * ratio);
             for (int attempt = 0; attempt <= o_.maximum_backtracks; ++attempt) {
-                be_->timedBegin(kTController);
-                be_->launches().propose(ProposeArgs{base, base_pre, base_raw, step, max_disp, Y_.get(), PXn_.get(),
-                                                    PGn_.get(), clamp_, P_, be_->partials(kRedPropose, kBankCandidate)});
-                be_->addReduction(kRedPropose, kBankCandidate, particleBlocks(P_));
-                be_->timedEnd(kTController);
-                const Backend::SessionOut so_cand = sessionOut(nullptr, nullptr, nullptr, nullptr);
-                const Backend::EvalSpec cand_spec{Y_.get(), &so_cand, kBankCandidate, fresh};
-                be_->enqueueEvaluate(&cand_spec, 1, smoothing_, false);
-                const HostScalars& h = co_await ScalarWait{this};
+                HostScalars fetched{};
+                if (early && pass == 0 && attempt == 0) {
+                    // proposed and evaluated already, from the device's decision: the same one
+                    const double* dv = first.bank(kBankCandidate) + kDecidedLook;
+                    if ((dv[0] != 0.0) != momentum_used || dv[1] != step || dv[2] != max_disp)
+                        throw std::logic_error("device step decision differs from the host's");
+                    fetched = first;
+                } else {
+                    be_->timedBegin(kTController);
+                    be_->launches().propose(ProposeArgs{base, base_pre, base_raw, step, max_disp, Y_.get(), PXn_.get(),
+                                                        PGn_.get(), clamp_, P_, be_->partials(kRedPropose, kBankCandidate)});
+                    be_->addReduction(kRedPropose, kBankCandidate, particleBlocks(P_));
+                    be_->timedEnd(kTController);
+                    const Backend::SessionOut so_cand = sessionOut(nullptr, nullptr, nullptr, nullptr);
+                    const Backend::EvalSpec cand_spec{Y_.get(), &so_cand, kBankCandidate, fresh};
+                    be_->enqueueEvaluate(&cand_spec, 1, smoothing_, false);
+                    fetched = co_await ScalarWait{this};
+                }
+                const HostScalars& h = fetched;
                 const double predicted = h.bank(kBankCandidate)[kPredicted];
                 const double disp = h.bank(kBankCandidate)[kMaxDisp];
                 if (predicted <= kEps) break;
@@ -654,6 +691,7 @@ private:
     ClampArgs clamp_{};
     DeviceArray<double2> X_, XP_, B_, Y_, PX_, PG_, PXn_, PGn_, Rc_, Gc_, Rl_, Gl_, CKf_, CKb_;
     DeviceArray<double> precond_;
+    DeviceArray<double> decided_;  // k_decide's first step of an iteration, read by k_propose
     std::vector<uint32_t> order_;  // device particle k is prepared particle order_[k]
     DeviceArray<double2>* pending_ck_ = nullptr;
     double min_bin_ = 0, max_bin_ = 0, bin_height_ = 1;
