@@ -681,6 +681,12 @@ void StreamLaunches::dct(int mode, const DctBatch& d, int batch, int rows, int c
 void StreamLaunches::gather(const GatherArgs& a, int count) { launchGather(a, count, s_); }
 void StreamLaunches::lookahead(const LookaheadArgs& l) { launchLookahead(l, s_); }
 void StreamLaunches::propose(const ProposeArgs& p) { launchPropose(p, s_); }
+void StreamLaunches::finalize(const RedList& l) {
+    RedList m = l;
+    m.flag = nullptr;
+    launchFinalize(m, s_);
+}
+void StreamLaunches::decide(const DecideArgs& a) { launchDecide(a, s_); }
 void StreamLaunches::copy(double2* dst, const double2* src, int n) {
     if (n > 0)
         hipCheck(hipMemcpyAsync(dst, src, static_cast<size_t>(n) * sizeof(double2), hipMemcpyDeviceToDevice, s_),
@@ -751,6 +757,10 @@ void RecordLaunches::propose(const ProposeArgs& p) {
     const ProposeEntry e = proposeEntry(p);
     add(kStagePropose, kBatchPropose, e, e.gx);
 }
+void RecordLaunches::finalize(const RedList& l) {
+    for (int k = 0; k < l.n; ++k) add(kStageFinalize, kBatchFinalize, l.job[k], 1);
+}
+void RecordLaunches::decide(const DecideArgs& a) { add(kStageDecide, kBatchDecide, a, 1); }
 void RecordLaunches::copy(double2* dst, const double2* src, int n) {
     const CopyEntry e = copyEntry(dst, src, n);
     add(kStageCopy, kBatchCopy, e, e.gx);
@@ -766,9 +776,7 @@ void Backend::finalizePending() {
     RedList l = pending_;
     l.seq = 0;
     l.flag = nullptr;
-    if (launches_ != own_launches_.get())
-        throw std::logic_error("finalizing early is for launches on the backend's own stream");
-    launchFinalize(l, stream_);
+    launches_->finalize(l);
     pending_.n = 0;
 }
 

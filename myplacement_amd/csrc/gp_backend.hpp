@@ -125,6 +125,9 @@ public:
     virtual void lookahead(const LookaheadArgs& l) = 0;
     virtual void propose(const ProposeArgs& p) = 0;
     virtual void copy(double2* dst, const double2* src, int n) = 0;
+    // reductions finalized into the scalar block for a later kernel (no completion words)
+    virtual void finalize(const RedList& l) = 0;
+    virtual void decide(const DecideArgs& a) = 0;
 };
 
 class StreamLaunches final : public Launches {
@@ -142,6 +145,8 @@ public:
     void lookahead(const LookaheadArgs& l) override;
     void propose(const ProposeArgs& p) override;
     void copy(double2* dst, const double2* src, int n) override;
+    void finalize(const RedList& l) override;
+    void decide(const DecideArgs& a) override;
 
 private:
     hipStream_t s_;
@@ -154,11 +159,15 @@ private:
 enum BatchStage : int {
     kStageCopy = 0, kStageLookahead, kStagePropose, kStageSortHist, kStageSortScan, kStageSortPlace, kStageWire,
     kStageZero, kStageDeposit, kStageBins, kStageSpectral,  // + product index 0..3
-    kStageGather = kStageSpectral + 4, kStageCount
+    kStageGather = kStageSpectral + 4,
+    kStageFinalize,  // reductions a later kernel of the round reads (the step decision's inputs)
+    kStageDecide,
+    kStageCount
 };
 constexpr std::size_t kMaxEntryBytes =
     std::max({sizeof(WireEntry), sizeof(SortEntry), sizeof(DensityEntry), sizeof(GemmEntry), sizeof(DctEntry),
-              sizeof(GatherEntry), sizeof(LookaheadEntry), sizeof(ProposeEntry), sizeof(CopyEntry), sizeof(ZeroEntry)});
+              sizeof(GatherEntry), sizeof(LookaheadEntry), sizeof(ProposeEntry), sizeof(CopyEntry), sizeof(ZeroEntry),
+              sizeof(RedJob), sizeof(DecideArgs)});
 
 // One recorded launch: the job's entry of a batched kernel.
 struct BatchOp {
@@ -185,6 +194,8 @@ public:
     void lookahead(const LookaheadArgs& l) override;
     void propose(const ProposeArgs& p) override;
     void copy(double2* dst, const double2* src, int n) override;
+    void finalize(const RedList& l) override;
+    void decide(const DecideArgs& a) override;
 
     std::vector<BatchOp> ops;
 

@@ -1131,9 +1131,8 @@ __global__ __launch_bounds__(kThreads) void k_propose(ProposeArgs pa) { proposeB
 
 // One thread: the host law's first decision of an iteration, operation for operation (no
 // contraction, correctly rounded square root and division), from the finalized scalars.
-__global__ void k_decide(DecideArgs a) {
+__device__ void decideBody(const DecideArgs& a) {
 #pragma clang fp contract(off)
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const double cur_objective = a.cur[kSmooth] + a.penalty * a.cur[kEnergy];
     bool restarted = a.restarted != 0, use_look = false;
     const double* base = a.cur;
@@ -1169,6 +1168,13 @@ __global__ void k_decide(DecideArgs a) {
         a.decided[k] = out[k];
         a.mirror[k] = out[k];
     }
+}
+__global__ void k_decide(DecideArgs a) {
+    if (threadIdx.x == 0) decideBody(a);
+}
+__global__ __launch_bounds__(64) void k_decide_batch(const DecideArgs* e, const int* first, int n) {
+    const int j = batchEntry<64>(first, n);
+    if (threadIdx.x == 0) decideBody(e[j]);
 }
 
 __global__ __launch_bounds__(kThreads) void k_zero(double* p, size_t count) {
@@ -1343,6 +1349,11 @@ __global__ __launch_bounds__(kThreads) void k_zero_batch(const ZeroEntry* e, con
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize_batch(const RedJob* jobs, volatile unsigned* flags,
                                                                     unsigned seq) {
     finalizeBody(jobs[blockIdx.x], flags + blockIdx.x, seq);
+}
+// reductions finalized inside a round, read by a later kernel of the round (no completion words)
+__global__ __launch_bounds__(kFinalizeThreads) void k_finalize_entries(const RedJob* e, const int* first, int n) {
+    const int j = batchEntry<kFinalizeThreads>(first, n);
+    finalizeBody(e[j], nullptr, 0U);
 }
 
 }  // namespace
@@ -1606,6 +1617,13 @@ void launchBatched(BatchKernel k, const void* entries, const int* first, int n, 
         case kBatchZero:
             hipLaunchKernelGGL(k_zero_batch, grid, dim3(kThreads), 0, s, static_cast<const ZeroEntry*>(entries), first, n);
             break;
+        case kBatchFinalize:
+            hipLaunchKernelGGL(k_finalize_entries, grid, dim3(kFinalizeThreads), 0, s, static_cast<const RedJob*>(entries),
+                               first, n);
+            break;
+        case kBatchDecide:
+            hipLaunchKernelGGL(k_decide_batch, grid, dim3(64), 0, s, static_cast<const DecideArgs*>(entries), first, n);
+            break;
         default: {
             // half-length products: kBatchDct + 2 * mode + (kpl == 8)
             const int v = static_cast<int>(k) - kBatchDct;
@@ -1697,6 +1715,16 @@ void launchEntry(BatchKernel k, const void* entry, std::size_t dyn_lds, hipStrea
             hipLaunchKernelGGL(k_zero, dim3(E.gx), dim3(kThreads), 0, s, E.p, static_cast<size_t>(E.n));
             break;
         }
+        case kBatchFinalize: {
+            RedList l{};
+            l.job[0] = *static_cast<const RedJob*>(entry);
+            l.n = 1;
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, s, l);
+            break;
+        }
+        case kBatchDecide:
+            hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, s, *static_cast<const DecideArgs*>(entry));
+            break;
         default: {
             const int v = static_cast<int>(k) - kBatchDct;
             if (v < 0 || v >= 8) throw std::invalid_argument("unknown batched kernel");

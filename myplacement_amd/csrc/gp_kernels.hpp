@@ -301,6 +301,8 @@ struct ZeroEntry {
 enum BatchKernel : int {
     kBatchWire = 0, kBatchSortHist, kBatchSortScan, kBatchSortPlace, kBatchDeposit, kBatchBins, kBatchGemmStore,
     kBatchGemmSpectral, kBatchGemmInterleave, kBatchGather, kBatchLookahead, kBatchPropose, kBatchCopy, kBatchZero,
+    kBatchFinalize,  // entries: RedJob (one workgroup each, no completion word)
+    kBatchDecide,    // entries: DecideArgs (one workgroup)
     kBatchDct  // + 2 * mode + (8 values per lane ? 1 : 0)
 };
 // Entry builders (the grid extent of the job's own launch; they validate as the launches do).

@@ -501,10 +501,10 @@ private:
                 if (try_momentum) be_->enqueueEvaluate(&specs[1], 1, smoothing_, true);
             }
         }
-        // On the session's own stream the first step is decided on the device (k_decide, the law
-        // below restated) and proposed and evaluated before the one read-back of the iteration;
-        // the host then replays the decision from the same scalars and checks it.
-        const bool early = it > 0 && !rec_;
+        // The first step is decided on the device (k_decide, the law below restated) and proposed
+        // and evaluated before the one read-back of the iteration; the host then replays the
+        // decision from the same scalars and checks it.
+        const bool early = it > 0;
         const double move_cap = o_.maximum_movement_in_bins * min_bin_;
         if (early) {
             be_->finalizePending();
@@ -513,7 +513,7 @@ private:
                           try_momentum ? 1 : 0, worse_streak_ >= 2 ? 1 : 0, has_previous_ ? 1 : 0, decided_.get(),
                           be_->deviceBank(kBankCandidate) + kDecidedLook};
             be_->timedBegin(kTController);
-            launchDecide(da, be_->stream());
+            be_->launches().decide(da);
             ProposeArgs pa{X_.get(), Gc_.get(), Rc_.get(), 0.0, 0.0, Y_.get(), PXn_.get(), PGn_.get(), clamp_, P_,
                            be_->partials(kRedPropose, kBankCandidate)};
             pa.decided = decided_.get();
