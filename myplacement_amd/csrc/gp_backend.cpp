@@ -300,6 +300,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
     for (int k = 0; k < kRedKinds * kScalarBanks; ++k)
         partials_[k].allocate(static_cast<std::size_t>(max_blocks) * redWidth(k / kScalarBanks), budget_,
                               "reduction partials");
+    scal_copy_.allocate(static_cast<std::size_t>(kScalarCount) * kScalarBanks, budget_, "scalar block copy");
     wire_out_.allocate(P_, budget_, "wire gradient");
     density_out_.allocate(P_, budget_, "density gradient");
     // coherent host memory: k_finalize's stores go straight to the host, no copy after it
@@ -686,7 +687,6 @@ void StreamLaunches::finalize(const RedList& l) {
     m.flag = nullptr;
     launchFinalize(m, s_);
 }
-void StreamLaunches::decide(const DecideArgs& a) { launchDecide(a, s_); }
 void StreamLaunches::copy(double2* dst, const double2* src, int n) {
     if (n > 0)
         hipCheck(hipMemcpyAsync(dst, src, static_cast<size_t>(n) * sizeof(double2), hipMemcpyDeviceToDevice, s_),
@@ -760,7 +760,6 @@ void RecordLaunches::propose(const ProposeArgs& p) {
 void RecordLaunches::finalize(const RedList& l) {
     for (int k = 0; k < l.n; ++k) add(kStageFinalize, kBatchFinalize, l.job[k], 1);
 }
-void RecordLaunches::decide(const DecideArgs& a) { add(kStageDecide, kBatchDecide, a, 1); }
 void RecordLaunches::copy(double2* dst, const double2* src, int n) {
     const CopyEntry e = copyEntry(dst, src, n);
     add(kStageCopy, kBatchCopy, e, e.gx);
@@ -768,7 +767,8 @@ void RecordLaunches::copy(double2* dst, const double2* src, int n) {
 
 void Backend::addReduction(int kind, int bank, int blocks, double p0, double p1, double p2) {
     if (pending_.n == kMaxRedJobs) throw std::logic_error("too many pending reductions");
-    pending_.job[pending_.n++] = RedJob{partials(kind, bank), blocks, kind, dev_scal_ + bank * kScalarCount, p0, p1, p2};
+    pending_.job[pending_.n++] = RedJob{partials(kind, bank), blocks, kind, dev_scal_ + bank * kScalarCount,
+                                        scal_copy_.get() + bank * kScalarCount, p0, p1, p2};
 }
 
 void Backend::finalizePending() {

@@ -127,7 +127,6 @@ public:
     virtual void copy(double2* dst, const double2* src, int n) = 0;
     // reductions finalized into the scalar block for a later kernel (no completion words)
     virtual void finalize(const RedList& l) = 0;
-    virtual void decide(const DecideArgs& a) = 0;
 };
 
 class StreamLaunches final : public Launches {
@@ -146,7 +145,6 @@ public:
     void propose(const ProposeArgs& p) override;
     void copy(double2* dst, const double2* src, int n) override;
     void finalize(const RedList& l) override;
-    void decide(const DecideArgs& a) override;
 
 private:
     hipStream_t s_;
@@ -161,13 +159,12 @@ enum BatchStage : int {
     kStageZero, kStageDeposit, kStageBins, kStageSpectral,  // + product index 0..3
     kStageGather = kStageSpectral + 4,
     kStageFinalize,  // reductions a later kernel of the round reads (the step decision's inputs)
-    kStageDecide,
     kStageCount
 };
 constexpr std::size_t kMaxEntryBytes =
     std::max({sizeof(WireEntry), sizeof(SortEntry), sizeof(DensityEntry), sizeof(GemmEntry), sizeof(DctEntry),
               sizeof(GatherEntry), sizeof(LookaheadEntry), sizeof(ProposeEntry), sizeof(CopyEntry), sizeof(ZeroEntry),
-              sizeof(RedJob), sizeof(DecideArgs)});
+              sizeof(RedJob)});
 
 // One recorded launch: the job's entry of a batched kernel.
 struct BatchOp {
@@ -195,7 +192,6 @@ public:
     void propose(const ProposeArgs& p) override;
     void copy(double2* dst, const double2* src, int n) override;
     void finalize(const RedList& l) override;
-    void decide(const DecideArgs& a) override;
 
     std::vector<BatchOp> ops;
 
@@ -259,8 +255,9 @@ public:
     // Finalizes the queued reductions into the scalar block without waiting for them (a later
     // kernel on the stream reads them: deviceBank).
     void finalizePending();
-    // device address of a scalar bank
-    double* deviceBank(int bank) const { return dev_scal_ + bank * kScalarCount; }
+    // a scalar bank: device address of the host-mapped block, and its copy in device memory
+    double* mappedBank(int bank) const { return dev_scal_ + bank * kScalarCount; }
+    const double* deviceBank(int bank) const { return scal_copy_.get() + bank * kScalarCount; }
     void toScalars(const HostScalars& h, mp_gp_scalars* out, int bank = 0) const;
 
     // Reference-shaped blocking evaluate (CudaPlacementBackend::evaluate).
@@ -350,6 +347,7 @@ private:
     bool owns_stream_ = true, owns_scalars_ = true;
     HostScalars* host_scal_ = nullptr;  // host-mapped, written by k_finalize
     double* dev_scal_ = nullptr;        // device address of host_scal_
+    DeviceArray<double> scal_copy_;     // the banks' copy in device memory (k_finalize writes both)
     unsigned* dev_done_ = nullptr;      // device address of host_scal_->done
     unsigned seq_ = 0;                  // completion value of the last finalize
 

@@ -1091,14 +1091,21 @@ __device__ __forceinline__ void lookaheadBody(const LookaheadArgs& l, const Blk&
 }
 __global__ __launch_bounds__(kThreads) void k_lookahead(LookaheadArgs l) { lookaheadBody(l, nativeBlk()); }
 
+__device__ double3 decideStep(const DecideArgs& a);
 __device__ __forceinline__ void proposeBody(const ProposeArgs& pa, const Blk& bk) {
     __shared__ double lds[(kWaves + 1) * 2];
-    const bool look = pa.decided && pa.decided[0] != 0.0;
+    // with decide_here every thread takes the same decision from the same device-memory scalars
+    const double3 dec = pa.decide_here ? decideStep(pa.decide) : make_double3(0.0, pa.step, pa.max_disp);
+    if (pa.decide_here && bk.x == 0 && threadIdx.x == 0) {  // for the host's check
+        pa.decide.mirror[0] = dec.x;
+        pa.decide.mirror[1] = dec.y;
+        pa.decide.mirror[2] = dec.z;
+    }
+    const bool look = dec.x != 0.0;
     const double2* b = look ? pa.b_look : pa.b;
     const double2* g = look ? pa.g_look : pa.g;
     const double2* raw = look ? pa.raw_look : pa.raw;
-    const double step = pa.decided ? pa.decided[1] : pa.step;
-    const double max_disp = pa.decided ? pa.decided[2] : pa.max_disp;
+    const double step = dec.y, max_disp = dec.z;
     double2* y = pa.y;
     double2* b_copy = pa.b_copy;
     double2* g_copy = pa.g_copy;
@@ -1129,9 +1136,10 @@ __device__ __forceinline__ void proposeBody(const ProposeArgs& pa, const Blk& bk
 }
 __global__ __launch_bounds__(kThreads) void k_propose(ProposeArgs pa) { proposeBody(pa, nativeBlk()); }
 
-// One thread: the host law's first decision of an iteration, operation for operation (no
-// contraction, correctly rounded square root and division), from the finalized scalars.
-__device__ void decideBody(const DecideArgs& a) {
+// The host law's first decision of an iteration, operation for operation (no contraction,
+// correctly rounded square root and division), from the finalized scalars: (1 for the look-ahead
+// base, step, displacement cap).
+__device__ double3 decideStep(const DecideArgs& a) {
 #pragma clang fp contract(off)
     const double cur_objective = a.cur[kSmooth] + a.penalty * a.cur[kEnergy];
     bool restarted = a.restarted != 0, use_look = false;
@@ -1162,19 +1170,7 @@ __device__ void decideBody(const DecideArgs& a) {
         }
     }
     const double ratio = fmax(0.0, fmin(__ddiv_rn(base[kOptDensity + 1], fmax(a.initial_overflow, kEps)), 1.0));
-    const double max_disp = a.move_cap * (0.35 + 0.65 * ratio);
-    const double out[3] = {use_look ? 1.0 : 0.0, step, max_disp};
-    for (int k = 0; k < 3; ++k) {
-        a.decided[k] = out[k];
-        a.mirror[k] = out[k];
-    }
-}
-__global__ void k_decide(DecideArgs a) {
-    if (threadIdx.x == 0) decideBody(a);
-}
-__global__ __launch_bounds__(64) void k_decide_batch(const DecideArgs* e, const int* first, int n) {
-    const int j = batchEntry<64>(first, n);
-    if (threadIdx.x == 0) decideBody(e[j]);
+    return make_double3(use_look ? 1.0 : 0.0, step, a.move_cap * (0.35 + 0.65 * ratio));
 }
 
 __global__ __launch_bounds__(kThreads) void k_zero(double* p, size_t count) {
@@ -1224,40 +1220,47 @@ __device__ __forceinline__ void finalizeBody(const RedJob& j, volatile unsigned*
     }
     blockReduce<9, kFinalizeThreads / 64>(acc, max_mask, lds);
     if (threadIdx.x != 0) return;
+    // results to the host-mapped bank and, when given, to its copy in device memory (which later
+    // kernels read at device speed)
     double* o = j.out;
+    double* od = j.out_dev;
+    auto put = [&](int slot, double v) {
+        o[slot] = v;
+        if (od) od[slot] = v;
+    };
     switch (j.kind) {
         case kRedWire:
-            o[kHpwl] = acc[0];
-            o[kSmooth] = acc[1];
+            put(kHpwl, acc[0]);
+            put(kSmooth, acc[1]);
             break;
         case kRedMetrics:
             for (int f = 0; f < 2; ++f) {
-                double* m = o + (f == 0 ? kOptDensity : kDesignDensity);
-                m[0] = acc[f];                        // total_overflow_area
-                m[5] = acc[4];                        // normalization_area
-                m[1] = m[0] / fmax(m[5], kEps);       // normalized_overflow
-                m[2] = acc[7 + f];                    // maximum_density (starts at 0)
-                m[3] = acc[5 + f] / j.p0;             // average_density
-                m[4] = acc[2 + f];                    // total_charge_area
-                m[6] = j.p2;                          // placeable_area
-                m[7] = j.p1;                          // dark_area
+                const int m = f == 0 ? kOptDensity : kDesignDensity;
+                put(m + 0, acc[f]);                        // total_overflow_area
+                put(m + 5, acc[4]);                        // normalization_area
+                put(m + 1, acc[f] / fmax(acc[4], kEps));   // normalized_overflow
+                put(m + 2, acc[7 + f]);                    // maximum_density (starts at 0)
+                put(m + 3, acc[5 + f] / j.p0);             // average_density
+                put(m + 4, acc[2 + f]);                    // total_charge_area
+                put(m + 6, j.p2);                          // placeable_area
+                put(m + 7, j.p1);                          // dark_area
             }
             break;
         case kRedEnergy:
-            o[kEnergy] = fmax(0.0, j.p0 * acc[0]);
+            put(kEnergy, fmax(0.0, j.p0 * acc[0]));
             break;
         case kRedGather:
-            o[kWireL1] = acc[0];
-            o[kDensityL1] = acc[1];
-            o[kRawNorm2] = acc[2];
-            o[kPreNorm2] = acc[3];
-            o[kAlignment] = -acc[4];
-            o[kCurvDp] = acc[5];
-            o[kCurvDg] = acc[6];
+            put(kWireL1, acc[0]);
+            put(kDensityL1, acc[1]);
+            put(kRawNorm2, acc[2]);
+            put(kPreNorm2, acc[3]);
+            put(kAlignment, -acc[4]);
+            put(kCurvDp, acc[5]);
+            put(kCurvDg, acc[6]);
             break;
         default:  // kRedPropose
-            o[kPredicted] = acc[0];
-            o[kMaxDisp] = acc[1];
+            put(kPredicted, acc[0]);
+            put(kMaxDisp, acc[1]);
             break;
     }
     // completion word for a host that polls the scalar block instead of waiting for the stream:
@@ -1477,11 +1480,6 @@ void launchLookahead(const LookaheadArgs& l, hipStream_t s) {
     check(hipGetLastError(), "launch look-ahead kernel");
 }
 
-void launchDecide(const DecideArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, s, a);
-    check(hipGetLastError(), "launch step decision kernel");
-}
-
 void launchPropose(const ProposeArgs& p, hipStream_t s) {
     hipLaunchKernelGGL(k_propose, dim3(particleBlocks(p.n)), dim3(kThreads), 0, s, p);
     check(hipGetLastError(), "launch propose kernel");
@@ -1621,9 +1619,6 @@ void launchBatched(BatchKernel k, const void* entries, const int* first, int n, 
             hipLaunchKernelGGL(k_finalize_entries, grid, dim3(kFinalizeThreads), 0, s, static_cast<const RedJob*>(entries),
                                first, n);
             break;
-        case kBatchDecide:
-            hipLaunchKernelGGL(k_decide_batch, grid, dim3(64), 0, s, static_cast<const DecideArgs*>(entries), first, n);
-            break;
         default: {
             // half-length products: kBatchDct + 2 * mode + (kpl == 8)
             const int v = static_cast<int>(k) - kBatchDct;
@@ -1722,9 +1717,7 @@ void launchEntry(BatchKernel k, const void* entry, std::size_t dyn_lds, hipStrea
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, s, l);
             break;
         }
-        case kBatchDecide:
-            hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, s, *static_cast<const DecideArgs*>(entry));
-            break;
+
         default: {
             const int v = static_cast<int>(k) - kBatchDct;
             if (v < 0 || v >= 8) throw std::invalid_argument("unknown batched kernel");

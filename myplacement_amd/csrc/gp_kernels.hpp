@@ -25,7 +25,7 @@ enum ScalarSlot : int {
     kAlignment = 25,  // -sum (pos - prev_pos) . pre of a gather with prev_pos
     kCurvDp = 26,     // sum |pos - px|^2 of a gather with px / pg
     kCurvDg = 27,     // sum |pre - pg|^2
-    kDecidedLook = 28,  // k_decide's choice (1: look-ahead base), step and displacement cap
+    kDecidedLook = 28,  // the device's step decision (1: look-ahead base), step and displacement cap
     kDecidedStep = 29,
     kDecidedDisp = 30,
     kScalarCount = 32
@@ -41,7 +41,8 @@ struct RedJob {
     const double* partials;
     int blocks;
     int kind;
-    double* out;        // scalar bank
+    double* out;        // scalar bank (host-mapped)
+    double* out_dev;    // its copy in device memory, or null
     double p0, p1, p2;  // energy: p0 half bin area; metrics: p0 bin count, p1 dark, p2 placeable
 };
 constexpr int kMaxRedJobs = 12;
@@ -219,20 +220,19 @@ struct LookaheadArgs {
 void launchLookahead(const LookaheadArgs& l, hipStream_t s);
 // The first step of an iteration, decided on the device from the scalars of the current and
 // look-ahead evaluations exactly as the host law decides it (AdaptiveGlobalPlacer.cpp:796-820 and
-// estimateStep :470-492), so that the proposal and its evaluation follow without a read-back.
-// decided[0] = 1 for the look-ahead base, decided[1] the step, decided[2] the displacement cap.
+// estimateStep :470-492), by the propose kernel itself, so that the proposal and its evaluation
+// follow without a read-back.  The decision (1 for the look-ahead base, step, displacement cap)
+// is also stored to `mirror` for the host's check.
 struct DecideArgs {
-    const double* cur;   // scalar banks (device addresses) of the current and look-ahead evaluations
+    const double* cur;   // device-memory copies of the current and look-ahead scalar banks
     const double* look;
     double penalty, growth_threshold, last_step, initial_overflow, move_cap;
     int try_momentum, restarted, has_previous;
-    double* decided;     // device memory, read by the propose kernel
-    double* mirror;      // scalar-bank slots kDecidedLook.. (host-mapped), for the host's check
+    double* mirror;      // scalar-bank slots kDecidedLook.. (host-mapped)
 };
-void launchDecide(const DecideArgs& a, hipStream_t s);
 // y = clamp(b + cap(-step * g, max_disp)); partials of sum raw.(b - y) and max |b - y|; optional
-// staged copies b_copy = b, g_copy = g.  With `decided` set, the base (b or b_look, with its
-// gradients), the step and the cap come from k_decide's output instead.
+// staged copies b_copy = b, g_copy = g.  With decide_here set, the base (b or b_look, with its
+// gradients), the step and the cap come from the decision taken from `decide` instead.
 struct ProposeArgs {
     const double2* b;
     const double2* g;
@@ -244,7 +244,8 @@ struct ProposeArgs {
     ClampArgs c;
     int n;
     double* partials;
-    const double* decided = nullptr;
+    int decide_here = 0;
+    DecideArgs decide{};
     const double2* b_look = nullptr;
     const double2* g_look = nullptr;
     const double2* raw_look = nullptr;
@@ -302,7 +303,6 @@ enum BatchKernel : int {
     kBatchWire = 0, kBatchSortHist, kBatchSortScan, kBatchSortPlace, kBatchDeposit, kBatchBins, kBatchGemmStore,
     kBatchGemmSpectral, kBatchGemmInterleave, kBatchGather, kBatchLookahead, kBatchPropose, kBatchCopy, kBatchZero,
     kBatchFinalize,  // entries: RedJob (one workgroup each, no completion word)
-    kBatchDecide,    // entries: DecideArgs (one workgroup)
     kBatchDct  // + 2 * mode + (8 values per lane ? 1 : 0)
 };
 // Entry builders (the grid extent of the job's own launch; they validate as the launches do).

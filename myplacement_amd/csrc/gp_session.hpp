@@ -181,7 +181,6 @@ public:
         for (DeviceArray<double2>* a : {&X_, &XP_, &B_, &Y_, &PX_, &PG_, &PXn_, &PGn_, &Rc_, &Gc_, &Rl_, &Gl_, &CKf_, &CKb_})
             a->allocate(P_, b, "controller vector");
         precond_.allocate(P_, b, "preconditioner");
-        decided_.allocate(4, b, "step decision");
         hipStream_t s = be_->stream();
         X_.upload(x.data(), P_, s, "upload positions");
         precond_.upload(precond.data(), P_, s, "upload preconditioner");
@@ -501,25 +500,24 @@ private:
                 if (try_momentum) be_->enqueueEvaluate(&specs[1], 1, smoothing_, true);
             }
         }
-        // The first step is decided on the device (k_decide, the law below restated) and proposed
+        // The first step is decided on the device (by k_propose, the law below restated) and proposed
         // and evaluated before the one read-back of the iteration; the host then replays the
         // decision from the same scalars and checks it.
         const bool early = it > 0;
         const double move_cap = o_.maximum_movement_in_bins * min_bin_;
         if (early) {
             be_->finalizePending();
-            DecideArgs da{be_->deviceBank(kBankCurrent), be_->deviceBank(kBankLook), penalty_,
-                          o_.hpwl_growth_restart_threshold, last_step_, initial_overflow_, move_cap,
-                          try_momentum ? 1 : 0, worse_streak_ >= 2 ? 1 : 0, has_previous_ ? 1 : 0, decided_.get(),
-                          be_->deviceBank(kBankCandidate) + kDecidedLook};
-            be_->timedBegin(kTController);
-            be_->launches().decide(da);
             ProposeArgs pa{X_.get(), Gc_.get(), Rc_.get(), 0.0, 0.0, Y_.get(), PXn_.get(), PGn_.get(), clamp_, P_,
                            be_->partials(kRedPropose, kBankCandidate)};
-            pa.decided = decided_.get();
+            pa.decide_here = 1;
+            pa.decide = DecideArgs{be_->deviceBank(kBankCurrent), be_->deviceBank(kBankLook), penalty_,
+                                   o_.hpwl_growth_restart_threshold, last_step_, initial_overflow_, move_cap,
+                                   try_momentum ? 1 : 0, worse_streak_ >= 2 ? 1 : 0, has_previous_ ? 1 : 0,
+                                   be_->mappedBank(kBankCandidate) + kDecidedLook};
             pa.b_look = B_.get();
             pa.g_look = Gl_.get();
             pa.raw_look = Rl_.get();
+            be_->timedBegin(kTController);
             be_->launches().propose(pa);
             be_->addReduction(kRedPropose, kBankCandidate, particleBlocks(P_));
             be_->timedEnd(kTController);
@@ -691,7 +689,6 @@ private:
     ClampArgs clamp_{};
     DeviceArray<double2> X_, XP_, B_, Y_, PX_, PG_, PXn_, PGn_, Rc_, Gc_, Rl_, Gl_, CKf_, CKb_;
     DeviceArray<double> precond_;
-    DeviceArray<double> decided_;  // k_decide's first step of an iteration, read by k_propose
     std::vector<uint32_t> order_;  // device particle k is prepared particle order_[k]
     DeviceArray<double2>* pending_ck_ = nullptr;
     double min_bin_ = 0, max_bin_ = 0, bin_height_ = 1;
