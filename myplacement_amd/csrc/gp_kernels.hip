@@ -209,6 +209,8 @@ __device__ inline float groupSum(float v) {
 // One net on a group of W lanes (W = 32 or 64); returns (hpwl, smooth) in lane 0 of the group.
 // Lane `sub` owns pins b + sub + W i.  Nets of up to W * PER pins keep their pins in registers
 // from one load pass (all loads of a lane in flight together); larger nets stream them three times.
+// The gradient pass recomputes the pin exponentials rather than holding them (71 instead of 78
+// VGPRs: 7 waves per SIMD instead of 6 for this gather-latency-bound kernel).
 template <int W, int PER>
 __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_grad, int net, int sub, double inv_gamma,
                           float gamma_f, double& hp, double& wl) {
@@ -230,15 +232,13 @@ __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_gra
             }
         }
         mx = groupMax<W>(mx); nx = groupMin<W>(nx); my = groupMax<W>(my); ny = groupMin<W>(ny);
-        float epx[PER], enx[PER], epy[PER], eny[PER];
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             if (b + sub + W * i < e) {
-                epx[i] = ex32(qx[i] - mx, inv_gamma);
-                enx[i] = ex32(nx - qx[i], inv_gamma);
-                epy[i] = ex32(qy[i] - my, inv_gamma);
-                eny[i] = ex32(ny - qy[i], inv_gamma);
-                spx += epx[i]; snx += enx[i]; spy += epy[i]; sny += eny[i];
+                spx += ex32(qx[i] - mx, inv_gamma);
+                snx += ex32(nx - qx[i], inv_gamma);
+                spy += ex32(qy[i] - my, inv_gamma);
+                sny += ex32(ny - qy[i], inv_gamma);
             }
         }
         spx = groupSum<W>(spx); snx = groupSum<W>(snx); spy = groupSum<W>(spy); sny = groupSum<W>(sny);
@@ -248,7 +248,9 @@ __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_gra
 #pragma unroll
             for (int i = 0; i < PER; ++i) {
                 const int p = b + sub + W * i;
-                if (p < e) pin_grad[p] = make_float2(epx[i] * ipx - enx[i] * inx, epy[i] * ipy - eny[i] * iny);
+                if (p < e)
+                    pin_grad[p] = make_float2(ex32(qx[i] - mx, inv_gamma) * ipx - ex32(nx - qx[i], inv_gamma) * inx,
+                                              ex32(qy[i] - my, inv_gamma) * ipy - ex32(ny - qy[i], inv_gamma) * iny);
             }
         }
     } else {
@@ -325,16 +327,14 @@ __device__ __forceinline__ void wireBody(const WireArgs& a, int large_blocks, in
                     my = fmax(my, q.y); ny = fmin(ny, q.y);
                 }
             }
-            float epx[kSmallDegree], enx[kSmallDegree], epy[kSmallDegree], eny[kSmallDegree];
             float spx = 0.f, snx = 0.f, spy = 0.f, sny = 0.f;
 #pragma unroll
             for (int i = 0; i < kSmallDegree; ++i) {
                 if (i < deg) {
-                    epx[i] = ex32(qx[i] - mx, inv_gamma);
-                    enx[i] = ex32(nx - qx[i], inv_gamma);
-                    epy[i] = ex32(qy[i] - my, inv_gamma);
-                    eny[i] = ex32(ny - qy[i], inv_gamma);
-                    spx += epx[i]; snx += enx[i]; spy += epy[i]; sny += eny[i];
+                    spx += ex32(qx[i] - mx, inv_gamma);
+                    snx += ex32(nx - qx[i], inv_gamma);
+                    spy += ex32(qy[i] - my, inv_gamma);
+                    sny += ex32(ny - qy[i], inv_gamma);
                 }
             }
             const double w = a.net_w[net];
@@ -347,7 +347,9 @@ __device__ __forceinline__ void wireBody(const WireArgs& a, int large_blocks, in
 #pragma unroll
                 for (int i = 0; i < kSmallDegree; ++i)
                     if (i < deg)
-                        pin_grad[b + i * size] = make_float2(epx[i] * ipx - enx[i] * inx, epy[i] * ipy - eny[i] * iny);
+                        pin_grad[b + i * size] =
+                            make_float2(ex32(qx[i] - mx, inv_gamma) * ipx - ex32(nx - qx[i], inv_gamma) * inx,
+                                        ex32(qy[i] - my, inv_gamma) * ipy - ex32(ny - qy[i], inv_gamma) * iny);
             }
         }
     }
