@@ -66,6 +66,12 @@ def oracle():
                                         C.POINTER(C.c_int32), C.POINTER(abi.MpDensityMetrics),
                                         C.POINTER(abi.MpDensityMetrics), C.c_void_p]
         lib.orc_set_method.argtypes = [C.c_int]
+        lib.orc_estimate_step.argtypes = [_dp, _dp, _dp, _dp, C.c_int64, C.c_int, C.c_double]
+        lib.orc_estimate_step.restype = C.c_double
+        lib.orc_smoothing.argtypes = [C.POINTER(abi.MpOptions), C.c_double, C.c_double, C.c_double]
+        lib.orc_smoothing.restype = C.c_double
+        lib.orc_accepts.argtypes = [C.POINTER(abi.MpOptions), C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_double]
         lib.orc_detailed_place.argtypes = [C.POINTER(abi.MpDesign), C.c_void_p, C.POINTER(abi.MpDpOptions), _dp,
                                            C.c_void_p, C.POINTER(abi.MpDpResult)]
         _oracle = lib

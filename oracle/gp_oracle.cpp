@@ -1982,4 +1982,34 @@ int orc_global_place(const mp_design* design, const mp_gp_options* options, doub
     });
 }
 
+// The controller's scalar laws, one at a time, so that tests can hold them to objectives whose
+// answers are known in closed form.  Vectors are interleaved (x, y) pairs.
+double orc_estimate_step(const double* pos, const double* pre, const double* prev_pos, const double* prev_grad,
+                         int64_t n, int has_prev, double fallback) {  // AdaptiveGlobalPlacer.cpp:470-492
+    std::vector<V2> p(static_cast<std::size_t>(n)), pp(p.size()), pg(p.size());
+    Eval e;
+    e.pre.resize(p.size());
+    for (std::size_t i = 0; i < p.size(); ++i) {
+        p[i] = V2{pos[2 * i], pos[2 * i + 1]};
+        e.pre[i] = V2{pre[2 * i], pre[2 * i + 1]};
+        pp[i] = V2{prev_pos[2 * i], prev_pos[2 * i + 1]};
+        pg[i] = V2{prev_grad[2 * i], prev_grad[2 * i + 1]};
+    }
+    return estimateStep(p, e, pp, pg, has_prev != 0, fallback).step;
+}
+
+double orc_smoothing(const mp_gp_options* o, double max_bin, double overflow, double initial) {  // :456-462
+    return smoothingFromOverflow(*o, max_bin, overflow, initial);
+}
+
+int orc_accepts(const mp_gp_options* o, double base_objective, double base_overflow, double cand_objective,
+                double cand_overflow, double predicted) {  // :521-534
+    Eval b, c;
+    b.objective = base_objective;
+    b.s.optimizer_density.normalized_overflow = base_overflow;
+    c.objective = cand_objective;
+    c.s.optimizer_density.normalized_overflow = cand_overflow;
+    return acceptsCandidate(b, c, predicted, *o) ? 1 : 0;
+}
+
 }  // extern "C"

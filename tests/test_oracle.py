@@ -22,7 +22,10 @@ Parity status of each oracle function (oracle/gp_oracle.cpp), and what pins it:
                                      (LAPACK); BiCGSTAB on a random SPD system against numpy;
                                      the reference's own invariants (PlacementFlowTests.cpp:121-129)
                                                                                       -> pinned
-  adaptive controller                no numeric pin exists in the reference; invariants of
+  controller laws                    estimateStep = 1/L on quadratics and within the inverse
+                                     extreme curvatures; Armijo acceptance iff tL <= 2(1-c) on a
+                                     quadratic line search; smoothing end points      -> pinned
+  adaptive trajectory                no numeric pin exists in the reference; invariants of
                                      PlacementFlowTests.cpp:153-216 only              -> parity unpinned
   window-reordering detailed pass    the reference's fixture (DetailedPlacementTests.cpp:55-70);
                                      bit-equal to the reference's own CPU placer (built from
@@ -581,3 +584,87 @@ def test_window_reordering_invalid_options_rejected():
                 dict(improvement_epsilon=float("nan")), dict(method=7)):
         with pytest.raises(ValueError):
             ob.detailed_place(d, abi.default_dp_options(**bad))
+
+
+# ---- controller laws on objectives whose answers are closed forms -------------------------------
+
+def _step(pos, pre, prev_pos, prev_grad, has_prev=True, fallback=0.3):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (pos, pre, prev_pos, prev_grad)]
+    ptrs = [a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]
+    return ob.oracle().orc_estimate_step(*ptrs, len(arrs[0]) // 2, 1 if has_prev else 0, fallback)
+
+
+def test_estimate_step_is_the_inverse_curvature_of_a_quadratic():
+    """estimateStep (AdaptiveGlobalPlacer.cpp:470-492) on f = L/2 |x|^2, whose preconditioned gradient
+    is L x: the secant curvature is exactly L, so the step is 1/L while that lies within [fallback/4,
+    4 fallback], and the nearer bound otherwise.  A dropped square root (1/L^2) or an inverted ratio
+    (L) fails."""
+    rng = np.random.default_rng(5)
+    x, xp = rng.normal(size=(50, 2)), rng.normal(size=(50, 2))
+    for L in (2.5, 1.0, 3.7):
+        assert _step(x, L * x, xp, L * xp) == pytest.approx(1.0 / L, rel=1e-12)
+    assert _step(x, 100.0 * x, xp, 100.0 * xp, fallback=0.3) == pytest.approx(0.075, rel=1e-15)  # clamped below
+    assert _step(x, 0.01 * x, xp, 0.01 * xp, fallback=0.3) == pytest.approx(1.2, rel=1e-15)      # clamped above
+    # no previous state, or no movement since it: the fallback
+    assert _step(x, 2.0 * x, xp, 2.0 * xp, has_prev=False) == 0.3
+    assert _step(x, 2.0 * x, x, 3.0 * x) == 0.3
+
+
+def test_estimate_step_lies_between_the_inverse_extreme_curvatures():
+    """On f = 1/2 x^T H x with H = diag(a, b) per particle the secant step is sqrt(|dx|^2 / |H dx|^2):
+    exactly 1/a (1/b) for motion along x (y) alone, and between 1/b and 1/a for any other motion
+    (a Rayleigh-quotient bound)."""
+    a, b = 1.0, 4.0
+    H = np.array([a, b])
+    rng = np.random.default_rng(6)
+    xp = rng.normal(size=(40, 2))
+    along_x = xp + np.column_stack([rng.normal(size=40), np.zeros(40)])
+    along_y = xp + np.column_stack([np.zeros(40), rng.normal(size=40)])
+    assert _step(along_x, along_x * H, xp, xp * H) == pytest.approx(1.0 / a, rel=1e-12)
+    assert _step(along_y, along_y * H, xp, xp * H) == pytest.approx(1.0 / b, rel=1e-12)
+    for _ in range(5):
+        x = xp + rng.normal(size=(40, 2))
+        s = _step(x, x * H, xp, xp * H)
+        assert 1.0 / b < s < 1.0 / a
+
+
+def test_smoothing_anneals_between_its_end_points():
+    """smoothingFromOverflow (:456-462): the initial smoothing (in bins) at or above the initial
+    overflow, the final one at zero overflow, and the overflow ratio raised to the exponent between."""
+    lib = ob.oracle()
+    o = abi.default_options(initial_smoothing=0.75, final_smoothing=0.12, smoothing_overflow_exponent=1.0)
+    mb = 3.0
+    assert lib.orc_smoothing(C.byref(o), mb, 0.4, 0.4) == pytest.approx(0.75 * mb, rel=1e-15)
+    assert lib.orc_smoothing(C.byref(o), mb, 0.9, 0.4) == pytest.approx(0.75 * mb, rel=1e-15)
+    assert lib.orc_smoothing(C.byref(o), mb, 0.0, 0.4) == pytest.approx(0.12 * mb, rel=1e-15)
+    assert lib.orc_smoothing(C.byref(o), mb, 0.2, 0.4) == pytest.approx((0.12 + 0.75) / 2 * mb, rel=1e-14)
+    o.smoothing_overflow_exponent = 2.0
+    assert lib.orc_smoothing(C.byref(o), mb, 0.2, 0.4) == pytest.approx((0.12 + 0.63 / 4) * mb, rel=1e-14)
+    values = [lib.orc_smoothing(C.byref(o), mb, v, 0.4) for v in np.linspace(0.0, 0.4, 9)]
+    assert all(u < w for u, w in zip(values, values[1:]))
+
+
+def test_armijo_acceptance_on_a_quadratic_line_search():
+    """acceptsCandidate (:521-534) on f(x) = L/2 x^2 from x = 1 with the gradient step y = 1 - tL:
+    the predicted decrease is g (x - y) = t L^2, and Armijo's condition f(y) <= f(x) - c t L^2 holds
+    exactly when tL <= 2 (1 - c).  With the overflow worse, the stability and density-trade rules stay
+    out; with the overflow unchanged, any candidate no worse than the base is accepted (stability);
+    with the overflow lower by more than the stall gain, a bounded objective increase is accepted."""
+    lib = ob.oracle()
+    c, L = 0.25, 1.0
+    o = abi.default_options(armijo_coefficient=c, density_stall_threshold=0.002, objective_increase_for_density=0.1)
+    base = 0.5 * L
+
+    def accepts(tL, cand_overflow):
+        t = tL / L
+        return lib.orc_accepts(C.byref(o), base, 0.3, 0.5 * L * (1.0 - tL) ** 2, cand_overflow, t * L * L)
+
+    limit = 2.0 * (1.0 - c)
+    for tL in (0.2, 1.0, limit - 0.01):
+        assert accepts(tL, 0.4) == 1
+    for tL in (limit + 0.01, 1.9):
+        assert accepts(tL, 0.4) == 0
+    assert accepts(1.9, 0.3) == 1    # f(y) = 0.405 <= 0.5 and the overflow did not grow
+    assert accepts(2.04, 0.3) == 0   # f(y) = 0.5408 > 0.5
+    assert accepts(2.04, 0.29) == 1  # overflow down by 0.01 > the stall gain, objective within 10 %
+    assert accepts(2.1, 0.29) == 0   # f(y) = 0.605: beyond the 10 % allowance (0.55)
