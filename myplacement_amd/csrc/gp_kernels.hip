@@ -578,40 +578,95 @@ __device__ __forceinline__ void dbinsBody(const DensityArgs& a, const Blk& bk) {
     const size_t i0 = (static_cast<size_t>(eb.y) * kThreads + threadIdx.x) * kBinsPerThread;
     unsigned long long gm[kBinsPerThread], gf[kBinsPerThread];
     double sc[kBinsPerThread];
+    float* rho = a.rho[eb.x];
+    // A thread's bins share one row and every array is 16-byte aligned at them when the row width,
+    // the transform's leading dimension and the bases allow: then 16-byte loads and stores
+    // (uniform test; the per-bin path below otherwise).
+    const bool wide = (nx % kBinsPerThread) == 0 && (a.ldrho % 4) == 0 &&
+                      ((reinterpret_cast<uintptr_t>(grid) | reinterpret_cast<uintptr_t>(a.static_charge) |
+                        reinterpret_cast<uintptr_t>(rho)) & 15) == 0;
+    static_assert(kBinsPerThread == 4, "the 16-byte path moves four bins");
+    if (wide) {  // uniform: every thread takes the same path to the one reduction below
+        if (i0 < G) {
+            const ulonglong2* m2 = reinterpret_cast<const ulonglong2*>(grid + i0);
+            const ulonglong2* f2 = reinterpret_cast<const ulonglong2*>(grid + G + i0);
+            const double2* s2 = reinterpret_cast<const double2*>(a.static_charge + i0);
+            const ulonglong2 m01 = m2[0], m23 = m2[1], f01 = f2[0], f23 = f2[1];
+            const double2 s01 = s2[0], s23 = s2[1];
+            gm[0] = m01.x; gm[1] = m01.y; gm[2] = m23.x; gm[3] = m23.y;
+            gf[0] = f01.x; gf[1] = f01.y; gf[2] = f23.x; gf[3] = f23.y;
+            sc[0] = s01.x; sc[1] = s01.y; sc[2] = s23.x; sc[3] = s23.y;
+            ulonglong2* z = reinterpret_cast<ulonglong2*>(grid + i0);
+            ulonglong2* zf = reinterpret_cast<ulonglong2*>(grid + G + i0);
+            z[0] = make_ulonglong2(0ULL, 0ULL);
+            z[1] = make_ulonglong2(0ULL, 0ULL);
+            zf[0] = make_ulonglong2(0ULL, 0ULL);
+            zf[1] = make_ulonglong2(0ULL, 0ULL);
+            const int row = static_cast<int>(i0 / nx), col0 = static_cast<int>(i0 - static_cast<size_t>(row) * nx);
+            const double bottom = r.lly + static_cast<double>(row) * r.bh;
+            const double height = fmax(0.0, (bottom + r.bh) - bottom);
+            float out[kBinsPerThread];
 #pragma unroll
-    for (int q = 0; q < kBinsPerThread; ++q) {
-        const size_t i = i0 + q;
-        gm[q] = i < G ? grid[i] : 0ULL;
-        gf[q] = i < G ? grid[G + i] : 0ULL;
-        sc[q] = i < G ? a.static_charge[i] : 0.0;
-    }
+            for (int q = 0; q < kBinsPerThread; ++q) {
+                const double left = r.llx + static_cast<double>(col0 + q) * r.bw;
+                const double area = fmax(0.0, (left + r.bw) - left) * height;
+                const double mov = static_cast<double>(gm[q]) * a.inv_fix;
+                const double fil = static_cast<double>(gf[q]) * a.inv_fix;
+                const double q_wo = mov + sc[q];
+                const double q_w = q_wo + a.t * fil;
+                const double cap = a.t * area;
+                const double inv_area = 1.0 / fmax(area, kEps);
+                const double d_w = q_w * inv_area, d_wo = q_wo * inv_area;
+                v[0] += fmax(0.0, q_w - cap);
+                v[1] += fmax(0.0, q_wo - cap);
+                v[2] += q_w;
+                v[3] += q_wo;
+                v[4] += mov;
+                v[5] += d_w;
+                v[6] += d_wo;
+                v[7] = fmax(v[7], d_w);
+                v[8] = fmax(v[8], d_wo);
+                out[q] = static_cast<float>(d_w - a.t);
+            }
+            *reinterpret_cast<float4*>(rho + static_cast<size_t>(row) * a.ldrho + col0) =
+                make_float4(out[0], out[1], out[2], out[3]);
+        }
+    } else {
 #pragma unroll
-    for (int q = 0; q < kBinsPerThread; ++q) {
-        const size_t i = i0 + q;
-        if (i >= G) break;
-        const int row = static_cast<int>(i / nx), col = static_cast<int>(i - static_cast<size_t>(row) * nx);
-        const double left = r.llx + static_cast<double>(col) * r.bw;
-        const double bottom = r.lly + static_cast<double>(row) * r.bh;
-        const double area = fmax(0.0, (left + r.bw) - left) * fmax(0.0, (bottom + r.bh) - bottom);
-        const double mov = static_cast<double>(gm[q]) * a.inv_fix;
-        const double fil = static_cast<double>(gf[q]) * a.inv_fix;
-        grid[i] = 0ULL;
-        grid[G + i] = 0ULL;
-        const double q_wo = mov + sc[q];
-        const double q_w = q_wo + a.t * fil;
-        const double cap = a.t * area;
-        const double inv_area = 1.0 / fmax(area, kEps);
-        const double d_w = q_w * inv_area, d_wo = q_wo * inv_area;
-        v[0] += fmax(0.0, q_w - cap);
-        v[1] += fmax(0.0, q_wo - cap);
-        v[2] += q_w;
-        v[3] += q_wo;
-        v[4] += mov;
-        v[5] += d_w;
-        v[6] += d_wo;
-        v[7] = fmax(v[7], d_w);
-        v[8] = fmax(v[8], d_wo);
-        a.rho[eb.x][static_cast<size_t>(row) * a.ldrho + col] = static_cast<float>(d_w - a.t);
+        for (int q = 0; q < kBinsPerThread; ++q) {
+            const size_t i = i0 + q;
+            gm[q] = i < G ? grid[i] : 0ULL;
+            gf[q] = i < G ? grid[G + i] : 0ULL;
+            sc[q] = i < G ? a.static_charge[i] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kBinsPerThread; ++q) {
+            const size_t i = i0 + q;
+            if (i >= G) break;
+            const int row = static_cast<int>(i / nx), col = static_cast<int>(i - static_cast<size_t>(row) * nx);
+            const double left = r.llx + static_cast<double>(col) * r.bw;
+            const double bottom = r.lly + static_cast<double>(row) * r.bh;
+            const double area = fmax(0.0, (left + r.bw) - left) * fmax(0.0, (bottom + r.bh) - bottom);
+            const double mov = static_cast<double>(gm[q]) * a.inv_fix;
+            const double fil = static_cast<double>(gf[q]) * a.inv_fix;
+            grid[i] = 0ULL;
+            grid[G + i] = 0ULL;
+            const double q_wo = mov + sc[q];
+            const double q_w = q_wo + a.t * fil;
+            const double cap = a.t * area;
+            const double inv_area = 1.0 / fmax(area, kEps);
+            const double d_w = q_w * inv_area, d_wo = q_wo * inv_area;
+            v[0] += fmax(0.0, q_w - cap);
+            v[1] += fmax(0.0, q_wo - cap);
+            v[2] += q_w;
+            v[3] += q_wo;
+            v[4] += mov;
+            v[5] += d_w;
+            v[6] += d_wo;
+            v[7] = fmax(v[7], d_w);
+            v[8] = fmax(v[8], d_wo);
+            rho[static_cast<size_t>(row) * a.ldrho + col] = static_cast<float>(d_w - a.t);
+        }
     }
     publishPartials<9>(v, (1U << 7) | (1U << 8), eb.y, a.partials[eb.x], lds);
 }
