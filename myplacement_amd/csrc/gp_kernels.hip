@@ -1078,26 +1078,27 @@ __device__ __forceinline__ void gatherBody(const GatherArgs& a, const Blk& bk) {
     if (p < a.n) {
         const double2 c = a.pos[e][p];
         const double q = a.charge[p];
-        const double2 f = sampleField(a.field[e], a.ldf, c.x, c.y, a.r);
-        const double2 dg = make_double2(-q * f.x, -q * f.y);
-        // the particle's pins, four index and gradient loads in flight at a time
+        // the particle's pins first (the longest chain of dependent loads), eight index and gradient
+        // loads in flight at a time; the field sample after them
         double2 wg = make_double2(0.0, 0.0);
         const int jb = a.ppin_off[p], je = a.ppin_off[p + 1];
         const float2* pg = a.pin_grad[e];
-        for (int j0 = jb; j0 < je; j0 += 4) {
-            int q[4];
+        for (int j0 = jb; j0 < je; j0 += 8) {
+            int q[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) q[u] = j0 + u < je ? a.ppin_idx[j0 + u] : -1;
-            float2 g[4];
+            for (int u = 0; u < 8; ++u) q[u] = j0 + u < je ? a.ppin_idx[j0 + u] : -1;
+            float2 g[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) g[u] = q[u] >= 0 ? pg[q[u]] : make_float2(0.f, 0.f);
+            for (int u = 0; u < 8; ++u) g[u] = q[u] >= 0 ? pg[q[u]] : make_float2(0.f, 0.f);
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < 8; ++u)
                 if (q[u] >= 0) {
                     wg.x += static_cast<double>(g[u].x);
                     wg.y += static_cast<double>(g[u].y);
                 }
         }
+        const double2 f = sampleField(a.field[e], a.ldf, c.x, c.y, a.r);
+        const double2 dg = make_double2(-q * f.x, -q * f.y);
         v[0] = fabs(wg.x) + fabs(wg.y);
         v[1] = fabs(dg.x) + fabs(dg.y);
         if (a.wire_out) {
