@@ -181,8 +181,8 @@ __device__ inline double2 pinPosition(const double2* pos, const int* pin_part, c
 
 // ---------------------------------------------------------------------------------------------
 // Wirelength.  Nets arrive ordered by degree class (the host permutes them once): nets of degree
-// 2..8 one thread each with their pins held in registers; 9..32 one half-wavefront (32 lanes)
-// each; larger nets one wavefront each.  Max/min and positions in FP64; exponentials, their sums
+// 2..8 one thread each with their pins held in registers; 9..32 a quarter-wavefront (16 lanes,
+// two pins per lane) each; larger nets one wavefront each.  Max/min and positions in FP64; exponentials, their sums
 // and the log in FP32; per-net HPWL and smooth WL accumulated in FP64.
 
 __device__ inline float ex32(double d, double inv_gamma) { return __expf(static_cast<float>(d * inv_gamma)); }
@@ -286,8 +286,8 @@ __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_gra
 }
 
 // Workgroups in the order large nets (one wavefront each, largest first), medium nets (a
-// half-wavefront each), small nets (a thread each): the longest-running workgroups are dispatched
-// first.
+// quarter-wavefront each), small nets (a thread each): the longest-running workgroups are
+// dispatched first.
 constexpr int kLargePins = 8;  // pins per lane held in registers for a large net (512 per net)
 __device__ __forceinline__ void wireBody(const WireArgs& a, int large_blocks, int medium_blocks, const Blk& bk) {
     __shared__ double lds[(kWaves + 1) * 2];
@@ -304,8 +304,8 @@ __device__ __forceinline__ void wireBody(const WireArgs& a, int large_blocks, in
             wireGroup<64, kLargePins>(a, pos, pin_grad, net, threadIdx.x & 63, inv_gamma, gamma_f, hp, wl);
     } else if (block < large_blocks + medium_blocks) {
         const int lane = threadIdx.x & 63;
-        const int net = a.n_small + (block - large_blocks) * (2 * kWaves) + 2 * (threadIdx.x >> 6) + (lane >> 5);
-        if (net < a.n_small + a.n_medium) wireGroup<32, 1>(a, pos, pin_grad, net, lane & 31, inv_gamma, gamma_f, hp, wl);
+        const int net = a.n_small + (block - large_blocks) * (4 * kWaves) + 4 * (threadIdx.x >> 6) + (lane >> 4);
+        if (net < a.n_small + a.n_medium) wireGroup<16, 2>(a, pos, pin_grad, net, lane & 15, inv_gamma, gamma_f, hp, wl);
     } else {
         const int net = (block - large_blocks - medium_blocks) * kThreads + threadIdx.x;
         if (net < a.n_small) {
@@ -1420,12 +1420,12 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize_entries(const Red
 }  // namespace
 
 int wireBlocks(int n_small, int n_medium, int n_large) {
-    return blocksFor(n_small) + (n_medium + 2 * kWaves - 1) / (2 * kWaves) + (n_large + kWaves - 1) / kWaves;
+    return blocksFor(n_small) + (n_medium + 4 * kWaves - 1) / (4 * kWaves) + (n_large + kWaves - 1) / kWaves;
 }
 
 void launchWire(const WireArgs& a, int count, hipStream_t s) {
     const int large_blocks = (a.n_large + kWaves - 1) / kWaves;
-    const int medium_blocks = (a.n_medium + 2 * kWaves - 1) / (2 * kWaves);
+    const int medium_blocks = (a.n_medium + 4 * kWaves - 1) / (4 * kWaves);
     const int blocks = wireBlocks(a.n_small, a.n_medium, a.n_large);
     if (blocks == 0) return;
     if (count < 1 || count > kMaxEval) throw std::invalid_argument("bad evaluation batch");
@@ -1571,7 +1571,7 @@ void launchZero(double* p, size_t count, hipStream_t s) {
 
 // ---- entries and launches batched over jobs ---------------------------------------------------------
 WireEntry wireEntry(const WireArgs& a, int count) {
-    WireEntry e{a, (a.n_large + kWaves - 1) / kWaves, (a.n_medium + 2 * kWaves - 1) / (2 * kWaves),
+    WireEntry e{a, (a.n_large + kWaves - 1) / kWaves, (a.n_medium + 4 * kWaves - 1) / (4 * kWaves),
                 wireBlocks(a.n_small, a.n_medium, a.n_large), count};
     if (a.small_net[2] != 0 || a.small_net[kSmallDegree + 1] != a.n_small)
         throw std::invalid_argument("small-net degree groups do not cover the small nets");
