@@ -114,18 +114,14 @@ def serving_sample(jobs: int, rate: float, slots: int, seed: int = 2026, mode: s
     import threading
     from myplacement_amd import abi, design
     from myplacement_amd import backend as hb
-    rng = np.random.default_rng(seed)
+    specs, gaps = design.served_stream_specs(jobs, rate, seed)
     work = []
-    for k in range(jobs):
-        cells = int(round(math.exp(rng.uniform(math.log(1_000), math.log(200_000)))))
-        t = float(rng.uniform(0.7, 0.9))
-        side = design.served_bins(cells)
-        d = design.served_design(seed + 1 + k, cells)
-        d.center = hb.random_placement(d, seed + 1 + k)
+    for job_seed, cells, side, t in specs:
+        d = design.served_design(job_seed, cells)
+        d.center = hb.random_placement(d, job_seed)
         o = abi.default_options(target_density=t, bins_x=side, bins_y=side, iterations=280,
-                                feasible_refinement_iterations=1, seed=seed + 1 + k)
+                                feasible_refinement_iterations=1, seed=job_seed)
         work.append((d, o, cells, side))
-    gaps = rng.exponential(1.0 / rate, size=jobs)
     server = hb.PlacementServer(max_active_jobs=slots, memory_budget_bytes=200 << 30, mode=mode)
     reports = [None] * jobs
     threads = []
@@ -208,15 +204,12 @@ def batch_sample(jobs: int, runs_spec, iterations: int, seed: int = 2026) -> dic
     total of the jobs' GP iterations over the makespan."""
     from myplacement_amd import abi, design
     from myplacement_amd import backend as hb
-    rng = np.random.default_rng(seed)
     work = []
-    for k in range(jobs):
-        cells = int(round(math.exp(rng.uniform(math.log(500), math.log(50_000)))))
-        side = design.served_bins(cells)
-        d = design.served_design(seed + 1 + k, cells)
-        d.center = hb.random_placement(d, seed + 1 + k)
+    for job_seed, cells, side in design.batch_job_specs(jobs, seed):
+        d = design.served_design(job_seed, cells)
+        d.center = hb.random_placement(d, job_seed)
         o = abi.default_options(target_density=0.8, bins_x=side, bins_y=side, iterations=iterations,
-                                feasible_refinement_iterations=2 ** 31 - 1, seed=seed + 1 + k)
+                                feasible_refinement_iterations=2 ** 31 - 1, seed=job_seed)
         work.append((d, o, cells, side))
     log(f"batch: {jobs} blocks generated, {sum(w[2] for w in work)} cells in total")
     runs = {}

@@ -36,6 +36,9 @@ def hip_flags() -> list:
         f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-munsafe-fp-atomics",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-Wall", "-Wno-unused-result", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}",
+        # the host replays the device's step decision bit for bit: no fused multiply-adds on the
+        # host side whatever the target (device code sets contraction per kernel)
+        "-Xarch_host", "-ffp-contract=off",
     ]
 
 

@@ -206,7 +206,7 @@ __device__ inline float groupSum(float v) {
     return v;
 }
 
-// One net on a group of W lanes (W = 32 or 64); returns (hpwl, smooth) in lane 0 of the group.
+// One net on a group of W lanes (W = 16 or 64); returns (hpwl, smooth) in lane 0 of the group.
 // Lane `sub` owns pins b + sub + W i.  Nets of up to W * PER pins keep their pins in registers
 // from one load pass (all loads of a lane in flight together); larger nets stream them three times.
 // The gradient pass recomputes the pin exponentials rather than holding them (71 instead of 78
@@ -1084,15 +1084,15 @@ __device__ __forceinline__ void gatherBody(const GatherArgs& a, const Blk& bk) {
         const int jb = a.ppin_off[p], je = a.ppin_off[p + 1];
         const float2* pg = a.pin_grad[e];
         for (int j0 = jb; j0 < je; j0 += 8) {
-            int q[8];
+            int idx[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) q[u] = j0 + u < je ? a.ppin_idx[j0 + u] : -1;
+            for (int u = 0; u < 8; ++u) idx[u] = j0 + u < je ? a.ppin_idx[j0 + u] : -1;
             float2 g[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) g[u] = q[u] >= 0 ? pg[q[u]] : make_float2(0.f, 0.f);
+            for (int u = 0; u < 8; ++u) g[u] = idx[u] >= 0 ? pg[idx[u]] : make_float2(0.f, 0.f);
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-                if (q[u] >= 0) {
+                if (idx[u] >= 0) {
                     wg.x += static_cast<double>(g[u].x);
                     wg.y += static_cast<double>(g[u].y);
                 }

@@ -441,3 +441,106 @@ def served_design(seed: int, cells: int) -> Design:
     d = large_design(seed=seed, cells=cells, nets=int(round(1.05 * cells)), row_count=side, sites=side,
                      macros=macros)
     return d
+
+
+def batch_job_specs(count: int, seed: int = 2026) -> list:
+    """The batching configuration's blocks: for block k, (job seed, cells, bin side) with cells
+    log-uniform in 500-50k drawn from default_rng(seed) in block order, job seed = seed + 1 + k, and
+    the bin side of served_bins.  Each block is served_design(job seed, cells) from
+    a random initial placement with that seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        cells = int(round(math.exp(rng.uniform(math.log(500), math.log(50_000)))))
+        out.append((seed + 1 + k, cells, served_bins(cells)))
+    return out
+
+
+def served_stream_specs(count: int, rate: float, seed: int = 2026) -> tuple:
+    """The served stream's requests: for request k, (job seed, cells, bin side, target density) with
+    cells log-uniform in 1k-200k and the target density uniform in 0.7-0.9, drawn in that order
+    from default_rng(seed), job seed = seed + 1 + k; then the `count` Poisson inter-arrival gaps
+    (exponential, mean 1 / rate) from the same generator.  Returns (specs, gaps)."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(count):
+        cells = int(round(math.exp(rng.uniform(math.log(1_000), math.log(200_000)))))
+        t = float(rng.uniform(0.7, 0.9))
+        specs.append((seed + 1 + k, cells, served_bins(cells), t))
+    gaps = rng.exponential(1.0 / rate, size=count)
+    return specs, gaps
+
+
+def _lower_left(c: float, half: float) -> float:
+    """A lower-left coordinate ll with ll + half == c in FP64 (the reader forms the centre as
+    lower-left + size / 2, PlacementDatabase.cpp Module::setLowerLeft), when one exists: c - half,
+    else its nearest neighbours."""
+    ll = c - half
+    if ll + half == c:
+        return ll
+    up, down = ll, ll
+    for _ in range(3):
+        up = float(np.nextafter(up, np.inf))
+        down = float(np.nextafter(down, -np.inf))
+        if up + half == c:
+            return up
+        if down + half == c:
+            return down
+    return ll
+
+
+def write_bookshelf(d: Design, directory: str, name: str) -> str:
+    """Writes `d` as a BookShelf design (.aux/.nodes/.nets/.wts/.pl/.scl) that
+    BookshelfParser::parseAux (BookshelfParser.cpp:62-348) and read_bookshelf read back into the same
+    arrays.  Numbers are written with Python's shortest round-trip repr, so every size, offset,
+    weight and row field comes back bit for bit; centres are written as lower-left corners chosen so
+    that lower-left + size / 2 gives the centre back exactly when such a corner exists.  Modules and
+    nets without names are called o<k> / n<k>.  Terminals are marked `terminal` in .nodes; other
+    fixed modules `/FIXED` in .pl.  Returns the .aux path."""
+    os.makedirs(directory, exist_ok=True)
+    M, N = d.module_count, d.net_count
+    names = d.names if d.names is not None else [f"o{k}" for k in range(M)]
+    nets = d.net_names if d.net_names is not None else [f"n{k}" for k in range(N)]
+    r = repr
+    base = os.path.join(directory, name)
+    with open(base + ".aux", "w") as f:
+        f.write(f"RowBasedPlacement : {name}.nodes {name}.nets {name}.wts {name}.pl {name}.scl\n")
+    with open(base + ".nodes", "w") as f:
+        f.write("UCLA nodes 1.0\n\n")
+        f.write(f"NumNodes : {M}\nNumTerminals : {int(d.is_terminal.sum())}\n")
+        for k in range(M):
+            f.write(f"{names[k]} {r(float(d.width[k]))} {r(float(d.height[k]))}"
+                    + (" terminal\n" if d.is_terminal[k] else "\n"))
+    with open(base + ".pl", "w") as f:
+        f.write("UCLA pl 1.0\n\n")
+        for k in range(M):
+            o = int(d.orientation[k])
+            ot = ORIENTATIONS[o] if 0 <= o < len(ORIENTATIONS) else "N"
+            x = _lower_left(float(d.center[k, 0]), float(d.width[k]) * 0.5)
+            y = _lower_left(float(d.center[k, 1]), float(d.height[k]) * 0.5)
+            fixed = " /FIXED" if (d.is_fixed[k] and not d.is_terminal[k]) else ""
+            f.write(f"{names[k]} {r(x)} {r(y)} : {ot}{fixed}\n")
+    with open(base + ".nets", "w") as f:
+        f.write("UCLA nets 1.0\n\n")
+        f.write(f"NumNets : {N}\nNumPins : {d.pin_count}\n")
+        for n in range(N):
+            a, b = int(d.net_pin_start[n]), int(d.net_pin_start[n + 1])
+            f.write(f"NetDegree : {b - a} {nets[n]}\n")
+            for p in range(a, b):
+                f.write(f"  {names[int(d.pin_module[p])]} I : {r(float(d.pin_offset[p, 0]))} "
+                        f"{r(float(d.pin_offset[p, 1]))}\n")
+    with open(base + ".wts", "w") as f:
+        f.write("UCLA wts 1.0\n\n")
+        for n in range(N):
+            f.write(f"{nets[n]} {r(float(d.net_weight[n]))}\n")
+    with open(base + ".scl", "w") as f:
+        f.write("UCLA scl 1.0\n\n")
+        f.write(f"NumRows : {d.rows.shape[0]}\n\n")
+        for bottom, height, x0, spacing, count in d.rows:
+            f.write("CoreRow Horizontal\n")
+            f.write(f"  Coordinate : {r(float(bottom))}\n  Height : {r(float(height))}\n")
+            f.write(f"  Sitewidth : {r(float(spacing))}\n  Sitespacing : {r(float(spacing))}\n")
+            f.write("  Siteorient : N\n  Sitesymmetry : Y\n")
+            f.write(f"  SubrowOrigin : {r(float(x0))} NumSites : {int(count)}\n")
+            f.write("End\n")
+    return base + ".aux"

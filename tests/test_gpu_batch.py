@@ -103,3 +103,25 @@ def test_service_batched_mode_is_bit_equal_to_single_runs():
         _same_as_alone(d, o, c, rep["result"])
         # the job's arrays were carved from one block sized by the footprint estimate
         assert rep["estimated_device_bytes"] >= rep["result"]["reserved_device_bytes"]
+
+
+def test_configured_256_block_batch_is_bit_equal_to_single_runs():
+    """The batching configuration at its size: 256 blocks of 500-50k cells (design.batch_job_specs,
+    the bench's blocks), 100 iterations each, every other 64 x 64 block with split-BF16 products,
+    all in one batched placement; each block must equal its own run bit for bit."""
+    specs = design.batch_job_specs(256)
+    jobs = []
+    for k, (seed, cells, side) in enumerate(specs):
+        d = design.served_design(seed, cells)
+        d.center = hb.random_placement(d, seed)
+        prec = abi.MP_DCT_BF16X3 if (side % 64 == 0 and k % 2 == 1) else abi.MP_DCT_FP32
+        o = abi.default_options(target_density=0.8, bins_x=side, bins_y=side, iterations=100,
+                                feasible_refinement_iterations=2 ** 31 - 1, seed=seed, dct_precision=prec)
+        jobs.append((d, o))
+    say(f"[gpu] configured batch: {len(jobs)} blocks, {sum(c for _, c, _ in specs)} cells, "
+        f"grids {sorted(set(s for _, _, s in specs))}")
+    out = hb.global_place_batch([d for d, _ in jobs], [o for _, o in jobs])
+    say("  batch done; comparing with single runs")
+    for (d, o), (c, r) in zip(jobs, out):
+        assert r["completed_iterations"] == 100
+        _same_as_alone(d, o, c, r)

@@ -3,9 +3,9 @@
 Tolerances (stated here, checked below):
   HPWL, density metrics        relative 1e-10 (FP64 on both sides; only summation order differs)
   smooth wirelength            relative 2e-6  (exp / log in FP32 per pin, sums per net in FP32)
-  electrostatic energy         relative 2e-4  (FP32 DCT products on MFMA; energy taken in the DCT basis)
+  electrostatic energy         relative 2e-5  (FP32 DCT products on MFMA; energy taken in the DCT basis)
   wire gradient                normwise relative 2e-5, elementwise 1e-4 of the largest entry
-  density gradient / field     normwise relative 5e-4
+  density gradient / field     normwise relative 1e-5 (FP32 products: measured ~1e-6)
   end-to-end adaptive run      first iterations row by row at 1e-5; final HPWL within 0.5 % and final
                                design overflow within 0.005 as sixteen-run ensemble means
 """
@@ -44,9 +44,31 @@ def nrel(a, b):
 
 
 def _setup(d, t, bins, seed=2026):
+    bx, by = bins if isinstance(bins, tuple) else (bins, bins)
     d.center = hb.random_placement(d, seed)
-    o = abi.default_options(target_density=t, bins_x=bins, bins_y=bins)
+    o = abi.default_options(target_density=t, bins_x=bx, bins_y=by)
     return d, o
+
+
+def with_wide_nets(d, degrees, seed=5):
+    """`d` with extra unit-weight nets of the given degrees appended, pins on distinct modules drawn
+    uniformly, offsets uniform within the module: nets at the edges of the wirelength kernel's
+    degree classes (8 / 9, 32 / 33 pins), at its register limit (512 / 513 pins) and far beyond it."""
+    rng = np.random.default_rng(seed)
+    M = d.module_count
+    mods = [rng.choice(M, size=k, replace=False) for k in degrees]
+    pm = np.concatenate([d.pin_module] + [m.astype(np.int32) for m in mods])
+    extra = np.concatenate(mods)
+    off = np.stack([(rng.random(extra.shape[0]) - 0.5) * d.width[extra],
+                    (rng.random(extra.shape[0]) - 0.5) * d.height[extra]], axis=1)
+    start = np.concatenate([d.net_pin_start, d.net_pin_start[-1] + np.cumsum(degrees)])
+    return design.Design(width=d.width, height=d.height, center=d.center, orientation=d.orientation,
+                         is_terminal=d.is_terminal, is_fixed=d.is_fixed,
+                         net_weight=np.concatenate([d.net_weight, np.ones(len(degrees))]), net_pin_start=start,
+                         pin_module=pm, pin_offset=np.concatenate([d.pin_offset, off]), rows=d.rows)
+
+
+WIDE_DEGREES = [8, 9, 16, 17, 31, 32, 32, 33, 64, 65, 511, 512, 513, 1024, 2000]
 
 
 def _cases():
@@ -56,6 +78,9 @@ def _cases():
         ("mixed-8x8", lambda: (design.read_bookshelf(os.path.join(fx, "mixed.aux")), 0.85, 8)),
         ("plain-64", lambda: (design.plain_design(), 0.9, 64)),
         ("plain-48", lambda: (design.plain_design(seed=7), 0.9, 48)),
+        ("plain-50x30", lambda: (design.plain_design(seed=8), 0.9, (50, 30))),
+        ("plain-30x52", lambda: (design.plain_design(seed=9), 0.9, (30, 52))),
+        ("wide-nets-64", lambda: (with_wide_nets(design.plain_design(seed=10), WIDE_DEGREES), 0.9, 64)),
         ("mid-128", lambda: (design.large_design(seed=3, cells=20000, nets=21000, row_count=200, sites=400,
                                                  macros=6), 0.8, 128)),
     ]
@@ -65,39 +90,45 @@ def _cases():
 def test_evaluate_matches_oracle(ob, name, make):
     d, t, bins = make()
     d, o = _setup(d, t, bins)
+    bx, by = o.bins_x, o.bins_y
     orc = ob.OracleSession(d, o)
     L = orc.layout()
-    say(f"[gpu] evaluate parity {name}: P={orc.particle_count} bins={bins}")
+    say(f"[gpu] evaluate parity {name}: P={orc.particle_count} bins={bx}x{by}")
     be = hb.HipPlacementBackend.create(L)
     xy0 = orc.arrays()["positions"]
     rng = np.random.default_rng(1)
     region = L["region"]
-    bw = (region[2] - region[0]) / bins
+    bw = (region[2] - region[0]) / bx
     for trial, gamma_bins in enumerate((0.75, 0.3, 0.12)):
         xy = xy0 + rng.normal(0.0, 2.0 * bw, size=xy0.shape) * (trial > 0)
-        gamma = gamma_bins * max(bw, (region[3] - region[1]) / bins)
+        gamma = gamma_bins * max(bw, (region[3] - region[1]) / by)
         for grad in (True, False):
             r = orc.evaluate(xy, gamma, grad)
             h = be.evaluate(xy, gamma, grad)
             assert rel(h.hpwl, r["hpwl"]) < 1e-10
             assert rel(h.smooth_wirelength, r["smooth_wirelength"]) < 2e-6
-            assert rel(h.electrostatic_energy, r["electrostatic_energy"]) < 2e-4, (h.electrostatic_energy, r["electrostatic_energy"])
+            say(f"  gamma {gamma_bins} grad={grad}: smooth {rel(h.smooth_wirelength, r['smooth_wirelength']):.1e} "
+                f"energy {rel(h.electrostatic_energy, r['electrostatic_energy']):.1e}")
+            assert rel(h.electrostatic_energy, r["electrostatic_energy"]) < 2e-5, (h.electrostatic_energy, r["electrostatic_energy"])
             for key in ("optimizer_density", "design_density"):
                 for m, v in r[key].items():
                     assert abs(h.__dict__[key][m] - v) <= 1e-10 * max(1.0, abs(v)), (key, m, h.__dict__[key][m], v)
             if grad:
+                say(f"    wire gradient {nrel(h.wire_gradient, r['wire_gradient']):.1e} "
+                    f"density gradient {nrel(h.density_gradient, r['density_gradient']):.1e}")
                 assert nrel(h.wire_gradient, r["wire_gradient"]) < 2e-5
                 assert np.abs(h.wire_gradient - r["wire_gradient"]).max() <= 1e-4 * np.abs(r["wire_gradient"]).max()
-                assert nrel(h.density_gradient, r["density_gradient"]) < 5e-4, nrel(h.density_gradient, r["density_gradient"])
+                assert nrel(h.density_gradient, r["density_gradient"]) < 1e-5, nrel(h.density_gradient, r["density_gradient"])
             else:
                 assert h.wire_gradient is None
     be.close()
 
 
-@pytest.mark.parametrize("bx,by", [(64, 64), (128, 64), (48, 48)])
+@pytest.mark.parametrize("bx,by", [(64, 64), (128, 64), (48, 48), (50, 30), (30, 52)])
 def test_fields_match_oracle(ob, bx, by):
     """Source, potential and field grids: half-length products (both sides multiples of 64,
-    square and not) and the dense products (48 x 48)."""
+    square and not), the dense products (48 x 48) and the dense products on grids whose width is not
+    a multiple of four (the bin kernel's per-bin path, padded non-square products)."""
     d = design.plain_design()
     d.center = hb.random_placement(d, 2026)
     o = abi.default_options(target_density=0.9, bins_x=bx, bins_y=by)
@@ -109,12 +140,13 @@ def test_fields_match_oracle(ob, bx, by):
     gamma = 0.75 * 360 / 64
     s_ref = orc.evaluate(xy, gamma, True)
     s = be.evaluate(xy, gamma, True)
-    assert rel(s.electrostatic_energy, s_ref["electrostatic_energy"]) < 2e-4
+    assert rel(s.electrostatic_energy, s_ref["electrostatic_energy"]) < 2e-5
     g_ref = orc.grids()
     g = be.debug_grids(bx * by)
     assert nrel(g["source"], g_ref["source"]) < 1e-6
     for k in ("potential", "field_x", "field_y"):
-        assert nrel(g[k], g_ref[k]) < 5e-4, (k, nrel(g[k], g_ref[k]))
+        say(f"  {k}: normwise relative {nrel(g[k], g_ref[k]):.1e}")
+        assert nrel(g[k], g_ref[k]) < 1e-5, (k, nrel(g[k], g_ref[k]))
     be.close()
 
 
@@ -260,3 +292,24 @@ def test_session_renumbering_is_invisible_outside():
     assert nrel(b.wire_gradient, a.wire_gradient[order]) < 1e-6
     ref.close()
     s.close()
+
+
+def test_exceeded_device_budget_is_unavailable_with_a_reason():
+    """DeviceMemoryBudget (.cu:60-80, :1156-1216): a backend whose allocations exceed
+    maximum_device_bytes is not created; the factory reports "unavailable" with the reason, and a
+    budget that fits creates it with reservedBytes() within the budget."""
+    d, o = _setup(design.plain_design(), 0.9, 64)
+    pp = hb.PreparedDesign(d, o)
+    L = pp.layout()
+    say("[gpu] budget injection")
+    be, reason = hb.try_create_hip_placement_backend(L, maximum_device_bytes=1 << 20)
+    assert be is None
+    assert "budget exceeded" in reason and "limit 1048576 bytes" in reason, reason
+    be, reason = hb.try_create_hip_placement_backend(L, maximum_device_bytes=64 << 20)
+    assert be is not None and reason == ""
+    assert 0 < be.reserved_bytes() <= 64 << 20
+    be.close()
+    o.maximum_device_bytes = 1 << 20
+    # the explicit hip backend throws when it cannot be created (AdaptiveGlobalPlacer.cpp:275-277)
+    with pytest.raises(RuntimeError, match="budget exceeded"):
+        hb.global_place(d, o)

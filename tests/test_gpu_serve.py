@@ -69,3 +69,28 @@ def test_admission_waits_for_memory(mode):
     with pytest.raises(ValueError):
         server.submit(*_job(9, 200000))
     server.close()
+
+
+@pytest.mark.parametrize("mode", ["batched", "streams"])
+def test_served_sizes_1k_to_200k_are_the_same_alone_and_in_the_mix(mode):
+    """Jobs across the served stream's whole size range (1k-200k cells, 32 x 32 to 256 x 256 bins),
+    each run until design overflow <= 0.10 or 280 iterations as in the stream: every result equal
+    alone and in the mix."""
+    jobs = []
+    for k, (cells, t) in enumerate(((1000, 0.7), (200000, 0.9), (40000, 0.8), (8000, 0.75), (120000, 0.85),
+                                    (2500, 0.9))):
+        d, o = _job(400 + k, cells, iterations=280)
+        o.target_density = t
+        jobs.append((d, o))
+    say(f"[gpu] served sizes 1k-200k, {mode} mode, 4 slots")
+    server = hb.PlacementServer(max_active_jobs=4, memory_budget_bytes=200 << 30, mode=mode)
+    ids = [server.submit(d, o) for d, o in jobs]
+    got = [server.wait(j, d.module_count) for j, (d, _) in zip(ids, jobs)]
+    server.close()
+    for (d, o), (c_mix, r_mix) in zip(jobs, got):
+        c_alone, r_alone, _ = hb.global_place(d, o)
+        say(f"  job {d.module_count} modules, {o.bins_x}x{o.bins_y} bins: {r_alone['completed_iterations']} "
+            f"iterations, overflow {r_alone['overflow_after']:.3f}")
+        assert np.array_equal(c_mix, c_alone)
+        for k in ("hpwl_after", "overflow_after", "completed_iterations", "accepted_iterations", "gradient_evaluations"):
+            assert r_mix["result"][k] == r_alone[k], k

@@ -229,25 +229,69 @@ def test_tiny_fixture_counts():
     assert hpwl > 0
 
 
+def _reference_parse(ref, aux):
+    """The database the reference's own BookshelfParser::parseAux builds from `aux`, as arrays."""
+    h = C.c_void_p()
+    assert ref.ref_parse_aux(aux.encode(), C.byref(h)) == 0, ref.ref_last_error()
+    sizes = np.zeros(7, dtype=np.int64)
+    ref.ref_db_sizes(h, sizes.ctypes.data_as(C.POINTER(C.c_int64)))
+    M, N, I, R = (int(v) for v in sizes[:4])
+    out = dict(width=np.zeros(M), height=np.zeros(M), center=np.zeros((M, 2)), orientation=np.zeros(M, np.int32),
+               is_terminal=np.zeros(M, np.uint8), is_fixed=np.zeros(M, np.uint8), net_weight=np.zeros(N),
+               net_pin_start=np.zeros(N + 1, np.int64), pin_module=np.zeros(I, np.int32),
+               pin_offset=np.zeros((I, 2)), rows=np.zeros((R, 5)))
+    core = np.zeros(4)
+    args = [out[k].ctypes.data_as(C.c_void_p) for k in out] + [core.ctypes.data_as(C.c_void_p)]
+    assert ref.ref_db_export(h, *args) == 0
+    ref.ref_db_destroy(h)
+    return out, tuple(core)
+
+
 def test_bookshelf_reader_matches_reference_parser(ref):
     for name in ("tiny", "mixed"):
         d = _fixture(name)
-        h = C.c_void_p()
-        assert ref.ref_parse_aux(os.path.join(BOOK, f"{name}.aux").encode(), C.byref(h)) == 0
-        sizes = np.zeros(7, dtype=np.int64)
-        ref.ref_db_sizes(h, sizes.ctypes.data_as(C.POINTER(C.c_int64)))
-        M, N, I, R = (int(v) for v in sizes[:4])
-        out = dict(width=np.zeros(M), height=np.zeros(M), center=np.zeros((M, 2)), orientation=np.zeros(M, np.int32),
-                   is_terminal=np.zeros(M, np.uint8), is_fixed=np.zeros(M, np.uint8), net_weight=np.zeros(N),
-                   net_pin_start=np.zeros(N + 1, np.int64), pin_module=np.zeros(I, np.int32),
-                   pin_offset=np.zeros((I, 2)), rows=np.zeros((R, 5)))
-        core = np.zeros(4)
-        args = [out[k].ctypes.data_as(C.c_void_p) for k in out] + [core.ctypes.data_as(C.c_void_p)]
-        assert ref.ref_db_export(h, *args) == 0
-        ref.ref_db_destroy(h)
+        out, core = _reference_parse(ref, os.path.join(BOOK, f"{name}.aux"))
         for k, v in out.items():
             assert np.array_equal(getattr(d, k), v), (name, k)
-        assert tuple(core) == d.core_region()
+        assert core == d.core_region()
+
+
+NETLIST_FIELDS = ("width", "height", "orientation", "is_terminal", "net_weight", "net_pin_start", "pin_module",
+                  "pin_offset", "rows")
+
+
+@pytest.mark.parametrize("which", ["tiny", "mixed", "plain-random", "served-12k", "large-30k-quadratic-like"])
+def test_bookshelf_writer_round_trips_through_reference_parser(ref, tmp_path, which):
+    """design.write_bookshelf -> the reference's BookshelfParser (oracle/_ref) and read_bookshelf:
+    both read back the same arrays bit for bit, and the netlist, rows, weights and centres are the
+    written design's own (centres are written as lower-left corners that give them back exactly;
+    terminals come back fixed, as refreshDerivedData makes them)."""
+    if which in ("tiny", "mixed"):
+        d = _fixture(which)
+    elif which == "plain-random":
+        d = _synthetic_with_random_start()
+    elif which == "served-12k":
+        d = design.served_design(77, 12000)
+        d.center = ob.random_placement(d, 77)
+        d.net_weight = np.random.default_rng(3).uniform(0.5, 2.0, d.net_count)
+        d.orientation[: d.module_count // 3] = np.arange(d.module_count // 3) % 8
+    else:
+        d = design.large_design(seed=9, cells=30000, nets=31000, row_count=300, sites=600, macros=6)
+        rng = np.random.default_rng(9)
+        mov = d.movable_ids()
+        core = d.core_region()
+        d.center[mov, 0] = rng.normal((core[0] + core[2]) / 2, (core[2] - core[0]) / 10, mov.shape[0])
+        d.center[mov, 1] = rng.normal((core[1] + core[3]) / 2, (core[3] - core[1]) / 10, mov.shape[0])
+    aux = design.write_bookshelf(d, str(tmp_path), "rt")
+    out, core = _reference_parse(ref, aux)
+    mine = design.read_bookshelf(aux)
+    for k, v in out.items():
+        assert np.array_equal(getattr(mine, k), v), (which, k)
+    assert core == mine.core_region() == d.core_region()
+    for k in NETLIST_FIELDS:
+        assert np.array_equal(getattr(d, k), out[k]), (which, k)
+    assert np.array_equal(out["is_fixed"], (d.is_fixed | d.is_terminal).astype(np.uint8))
+    assert np.array_equal(out["center"], d.center), (which, np.abs(out["center"] - d.center).max())
 
 
 def _synthetic_with_random_start(seed=2026):
