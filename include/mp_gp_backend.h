@@ -404,6 +404,20 @@ void mp_dp_default_options(mp_dp_options* options);
 mp_gp_status mp_dp_run(const mp_design* design, const int32_t* row_orientation, const mp_dp_options* options,
                        double* centers, int32_t* orientation, mp_dp_result* result);
 
+/* mp_dp_run without any device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, results and statuses as mp_dp_run, bit for bit; every
+ * device buffer is carved from a fixed scratch array the library holds on each GPU it runs on
+ * (mp_dp_noalloc_scratch_bytes(), 64 MiB, kept until the process ends), and the work runs on the
+ * null stream (no stream is created).  The design must fit that scratch: 56 bytes per module
+ * + 24 per pin + 12 per net, each array rounded up to 256 bytes; a larger design gives
+ * MP_GP_INVALID_ARGUMENT with the sizes in mp_gp_last_error() before anything runs.
+ * maximum_device_bytes is not used.  Calls on one GPU take turns (they share its scratch); each
+ * call overwrites what the previous one left there. */
+mp_gp_status mp_dp_run_noalloc(const mp_design* design, const int32_t* row_orientation, const mp_dp_options* options,
+                               double* centers, int32_t* orientation, mp_dp_result* result);
+uint64_t mp_dp_noalloc_scratch_bytes(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,8 @@ def library():
         lib.mp_dp_default_options.argtypes = [C.POINTER(abi.MpDpOptions)]
         lib.mp_dp_run.argtypes = [C.POINTER(abi.MpDesign), V, C.POINTER(abi.MpDpOptions), V, V,
                                   C.POINTER(abi.MpDpResult)]
+        lib.mp_dp_run_noalloc.argtypes = lib.mp_dp_run.argtypes
+        lib.mp_dp_noalloc_scratch_bytes.restype = C.c_uint64
         lib.mp_gp_quadratic_placement.argtypes = [C.POINTER(abi.MpDesign), C.POINTER(abi.MpQpOptions), V,
                                                   C.POINTER(abi.MpQpResult)]
         lib.mp_gp_session_create.argtypes = [V, C.POINTER(abi.MpOptions), V, C.POINTER(V)]
@@ -442,6 +444,26 @@ def detailed_place(design, options: abi.MpDpOptions | None = None, center=None, 
     r = abi.MpDpResult()
     _raise(library().mp_dp_run(C.byref(d), _p(ro), C.byref(o), _p(out_c), _p(out_o), C.byref(r)))
     return out_c, out_o, r.as_dict()
+
+
+def detailed_place_noalloc(design, options: abi.MpDpOptions | None = None, center=None, orientation=None,
+                           row_orientation=None):
+    """detailed_place through mp_dp_run_noalloc: the same pass with every device buffer carved from
+    the library's fixed per-device scratch (no device allocation, null stream)."""
+    o = abi.default_dp_options() if options is None else options
+    c = np.ascontiguousarray(design.center if center is None else center, dtype=np.float64).reshape(-1, 2).copy()
+    ori = np.ascontiguousarray(design.orientation if orientation is None else orientation, dtype=np.int32).copy()
+    d = design.to_c(c, ori)
+    ro = None if row_orientation is None else np.ascontiguousarray(row_orientation, dtype=np.int32)
+    out_c = c.copy()
+    out_o = ori.copy()
+    r = abi.MpDpResult()
+    _raise(library().mp_dp_run_noalloc(C.byref(d), _p(ro), C.byref(o), _p(out_c), _p(out_o), C.byref(r)))
+    return out_c, out_o, r.as_dict()
+
+
+def noalloc_scratch_bytes() -> int:
+    return int(library().mp_dp_noalloc_scratch_bytes())
 
 
 def estimate_device_bytes(design, options: abi.MpOptions) -> int:

@@ -35,6 +35,7 @@
 #include <climits>
 #include <cmath>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -182,18 +183,47 @@ __global__ __launch_bounds__(kDpThreads) void k_dp_windows(DpArgs a) {
     if (lane == 0) a.best[window] = idx;
 }
 
+// Device memory carved from a caller-provided block (the no-allocation entry point's scratch):
+// a bump pointer over [base, base + size), 256-byte aligned pieces; mark / release rewind it.
+class Arena {
+public:
+    Arena(unsigned char* base, std::size_t size) : base_(base), size_(size) {}
+    void* take(std::size_t bytes, const char* what) {
+        const std::size_t at = (used_ + 255) / 256 * 256;
+        if (at > size_ || bytes > size_ - at)
+            throw std::logic_error(std::string("detailed-placement scratch overrun while carving ") + what);
+        used_ = at + bytes;
+        peak_ = std::max(peak_, used_);
+        return base_ + at;
+    }
+    std::size_t mark() const { return used_; }
+    void release(std::size_t mark) { used_ = mark; }
+    std::size_t peak() const { return peak_; }
+
+private:
+    unsigned char* base_;
+    std::size_t size_, used_ = 0, peak_ = 0;
+};
+
 template <typename T>
 class DeviceVec {
 public:
     DeviceVec() = default;
     ~DeviceVec() {
-        if (p_) (void)hipFree(p_);
+        if (p_ && owned_) (void)hipFree(p_);
     }
     DeviceVec(const DeviceVec&) = delete;
     DeviceVec& operator=(const DeviceVec&) = delete;
-    // (re)allocates for at least n elements, charging the budget with the growth
-    void reserve(std::size_t n, std::size_t& used, std::size_t limit, const char* what) {
+    // (re)allocates for at least n elements, charging the budget with the growth; with an arena,
+    // carves n elements from it instead (the arena's owner rewinds it between passes)
+    void reserve(std::size_t n, std::size_t& used, std::size_t limit, const char* what, Arena* arena = nullptr) {
         if (n == 0) n = 1;
+        if (arena) {
+            p_ = static_cast<T*>(arena->take(n * sizeof(T), what));
+            owned_ = false;
+            n_ = n;
+            return;
+        }
         if (n <= n_) return;
         const std::size_t grow = (n - n_) * sizeof(T);
         if (grow > limit - used)
@@ -201,6 +231,7 @@ public:
         if (p_) check(hipFree(p_), what);
         p_ = nullptr;
         check(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)), what);
+        owned_ = true;
         used += grow;
         n_ = n;
     }
@@ -213,6 +244,7 @@ public:
 private:
     T* p_ = nullptr;
     std::size_t n_ = 0;
+    bool owned_ = false;
 };
 
 // The placement the pass works on: the PlacementDatabase fields window reordering reads and writes.
@@ -505,19 +537,24 @@ Placement loadPlacement(const mp_design& d, const int32_t* row_orientation) {
 // The device pass of one run: static netlist resident, per-pass placement and window lists.
 class DeviceWindows {
 public:
-    DeviceWindows(const Placement& P, int device, std::size_t limit) : limit_(limit) {
+    // arena == nullptr: buffers from hipMalloc on a stream of its own; otherwise every buffer is
+    // carved from the arena and the work runs on the null stream (no allocation of any kind)
+    DeviceWindows(const Placement& P, int device, std::size_t limit, Arena* arena = nullptr)
+        : limit_(limit), arena_(arena) {
         check(hipSetDevice(device), "select HIP device");
-        check(hipStreamCreate(&s_), "create detailed-placement stream");
+        if (!arena_) check(hipStreamCreate(&s_), "create detailed-placement stream");
         std::vector<int> net_start(P.N + 1);
         for (int64_t n = 0; n <= P.N; ++n) net_start[n] = static_cast<int>(P.net_start[n]);
         std::vector<int> pin_module(P.pin_module.begin(), P.pin_module.end());
-        width_.reserve(P.M, used_, limit_, "module widths");
-        height_.reserve(P.M, used_, limit_, "module heights");
-        centers_.reserve(P.M, used_, limit_, "module centres");
-        pin_module_.reserve(P.I, used_, limit_, "pin modules");
-        pin_off_.reserve(P.I, used_, limit_, "pin offsets");
-        net_start_.reserve(P.N + 1, used_, limit_, "net pin starts");
-        net_w_.reserve(P.N, used_, limit_, "net weights");
+        width_.reserve(P.M, used_, limit_, "module widths", arena_);
+        height_.reserve(P.M, used_, limit_, "module heights", arena_);
+        centers_.reserve(P.M, used_, limit_, "module centres", arena_);
+        pin_module_.reserve(P.I, used_, limit_, "pin modules", arena_);
+        pin_off_.reserve(P.I, used_, limit_, "pin offsets", arena_);
+        net_start_.reserve(P.N + 1, used_, limit_, "net pin starts", arena_);
+        net_w_.reserve(P.N, used_, limit_, "net weights", arena_);
+        if (arena_) window_mark_ = arena_->mark();
+        peak_ = arena_ ? arena_->peak() : used_;
         width_.upload(P.w, s_, "upload module widths");
         height_.upload(P.h, s_, "upload module heights");
         pin_module_.upload(pin_module, s_, "upload pin modules");
@@ -529,6 +566,8 @@ public:
         if (s_) {
             (void)hipStreamSynchronize(s_);
             (void)hipStreamDestroy(s_);
+        } else if (arena_) {
+            (void)hipStreamSynchronize(nullptr);
         }
     }
     DeviceWindows(const DeviceWindows&) = delete;
@@ -558,14 +597,15 @@ public:
             if (nets.size() > static_cast<std::size_t>(INT_MAX)) throw std::invalid_argument("too many window nets");
             net_start[k + 1] = static_cast<int>(nets.size());
         }
-        win_mod_.reserve(mods.size(), used_, limit_, "window modules");
-        win_x_.reserve(W, used_, limit_, "window starts");
-        win_bottom_.reserve(W, used_, limit_, "window rows");
-        win_spacing_.reserve(W, used_, limit_, "window site spacing");
-        win_net_start_.reserve(W + 1, used_, limit_, "window net starts");
-        win_nets_.reserve(nets.size(), used_, limit_, "window nets");
-        best_.reserve(W, used_, limit_, "window orderings");
-        peak_ = std::max(peak_, used_);
+        if (arena_) arena_->release(window_mark_);  // the previous pass's window arrays are done
+        win_mod_.reserve(mods.size(), used_, limit_, "window modules", arena_);
+        win_x_.reserve(W, used_, limit_, "window starts", arena_);
+        win_bottom_.reserve(W, used_, limit_, "window rows", arena_);
+        win_spacing_.reserve(W, used_, limit_, "window site spacing", arena_);
+        win_net_start_.reserve(W + 1, used_, limit_, "window net starts", arena_);
+        win_nets_.reserve(nets.size(), used_, limit_, "window nets", arena_);
+        best_.reserve(W, used_, limit_, "window orderings", arena_);
+        peak_ = arena_ ? arena_->peak() : std::max(peak_, used_);
         centers_.upload(centers, s_, "upload module centres");
         pin_off_.upload(off, s_, "upload pin offsets");
         win_mod_.upload(mods, s_, "upload window modules");
@@ -588,16 +628,67 @@ public:
     std::size_t peakBytes() const { return peak_; }
 
 private:
-    std::size_t limit_ = 0, used_ = 0, peak_ = 0;
+    std::size_t limit_ = 0, used_ = 0, peak_ = 0, window_mark_ = 0;
+    Arena* arena_ = nullptr;
     hipStream_t s_ = nullptr;
     DeviceVec<double> width_, height_, net_w_, win_x_, win_bottom_, win_spacing_;
     DeviceVec<double2> centers_, pin_off_;
     DeviceVec<int> pin_module_, net_start_, win_mod_, win_net_start_, win_nets_, best_;
 };
 
+// Device bytes the no-allocation pass may carve, from the design's counts alone: the resident
+// netlist (widths, heights, centres, pin modules and offsets, net starts and weights) and the
+// window arrays of the largest possible pass (at most one window per two cells, and a window's
+// nets are at most its cells' pins), each piece rounded up to 256 bytes.
+std::size_t scratchBytesNeeded(int64_t M, int64_t N, int64_t I) {
+    auto piece = [](std::size_t n, std::size_t elem) { return ((n == 0 ? 1 : n) * elem + 255) / 256 * 256; };
+    const std::size_t m = static_cast<std::size_t>(M), n = static_cast<std::size_t>(N), i = static_cast<std::size_t>(I);
+    const std::size_t w = m / 2 + 1;
+    return piece(m, 8) * 2 + piece(m, 16) + piece(i, 4) + piece(i, 16) + piece(n + 1, 4) + piece(n, 8) +
+           piece(kDpWindow * w, 4) + piece(w, 8) * 3 + piece(w + 1, 4) + piece(i, 4) + piece(w, 4);
+}
+
+// Scratch of the no-allocation entry point: one per GPU the library runs on, held until the
+// process ends.  Host code never uses the symbol's name as an address: kScratch[device] holds the
+// device address that hipGetSymbolAddress returned with that device current.
+constexpr std::size_t kDpScratchBytes = 64ull << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kDpScratchBytes];
+
+struct DeviceScratch {
+    std::once_flag once;
+    unsigned char* ptr = nullptr;
+    hipError_t status = hipSuccess;
+    std::mutex lock;  // calls on one device take turns: they share its scratch
+};
+DeviceScratch g_scratch[kMaxDevices];
+
+DeviceScratch& deviceScratch(int device) {
+    DeviceScratch& d = g_scratch[device];
+    std::call_once(d.once, [&] {
+        d.status = hipSetDevice(device);
+        if (d.status == hipSuccess) d.status = hipGetSymbolAddress(reinterpret_cast<void**>(&d.ptr), HIP_SYMBOL(scratch));
+        if (d.status != hipSuccess) d.ptr = nullptr;
+    });
+    if (d.status != hipSuccess || d.ptr == nullptr)
+        throw std::runtime_error(std::string("cannot locate the detailed-placement scratch on the device: ") +
+                                 hipGetErrorString(d.status));
+    return d;
+}
+
 void run(const mp_design& d, const int32_t* row_orientation, const mp_dp_options& o, double* centers,
-         int32_t* orientation, mp_dp_result& r) {
+         int32_t* orientation, mp_dp_result& r, bool no_alloc = false) {
     validate(o);
+    if (no_alloc) {
+        // size check first, from the counts alone: nothing runs for a design that does not fit
+        const std::size_t need = scratchBytesNeeded(d.module_count, d.net_count, d.pin_count);
+        if (d.module_count < 0 || d.net_count < 0 || d.pin_count < 0)
+            throw std::invalid_argument("negative counts in the design");
+        if (need > kDpScratchBytes)
+            throw std::invalid_argument("design too large for the no-allocation detailed placement: it needs up to " +
+                                        std::to_string(need) + " bytes of device scratch, the library holds " +
+                                        std::to_string(kDpScratchBytes) + " per device");
+    }
     Placement P = loadPlacement(d, row_orientation);
     r = {};
     r.device_used = -1;
@@ -616,10 +707,27 @@ void run(const mp_design& d, const int32_t* row_orientation, const mp_dp_options
     }
     if (o.method != MP_DP_WINDOW_REORDER || o.window_size != kDpWindow)
         throw Unavailable("HIP detailed placement supports window reordering with a four-cell window only.");
-    std::string reason;
-    const std::size_t limit = allocationLimit(o.device, o.maximum_device_bytes, reason);
-    if (limit == 0) throw Unavailable("HIP detailed-placement backend is unavailable: " + reason);
-    DeviceWindows dev(P, o.device, limit);
+    std::unique_ptr<std::lock_guard<std::mutex>> turn;
+    std::unique_ptr<Arena> arena;
+    std::size_t limit = 0;
+    if (no_alloc) {
+        int count = 0;
+        const hipError_t e = hipGetDeviceCount(&count);
+        if (e != hipSuccess || count == 0)
+            throw Unavailable(std::string("HIP detailed-placement backend is unavailable: HIP device discovery failed: ") +
+                              (e == hipSuccess ? "no device" : hipGetErrorString(e)));
+        if (o.device < 0 || o.device >= count || o.device >= kMaxDevices)
+            throw Unavailable("HIP detailed-placement backend is unavailable: Requested HIP device does not exist on this host.");
+        DeviceScratch& ds = deviceScratch(o.device);
+        turn = std::make_unique<std::lock_guard<std::mutex>>(ds.lock);
+        arena = std::make_unique<Arena>(ds.ptr, kDpScratchBytes);
+        limit = kDpScratchBytes;
+    } else {
+        std::string reason;
+        limit = allocationLimit(o.device, o.maximum_device_bytes, reason);
+        if (limit == 0) throw Unavailable("HIP detailed-placement backend is unavailable: " + reason);
+    }
+    DeviceWindows dev(P, o.device, limit, arena.get());
     r.device_used = o.device;
     const auto started = std::chrono::steady_clock::now();
     std::vector<double> wx, wy;
@@ -715,5 +823,24 @@ mp_gp_status mp_dp_run(const mp_design* design, const int32_t* row_orientation, 
         return translate(std::current_exception());
     }
 }
+
+mp_gp_status mp_dp_run_noalloc(const mp_design* design, const int32_t* row_orientation, const mp_dp_options* options,
+                               double* centers, int32_t* orientation, mp_dp_result* result) {
+    g_last_error.clear();
+    try {
+        if (!design || !options || !centers || !result) throw std::invalid_argument("null design, options, centres or result");
+        mp_dp_result r{};
+        run(*design, row_orientation, *options, centers, orientation, r, true);
+        *result = r;
+        return MP_GP_OK;
+    } catch (const Unavailable& e) {
+        g_last_error = e.what();
+        return MP_GP_UNAVAILABLE;
+    } catch (...) {
+        return translate(std::current_exception());
+    }
+}
+
+uint64_t mp_dp_noalloc_scratch_bytes(void) { return kDpScratchBytes; }
 
 }  // extern "C"
