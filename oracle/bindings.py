@@ -72,6 +72,10 @@ def oracle():
         lib.orc_smoothing.restype = C.c_double
         lib.orc_accepts.argtypes = [C.POINTER(abi.MpOptions), C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_double]
+        lib.orc_update_penalty.argtypes = [C.POINTER(abi.MpOptions)] + [C.c_double] * 7
+        lib.orc_update_penalty.restype = C.c_double
+        lib.orc_checkpoint_sequence.argtypes = [C.POINTER(abi.MpOptions), C.c_int64, _dp, _dp, C.c_void_p, C.c_void_p]
+        lib.orc_checkpoint_sequence.restype = C.c_int32
         lib.orc_detailed_place.argtypes = [C.POINTER(abi.MpDesign), C.c_void_p, C.POINTER(abi.MpDpOptions), _dp,
                                            C.c_void_p, C.POINTER(abi.MpDpResult)]
         _oracle = lib

@@ -2002,6 +2002,38 @@ double orc_smoothing(const mp_gp_options* o, double max_bin, double overflow, do
     return smoothingFromOverflow(*o, max_bin, overflow, initial);
 }
 
+double orc_update_penalty(const mp_gp_options* o, double penalty, double reference_penalty, double before_hpwl,
+                          double before_overflow, double after_hpwl, double after_overflow,
+                          double after_design_overflow) {  // :536-557
+    Eval b, a;
+    b.s.hpwl = before_hpwl;
+    b.s.optimizer_density.normalized_overflow = before_overflow;
+    a.s.hpwl = after_hpwl;
+    a.s.optimizer_density.normalized_overflow = after_overflow;
+    a.s.design_density.normalized_overflow = after_design_overflow;
+    return updateDensityPenalty(penalty, reference_penalty, b, a, *o);
+}
+
+// considerCheckpoint (:563-596) over a sequence of (HPWL, design overflow) records, iterations 0..n-1:
+// new_best[i] receives is_new_best of record i; returns the iteration runAdaptiveGlobalPlacement
+// restores (:930-935), -1 when none; *feasible: whether it is the feasible checkpoint.
+int32_t orc_checkpoint_sequence(const mp_gp_options* o, int64_t n, const double* hpwl, const double* design_overflow,
+                                int32_t* new_best, int32_t* feasible) {
+    Checkpoint feas, fall;
+    const std::vector<V2> none;
+    for (int64_t i = 0; i < n; ++i) {
+        Eval e;
+        e.s.hpwl = hpwl[i];
+        e.s.design_density.normalized_overflow = design_overflow[i];
+        bool is_new = false;
+        considerCheckpoint(feas, fall, none, e, static_cast<int>(i), *o, is_new);
+        if (new_best) new_best[i] = is_new ? 1 : 0;
+    }
+    const Checkpoint& chosen = feas.found ? feas : fall;
+    if (feasible) *feasible = feas.found ? 1 : 0;
+    return chosen.found ? chosen.iteration : -1;
+}
+
 int orc_accepts(const mp_gp_options* o, double base_objective, double base_overflow, double cand_objective,
                 double cand_overflow, double predicted) {  // :521-534
     Eval b, c;

@@ -313,3 +313,31 @@ def test_exceeded_device_budget_is_unavailable_with_a_reason():
     # the explicit hip backend throws when it cannot be created (AdaptiveGlobalPlacer.cpp:275-277)
     with pytest.raises(RuntimeError, match="budget exceeded"):
         hb.global_place(d, o)
+
+
+@pytest.mark.parametrize("case", ["plain-default", "plain-aggressive", "mid-quadratic"])
+def test_device_adaptive_run_composes_the_pinned_laws(ob, case):
+    """The device controller's whole run, row by row, against the oracle's pinned laws
+    (tests/controller_checks.py): objective, smoothing, penalty, checkpoint, restart and
+    displacement composition of AdaptiveGlobalPlacer.cpp:652-953."""
+    from controller_checks import check_history_composition
+    if case.startswith("plain"):
+        d, o = _setup(design.plain_design(), 0.9, 64)
+        if case == "plain-aggressive":
+            o.initial_movement_in_bins = 1.0
+            o.maximum_movement_in_bins = 1.0
+            o.armijo_coefficient = 0.5
+            o.objective_increase_for_density = 0.0
+    else:
+        d = design.large_design(seed=3, cells=5000, nets=5250, row_count=80, sites=300, macros=3)
+        d.center, _ = hb.quadratic_placement(d)
+        o = abi.default_options(target_density=0.8, bins_x=32, bins_y=32, iterations=150)
+    o.feasible_refinement_iterations = 2 ** 31 - 1
+    say(f"[gpu] controller composition on the device: {case}")
+    _, r, hist = hb.global_place(d, o)
+    core = d.core_region()
+    stats = check_history_composition(o, r, hist, (core[2] - core[0]) / o.bins_x, (core[3] - core[1]) / o.bins_y)
+    say(f"  {stats}")
+    assert stats["accepted"] > 10
+    if case == "plain-aggressive":
+        assert r["rejected_candidates"] > 0 and r["momentum_restarts"] > 0

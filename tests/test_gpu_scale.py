@@ -164,11 +164,16 @@ def test_headline_random_start_end_state_matches_oracle(ob, headline):
     o = abi.default_options(target_density=0.8, bins_x=512, bins_y=512)
     o.feasible_refinement_iterations = 2 ** 31 - 1
     say("[gpu] headline random start, 280 iterations on the device and in the oracle")
-    _, a, _ = hb.global_place(d, o)
+    _, a, hist_a = hb.global_place(d, o)
     say(f"  device: HPWL {a['hpwl_after']:.1f} overflow {a['overflow_after']:.4f} in {a['loop_seconds']:.3f}s")
-    _, b, _ = ob.global_place(d, o)
+    _, b, hist_b = ob.global_place(d, o)
     say(f"  oracle: HPWL {b['hpwl_after']:.1f} overflow {b['overflow_after']:.4f} in {b['loop_seconds']:.1f}s")
     assert a["completed_iterations"] == b["completed_iterations"] == 280
     assert rel(a["hpwl_before"], b["hpwl_before"]) < 1e-12
     assert abs(a["hpwl_after"] / b["hpwl_after"] - 1.0) < 5e-3
     assert abs(a["overflow_after"] - b["overflow_after"]) < 5e-3
+    # both runs compose the pinned controller laws row by row
+    from controller_checks import check_history_composition
+    core = d.core_region()
+    for r, hist in ((a, hist_a), (b, hist_b)):
+        check_history_composition(o, r, hist, (core[2] - core[0]) / 512, (core[3] - core[1]) / 512)

@@ -24,9 +24,19 @@ Parity status of each oracle function (oracle/gp_oracle.cpp), and what pins it:
                                                                                       -> pinned
   controller laws                    estimateStep = 1/L on quadratics and within the inverse
                                      extreme curvatures; Armijo acceptance iff tL <= 2(1-c) on a
-                                     quadratic line search; smoothing end points      -> pinned
-  adaptive trajectory                no numeric pin exists in the reference; invariants of
-                                     PlacementFlowTests.cpp:153-216 only              -> parity unpinned
+                                     quadratic line search; smoothing end points; updateDensityPenalty
+                                     and considerCheckpoint on known-answer tables (every branch,
+                                     both clamps, the tie rules)                      -> pinned
+  adaptive loop composition          every row of whole runs (default and the reference's aggressive
+                                     options, random and quadratic starts) re-derived from the pinned
+                                     laws: objective, smoothing, penalty, checkpoint, restart and
+                                     displacement composition; the non-accepted branch at a symmetric
+                                     fixed point (exact penalty sequence, clamp at the maximum option,
+                                     no move, no candidate)                           -> pinned
+  adaptive trajectory                the reference ships no trajectory; the momentum coefficient
+                                     min(0.92, age / (age + 3)) and the look-ahead restart test
+                                     (:751-773) are covered only through the invariants of
+                                     PlacementFlowTests.cpp:153-216                   -> parity unpinned
   window-reordering detailed pass    the reference's fixture (DetailedPlacementTests.cpp:55-70);
                                      bit-equal to the reference's own CPU placer (built from
                                      DetailedPlacer.cpp) when no two windows of a pass share a net;
@@ -712,3 +722,145 @@ def test_armijo_acceptance_on_a_quadratic_line_search():
     assert accepts(2.04, 0.3) == 0   # f(y) = 0.5408 > 0.5
     assert accepts(2.04, 0.29) == 1  # overflow down by 0.01 > the stall gain, objective within 10 %
     assert accepts(2.1, 0.29) == 0   # f(y) = 0.605: beyond the 10 % allowance (0.55)
+
+
+# ---- the rest of the controller: penalty law, checkpoint rules, and their composition ----------
+
+def test_density_penalty_law_known_answers():
+    """updateDensityPenalty (AdaptiveGlobalPlacer.cpp:536-557), each branch and both clamps, with
+    the default options (bound 0.1, critical above 0.2, stall 0.002, HPWL growth 2 %, x1.05 / x0.88):
+    a swapped branch order, a wrong comparison or the wrong clamp bound fails one of these."""
+    lib = ob.oracle()
+    o = abi.default_options()
+
+    def upd(pen, ref, b_hpwl, b_ovf, a_hpwl, a_ovf, a_design, opts=o):
+        return lib.orc_update_penalty(C.byref(opts), pen, ref, b_hpwl, b_ovf, a_hpwl, a_ovf, a_design)
+
+    # critical (0.5 > 0.2) and stalled (improvement 0.001 < 0.002): increase
+    assert upd(10.0, 10.0, 100.0, 0.501, 100.0, 0.5, 0.5) == 10.0 * 1.05
+    # critical but improving, design overflow high: unchanged
+    assert upd(10.0, 10.0, 100.0, 0.6, 100.0, 0.5, 0.5) == 10.0
+    # not critical (0.15), stalled, HPWL grew 3 % > 2 %: decrease
+    assert upd(10.0, 10.0, 100.0, 0.151, 103.0, 0.15, 0.15) == 10.0 * 0.88
+    # not critical, stalled, HPWL grew 1 %, overflow above the 0.1 bound: increase
+    assert upd(10.0, 10.0, 100.0, 0.151, 101.0, 0.15, 0.15) == 10.0 * 1.05
+    # exactly at the critical threshold (0.2 is not above it): the HPWL-growth decrease applies
+    assert upd(10.0, 10.0, 100.0, 0.2005, 103.0, 0.2, 0.2) == 10.0 * 0.88
+    # improving into the feasible range (design overflow <= 0.1): decrease
+    assert upd(10.0, 10.0, 100.0, 0.2, 100.0, 0.05, 0.05) == 10.0 * 0.88
+    # improving, overflow 0.15 > 0.1: unchanged
+    assert upd(10.0, 10.0, 100.0, 0.3, 100.0, 0.15, 0.15) == 10.0
+    # HPWL growth is measured against max(HPWL, 1): 0.5 -> 0.515 is 1.5 % of 1 (not 3 % of 0.5), so
+    # no decrease; the overflow above its bound then increases the penalty
+    assert upd(10.0, 10.0, 0.5, 0.151, 0.515, 0.15, 0.15) == 10.0 * 1.05
+    # lower clamp max(minimum, 0.02 reference): 0.21 x 0.88 = 0.1848 -> 0.2
+    assert upd(0.21, 10.0, 100.0, 0.2, 100.0, 0.05, 0.05) == 0.2
+    assert upd(1e-4, 1e-3, 100.0, 0.2, 100.0, 0.05, 0.05) == 1e-4  # max(1e-4, 2e-5)
+    # upper clamp max(maximum, 10 x lower)
+    tight = abi.default_options(maximum_density_penalty=11.0)
+    assert upd(10.5, 10.0, 100.0, 0.501, 100.0, 0.5, 0.5, tight) == 11.0
+    wide = abi.default_options(minimum_density_penalty=5.0, maximum_density_penalty=6.0)
+    assert upd(40.0, 10.0, 100.0, 0.501, 100.0, 0.5, 0.5, wide) == 42.0  # 10 x lower = 50 > 6
+    assert upd(49.0, 10.0, 100.0, 0.501, 100.0, 0.5, 0.5, wide) == 50.0
+
+
+def _checkpoints(hpwl, ovf, **kw):
+    o = abi.default_options(**kw)
+    hp = np.ascontiguousarray(hpwl, dtype=np.float64)
+    ov = np.ascontiguousarray(ovf, dtype=np.float64)
+    flags = np.zeros(len(hp), dtype=np.int32)
+    feas = C.c_int32()
+    chosen = ob.oracle().orc_checkpoint_sequence(C.byref(o), len(hp), hp.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 ov.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 flags.ctypes.data_as(C.c_void_p), C.byref(feas))
+    return chosen, feas.value, flags.tolist()
+
+
+def test_checkpoint_selection_rules():
+    """considerCheckpoint (:563-596) and the restore choice (:930-935) on constructed sequences:
+    feasible (design overflow <= 0.1, the bound included) beats any infeasible record whatever its
+    HPWL; among feasible records the lowest HPWL, a record within 1e-10 counting only if its HPWL is
+    lower; before any feasible record the lowest overflow, within 1e-12 the lower HPWL; once a
+    feasible record exists the fallback no longer moves."""
+    # never feasible: overflow 0.4 at iteration 1, then a 1e-13-higher overflow with lower HPWL
+    # ties and wins; a lower overflow by less than 1e-12 with higher HPWL does not
+    assert _checkpoints([100, 120, 110, 90, 130], [0.5, 0.4, 0.4 + 1e-13, 0.45, 0.4]) == (2, 0, [1, 1, 1, 0, 0])
+    # feasible from iteration 1; the infeasible 70 never wins; 0.1 itself is feasible; a 5e-11 lower
+    # HPWL is a tie that still replaces (lower HPWL), 5e-10 lower replaces outright
+    seq_h = [80, 120, 70, 125, 119, 119 - 5e-11, 119 - 5e-11 - 5e-10, 200]
+    seq_o = [0.5, 0.09, 0.3, 0.08, 0.1, 0.05, 0.07, 0.0]
+    assert _checkpoints(seq_h, seq_o) == (6, 1, [1, 1, 0, 0, 1, 1, 1, 0])
+    # an equal HPWL does not replace the earlier record
+    assert _checkpoints([100, 100, 100], [0.05, 0.05, 0.01]) == (0, 1, [1, 0, 0])
+    # a tighter bound makes all but the last record infeasible: the fallback follows the overflow down
+    # (0.5, 0.09, 0.08, 0.05) until the feasible record takes over
+    assert _checkpoints(seq_h, seq_o, maximum_bin_overflow=0.04) == (7, 1, [1, 1, 0, 1, 0, 1, 0, 1])
+    assert _checkpoints([], []) == (-1, 0, [])
+
+
+def _two_terminal_fixed_point():
+    """One movable 20 x 10 standard cell at the centre of a 320 x 320 core, midway between two fixed
+    1 x 1 terminals at (156, 160) and (164, 160), one two-pin net to each; no fillers."""
+    rows = np.array([(r * 10.0, 10.0, 0.0, 1.0, 320.0) for r in range(32)])
+    term = np.array([0, 1, 1], np.uint8)
+    return design.Design(width=np.array([20.0, 1.0, 1.0]), height=np.array([10.0, 1.0, 1.0]),
+                         center=np.array([[160.0, 160.0], [156.0, 160.0], [164.0, 160.0]]),
+                         orientation=np.zeros(3, np.int32), is_terminal=term, is_fixed=term.copy(),
+                         net_weight=np.ones(2), net_pin_start=np.array([0, 2, 4]),
+                         pin_module=np.array([0, 1, 0, 2], np.int32), pin_offset=np.zeros((4, 2)), rows=rows)
+
+
+def test_non_accepted_iterations_at_a_symmetric_fixed_point():
+    """At a point where the gradient vanishes by symmetry (the cell midway between its two
+    terminals at the centre of the core: every wirelength term cancels exactly and the field is
+    zero there) every proposal predicts no reduction, so no iteration is accepted (:818-822, :909):
+    the cell never moves, no candidate is evaluated, momentum never starts, and since the overflow
+    (0.602 at t = 0.2) stays above its bound the penalty grows by exactly x1.05 per row, clamped at
+    maximum_density_penalty itself -- not at max(maximum, 10 x lower), the accepted branch's bound
+    (the options make the two differ: lower 0.5, 10 x lower 5, maximum 1.2)."""
+    d = _two_terminal_fixed_point()
+    o = abi.default_options(target_density=0.2, bins_x=32, bins_y=32, maximum_fillers=0, iterations=12,
+                            minimum_density_penalty=0.5, maximum_density_penalty=1.2)
+    o.feasible_refinement_iterations = 2 ** 31 - 1
+    c, r, hist = ob.global_place(d, o)
+    assert r["completed_iterations"] == 12 and r["accepted_iterations"] == 0
+    assert r["value_evaluations"] == 0 and r["gradient_evaluations"] == 2 + 11
+    assert r["momentum_restarts"] == 0 and r["rejected_candidates"] == 0
+    assert np.array_equal(c, d.center)
+    pen = [h["penalty"] for h in hist]
+    assert pen[0] == 1.0  # no density gradient at all: the reference penalty falls back to 1
+    expect = [1.0]
+    for _ in range(11):
+        expect.append(min(max(expect[-1] * 1.05, 0.5), 1.2))
+    assert pen == expect and pen[-1] == 1.2
+    for h in hist:  # 0.6 from the cell, 0.002 from the terminals' fixed area
+        assert h["overflow"] == pytest.approx(0.602, abs=1e-12) and not h["accepted"] and h["step_size"] == 0.0
+
+
+@pytest.mark.parametrize("case", ["plain-default", "plain-aggressive", "mid-quadratic"])
+def test_adaptive_run_composes_the_pinned_laws(case):
+    """The oracle's whole adaptive run, row by row, against the pinned laws (see controller_checks):
+    objective, smoothing, penalty, checkpoint, restart and displacement composition.  The
+    aggressive options are the reference's own controller test (PlacementFlowTests.cpp:197-216):
+    larger moves and a strict Armijo coefficient, so backtracking and momentum restarts occur."""
+    from controller_checks import check_history_composition
+    if case.startswith("plain"):
+        d = design.plain_design()
+        d.center = ob.random_placement(d, 2026)
+        o = abi.default_options(target_density=0.9, bins_x=64, bins_y=64)
+        if case == "plain-aggressive":
+            o.initial_movement_in_bins = 1.0
+            o.maximum_movement_in_bins = 1.0
+            o.armijo_coefficient = 0.5
+            o.objective_increase_for_density = 0.0
+    else:
+        d = design.large_design(seed=3, cells=5000, nets=5250, row_count=80, sites=300, macros=3)
+        d.center, _ = ob.quadratic_placement(d)
+        o = abi.default_options(target_density=0.8, bins_x=32, bins_y=32, iterations=150)
+    o.feasible_refinement_iterations = 2 ** 31 - 1
+    _, r, hist = ob.global_place(d, o)
+    core = d.core_region()
+    stats = check_history_composition(o, r, hist, (core[2] - core[0]) / o.bins_x, (core[3] - core[1]) / o.bins_y)
+    assert stats["accepted"] > 10 and stats["penalty_checks"] == len(hist) - 1 - 1 * bool(hist[0]["accepted"])
+    if case == "plain-aggressive":
+        assert r["rejected_candidates"] > 0 and r["momentum_restarts"] > 0
