@@ -63,7 +63,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_workload(name: str):
+def make_workload(name: str, center=None):
+    """The design, options and description of a workload; `center` replaces the computed start (the
+    CPU baseline's child runs from the start the GPU run used)."""
     from myplacement_amd import abi, design
     from myplacement_amd import backend as hb
     if name == "large":
@@ -71,7 +73,7 @@ def make_workload(name: str):
         o = abi.default_options(target_density=0.8, bins_x=512, bins_y=512)
         desc = {"workload": "synthetic 210k cells, 12 macros (15%), 220k nets (~0.9M pins), 1200x1200 sites, "
                             "512x512 bins, t=0.8, quadratic init", "cells": 210000, "bins": 512, "target_density": 0.8}
-        d.center, _ = hb.quadratic_placement(d)
+        d.center = center if center is not None else hb.quadratic_placement(d)[0]
     elif name == "precision":
         d = design.large_design(seed=2026, cells=1_300_000, nets=1_400_000, row_count=3000, sites=3000, macros=20,
                                 macro_fraction=0.12)
@@ -79,13 +81,13 @@ def make_workload(name: str):
         desc = {"workload": "synthetic 1.3M cells, 20 macros (12%), 1.4M nets (~5.7M pins), 3000x3000 sites, "
                             "1024x1024 bins, t=0.7, quadratic init, split-BF16 (bf16x3) DCT products",
                 "cells": 1_300_000, "bins": 1024, "target_density": 0.7}
-        d.center, _ = hb.quadratic_placement(d)
+        d.center = center if center is not None else hb.quadratic_placement(d)[0]
     elif name == "plain":
         d = design.plain_design(seed=2026)
         o = abi.default_options(target_density=0.9, bins_x=64, bins_y=64)
         desc = {"workload": "synthetic 2k cells, 2.3k nets, 40x400 sites, 64x64 bins, t=0.9, random init",
                 "cells": 2000, "bins": 64, "target_density": 0.9}
-        d.center = hb.random_placement(d, 2026)
+        d.center = center if center is not None else hb.random_placement(d, 2026)
     else:
         raise SystemExit(f"unknown workload {name}")
     o.iterations = 1_000_000
@@ -93,70 +95,194 @@ def make_workload(name: str):
     return d, o, desc
 
 
-def cpu_baseline(name: str, iterations: int) -> dict:
-    """The oracle's single-threaded FP64 GP loop on the same design (checker, not product)."""
+def dropin_sample(prep, calls: int) -> dict:
+    """The drop-in boundary as the reference defines it (CudaPlacementBackend::evaluate): positions
+    copied in from host memory and, with gradients, two gradient vectors copied out, on every call.
+    PCIe-inclusive time per call on the workload's design at its starting positions; never `value`."""
+    from myplacement_amd import backend as hb
+    layout = prep.layout()
+    be = hb.HipPlacementBackend.create(layout)
+    xy = prep.initial_positions()
+    r = layout["region"]
+    gamma = 0.5 * max((r[2] - r[0]) / layout["bins_x"], (r[3] - r[1]) / layout["bins_y"])
+    out = {}
+    for grad in (True, False):
+        for _ in range(3):
+            be.evaluate(xy, gamma, grad)
+        t = time.perf_counter()
+        for _ in range(calls):
+            be.evaluate(xy, gamma, grad)
+        out["gradient_ms" if grad else "value_ms"] = 1000.0 * (time.perf_counter() - t) / calls
+    be.close()
+    out["calls"] = calls
+    out["note"] = "mp_gp_evaluate with pageable host buffers, copies staged through pinned memory"
+    log(f"drop-in evaluate: {out['gradient_ms']:.3f} ms with gradients, {out['value_ms']:.3f} ms value-only")
+    return out
+
+
+def host_description() -> dict:
+    """nproc, the CPU model and the cores this process may use (the CPU baseline's hardware)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0)), "cpu_model": model}
+
+
+def cpu_baseline_run(name: str, iterations: int, start_file: str) -> dict:
+    """One oracle run (checker, not product) of the workload in this process: FP64, single thread."""
     from oracle import bindings as ob
-    d, o, _ = make_workload(name)
+    d, o, _ = make_workload(name, center=np.load(start_file))
     o.iterations = iterations
     _, r, _ = ob.global_place(d, o)
-    return {"value": iterations / r["loop_seconds"], "unit": "GP iterations/s", "cores": 1, "kind": "port",
-            "sample": f"{iterations} adaptive iterations of the same workload, FP64 oracle (own FFT DCT), 1 thread",
-            "loop_seconds": r["loop_seconds"]}
+    return {"loop_seconds": r["loop_seconds"], "elapsed_seconds": r["elapsed_seconds"],
+            "iterations": r["completed_iterations"]}
 
 
-def serving_sample(jobs: int, rate: float, slots: int, seed: int = 2026, mode: str = "streams") -> dict:
-    """A bounded sample of the served mixed-size stream: `jobs` requests with Poisson arrivals at
-    `rate` per second, sizes log-uniform in 1k-200k cells (designs.served_design), bin grids by
-    designs.served_bins, target density uniform in 0.7-0.9, random initial placement, each run until
-    design overflow <= 0.10 (checked from iteration 21 on) or 280 iterations.  Jobs are generated
-    before the stream starts; latency is submit -> result.  mode: the service's mode ("streams": a
+def cpu_baseline(name: str, iterations: int, repeats: int, start_file: str) -> dict:
+    """The oracle's FP64 GP loop on the same design and start, `repeats` runs, each in a child
+    process pinned to one core (the reference's CPU path is single-threaded), median reported:
+    loop-only iterations/s (the metric's unit) and set-up-inclusive iterations/s (the reference's
+    elapsed_seconds counts set-up too)."""
+    import statistics
+    import subprocess
+    core = min(os.sched_getaffinity(0))
+    runs = []
+    for k in range(repeats):
+        out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", name,
+                              "--cpu-iterations", str(iterations), "--start-file", start_file],
+                             capture_output=True, text=True, preexec_fn=lambda: os.sched_setaffinity(0, {core}))
+        if out.returncode != 0:
+            raise RuntimeError("cpu baseline run failed: " + out.stderr[-2000:])
+        runs.append(json.loads(out.stdout.strip().splitlines()[-1]))
+        log(f"cpu baseline repeat {k + 1}/{repeats}: {runs[-1]['iterations'] / runs[-1]['loop_seconds']:.2f} it/s")
+    loop = statistics.median(r["iterations"] / r["loop_seconds"] for r in runs)
+    total = statistics.median(r["iterations"] / r["elapsed_seconds"] for r in runs)
+    return dict({"value": loop, "unit": "GP iterations/s", "cores": 1, "kind": "port",
+                 "sample": f"{iterations} adaptive iterations of the same workload from the same start, FP64 oracle "
+                           f"(own FFT DCT, FFTW absent), 1 thread pinned to one core, median of {repeats} runs",
+                 "setup_inclusive_value": total, "repeats": [r["iterations"] / r["loop_seconds"] for r in runs],
+                 "pinned_core": core}, **host_description())
+
+
+def hpwl_deviation_sample(work, reports, pick, max_cells: int) -> dict:
+    """Per-job final HPWL and overflow of served jobs against the FP64 oracle's run of the same job
+    (same design, start and options), outside the timed stream, for the requests `pick` (chosen
+    evenly spaced in stream order among those of at most `max_cells` cells: the oracle runs ~10^4 x
+    slower than the device).  Each pair is one run per side, so at small sizes the chaotic spread
+    between runs (~0.5 % at 2k cells) is part of the deviation."""
+    from oracle import bindings as ob
+    if not pick:
+        return {"jobs": 0}
+    rows = []
+    t = time.perf_counter()
+    for i in pick:
+        d, o, cells, side = work[i]
+        _, r, _ = ob.global_place(d, o)
+        mine = reports[i]["result"]
+        rows.append({"request": i, "cells": cells, "bins": side,
+                     "hpwl_deviation": mine["hpwl_after"] / r["hpwl_after"] - 1.0,
+                     "overflow_deviation": mine["overflow_after"] - r["overflow_after"],
+                     "iterations": [mine["completed_iterations"], r["completed_iterations"]]})
+    dev = np.array([r["hpwl_deviation"] for r in rows])
+    ovf = np.array([r["overflow_deviation"] for r in rows])
+    log(f"served-job HPWL deviation vs the FP64 oracle: {len(rows)} jobs in {time.perf_counter() - t:.1f}s, "
+        f"mean {dev.mean():+.3%}, max |{np.abs(dev).max():.3%}|")
+    return {"jobs": len(rows), "max_cells": max_cells, "mean_hpwl_deviation": float(dev.mean()),
+            "mean_abs_hpwl_deviation": float(np.abs(dev).mean()), "max_abs_hpwl_deviation": float(np.abs(dev).max()),
+            "mean_overflow_deviation": float(ovf.mean()), "max_abs_overflow_deviation": float(np.abs(ovf).max()),
+            "per_job": rows}
+
+
+def serving_sample(jobs: int, rate: float, slots: int, seed: int = 2026, mode: str = "streams",
+                   capacity_jobs: int = 200, deviation_jobs: int = 16, deviation_max_cells: int = 30000) -> dict:
+    """The served mixed-size stream: `jobs` requests with Poisson arrivals at `rate` per second,
+    sizes log-uniform in 1k-200k cells (design.served_stream_specs, design.served_design), bin grids
+    by design.served_bins, target density uniform in 0.7-0.9, random initial placement, each run
+    until design overflow <= 0.10 (checked from iteration 21 on) or 280 iterations.  A producer
+    thread builds each request's design ahead of its arrival time (the designs of ~75M cells in all
+    are never held at once); latency is submit -> result.  mode: the service's mode ("streams": a
     stream per job; "batched": lockstep rounds of batched launches, jobs joining between rounds)."""
+    import queue
     import threading
     from myplacement_amd import abi, design
     from myplacement_amd import backend as hb
     specs, gaps = design.served_stream_specs(jobs, rate, seed)
-    work = []
-    for job_seed, cells, side, t in specs:
+    eligible = [i for i, sp in enumerate(specs) if sp[1] <= deviation_max_cells]
+    picks = {eligible[int(k * len(eligible) / deviation_jobs)] for k in range(min(deviation_jobs, len(eligible)))}
+    keep = set(range(min(capacity_jobs, jobs))) | picks
+
+    def build(i):
+        job_seed, cells, side, t = specs[i]
         d = design.served_design(job_seed, cells)
         d.center = hb.random_placement(d, job_seed)
         o = abi.default_options(target_density=t, bins_x=side, bins_y=side, iterations=280,
                                 feasible_refinement_iterations=1, seed=job_seed)
-        work.append((d, o, cells, side))
+        return d, o, cells, side
+
+    ready = queue.Queue(maxsize=64)
+
+    def produce():
+        for i in range(jobs):
+            ready.put(build(i))
+
+    producer = threading.Thread(target=produce, daemon=True)
+    producer.start()
     server = hb.PlacementServer(max_active_jobs=slots, memory_budget_bytes=200 << 30, mode=mode)
     reports = [None] * jobs
+    kept = {}
     threads = []
 
     def collect(i, job, m):
         reports[i] = server.wait(job, m)[1]
 
+    log(f"served stream: {jobs} requests, {sum(sp[1] for sp in specs)} cells in total")
+    late = 0.0
     t0 = time.perf_counter()
     due = 0.0
-    for i, (d, o, cells, side) in enumerate(work):
+    for i in range(jobs):
+        w = ready.get()
         due += gaps[i]
         delay = due - (time.perf_counter() - t0)
         if delay > 0:
             time.sleep(delay)
-        job = server.submit(d, o)
-        th = threading.Thread(target=collect, args=(i, job, d.module_count))
+        else:
+            late = max(late, -delay)
+        job = server.submit(w[0], w[1])
+        if i in keep:
+            kept[i] = w
+        th = threading.Thread(target=collect, args=(i, job, w[0].module_count))
         th.start()
         threads.append(th)
+        if (i + 1) % 100 == 0:
+            log(f"served stream: {i + 1} requests submitted at {time.perf_counter() - t0:.0f}s, "
+                f"{sum(r is not None for r in reports)} done")
     for th in threads:
         th.join()
+    producer.join()
+    work = [kept.get(i) for i in range(jobs)]
     wall = time.perf_counter() - t0
     server.close()
     lat = np.array([r["latency_seconds"] for r in reports])
     conv = [r["result"]["overflow_after"] <= 0.10 + 1e-12 for r in reports]
     first = min(r["submit_seconds"] for r in reports)
     last = max(r["finish_seconds"] for r in reports)
-    # capacity: the same requests all submitted at once, to the plain backend one job at a time and
-    # to the batched service
-    capacity = {}
-    for name, m, k in (("plain_one_at_a_time", "streams", 1), ("batched", "batched", jobs)):
-        span, reps = run_all_at_once([(d, o) for d, o, _, _ in work], m, k)
-        capacity[name] = {"mode": m, "slots": k, "makespan_s": span, "jobs_per_hour": 3600.0 * jobs / span,
+    # capacity: the stream's first `capacity_jobs` requests all submitted at once, to the plain
+    # backend one job at a time and to the batched service
+    capacity = {"jobs": min(capacity_jobs, jobs)}
+    sub = [(w[0], w[1]) for w in work[:capacity["jobs"]]]
+    for name, m, k in (("plain_one_at_a_time", "streams", 1), ("batched", "batched", len(sub))):
+        span, reps = run_all_at_once(sub, m, k)
+        capacity[name] = {"mode": m, "slots": k, "makespan_s": span, "jobs_per_hour": 3600.0 * len(sub) / span,
                           "gp_iterations_per_s": sum(r["result"]["completed_iterations"] for r in reps) / span}
         log(f"serving capacity: {name}, makespan {span:.2f}s")
     capacity["ratio"] = capacity["batched"]["jobs_per_hour"] / capacity["plain_one_at_a_time"]["jobs_per_hour"]
+    deviation = hpwl_deviation_sample(work, reports, sorted(picks), deviation_max_cells)
     return {
         "capacity_all_at_once": capacity,
         "jobs": jobs, "arrival_rate_per_s": rate, "slots": slots, "mode": mode,
@@ -164,9 +290,11 @@ def serving_sample(jobs: int, rate: float, slots: int, seed: int = 2026, mode: s
         "mean_latency_s": float(lat.mean()), "jobs_per_hour": 3600.0 * jobs / (last - first),
         "converged_fraction": float(np.mean(conv)),
         "mean_iterations": float(np.mean([r["result"]["completed_iterations"] for r in reports])),
-        "cells_range": [int(min(w[2] for w in work)), int(max(w[2] for w in work))], "wall_s": wall,
+        "cells_range": [int(min(sp[1] for sp in specs)), int(max(sp[1] for sp in specs))], "wall_s": wall,
+        "latest_submission_s": late,
         "sample": f"{jobs} requests, Poisson {rate}/s, 1k-200k cells log-uniform, t~U(0.7,0.9), "
                   "bins 32-256 by >=4 cells/bin, stop at overflow<=0.10 or 280 iterations",
+        "hpwl_vs_oracle": deviation,
     }
 
 
@@ -267,9 +395,18 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=270)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="large")
-    ap.add_argument("--cpu-iterations", type=int, default=60)
+    ap.add_argument("--cpu-iterations", type=int, default=40, help="oracle iterations per CPU-baseline repeat")
+    ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--cpu-baseline-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--start-file", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--dropin-evaluations", type=int, default=30,
+                    help="timed calls of the reference-shaped mp_gp_evaluate with host buffers (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0)
-    ap.add_argument("--serve-jobs", type=int, default=40, help="served-stream sample size (0: skip)")
+    ap.add_argument("--serve-jobs", type=int, default=2000, help="served-stream requests (0: skip)")
+    ap.add_argument("--serve-capacity-jobs", type=int, default=200,
+                    help="requests of the stream submitted at once for the capacity comparison")
+    ap.add_argument("--serve-deviation-jobs", type=int, default=16,
+                    help="served jobs re-run in the FP64 oracle for the per-job HPWL deviation")
     ap.add_argument("--serve-rate", type=float, default=8.0)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-mode", default="batched", choices=("streams", "batched"))
@@ -283,6 +420,9 @@ def main() -> int:
                     help="service mode:slots[:bf16x3] runs of the batch workload")
     ap.add_argument("--batch-iterations", type=int, default=100)
     args = ap.parse_args()
+    if args.cpu_baseline_child:
+        print(json.dumps(cpu_baseline_run(args.cpu_baseline_child, args.cpu_iterations, args.start_file)), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU split is a design note, "
               "not a measured path", file=sys.stderr)
@@ -298,7 +438,8 @@ def main() -> int:
         return 0
     from myplacement_amd import backend as hb
     if args.serve_only:
-        sv = serving_sample(args.serve_jobs, args.serve_rate, args.serve_slots, mode=args.serve_mode)
+        sv = serving_sample(args.serve_jobs, args.serve_rate, args.serve_slots, mode=args.serve_mode,
+                            capacity_jobs=args.serve_capacity_jobs, deviation_jobs=args.serve_deviation_jobs)
         print(json.dumps({"serving": sv}), flush=True)
         return 0
     if args.workload == "batch":
@@ -314,6 +455,7 @@ def main() -> int:
         print(json.dumps(line), flush=True)
         return 0
     d, o, desc = make_workload(args.workload)
+    start_center = d.center.copy()
     t0 = time.perf_counter()
     prep = hb.PreparedDesign(d, o)
     sess = hb.PlacementSession(prep, o)
@@ -348,6 +490,7 @@ def main() -> int:
                 "avg_launch_us": avg_us, "launches": live["launches"],
                 "algorithmic_bytes_per_launch": live["work"] / live["launches"]}
     _, res = sess.finish()
+    dropin = dropin_sample(prep, args.dropin_evaluations) if args.dropin_evaluations > 0 else None
 
     line = {
         "metric": METRICS[args.workload],
@@ -370,11 +513,14 @@ def main() -> int:
                              if v["launches"]},
         "roofline_dct": dct_roofline(timing.get("k_gemm"), o.dct_precision),
         "setup_seconds": setup_s,
+        "dropin_boundary": dropin,
     }
     sess.close()
     if args.serve_jobs > 0:
         log(f"served stream sample: {args.serve_jobs} jobs ...")
-        line["serving"] = serving_sample(args.serve_jobs, args.serve_rate, args.serve_slots, mode=args.serve_mode)
+        line["serving"] = serving_sample(args.serve_jobs, args.serve_rate, args.serve_slots, mode=args.serve_mode,
+                                         capacity_jobs=args.serve_capacity_jobs,
+                                         deviation_jobs=args.serve_deviation_jobs)
     if args.batch_jobs > 0:
         log(f"batching sample: {args.batch_jobs} blocks ...")
         b = batch_sample(args.batch_jobs, parse_batch_runs(args.batch_runs), args.batch_iterations)
@@ -383,9 +529,12 @@ def main() -> int:
         b["best"] = f"{best['mode']}/{best['slots']}/{best['precision']}"
         line["batching"] = b
     if not args.no_cpu_baseline:
+        import tempfile
         log("cpu baseline ...")
-        cb = cpu_baseline(args.workload, args.cpu_iterations)
-        line["cpu_baseline"] = cb
+        with tempfile.TemporaryDirectory() as tmp:
+            start_file = os.path.join(tmp, "start.npy")
+            np.save(start_file, start_center)
+            line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_iterations, args.cpu_repeats, start_file)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -395,6 +395,7 @@ Backend::~Backend() {
         if (owns_stream_) (void)hipStreamDestroy(stream_);
     }
     if (host_scal_ && owns_scalars_) (void)hipHostFree(host_scal_);
+    if (host_stage_) (void)hipHostFree(host_stage_);
 }
 
 void Backend::setDevice() const { hipCheck(hipSetDevice(device_), "select HIP device"); }
@@ -828,6 +829,11 @@ void Backend::toScalars(const HostScalars& hs, mp_gp_scalars* out, int bank) con
     fill(kDesignDensity, out->design_density);
 }
 
+namespace {
+constexpr std::size_t kStageMinBytes = 1u << 20;  // designs of 65,536 particles and more
+constexpr std::size_t kStageGrain = 1u << 19;     // bytes per host copy thread
+}  // namespace
+
 void Backend::evaluateHost(const double* xy, double gamma, bool want_grad, mp_gp_scalars* out, double* wire_grad,
                            double* density_grad) {
     if (!std::isfinite(gamma) || gamma <= 0.0)
@@ -837,18 +843,43 @@ void Backend::evaluateHost(const double* xy, double gamma, bool want_grad, mp_gp
         throw InvalidArgument("gradient outputs are required when want_grad is set");
     setDevice();
     hipStream_t s = stream_;
-    pos_.upload(reinterpret_cast<const double2*>(xy), P_, s, "upload particle positions");
+    const std::size_t bytes = static_cast<std::size_t>(P_) * sizeof(double2);
+    // Large designs stage through pinned memory: the host copies run on several threads and the
+    // transfers at the link's pinned rate, instead of the runtime's single-threaded pageable path.
+    if (bytes >= kStageMinBytes && host_stage_ == nullptr)
+        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&host_stage_), 3 * bytes, hipHostMallocDefault),
+                 "allocate pinned staging");
+    const bool staged = bytes >= kStageMinBytes;
+    auto hostCopy = [](void* dst, const void* src, std::size_t n) {
+        parallelFor(n, kStageGrain, [&](std::size_t a, std::size_t b) {
+            std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
+        });
+    };
+    if (staged) {
+        hostCopy(host_stage_, xy, bytes);
+        hipCheck(hipMemcpyAsync(pos_.get(), host_stage_, bytes, hipMemcpyHostToDevice, s), "upload particle positions");
+    } else {
+        pos_.upload(reinterpret_cast<const double2*>(xy), P_, s, "upload particle positions");
+    }
     const EvalSpec spec{pos_.get(), nullptr, 0, true};
     enqueueEvaluate(&spec, 1, gamma, want_grad);
     if (want_grad && P_ > 0) {
-        hipCheck(hipMemcpyAsync(wire_grad, wire_out_.get(), P_ * sizeof(double2), hipMemcpyDeviceToHost, s),
+        hipCheck(hipMemcpyAsync(staged ? static_cast<void*>(host_stage_ + P_) : wire_grad, wire_out_.get(), bytes,
+                                hipMemcpyDeviceToHost, s),
                  "download wire gradients");
-        hipCheck(hipMemcpyAsync(density_grad, density_out_.get(), P_ * sizeof(double2), hipMemcpyDeviceToHost, s),
+        hipCheck(hipMemcpyAsync(staged ? static_cast<void*>(host_stage_ + 2 * static_cast<std::size_t>(P_)) : density_grad,
+                                density_out_.get(), bytes, hipMemcpyDeviceToHost, s),
                  "download density gradients");
     }
     const HostScalars& h = fetchScalars();
-    // the gradient copies go to pageable host memory: wait for the stream to finish them
-    if (want_grad && P_ > 0) hipCheck(hipStreamSynchronize(s), "download gradients");
+    // the gradient copies follow the reductions on the stream: wait for them to finish
+    if (want_grad && P_ > 0) {
+        hipCheck(hipStreamSynchronize(s), "download gradients");
+        if (staged) {
+            hostCopy(wire_grad, host_stage_ + P_, bytes);
+            hostCopy(density_grad, host_stage_ + 2 * static_cast<std::size_t>(P_), bytes);
+        }
+    }
     toScalars(h, out);
     if (!std::isfinite(out->smooth_wirelength) || !std::isfinite(out->hpwl) || !std::isfinite(out->electrostatic_energy))
         throw std::runtime_error("HIP global-placement evaluation produced a non-finite metric.");

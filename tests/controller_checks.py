@@ -112,7 +112,8 @@ def check_history_composition(o: abi.MpOptions, result: dict, hist: list, bin_w:
             streak = 0
 
     # displacement bound of accepted moves
-    cap = o.maximum_movement_in_bins * minbin * (1.0 + 1e-12)
+    # (the move is measured as |base - new| after both were rounded at the positions' magnitude)
+    cap = o.maximum_movement_in_bins * minbin * (1.0 + 1e-9)
     for h in hist:
         if h["accepted"]:
             assert 0.0 < h["maximum_displacement"] <= cap, (h["iteration"], h["maximum_displacement"], cap)
