@@ -844,31 +844,22 @@ void Backend::evaluateHost(const double* xy, double gamma, bool want_grad, mp_gp
     setDevice();
     hipStream_t s = stream_;
     const std::size_t bytes = static_cast<std::size_t>(P_) * sizeof(double2);
-    // Large designs stage through pinned memory: the host copies run on several threads and the
-    // transfers at the link's pinned rate, instead of the runtime's single-threaded pageable path.
-    if (bytes >= kStageMinBytes && host_stage_ == nullptr)
-        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&host_stage_), 3 * bytes, hipHostMallocDefault),
+    // The positions go up through the runtime's pageable path (measured faster than staging them
+    // for one 4 MB vector); a large design's two gradient vectors come down into pinned memory at the
+    // link's pinned rate and are copied out on several host threads.
+    pos_.upload(reinterpret_cast<const double2*>(xy), P_, s, "upload particle positions");
+    const bool staged = want_grad && bytes >= kStageMinBytes;
+    if (staged && host_stage_ == nullptr)
+        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&host_stage_), 2 * bytes, hipHostMallocDefault),
                  "allocate pinned staging");
-    const bool staged = bytes >= kStageMinBytes;
-    auto hostCopy = [](void* dst, const void* src, std::size_t n) {
-        parallelFor(n, kStageGrain, [&](std::size_t a, std::size_t b) {
-            std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
-        });
-    };
-    if (staged) {
-        hostCopy(host_stage_, xy, bytes);
-        hipCheck(hipMemcpyAsync(pos_.get(), host_stage_, bytes, hipMemcpyHostToDevice, s), "upload particle positions");
-    } else {
-        pos_.upload(reinterpret_cast<const double2*>(xy), P_, s, "upload particle positions");
-    }
     const EvalSpec spec{pos_.get(), nullptr, 0, true};
     enqueueEvaluate(&spec, 1, gamma, want_grad);
     if (want_grad && P_ > 0) {
-        hipCheck(hipMemcpyAsync(staged ? static_cast<void*>(host_stage_ + P_) : wire_grad, wire_out_.get(), bytes,
+        hipCheck(hipMemcpyAsync(staged ? static_cast<void*>(host_stage_) : wire_grad, wire_out_.get(), bytes,
                                 hipMemcpyDeviceToHost, s),
                  "download wire gradients");
-        hipCheck(hipMemcpyAsync(staged ? static_cast<void*>(host_stage_ + 2 * static_cast<std::size_t>(P_)) : density_grad,
-                                density_out_.get(), bytes, hipMemcpyDeviceToHost, s),
+        hipCheck(hipMemcpyAsync(staged ? static_cast<void*>(host_stage_ + P_) : density_grad, density_out_.get(), bytes,
+                                hipMemcpyDeviceToHost, s),
                  "download density gradients");
     }
     const HostScalars& h = fetchScalars();
@@ -876,8 +867,17 @@ void Backend::evaluateHost(const double* xy, double gamma, bool want_grad, mp_gp
     if (want_grad && P_ > 0) {
         hipCheck(hipStreamSynchronize(s), "download gradients");
         if (staged) {
-            hostCopy(wire_grad, host_stage_ + P_, bytes);
-            hostCopy(density_grad, host_stage_ + 2 * static_cast<std::size_t>(P_), bytes);
+            parallelFor(2 * bytes, kStageGrain, [&](std::size_t a, std::size_t b) {
+                // [0, bytes): wire gradient, [bytes, 2 bytes): density gradient
+                const char* src = reinterpret_cast<const char*>(host_stage_);
+                while (a < b) {
+                    const bool wire = a < bytes;
+                    const std::size_t end = wire ? std::min(b, bytes) : b;
+                    char* dst = wire ? reinterpret_cast<char*>(wire_grad) + a : reinterpret_cast<char*>(density_grad) + (a - bytes);
+                    std::memcpy(dst, src + a, end - a);
+                    a = end;
+                }
+            });
         }
     }
     toScalars(h, out);

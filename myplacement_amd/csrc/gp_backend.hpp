@@ -346,8 +346,8 @@ private:
     DeviceArray<double2> wire_out_, density_out_;
     bool owns_stream_ = true, owns_scalars_ = true;
     HostScalars* host_scal_ = nullptr;  // host-mapped, written by k_finalize
-    // pinned staging of evaluateHost's positions and gradients (3 x P double2), allocated on the
-    // first evaluateHost call of a large design
+    // pinned staging of evaluateHost's gradients (2 x P double2), allocated on the first gradient
+    // evaluateHost call of a large design
     double2* host_stage_ = nullptr;
     double* dev_scal_ = nullptr;        // device address of host_scal_
     DeviceArray<double> scal_copy_;     // the banks' copy in device memory (k_finalize writes both)

@@ -117,15 +117,18 @@ int adp_positions(const adp_run* r, double* xy) {
 
 int64_t adp_particle_count(const adp_run* r) { return static_cast<int64_t>(r->movable.size() + r->fillers.size()); }
 
-// CudaPlacementBackend::evaluate through the adapter; gradients [2P] when want_grad.
-int adp_evaluate(adp_run* r, const double* xy, double smoothing, int want_grad, mp_gp_scalars* out, double* wire_grad,
-                 double* density_grad) {
+// CudaPlacementBackend::evaluate through the adapter on `count` positions xy [2 count]; gradients
+// [2 count] when want_grad (the adapter rejects a count other than the particle count before it
+// writes anything).
+int adp_evaluate(adp_run* r, const double* xy, int64_t count, double smoothing, int want_grad, mp_gp_scalars* out,
+                 double* wire_grad, double* density_grad) {
     return adpGuard([&] {
         if (!r->backend) throw std::runtime_error("no backend");
-        const std::size_t P = r->movable.size() + r->fillers.size();
-        std::vector<Vec2> positions(P);
-        for (std::size_t i = 0; i < P; ++i) positions[i] = {xy[2 * i], xy[2 * i + 1]};
+        if (count < 0) throw std::invalid_argument("negative position count");
+        std::vector<Vec2> positions(static_cast<std::size_t>(count));
+        for (std::size_t i = 0; i < positions.size(); ++i) positions[i] = {xy[2 * i], xy[2 * i + 1]};
         const CudaPlacementEvaluation e = r->backend->evaluate(positions, smoothing, want_grad != 0);
+        const std::size_t P = positions.size();
         out->hpwl = e.hpwl;
         out->smooth_wirelength = e.smooth_wirelength;
         out->electrostatic_energy = e.electrostatic_energy;

@@ -385,7 +385,7 @@ def adapter():
         lib.adp_positions.argtypes = [V, V]
         lib.adp_particle_count.argtypes = [V]
         lib.adp_particle_count.restype = C.c_int64
-        lib.adp_evaluate.argtypes = [V, V, C.c_double, C.c_int, C.POINTER(abi.MpScalars), V, V]
+        lib.adp_evaluate.argtypes = [V, V, C.c_int64, C.c_double, C.c_int, C.POINTER(abi.MpScalars), V, V]
         lib.adp_device.argtypes = [V]
         lib.adp_reserved_bytes.argtypes = [V]
         lib.adp_reserved_bytes.restype = C.c_uint64
@@ -453,7 +453,7 @@ class AdapterRun:
         s = abi.MpScalars()
         wg = np.zeros_like(xy) if want_grad else None
         dg = np.zeros_like(xy) if want_grad else None
-        st = lib.adp_evaluate(self.h, _p(xy), smoothing, int(want_grad), C.byref(s), _p(wg), _p(dg))
+        st = lib.adp_evaluate(self.h, _p(xy), xy.shape[0], smoothing, int(want_grad), C.byref(s), _p(wg), _p(dg))
         if st:
             raise (ValueError if st == 1 else RuntimeError)(lib.adp_last_error().decode())
         return dict(hpwl=s.hpwl, smooth_wirelength=s.smooth_wirelength, electrostatic_energy=s.electrostatic_energy,

@@ -40,11 +40,9 @@ def adapter():
 
 
 def _gpu_present() -> bool:
-    try:
-        import torch
-        return torch.cuda.device_count() > 0
-    except Exception:
-        return False
+    # the kernel driver's device node, not torch: torch brings its own HIP runtime, which must not
+    # be loaded beside the one the product library links
+    return os.path.exists("/dev/kfd")
 
 
 def _aux_cases(tmp_path_factory):
