@@ -207,6 +207,18 @@ void mp_gp_destroy(mp_gp_backend* backend);
 mp_gp_status mp_gp_debug_grids(mp_gp_backend* backend, double* source, double* potential,
                                double* field_x, double* field_y);
 
+/* Autotuner support (no counterpart in the reference, whose cuFFT transforms have no tile shapes):
+ * the average time in microseconds of `reps` launches of one half-length DCT product of an
+ * nx x ny grid (both multiples of 64) with tile shape `tile` (0..7, see gp_kernels.hpp), for
+ * `evaluations` evaluations batched as the evaluation chain batches them, on device 0, with
+ * operands of the chain's shapes.  myplacement_amd/autotune.py times every valid tile this way and
+ * writes the table the library compiles in (csrc/dct_tiles.inc).  MP_GP_INVALID_ARGUMENT for a
+ * tile that does not divide the product. */
+mp_gp_status mp_gp_dct_tile_time(int32_t nx, int32_t ny, int32_t precision, int32_t mode, int32_t tile,
+                                 int32_t evaluations, int32_t reps, double* microseconds);
+/* The tile the library uses for a product (from its compiled table; 0 where the table names none). */
+int32_t mp_gp_dct_tile_for(int32_t nx, int32_t ny, int32_t precision, int32_t mode);
+
 /* ---- host layout services (GlobalPlacementSession set-up) --------------------------------- */
 
 /* Default option values of GlobalPlacementOptions (GlobalPlacer.hpp:26-80). */

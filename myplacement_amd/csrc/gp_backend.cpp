@@ -27,6 +27,8 @@
 #include <memory>
 #include <numeric>
 #include <sstream>
+#include <type_traits>
+#include <vector>
 
 namespace mpgp {
 
@@ -577,7 +579,9 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
         dct(kFoldCols, f1, count, nxp_, 0, nxp_, 1);
         dct(kFoldRows, f2, count, 0, nxp_, nxp_, 1);
         for (int e = 0; e < count; ++e)
-            addReduction(kRedEnergy, specs[e].bank, (nx_ / 32) * (ny_ / 64), 0.5 * region_.bw * region_.bh);
+            addReduction(kRedEnergy, specs[e].bank,
+                         dctWorkgroups(dctTileFor(kFoldRows, dct_precision_, ny_, nx_), kFoldRows, ny_, nx_),
+                         0.5 * region_.bw * region_.bh);
         if (!want_grad) return;
         // [T1 | T2] = mirror_x(phihat) ; Ex = mirror_y(T2, cosine) ; Ey = mirror_y(T1, sine)
         DctBatch i1{}, i2{};
@@ -743,7 +747,7 @@ void RecordLaunches::dct(int mode, const DctBatch& d, int batch, int rows, int c
                          int step, int precision, const SpectralArgs& sp) {
     if (spectral_ >= 4) throw std::logic_error("more than four spectral products in one evaluation");
     const DctEntry e = dctEntry(mode, d, batch, rows, cols, lda, ldb, ldc, step, precision, sp);
-    add(kStageSpectral + spectral_++, kBatchDct + 2 * e.mode + (e.kpl == 8 ? 1 : 0), e, e.gx * e.gy * e.gz);
+    add(kStageSpectral + spectral_++, kBatchDct + dctVariant(e), e, e.gx * e.gy * e.gz);
 }
 void RecordLaunches::gather(const GatherArgs& a, int count) {
     if (count < 1 || count > kMaxEval) throw std::invalid_argument("bad evaluation batch");
@@ -1052,3 +1056,89 @@ mp_gp_status mp_gp_debug_grids(mp_gp_backend* b, double* source, double* potenti
 }
 
 }  // extern "C"
+
+// ---- autotuner timing of the half-length product tiles ---------------------------------------
+namespace mpgp {
+namespace {
+struct DeviceScratchBuffer {
+    void* p = nullptr;
+    explicit DeviceScratchBuffer(std::size_t bytes) { hipCheck(hipMalloc(&p, bytes), "allocate tuning buffer"); }
+    ~DeviceScratchBuffer() {
+        if (p) (void)hipFree(p);
+    }
+    DeviceScratchBuffer(const DeviceScratchBuffer&) = delete;
+    DeviceScratchBuffer& operator=(const DeviceScratchBuffer&) = delete;
+};
+}  // namespace
+}  // namespace mpgp
+
+extern "C" mp_gp_status mp_gp_dct_tile_time(int32_t nx, int32_t ny, int32_t precision, int32_t mode, int32_t tile,
+                                            int32_t evaluations, int32_t reps, double* microseconds) {
+    using namespace mpgp;
+    g_last_error.clear();
+    try {
+        if (!microseconds || reps < 1 || evaluations < 1 || evaluations > kMaxEval)
+            throw std::invalid_argument("bad tuning arguments");
+        if (!foldedDct(nx, ny)) throw std::invalid_argument("half-length products need both bin counts multiples of 64");
+        if (mode < kFoldCols || mode > kMirrorRows) throw std::invalid_argument("unknown DCT product mode");
+        if (precision != MP_DCT_FP32 && precision != MP_DCT_BF16X3) throw std::invalid_argument("unknown DCT precision");
+        if (!dctTileValid(tile, mode, ny, nx)) throw std::invalid_argument("DCT tile does not divide the product");
+        std::string reason;
+        if (allocationLimit(0, ~0ull, reason) == 0) throw std::runtime_error(reason);
+        // operands shaped as the evaluation's products (gp_backend.cpp, enqueueEvaluate): data grids of
+        // ny x 2 nx floats (the widest, [T1 | T2] and the interleaved field), factors of n x 2n
+        const std::size_t grid = static_cast<std::size_t>(ny) * 2 * nx, factor = 2 * static_cast<std::size_t>(std::max(nx, ny)) * std::max(nx, ny);
+        const int batch = mode == kMirrorRows ? 2 * evaluations : evaluations;
+        DeviceScratchBuffer data(sizeof(float) * grid * batch), fac(sizeof(float) * factor), out(sizeof(float) * grid * batch),
+            part(sizeof(double) * static_cast<std::size_t>(dctWorkgroups(tile, mode, ny, nx)) * batch);
+        hipStream_t s = nullptr;
+        hipCheck(hipStreamCreate(&s), "create tuning stream");
+        std::unique_ptr<std::remove_pointer_t<hipStream_t>, decltype(&hipStreamDestroy)> own(s, hipStreamDestroy);
+        {
+            std::vector<float> h(std::max(grid * batch, factor));
+            for (std::size_t i = 0; i < h.size(); ++i) h[i] = static_cast<float>((i * 2654435761u % 1000) * 1e-3 - 0.5);
+            hipCheck(hipMemcpy(data.p, h.data(), sizeof(float) * grid * batch, hipMemcpyHostToDevice), "fill tuning data");
+            hipCheck(hipMemcpy(fac.p, h.data(), sizeof(float) * factor, hipMemcpyHostToDevice), "fill tuning factors");
+        }
+        const int nxp = nx;  // multiples of 64 need no padding
+        DctBatch d{};
+        SpectralArgs sp{nx, ny, static_cast<double>(nx), static_cast<double>(ny), 1.0 / (4.0 * nx * ny), {}};
+        for (int e = 0; e < batch; ++e) {
+            const bool data_left = mode == kFoldCols || mode == kMirrorCols;
+            float* dg = static_cast<float*>(data.p) + grid * e;
+            d.A[e] = data_left ? dg : static_cast<float*>(fac.p);
+            d.B[e] = data_left ? static_cast<float*>(fac.p) : dg;
+            d.C[e] = static_cast<float*>(out.p) + grid * e;
+            d.sigma[e] = e % 2 ? -1.f : 1.f;
+            sp.partials[e] = static_cast<double*>(part.p) + static_cast<std::size_t>(dctWorkgroups(tile, mode, ny, nx)) * e;
+        }
+        int lda = 0, ldb = 0, ldc = nxp, step = 1;
+        switch (mode) {
+            case kFoldCols: lda = nxp; break;
+            case kFoldRows: ldb = nxp; break;
+            case kMirrorCols: lda = nxp; ldc = 2 * nxp; break;
+            default: ldb = 2 * nxp; step = 2; break;
+        }
+        for (int r = 0; r < 3; ++r)
+            launchDctTile(tile, mode, d, batch, ny, nx, lda, ldb, ldc, step, precision, sp, s);
+        hipEvent_t a, b;
+        hipCheck(hipEventCreate(&a), "create tuning event");
+        hipCheck(hipEventCreate(&b), "create tuning event");
+        hipCheck(hipEventRecord(a, s), "record tuning event");
+        for (int r = 0; r < reps; ++r) launchDctTile(tile, mode, d, batch, ny, nx, lda, ldb, ldc, step, precision, sp, s);
+        hipCheck(hipEventRecord(b, s), "record tuning event");
+        hipCheck(hipEventSynchronize(b), "time tile");
+        float ms = 0.f;
+        hipCheck(hipEventElapsedTime(&ms, a, b), "read tuning events");
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        *microseconds = 1000.0 * ms / reps;
+        return MP_GP_OK;
+    } catch (...) {
+        return translate(std::current_exception());
+    }
+}
+
+extern "C" int32_t mp_gp_dct_tile_for(int32_t nx, int32_t ny, int32_t precision, int32_t mode) {
+    return mpgp::foldedDct(nx, ny) ? mpgp::dctTileFor(mode, precision, ny, nx) : -1;
+}

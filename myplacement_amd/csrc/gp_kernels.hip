@@ -21,6 +21,8 @@
 #include "gp_kernels.hpp"
 
 #include <algorithm>
+#include <array>
+#include <utility>
 #include <climits>
 #include <cmath>
 #include <stdexcept>
@@ -802,8 +804,6 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g, int K, int lda, 
 // x = hi + lo with hi = bf16(x), lo = bf16(x - hi), and accumulates lo*hi + hi*lo + hi*hi in FP32
 // (the lo*lo term, below 2^-16 of the product, is dropped).  The bf16 instruction takes A[i][k]
 // from lane i + 16 (k / 8), element k % 8, and B[k][j] from lane j + 16 (k / 8).
-constexpr int kDctThreads = 512;
-constexpr int kDctWaves = kDctThreads / 64;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 template <int N>
@@ -820,13 +820,13 @@ __device__ inline void loadFloats(const float* p, float (&v)[N]) {
 
 // Operands of one chunk for this lane: a[p][row half][j] = A-side value at contraction index t + j
 // for rows r16 (+16), b[p][col half][j] likewise for columns r16 (+16), j < KPL.
-template <int MODE, int KPL>
+template <int MODE, int KPL, int RH, int CH>
 __device__ inline void dctLoad(const float* __restrict__ A, const float* __restrict__ B, int n, int h, int lda,
-                               int ldb, int row0, int col0, int t, int r16, float (&a)[2][2][KPL],
-                               float (&b)[2][2][KPL]) {
+                               int ldb, int row0, int col0, int t, int r16, float (&a)[2][RH][KPL],
+                               float (&b)[2][CH][KPL]) {
     if (MODE == kFoldCols) {
 #pragma unroll
-        for (int rh = 0; rh < 2; ++rh) {
+        for (int rh = 0; rh < RH; ++rh) {
             const float* ar = A + static_cast<size_t>(row0 + 16 * rh + r16) * lda;
             float x[KPL], y[KPL];
             loadFloats<KPL>(ar + t, x);
@@ -840,7 +840,7 @@ __device__ inline void dctLoad(const float* __restrict__ A, const float* __restr
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
-            for (int ch = 0; ch < 2; ++ch)
+            for (int ch = 0; ch < CH; ++ch)
 #pragma unroll
                 for (int j = 0; j < KPL; ++j)
                     b[p][ch][j] = B[static_cast<size_t>(p) * h * h + static_cast<size_t>(t + j) * h + col0 + 16 * ch + r16];
@@ -848,10 +848,10 @@ __device__ inline void dctLoad(const float* __restrict__ A, const float* __restr
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
-            for (int rh = 0; rh < 2; ++rh)
+            for (int rh = 0; rh < RH; ++rh)
                 loadFloats<KPL>(A + static_cast<size_t>(p) * h * h + static_cast<size_t>(row0 + 16 * rh + r16) * h + t, a[p][rh]);
 #pragma unroll
-        for (int ch = 0; ch < 2; ++ch)
+        for (int ch = 0; ch < CH; ++ch)
 #pragma unroll
             for (int j = 0; j < KPL; ++j) {
                 const int col = col0 + 16 * ch + r16;
@@ -862,7 +862,7 @@ __device__ inline void dctLoad(const float* __restrict__ A, const float* __restr
             }
     } else if (MODE == kMirrorCols) {
 #pragma unroll
-        for (int rh = 0; rh < 2; ++rh) {
+        for (int rh = 0; rh < RH; ++rh) {
             float x[2 * KPL];
             loadFloats<2 * KPL>(A + static_cast<size_t>(row0 + 16 * rh + r16) * lda + 2 * t, x);
 #pragma unroll
@@ -874,7 +874,7 @@ __device__ inline void dctLoad(const float* __restrict__ A, const float* __restr
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
-            for (int ch = 0; ch < 2; ++ch)
+            for (int ch = 0; ch < CH; ++ch)
 #pragma unroll
                 for (int j = 0; j < KPL; ++j)
                     b[p][ch][j] = B[static_cast<size_t>(p) * h * n + static_cast<size_t>(t + j) * n + col0 + 16 * ch + r16];
@@ -882,27 +882,28 @@ __device__ inline void dctLoad(const float* __restrict__ A, const float* __restr
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
-            for (int rh = 0; rh < 2; ++rh)
+            for (int rh = 0; rh < RH; ++rh)
                 loadFloats<KPL>(A + static_cast<size_t>(p) * h * h + static_cast<size_t>(row0 + 16 * rh + r16) * h + t, a[p][rh]);
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
-            for (int ch = 0; ch < 2; ++ch)
+            for (int ch = 0; ch < CH; ++ch)
 #pragma unroll
                 for (int j = 0; j < KPL; ++j)
                     b[p][ch][j] = B[static_cast<size_t>(2 * (t + j) + p) * ldb + col0 + 16 * ch + r16];
     }
 }
 
-__device__ inline void dctMma(const float (&a)[2][2][4], const float (&b)[2][2][4], floatx4 (&acc)[2][2][2]) {
+template <int RH, int CH>
+__device__ inline void dctMma(const float (&a)[2][RH][4], const float (&b)[2][CH][4], floatx4 (&acc)[2][RH][CH]) {
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int rh = 0; rh < 2; ++rh)
+            for (int rh = 0; rh < RH; ++rh)
 #pragma unroll
-                for (int ch = 0; ch < 2; ++ch)
+                for (int ch = 0; ch < CH; ++ch)
                     acc[p][rh][ch] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[p][rh][j], b[p][ch][j], acc[p][rh][ch], 0, 0, 0);
 }
 
@@ -915,19 +916,19 @@ __device__ inline void splitBf16(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
     }
 }
 
-__device__ inline void dctMma(const float (&a)[2][2][8], const float (&b)[2][2][8], floatx4 (&acc)[2][2][2]) {
+template <int RH, int CH>
+__device__ inline void dctMma(const float (&a)[2][RH][8], const float (&b)[2][CH][8], floatx4 (&acc)[2][RH][CH]) {
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        bf16x8 ah[2], al[2], bh[2], bl[2];
+        bf16x8 ah[RH], al[RH], bh[CH], bl[CH];
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            splitBf16(a[p][q], ah[q], al[q]);
-            splitBf16(b[p][q], bh[q], bl[q]);
-        }
+        for (int q = 0; q < RH; ++q) splitBf16(a[p][q], ah[q], al[q]);
 #pragma unroll
-        for (int rh = 0; rh < 2; ++rh)
+        for (int q = 0; q < CH; ++q) splitBf16(b[p][q], bh[q], bl[q]);
 #pragma unroll
-            for (int ch = 0; ch < 2; ++ch) {
+        for (int rh = 0; rh < RH; ++rh)
+#pragma unroll
+            for (int ch = 0; ch < CH; ++ch) {
                 floatx4 c = acc[p][rh][ch];
                 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rh], bh[ch], c, 0, 0, 0);
                 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rh], bl[ch], c, 0, 0, 0);
@@ -936,100 +937,132 @@ __device__ inline void dctMma(const float (&a)[2][2][8], const float (&b)[2][2][
     }
 }
 
-template <int MODE, int KPL>
+// Epilogue of one output element of the half-size product: (row, col) in half-size coordinates, ev /
+// od the two parities' sums.
+template <int MODE>
+__device__ inline void dctStore(const DctBatch& d, int z, float* __restrict__ C, int n, int ldc, int step,
+                                const SpectralArgs& sp, int row, int col, int tile_col0, float ev, float od,
+                                double& energy) {
+    const int h = n >> 1;
+    if (MODE == kFoldCols) {
+        *reinterpret_cast<float2*>(C + static_cast<size_t>(row) * ldc + 2 * col) = make_float2(ev, od);
+    } else if (MODE == kFoldRows) {
+        const int k = col;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int l = 2 * row + p;
+            float out = 0.f;
+            if (l < sp.ny && k < sp.nx) {
+                const double wx = kPi * static_cast<double>(k) / sp.W;
+                const double wy = kPi * static_cast<double>(l) / sp.H;
+                const double den = wx * wx + wy * wy;
+                if (den > kEps) {
+                    const double chat = static_cast<double>(p ? od : ev);
+                    const double phi = chat * sp.normalization / den;
+                    out = static_cast<float>(phi);
+                    const double weight = (k == 0 || l == 0) ? 0.5 : 1.0;
+                    energy += weight * chat * phi;
+                }
+            }
+            C[static_cast<size_t>(l) * ldc + k] = out;
+        }
+    } else if (MODE == kMirrorCols) {
+        const bool sine = tile_col0 >= h;  // a tile lies wholly in the cosine or the sine half
+        const int c = col - (sine ? h : 0);
+        float* out = C + static_cast<size_t>(row) * ldc + (sine ? n : 0);
+        out[c] = ev + od;
+        out[n - 1 - c] = sine ? od - ev : ev - od;
+    } else {  // kMirrorRows
+        C[static_cast<size_t>(step) * (static_cast<size_t>(row) * ldc + col)] = ev + od;
+        C[static_cast<size_t>(step) * (static_cast<size_t>(n - 1 - row) * ldc + col)] = d.sigma[z] * (ev - od);
+    }
+}
+
+// A workgroup of W x S wavefronts holds W tiles of TR x TC half-size outputs side by side, both
+// parities.  Tile t's contraction is split over its S wavefronts: wavefront s takes chunks s, s + S,
+// s + 2S, ... in order (two in flight), and with S > 1 the partial tiles meet in LDS, summed in
+// wavefront order, before the epilogue; with S = 1 the epilogue runs from the accumulators.  The
+// per-element summation order depends on S only, never on TR, TC or W.
+template <int MODE, int KPL, int TR, int TC, int W, int S>
 __device__ __forceinline__ void dctBody(const DctBatch& d, int n, int lda, int ldb, int ldc, int step,
                                         const SpectralArgs& sp, const Blk& bk) {
-    __shared__ float red[2][kDctWaves][32][33];
-    __shared__ double lds[kDctWaves + 1];
+    constexpr int RH = TR / 16, CH = TC / 16;
+    constexpr int kWaves = W * S;
+    __shared__ float red[S > 1 ? 2 * kWaves * TR * (TC + 1) : 1];
+    __shared__ double lds[kWaves + 1];
     const int z = bk.z;
     const float* __restrict__ A = d.A[z];
     const float* __restrict__ B = d.B[z];
     float* __restrict__ C = d.C[z];
     const int h = n >> 1;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r16 = lane & 15, grp = lane >> 4;
-    const int row0 = bk.y * 32, col0 = bk.x * 32;
-    floatx4 acc[2][2][2];
+    const int tile = wave / S, part = wave % S;
+    const int row0 = bk.y * TR, col0 = (bk.x * W + tile) * TC;
+    floatx4 acc[2][RH][CH];
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int rh = 0; rh < 2; ++rh)
+        for (int rh = 0; rh < RH; ++rh)
 #pragma unroll
-            for (int ch = 0; ch < 2; ++ch) acc[p][rh][ch] = floatx4{0.f, 0.f, 0.f, 0.f};
-    // chunks of 4 KPL contraction indices, two per pass, both loaded before the products run
-    // (h is a multiple of 32)
+            for (int ch = 0; ch < CH; ++ch) acc[p][rh][ch] = floatx4{0.f, 0.f, 0.f, 0.f};
     constexpr int chunk = 4 * KPL;
-    constexpr int stride = chunk * kDctWaves;
-    for (int k0 = wave * chunk; k0 < h; k0 += 2 * stride) {
-        float a0[2][2][KPL], b0[2][2][KPL], a1[2][2][KPL], b1[2][2][KPL];
+    constexpr int stride = chunk * S;  // between this wavefront's chunks
+    for (int k0 = part * chunk; k0 < h; k0 += 2 * stride) {
+        float a0[2][RH][KPL], b0[2][CH][KPL], a1[2][RH][KPL], b1[2][CH][KPL];
         const bool second = k0 + stride < h;
         dctLoad<MODE, KPL>(A, B, n, h, lda, ldb, row0, col0, k0 + KPL * grp, r16, a0, b0);
         if (second) dctLoad<MODE, KPL>(A, B, n, h, lda, ldb, row0, col0, k0 + stride + KPL * grp, r16, a1, b1);
         dctMma(a0, b0, acc);
         if (second) dctMma(a1, b1, acc);
     }
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-        for (int rh = 0; rh < 2; ++rh)
-#pragma unroll
-            for (int ch = 0; ch < 2; ++ch)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) red[p][wave][16 * rh + 4 * grp + v][16 * ch + r16] = acc[p][rh][ch][v];
-    __syncthreads();
     double energy = 0.0;
+    if constexpr (S == 1) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int e = tid + kDctThreads * q;
-        const int row = e >> 5, col = e & 31;
-        float ev = 0.f, od = 0.f;
+        for (int rh = 0; rh < RH; ++rh)
 #pragma unroll
-        for (int w = 0; w < kDctWaves; ++w) {
-            ev += red[0][w][row][col];
-            od += red[1][w][row][col];
-        }
-        if (MODE == kFoldCols) {
-            *reinterpret_cast<float2*>(C + static_cast<size_t>(row0 + row) * ldc + 2 * (col0 + col)) = make_float2(ev, od);
-        } else if (MODE == kFoldRows) {
-            const int k = col0 + col;
+            for (int ch = 0; ch < CH; ++ch)
 #pragma unroll
-            for (int p = 0; p < 2; ++p) {
-                const int l = 2 * (row0 + row) + p;
-                float out = 0.f;
-                if (l < sp.ny && k < sp.nx) {
-                    const double wx = kPi * static_cast<double>(k) / sp.W;
-                    const double wy = kPi * static_cast<double>(l) / sp.H;
-                    const double den = wx * wx + wy * wy;
-                    if (den > kEps) {
-                        const double chat = static_cast<double>(p ? od : ev);
-                        const double phi = chat * sp.normalization / den;
-                        out = static_cast<float>(phi);
-                        const double weight = (k == 0 || l == 0) ? 0.5 : 1.0;
-                        energy += weight * chat * phi;
-                    }
-                }
-                C[static_cast<size_t>(l) * ldc + k] = out;
+                for (int v = 0; v < 4; ++v)
+                    dctStore<MODE>(d, z, C, n, ldc, step, sp, row0 + 16 * rh + 4 * grp + v, col0 + 16 * ch + r16, col0,
+                                   acc[0][rh][ch][v], acc[1][rh][ch][v], energy);
+    } else {
+        // red[p][wave][row][col], row-major within a tile, padded by one column
+        auto at = [&](int p, int w, int row, int col) -> float& {
+            return red[((p * kWaves + w) * TR + row) * (TC + 1) + col];
+        };
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int rh = 0; rh < RH; ++rh)
+#pragma unroll
+                for (int ch = 0; ch < CH; ++ch)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) at(p, wave, 16 * rh + 4 * grp + v, 16 * ch + r16) = acc[p][rh][ch][v];
+        __syncthreads();
+        constexpr int per = W * TR * TC / (64 * kWaves);  // elements per thread
+#pragma unroll
+        for (int q = 0; q < per; ++q) {
+            const int e = tid + 64 * kWaves * q;
+            const int t = e / (TR * TC), row = (e / TC) % TR, col = e % TC;
+            float ev = 0.f, od = 0.f;
+#pragma unroll
+            for (int w = 0; w < S; ++w) {
+                ev += at(0, t * S + w, row, col);
+                od += at(1, t * S + w, row, col);
             }
-        } else if (MODE == kMirrorCols) {
-            const bool sine = col0 >= h;
-            const int c = col0 + col - (sine ? h : 0);
-            float* out = C + static_cast<size_t>(row0 + row) * ldc + (sine ? n : 0);
-            out[c] = ev + od;
-            out[n - 1 - c] = sine ? od - ev : ev - od;
-        } else {  // kMirrorRows
-            const int r = row0 + row, k = col0 + col;
-            C[static_cast<size_t>(step) * (static_cast<size_t>(r) * ldc + k)] = ev + od;
-            C[static_cast<size_t>(step) * (static_cast<size_t>(n - 1 - r) * ldc + k)] = d.sigma[z] * (ev - od);
+            const int tc0 = (bk.x * W + t) * TC;
+            dctStore<MODE>(d, z, C, n, ldc, step, sp, row0 + row, tc0 + col, tc0, ev, od, energy);
         }
     }
     if (MODE == kFoldRows) {
         double v[1] = {energy};
-        publishPartials<1, kDctWaves>(v, 0U, bk.y * bk.gx + bk.x, sp.partials[z], lds);
+        publishPartials<1, kWaves>(v, 0U, bk.y * bk.gx + bk.x, sp.partials[z], lds);
     }
 }
-template <int MODE, int KPL>
-__global__ __launch_bounds__(kDctThreads) void k_dct(DctBatch d, int n, int lda, int ldb, int ldc, int step,
+template <int MODE, int KPL, int TR, int TC, int W, int S>
+__global__ __launch_bounds__(64 * W * S) void k_dct(DctBatch d, int n, int lda, int ldb, int ldc, int step,
                                                      SpectralArgs sp) {
-    dctBody<MODE, KPL>(d, n, lda, ldb, ldc, step, sp, nativeBlk());
+    dctBody<MODE, KPL, TR, TC, W, S>(d, n, lda, ldb, ldc, step, sp, nativeBlk());
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1373,12 +1406,12 @@ __global__ __launch_bounds__(kThreads) void k_gemm_batch(const GemmEntry* e, con
     const GemmEntry& E = e[j];
     gemmBody<EPI>(E.g, E.K, E.lda, E.ldb, E.ldc, E.sp, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, E.gz));
 }
-template <int MODE, int KPL>
-__global__ __launch_bounds__(kDctThreads) void k_dct_batch(const DctEntry* e, const int* first, int n) {
-    const int j = batchEntry<kDctThreads>(first, n);
+template <int MODE, int KPL, int TR, int TC, int W, int S>
+__global__ __launch_bounds__(64 * W * S) void k_dct_batch(const DctEntry* e, const int* first, int n) {
+    const int j = batchEntry<64 * W * S>(first, n);
     const DctEntry& E = e[j];
-    dctBody<MODE, KPL>(E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp,
-                       entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, E.gz));
+    dctBody<MODE, KPL, TR, TC, W, S>(E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp,
+                                     entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, E.gz));
 }
 __global__ __launch_bounds__(kThreads) void k_gather_batch(const GatherEntry* e, const int* first, int n) {
     const int j = batchEntry<kThreads>(first, n);
@@ -1483,45 +1516,115 @@ void launchGemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int
 
 bool foldedDct(int nx, int ny) { return nx > 0 && ny > 0 && nx % 64 == 0 && ny % 64 == 0; }
 
-template <int KPL>
-void launchDctPrec(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
-                   const SpectralArgs& sp, hipStream_t s) {
-    const dim3 block(kDctThreads);
+// ---- tile variants of the half-length products ---------------------------------------------------
+// Variant index v = (tile * 4 + mode) * 2 + (split-BF16 ? 1 : 0); every variant is instantiated for
+// the native and the batched launch, dispatched through tables built at compile time.
+
+namespace {
+
+constexpr int kDctVariants = kDctTileCount * 8;
+
+template <int V>
+struct DctVariant {
+    static constexpr int mode = (V / 2) % 4, kpl = (V % 2) ? 8 : 4;
+    static constexpr DctTile t = kDctTiles[V / 8];
+};
+using DctLaunchFn = void (*)(dim3, const DctBatch&, int, int, int, int, int, const SpectralArgs&, hipStream_t);
+using DctBatchFn = void (*)(dim3, const DctEntry*, const int*, int, hipStream_t);
+
+template <int V>
+void launchDctV(dim3 grid, const DctBatch& d, int n, int lda, int ldb, int ldc, int step, const SpectralArgs& sp,
+                hipStream_t s) {
+    using X = DctVariant<V>;
+    hipLaunchKernelGGL((k_dct<X::mode, X::kpl, X::t.tr, X::t.tc, X::t.w, X::t.s>), grid, dim3(64 * X::t.w * X::t.s), 0,
+                       s, d, n, lda, ldb, ldc, step, sp);
+}
+template <int V>
+void launchDctBatchV(dim3 grid, const DctEntry* e, const int* first, int n, hipStream_t s) {
+    using X = DctVariant<V>;
+    hipLaunchKernelGGL((k_dct_batch<X::mode, X::kpl, X::t.tr, X::t.tc, X::t.w, X::t.s>), grid,
+                       dim3(64 * X::t.w * X::t.s), 0, s, e, first, n);
+}
+template <int... V>
+constexpr std::array<DctLaunchFn, sizeof...(V)> dctLaunchTable(std::integer_sequence<int, V...>) {
+    return {{&launchDctV<V>...}};
+}
+template <int... V>
+constexpr std::array<DctBatchFn, sizeof...(V)> dctBatchTable(std::integer_sequence<int, V...>) {
+    return {{&launchDctBatchV<V>...}};
+}
+constexpr std::array<DctLaunchFn, kDctVariants> kDctLaunch = dctLaunchTable(std::make_integer_sequence<int, kDctVariants>{});
+constexpr std::array<DctBatchFn, kDctVariants> kDctBatchLaunch = dctBatchTable(std::make_integer_sequence<int, kDctVariants>{});
+
+// Half-size output extent of a product: rows x cols of (row, col) indices the tiles cover.
+void dctHalfExtent(int mode, int rows, int cols, int& hr, int& hc) {
     switch (mode) {
-        case kFoldCols:
-            hipLaunchKernelGGL((k_dct<kFoldCols, KPL>), dim3(cols / 64, rows / 32, batch), block, 0, s, d, cols, lda, ldb,
-                               ldc, 1, sp);
-            break;
-        case kFoldRows:
-            hipLaunchKernelGGL((k_dct<kFoldRows, KPL>), dim3(cols / 32, rows / 64, batch), block, 0, s, d, rows, lda, ldb,
-                               ldc, 1, sp);
-            break;
-        case kMirrorCols:
-            hipLaunchKernelGGL((k_dct<kMirrorCols, KPL>), dim3(cols / 32, rows / 32, batch), block, 0, s, d, cols, lda,
-                               ldb, ldc, 1, sp);
-            break;
-        case kMirrorRows:
-            hipLaunchKernelGGL((k_dct<kMirrorRows, KPL>), dim3(cols / 32, rows / 64, batch), block, 0, s, d, rows, lda,
-                               ldb, ldc, step, sp);
-            break;
-        default:
-            throw std::invalid_argument("unknown DCT product mode");
+        case kFoldCols: hr = rows; hc = cols / 2; break;
+        case kFoldRows: hr = rows / 2; hc = cols; break;
+        case kMirrorCols: hr = rows; hc = cols; break;
+        case kMirrorRows: hr = rows / 2; hc = cols; break;
+        default: throw std::invalid_argument("unknown DCT product mode");
     }
 }
 
-void launchDct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
-               int precision, const SpectralArgs& sp, hipStream_t s) {
+// The tuned table: {mode, precision, rows, cols, tile}, written by myplacement_amd/autotune.py.
+struct DctTileChoice {
+    int mode, precision, rows, cols, tile;
+};
+constexpr DctTileChoice kDctTuned[] = {
+#include "dct_tiles.inc"
+};
+
+// Transform length of a product (the contraction is half of it).
+int dctLength(int mode, int rows, int cols) { return (mode == kFoldRows || mode == kMirrorRows) ? rows : cols; }
+
+}  // namespace
+
+bool dctTileValid(int tile, int mode, int rows, int cols) {
+    if (tile < 0 || tile >= kDctTileCount || rows <= 0 || cols <= 0 || rows % 64 || cols % 64) return false;
+    int hr = 0, hc = 0;
+    dctHalfExtent(mode, rows, cols, hr, hc);
+    const DctTile& t = kDctTiles[tile];
+    return hr % t.tr == 0 && hc % (t.w * t.tc) == 0;
+}
+
+int dctTileFor(int mode, int precision, int rows, int cols) {
+    for (const DctTileChoice& c : kDctTuned)
+        if (c.mode == mode && c.precision == precision && c.rows == rows && c.cols == cols &&
+            dctTileValid(c.tile, mode, rows, cols))
+            return c.tile;
+    return 0;  // the eight-way split of 32 x 32 tiles, valid for every folded grid
+}
+
+int dctWorkgroups(int tile, int mode, int rows, int cols) {
+    int hr = 0, hc = 0;
+    dctHalfExtent(mode, rows, cols, hr, hc);
+    const DctTile& t = kDctTiles[tile];
+    return (hc / (t.w * t.tc)) * (hr / t.tr);
+}
+
+
+void launchDctTile(int tile, int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc,
+                   int step, int precision, const SpectralArgs& sp, hipStream_t s) {
     if (rows <= 0 || cols <= 0 || rows % 64 || cols % 64)
         throw std::invalid_argument("half-length DCT products need both grid sides to be multiples of 64");
     if (batch < 1 || batch > kMaxGemmBatch) throw std::invalid_argument("bad DCT product batch");
     if (mode == kMirrorRows && step != 1 && step != 2) throw std::invalid_argument("bad DCT output element stride");
-    if (precision == 0)
-        launchDctPrec<4>(mode, d, batch, rows, cols, lda, ldb, ldc, step, sp, s);
-    else if (precision == 1)
-        launchDctPrec<8>(mode, d, batch, rows, cols, lda, ldb, ldc, step, sp, s);
-    else
-        throw std::invalid_argument("unknown DCT product precision");
+    if (precision != 0 && precision != 1) throw std::invalid_argument("unknown DCT product precision");
+    if (!dctTileValid(tile, mode, rows, cols)) throw std::invalid_argument("DCT tile does not divide the product");
+    int hr = 0, hc = 0;
+    dctHalfExtent(mode, rows, cols, hr, hc);
+    const DctTile& t = kDctTiles[tile];
+    const dim3 grid(hc / (t.w * t.tc), hr / t.tr, batch);
+    kDctLaunch[(tile * 4 + mode) * 2 + precision](grid, d, dctLength(mode, rows, cols), lda, ldb, ldc,
+                                                  mode == kMirrorRows ? step : 1, sp, s);
     check(hipGetLastError(), "launch half-length DCT product kernel");
+}
+
+void launchDct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
+               int precision, const SpectralArgs& sp, hipStream_t s) {
+    launchDctTile(dctTileFor(mode, precision, rows, cols), mode, d, batch, rows, cols, lda, ldb, ldc, step, precision,
+                  sp, s);
 }
 
 int particleBlocks(int n) { return std::max(1, blocksFor(n)); }
@@ -1599,19 +1702,16 @@ DctEntry dctEntry(int mode, const DctBatch& d, int batch, int rows, int cols, in
         throw std::invalid_argument("half-length DCT products need both grid sides to be multiples of 64");
     if (batch < 1 || batch > kMaxGemmBatch) throw std::invalid_argument("bad DCT product batch");
     if (precision != 0 && precision != 1) throw std::invalid_argument("unknown DCT product precision");
-    DctEntry e{d, sp, mode, precision == 0 ? 4 : 8, 0, lda, ldb, ldc, step, 0, 0, batch};
-    switch (mode) {
-        case kFoldCols: e.n = cols; e.gx = cols / 64; e.gy = rows / 32; e.step = 1; break;
-        case kFoldRows: e.n = rows; e.gx = cols / 32; e.gy = rows / 64; e.step = 1; break;
-        case kMirrorCols: e.n = cols; e.gx = cols / 32; e.gy = rows / 32; e.step = 1; break;
-        case kMirrorRows:
-            if (step != 1 && step != 2) throw std::invalid_argument("bad DCT output element stride");
-            e.n = rows; e.gx = cols / 32; e.gy = rows / 64;
-            break;
-        default: throw std::invalid_argument("unknown DCT product mode");
-    }
+    if (mode == kMirrorRows && step != 1 && step != 2) throw std::invalid_argument("bad DCT output element stride");
+    const int tile = dctTileFor(mode, precision, rows, cols);
+    int hr = 0, hc = 0;
+    dctHalfExtent(mode, rows, cols, hr, hc);
+    const DctTile& t = kDctTiles[tile];
+    DctEntry e{d, sp, mode, precision == 0 ? 4 : 8, dctLength(mode, rows, cols), lda, ldb, ldc,
+               mode == kMirrorRows ? step : 1, hc / (t.w * t.tc), hr / t.tr, batch, tile};
     return e;
 }
+int dctVariant(const DctEntry& e) { return (e.tile * 4 + e.mode) * 2 + (e.kpl == 8 ? 1 : 0); }
 GatherEntry gatherEntry(const GatherArgs& a, int count) { return GatherEntry{a, particleBlocks(a.n), count}; }
 LookaheadEntry lookaheadEntry(const LookaheadArgs& l) { return LookaheadEntry{l, blocksFor(l.n)}; }
 ProposeEntry proposeEntry(const ProposeArgs& p) { return ProposeEntry{p, particleBlocks(p.n)}; }
@@ -1678,20 +1778,10 @@ void launchBatched(BatchKernel k, const void* entries, const int* first, int n, 
                                first, n);
             break;
         default: {
-            // half-length products: kBatchDct + 2 * mode + (kpl == 8)
+            // half-length products: kBatchDct + dctVariant(entry)
             const int v = static_cast<int>(k) - kBatchDct;
-            if (v < 0 || v >= 8) throw std::invalid_argument("unknown batched kernel");
-            const DctEntry* de = static_cast<const DctEntry*>(entries);
-            switch (v) {
-                case 0: hipLaunchKernelGGL((k_dct_batch<kFoldCols, 4>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
-                case 1: hipLaunchKernelGGL((k_dct_batch<kFoldCols, 8>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
-                case 2: hipLaunchKernelGGL((k_dct_batch<kFoldRows, 4>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
-                case 3: hipLaunchKernelGGL((k_dct_batch<kFoldRows, 8>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
-                case 4: hipLaunchKernelGGL((k_dct_batch<kMirrorCols, 4>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
-                case 5: hipLaunchKernelGGL((k_dct_batch<kMirrorCols, 8>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
-                case 6: hipLaunchKernelGGL((k_dct_batch<kMirrorRows, 4>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
-                default: hipLaunchKernelGGL((k_dct_batch<kMirrorRows, 8>), grid, dim3(kDctThreads), 0, s, de, first, n); break;
-            }
+            if (v < 0 || v >= kDctVariants) throw std::invalid_argument("unknown batched kernel");
+            kDctBatchLaunch[v](grid, static_cast<const DctEntry*>(entries), first, n, s);
         }
     }
     check(hipGetLastError(), "launch batched kernel");
@@ -1778,19 +1868,9 @@ void launchEntry(BatchKernel k, const void* entry, std::size_t dyn_lds, hipStrea
 
         default: {
             const int v = static_cast<int>(k) - kBatchDct;
-            if (v < 0 || v >= 8) throw std::invalid_argument("unknown batched kernel");
+            if (v < 0 || v >= kDctVariants) throw std::invalid_argument("unknown batched kernel");
             const DctEntry& E = *static_cast<const DctEntry*>(entry);
-            const dim3 grid(E.gx, E.gy, E.gz), block(kDctThreads);
-            switch (v) {
-                case 0: hipLaunchKernelGGL((k_dct<kFoldCols, 4>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
-                case 1: hipLaunchKernelGGL((k_dct<kFoldCols, 8>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
-                case 2: hipLaunchKernelGGL((k_dct<kFoldRows, 4>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
-                case 3: hipLaunchKernelGGL((k_dct<kFoldRows, 8>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
-                case 4: hipLaunchKernelGGL((k_dct<kMirrorCols, 4>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
-                case 5: hipLaunchKernelGGL((k_dct<kMirrorCols, 8>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
-                case 6: hipLaunchKernelGGL((k_dct<kMirrorRows, 4>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
-                default: hipLaunchKernelGGL((k_dct<kMirrorRows, 8>), grid, block, 0, s, E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp); break;
-            }
+            kDctLaunch[v](dim3(E.gx, E.gy, E.gz), E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp, s);
         }
     }
     check(hipGetLastError(), "launch kernel from its entry");

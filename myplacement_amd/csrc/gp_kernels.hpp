@@ -160,6 +160,20 @@ void launchGemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int
 //               stored at out + step (row ldc + k) (step 2: one of two interleaved grids);
 //               A: factor halves [2][h][h], B: n x N
 enum DctMode : int { kFoldCols = 0, kFoldRows = 1, kMirrorCols = 2, kMirrorRows = 3 };
+// Tile shapes of the half-length products: a workgroup holds w tiles of tr x tc half-size outputs
+// (both parities) side by side, each tile's contraction split over s wavefronts.  Results depend on
+// s only (the order of each element's sum), never on tr, tc or w.  Tile 0 is valid for every folded
+// grid and is the default where the tuned table (dct_tiles.inc, written by
+// myplacement_amd/autotune.py on an MI355X) names none.
+struct DctTile {
+    int tr, tc, w, s;
+};
+constexpr int kDctTileCount = 8;
+constexpr DctTile kDctTiles[kDctTileCount] = {{32, 32, 1, 8}, {32, 32, 1, 4}, {16, 32, 1, 4}, {32, 32, 1, 2},
+                                              {32, 32, 1, 1}, {32, 32, 2, 1}, {16, 32, 2, 1}, {16, 16, 4, 1}};
+bool dctTileValid(int tile, int mode, int rows, int cols);
+int dctTileFor(int mode, int precision, int rows, int cols);  // the tuned tile of a product
+int dctWorkgroups(int tile, int mode, int rows, int cols);    // per product (energy partial rows)
 struct DctBatch {
     const float* A[kMaxGemmBatch];
     const float* B[kMaxGemmBatch];
@@ -172,6 +186,9 @@ bool foldedDct(int nx, int ny);
 // FP32 products, 1: split-BF16 (bf16x3) products with FP32 accumulation.
 void launchDct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
                int precision, const SpectralArgs& sp, hipStream_t s);
+// launchDct with a given tile (the autotuner's timing runs)
+void launchDctTile(int tile, int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc,
+                   int step, int precision, const SpectralArgs& sp, hipStream_t s);
 
 // ---- field gather + gradient assembly ---------------------------------------------------------
 struct GatherArgs {
@@ -276,7 +293,7 @@ struct GemmEntry {
 struct DctEntry {
     DctBatch d;
     SpectralArgs sp;
-    int mode, kpl, n, lda, ldb, ldc, step, gx, gy, gz;
+    int mode, kpl, n, lda, ldb, ldc, step, gx, gy, gz, tile;
 };
 struct GatherEntry {
     GatherArgs a;
@@ -303,7 +320,7 @@ enum BatchKernel : int {
     kBatchWire = 0, kBatchSortHist, kBatchSortScan, kBatchSortPlace, kBatchDeposit, kBatchBins, kBatchGemmStore,
     kBatchGemmSpectral, kBatchGemmInterleave, kBatchGather, kBatchLookahead, kBatchPropose, kBatchCopy, kBatchZero,
     kBatchFinalize,  // entries: RedJob (one workgroup each, no completion word)
-    kBatchDct  // + 2 * mode + (8 values per lane ? 1 : 0)
+    kBatchDct  // + dctVariant(entry): (tile * 4 + mode) * 2 + (8 values per lane ? 1 : 0)
 };
 // Entry builders (the grid extent of the job's own launch; they validate as the launches do).
 WireEntry wireEntry(const WireArgs& a, int count);
@@ -314,6 +331,7 @@ GemmEntry gemmEntry(const GemmBatch& g, int batch, int M, int N, int K, int lda,
                     const SpectralArgs& sp);
 DctEntry dctEntry(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
                   int precision, const SpectralArgs& sp);
+int dctVariant(const DctEntry& e);
 GatherEntry gatherEntry(const GatherArgs& a, int count);
 LookaheadEntry lookaheadEntry(const LookaheadArgs& l);
 ProposeEntry proposeEntry(const ProposeArgs& p);
