@@ -201,7 +201,7 @@ def hpwl_deviation_sample(work, reports, pick, max_cells: int) -> dict:
 
 
 def serving_sample(jobs: int, rate: float, slots: int, seed: int = 2026, mode: str = "streams",
-                   capacity_jobs: int = 200, deviation_jobs: int = 16, deviation_max_cells: int = 30000) -> dict:
+                   capacity_jobs: int = 200, deviation_jobs: int = 16, deviation_max_cells: int = 10000) -> dict:
     """The served mixed-size stream: `jobs` requests with Poisson arrivals at `rate` per second,
     sizes log-uniform in 1k-200k cells (design.served_stream_specs, design.served_design), bin grids
     by design.served_bins, target density uniform in 0.7-0.9, random initial placement, each run
