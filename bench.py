@@ -37,6 +37,7 @@ of known traffic whose FETCH_SIZE / WRITE_SIZE counters scale those of k_wire in
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import math
 import os
@@ -74,6 +75,7 @@ def make_workload(name: str, center=None):
         desc = {"workload": "synthetic 210k cells, 12 macros (15%), 220k nets (~0.9M pins), 1200x1200 sites, "
                             "512x512 bins, t=0.8, quadratic init", "cells": 210000, "bins": 512, "target_density": 0.8}
         d.center = center if center is not None else hb.quadratic_placement(d)[0]
+        desc["start_sha256"] = hashlib.sha256(np.ascontiguousarray(d.center, dtype="<f8").tobytes()).hexdigest()
     elif name == "precision":
         d = design.large_design(seed=2026, cells=1_300_000, nets=1_400_000, row_count=3000, sites=3000, macros=20,
                                 macro_fraction=0.12)

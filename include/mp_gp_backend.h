@@ -146,6 +146,11 @@ typedef struct mp_gp_options {
                                               until positions may have moved this many bin rows;
                                               0 rebuilds it for every evaluation (results are
                                               identical either way; this is a speed setting) */
+    int32_t capture_graphs;                /* 0: each iteration's launch chain is launched kernel by
+                                              kernel (default); 1: it is captured into a HIP graph
+                                              (one executable graph per chain shape, updated in place)
+                                              and launched as one; results are identical */
+    int32_t reserved_options;
 } mp_gp_options;
 
 /* GlobalPlacementIteration, GlobalPlacer.hpp:82-105 (same 22 columns as global_history.csv) */

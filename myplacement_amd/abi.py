@@ -132,6 +132,8 @@ class MpOptions(C.Structure):
         ("dct_precision", C.c_int32),
         ("maximum_device_bytes", C.c_uint64),
         ("density_order_reuse_rows", C.c_double),
+        ("capture_graphs", C.c_int32),
+        ("reserved_options", C.c_int32),
     ]
 
 

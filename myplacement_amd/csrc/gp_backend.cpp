@@ -392,7 +392,13 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
 Backend::~Backend() {
     if (stream_) {
         (void)hipSetDevice(device_);
+        if (capture_shape_ >= 0) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
         (void)hipStreamSynchronize(stream_);
+        for (auto& e : graphs_) (void)hipGraphExecDestroy(e.second);
         for (auto e : ev_pool_) (void)hipEventDestroy(e);
         if (owns_stream_) (void)hipStreamDestroy(stream_);
     }
@@ -785,16 +791,52 @@ void Backend::finalizePending() {
     pending_.n = 0;
 }
 
+void Backend::beginCapture(int shape) {
+    if (timing_mask_ != 0 || !dynamic_cast<StreamLaunches*>(launches_) || capture_shape_ >= 0) return;
+    hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin graph capture");
+    capture_shape_ = shape;
+}
+
 const HostScalars& Backend::fetchScalars() {
     const int jobs = pending_.n;
-    if (jobs == 0) {
+    if (jobs == 0 && capture_shape_ < 0) {
         hipCheck(hipStreamSynchronize(stream_), "synchronize placement evaluation");
         return *host_scal_;
     }
     pending_.seq = ++seq_;
     pending_.flag = dev_done_;
-    launchFinalize(pending_, stream_);
+    if (jobs > 0) launchFinalize(pending_, stream_);
     pending_.n = 0;
+    if (capture_shape_ >= 0) {
+        const int shape = capture_shape_;
+        capture_shape_ = -1;
+        hipGraph_t g = nullptr;
+        hipCheck(hipStreamEndCapture(stream_, &g), "end graph capture");
+        std::unique_ptr<std::remove_pointer_t<hipGraph_t>, decltype(&hipGraphDestroy)> own(g, hipGraphDestroy);
+        auto it = std::find_if(graphs_.begin(), graphs_.end(), [&](const auto& e) { return e.first == shape; });
+        bool updated = false;
+        if (it != graphs_.end()) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult why{};
+            updated = hipGraphExecUpdate(it->second, g, &bad, &why) == hipSuccess && why == hipGraphExecUpdateSuccess;
+            if (!updated) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(it->second);
+                graphs_.erase(it);
+            }
+        }
+        if (!updated) {
+            hipGraphExec_t ex = nullptr;
+            hipCheck(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0), "instantiate graph");
+            graphs_.emplace_back(shape, ex);
+            it = graphs_.end() - 1;
+        }
+        hipCheck(hipGraphLaunch(it->second, stream_), "launch graph");
+        if (jobs == 0) {
+            hipCheck(hipStreamSynchronize(stream_), "synchronize placement evaluation");
+            return *host_scal_;
+        }
+    }
     // Poll the completion words: the results are on the host as soon as the last one lands, some
     // microseconds before the stream reports the kernel retired.  The stream is queried now and
     // then so that a failed launch surfaces as an error instead of a hang.

@@ -252,6 +252,11 @@ public:
     // Reduces the pending partials into the scalar block and waits until k_finalize has written
     // it (by polling its completion words; the rest of the kernel's retirement is not waited for).
     const HostScalars& fetchScalars();
+    // Captures the launches from here to the next fetchScalars() (its finalize included) into a HIP
+    // graph and launches that graph instead: one executable graph per `shape`, updated in place with
+    // the new launch arguments while the chain's shape repeats.  No effect while kernel timing is on
+    // or the launches are recorded for a batch.
+    void beginCapture(int shape);
     // Finalizes the queued reductions into the scalar block without waiting for them (a later
     // kernel on the stream reads them: deviceBank).
     void finalizePending();
@@ -358,6 +363,8 @@ private:
     static constexpr int kTimingClasses = 6;
     hipEvent_t poolEvent();
     unsigned timing_mask_ = 0;
+    int capture_shape_ = -1;                              // the chain being captured, or -1
+    std::vector<std::pair<int, hipGraphExec_t>> graphs_;  // executable graph per chain shape
     std::vector<hipEvent_t> ev_pool_;
     std::size_t ev_next_ = 0;
     std::vector<hipEvent_t> ev_begin_[kTimingClasses], ev_end_[kTimingClasses];

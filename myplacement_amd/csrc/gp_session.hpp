@@ -486,6 +486,9 @@ private:
         const Backend::SessionOut so_look = sessionOut(Rl_.get(), Gl_.get(), nullptr, nullptr);
         Backend::EvalSpec specs[2] = {{X_.get(), &so_cur, kBankCurrent, reorder},
                                       {B_.get(), &so_look, kBankLook, fresh}};
+        // the chain up to the first read-back, as one graph when asked (its shape: the momentum
+        // look-ahead, the row-order rebuild, the evaluations one after another)
+        if (it > 0 && o_.capture_graphs) be_->beginCapture((try_momentum ? 1 : 0) | (reorder ? 2 : 0) | (fresh ? 4 : 0));
         if (try_momentum) {
             be_->timedBegin(kTController);
             be_->launches().lookahead(LookaheadArgs{X_.get(), XP_.get(), beta, B_.get(), clamp_, P_});

@@ -341,3 +341,27 @@ def test_device_adaptive_run_composes_the_pinned_laws(ob, case):
     assert stats["accepted"] > 10
     if case == "plain-aggressive":
         assert r["rejected_candidates"] > 0 and r["momentum_restarts"] > 0
+
+
+@pytest.mark.parametrize("case", ["plain-64", "mid-128-quadratic"])
+def test_graph_captured_iterations_equal_direct_launches(case):
+    """capture_graphs = 1 launches each iteration's chain up to its read-back as one HIP graph (one
+    executable graph per chain shape, updated in place): the runs must be bit-identical to direct
+    launches, whatever shapes the chain takes (momentum look-ahead, row-order rebuilds)."""
+    if case == "plain-64":
+        d, o = _setup(design.plain_design(), 0.9, 64)
+    else:
+        d = design.large_design(seed=3, cells=20000, nets=21000, row_count=200, sites=400, macros=6)
+        d.center, _ = hb.quadratic_placement(d)
+        o = abi.default_options(target_density=0.8, bins_x=128, bins_y=128)
+    o.iterations = 150
+    o.feasible_refinement_iterations = 2 ** 31 - 1
+    runs = []
+    for cg in (0, 1):
+        o.capture_graphs = cg
+        say(f"[gpu] {case}: capture_graphs={cg}")
+        c, r, hist = hb.global_place(d, o)
+        runs.append((c, r, np.array([[row[k] for k in abi.HISTORY_COLUMNS] for row in hist])))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][2], runs[1][2])
+    assert runs[0][1]["gradient_evaluations"] == runs[1][1]["gradient_evaluations"]

@@ -88,3 +88,16 @@ def test_quadratic_one_cell_closed_form_on_device():
     wx, wy = 2.0 / abs(x0 - 80.0), 2.0 / abs(y0 - 30.0)
     assert abs(c[0, 0] - (wx * 80.0 + 1e-4 * x0) / (wx + 1e-4)) < 1e-10
     assert abs(c[0, 1] - (wy * 30.0 + 1e-4 * y0) / (wy + 1e-4)) < 1e-10
+
+
+def test_headline_quadratic_start_is_the_committed_one():
+    """The 210k-cell design's quadratic start, which bench.py times from and hands to the CPU
+    baseline, reproduces the committed bytes (tests/golden/cfg2_quadratic_start.json)."""
+    import hashlib
+    import json
+    import os
+    ref = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cfg2_quadratic_start.json")))
+    d = design.large_design(seed=2026)
+    c, r = hb.quadratic_placement(d)
+    assert hashlib.sha256(np.ascontiguousarray(c, dtype="<f8").tobytes()).hexdigest() == ref["centers_sha256"]
+    assert r["hpwl_after"] == ref["hpwl_after"] and r["solver_iterations"] == ref["solver_iterations"]
