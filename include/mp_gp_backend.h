@@ -150,7 +150,10 @@ typedef struct mp_gp_options {
                                               kernel (default); 1: it is captured into a HIP graph
                                               (one executable graph per chain shape, updated in place)
                                               and launched as one; results are identical */
-    int32_t reserved_options;
+    int32_t recompute_density;             /* 0: an iteration's evaluation at the candidate accepted by
+                                              the iteration before takes that candidate's density,
+                                              potential and energy, which are the same bits (default);
+                                              1: it computes them again; results are identical */
 } mp_gp_options;
 
 /* GlobalPlacementIteration, GlobalPlacer.hpp:82-105 (same 22 columns as global_history.csv) */
@@ -214,13 +217,21 @@ mp_gp_status mp_gp_debug_grids(mp_gp_backend* backend, double* source, double* p
 
 /* Autotuner support (no counterpart in the reference, whose cuFFT transforms have no tile shapes):
  * the average time in microseconds of `reps` launches of one half-length DCT product of an
- * nx x ny grid (both multiples of 64) with tile shape `tile` (0..7, see gp_kernels.hpp), for
+ * nx x ny grid (both multiples of 64) with tile shape `tile` (0..15, see gp_kernels.hpp), for
  * `evaluations` evaluations batched as the evaluation chain batches them, on device 0, with
  * operands of the chain's shapes.  myplacement_amd/autotune.py times every valid tile this way and
  * writes the table the library compiles in (csrc/dct_tiles.inc).  MP_GP_INVALID_ARGUMENT for a
  * tile that does not divide the product. */
 mp_gp_status mp_gp_dct_tile_time(int32_t nx, int32_t ny, int32_t precision, int32_t mode, int32_t tile,
                                  int32_t evaluations, int32_t reps, double* microseconds);
+/* One product of the same kind with tile `tile` on the caller's operands (no counterpart in the
+ * reference; for tests of every tile): `data` holds the data grid of ny x 2 nx floats (two of them
+ * for mode 3, the mirror-rows product), `factors` the 2 max(nx, ny)^2 factor floats, and `out`
+ * receives as many floats as `data` holds.  The layouts are those of the evaluation chain's
+ * products, mode 0 fold-cols, 1 fold-rows, 2 mirror-cols, 3 mirror-rows (gp_kernels.hpp). */
+mp_gp_status mp_gp_dct_tile_run(int32_t nx, int32_t ny, int32_t precision, int32_t mode, int32_t tile,
+                                const float* data, int64_t data_count, const float* factors, int64_t factor_count,
+                                float* out);
 /* The tile the library uses for a product (from its compiled table; 0 where the table names none). */
 int32_t mp_gp_dct_tile_for(int32_t nx, int32_t ny, int32_t precision, int32_t mode);
 

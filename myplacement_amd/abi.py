@@ -133,7 +133,7 @@ class MpOptions(C.Structure):
         ("maximum_device_bytes", C.c_uint64),
         ("density_order_reuse_rows", C.c_double),
         ("capture_graphs", C.c_int32),
-        ("reserved_options", C.c_int32),
+        ("recompute_density", C.c_int32),
     ]
 
 

@@ -28,8 +28,9 @@ if os.path.dirname(HERE) not in sys.path:
 from myplacement_amd import backend as hb  # noqa: E402
 
 MODES = ("fold-cols", "fold-rows", "mirror-cols", "mirror-rows")
-TILES = ((32, 32, 1, 8), (32, 32, 1, 4), (16, 32, 1, 4), (32, 32, 1, 2), (32, 32, 1, 1), (32, 32, 2, 1), (16, 32, 2, 1),
-         (16, 16, 4, 1))  # (tr, tc, w, s): kDctTiles of gp_kernels.hpp, in order
+_SHAPES = ((32, 32, 1, 8), (32, 32, 1, 4), (16, 32, 1, 4), (32, 32, 1, 2), (32, 32, 1, 1), (32, 32, 2, 1), (16, 32, 2, 1),
+           (16, 16, 4, 1))
+TILES = tuple(t + (f,) for f in (0, 1) for t in _SHAPES)  # (tr, tc, w, s, f): kDctTiles of gp_kernels.hpp, in order
 SHAPES = ((64, 64), (128, 64), (64, 128), (128, 128), (256, 256), (512, 512), (1024, 1024))
 
 
@@ -75,7 +76,7 @@ def main() -> int:
         for mode, precision, r, c, tile in rows:
             f.write(f"{{{mode}, {precision}, {r}, {c}, {tile}}},\n")
     # markdown summary on stdout
-    print("| grid | precision | product | chosen tile (tr, tc, w, s) | us, 1 / 2 evaluations | tile 0 us | gain |")
+    print("| grid | precision | product | chosen tile (tr, tc, w, s, f) | us, 1 / 2 evaluations | tile 0 us | gain |")
     print("|---|---|---|---|---|---|---|")
     for nx, ny, precision, mode, best, times in summary:
         b, z = times[best], times[0]

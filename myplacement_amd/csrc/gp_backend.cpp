@@ -410,8 +410,11 @@ void Backend::setDevice() const { hipCheck(hipSetDevice(device_), "select HIP de
 
 hipEvent_t Backend::poolEvent() {
     if (ev_next_ == ev_pool_.size()) {
+        // timing only: without the system-scope fence an event neither writes back nor invalidates
+        // the caches, so the kernels after it run as they do untimed (the timestamps are read after
+        // a stream synchronize)
         hipEvent_t e;
-        hipCheck(hipEventCreate(&e), "create timing event");
+        hipCheck(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "create timing event");
         ev_pool_.push_back(e);
     }
     return ev_pool_[ev_next_++];
@@ -505,8 +508,11 @@ void Backend::sortParticles(const double2* pos) {
 
 void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bool want_grad) {
     if (count < 1 || count > kSlots) throw std::logic_error("one or two evaluations per batch");
-    for (int e = 0; e < count; ++e)
+    for (int e = 0; e < count; ++e) {
         if (!specs[e].session && count != 1) throw std::logic_error("backend-mode evaluations are not batched");
+        if (specs[e].density_from >= 0 && (e != 0 || specs[e].density_from >= kScalarBanks))
+            throw std::logic_error("only the first evaluation may take the density of slot 0");
+    }
     if (specs[0].reorder) sortParticles(specs[0].pos);
 
     WireArgs wa{};
@@ -540,28 +546,47 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
     // particle with the gradient and 8 without
     timedEnd(kTWire, count * (12.0 * I_ + 8.0 * N_ + (want_grad ? 16.0 : 8.0) * P_));
 
-    DensityArgs da = densityArgs();
-    for (int e = 0; e < count; ++e) {
-        da.pos[e] = specs[e].pos;
-        da.grid[e] = slots_[e].grid.get();
-        da.rho[e] = slots_[e].rho.get();
-        da.partials[e] = partials(kRedMetrics, specs[e].bank);
-    }
-    timedBegin(kTBand);
-    launches_->density(da, count);
-    // algorithmic bytes per evaluation (same convention): scatter 20 per particle and 12 per bin,
-    // metrics and deviation 24 per bin
-    timedEnd(kTBand, count * (20.0 * P_ + 36.0 * static_cast<double>(nx_) * ny_));
+    // The evaluations whose density is computed here (fwd[0..nf)); an evaluation at the positions
+    // of the one before it takes that one's density, potential and partial rows instead.
+    int fwd[kSlots], nf = 0;
     for (int e = 0; e < count; ++e)
-        addReduction(kRedMetrics, specs[e].bank, densityBinBlocks(region_), static_cast<double>(nx_) * ny_, dark_total_,
-                     placeable_total_);
+        if (specs[e].density_from < 0) fwd[nf++] = e;
+    const int energy_rows = folded_ ? dctWorkgroups(dctTileFor(kFoldRows, dct_precision_, ny_, nx_), kFoldRows, ny_, nx_)
+                                    : (nxp_ / 32) * (nyp_ / 32);
+    const double bins = static_cast<double>(nx_) * ny_;
+    for (int e = 0; e < count; ++e)
+        if (specs[e].density_from >= 0) {
+            addReduction(kRedMetrics, specs[e].bank, densityBinBlocks(region_), bins, dark_total_, placeable_total_,
+                         specs[e].density_from);
+            addReduction(kRedEnergy, specs[e].bank, energy_rows, 0.5 * region_.bw * region_.bh,
+                         0.0, 0.0, specs[e].density_from);
+        }
 
     const double W = region_.urx - region_.llx, H = region_.ury - region_.lly;
     SpectralArgs sp{nx_, ny_, W, H, 1.0 / (4.0 * nx_ * static_cast<double>(ny_)), {}};
-    for (int e = 0; e < count; ++e) sp.partials[e] = partials(kRedEnergy, specs[e].bank);
-    auto gemm = [&](const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epi) {
+    if (nf > 0) {
+        DensityArgs da = densityArgs();
+        for (int k = 0; k < nf; ++k) {
+            const int e = fwd[k];
+            da.pos[k] = specs[e].pos;
+            da.grid[k] = slots_[e].grid.get();
+            da.rho[k] = slots_[e].rho.get();
+            da.partials[k] = partials(kRedMetrics, specs[e].bank);
+            sp.partials[k] = partials(kRedEnergy, specs[e].bank);
+        }
+        timedBegin(kTBand);
+        launches_->density(da, nf);
+        // algorithmic bytes per evaluation (same convention): scatter 20 per particle and 12 per bin,
+        // metrics and deviation 24 per bin
+        timedEnd(kTBand, nf * (20.0 * P_ + 36.0 * bins));
+        for (int k = 0; k < nf; ++k)
+            addReduction(kRedMetrics, specs[fwd[k]].bank, densityBinBlocks(region_), bins, dark_total_,
+                         placeable_total_);
+    }
+    auto gemm = [&](const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epi,
+                    int product) {
         timedBegin(kTDct);
-        launches_->gemm(g, batch, M, N, K, lda, ldb, ldc, epi, sp, dct_precision_);
+        launches_->gemm(g, batch, M, N, K, lda, ldb, ldc, epi, sp, dct_precision_, product);
         timedEnd(kTDct, 2.0 * M * static_cast<double>(N) * K * batch);
     };
     // half-length products: `rows` x `cols` outputs over a contraction of half the transform side
@@ -573,21 +598,22 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
     };
     if (folded_) {
         // forward: T = fold_x(rho) ; chat = fold_y(T) -> phihat (+ energy)
-        DctBatch f1{}, f2{};
-        for (int e = 0; e < count; ++e) {
-            f1.A[e] = slots_[e].rho.get();
-            f1.B[e] = bxf_.get();
-            f1.C[e] = slots_[e].tmp.get();
-            f2.A[e] = ayf_.get();
-            f2.B[e] = slots_[e].tmp.get();
-            f2.C[e] = slots_[e].phi.get();
+        if (nf > 0) {
+            DctBatch f1{}, f2{};
+            for (int k = 0; k < nf; ++k) {
+                const Slot& sl = slots_[fwd[k]];
+                f1.A[k] = sl.rho.get();
+                f1.B[k] = bxf_.get();
+                f1.C[k] = sl.tmp.get();
+                f2.A[k] = ayf_.get();
+                f2.B[k] = sl.tmp.get();
+                f2.C[k] = sl.phi.get();
+            }
+            dct(kFoldCols, f1, nf, nxp_, 0, nxp_, 1);
+            dct(kFoldRows, f2, nf, 0, nxp_, nxp_, 1);
+            for (int k = 0; k < nf; ++k)
+                addReduction(kRedEnergy, specs[fwd[k]].bank, energy_rows, 0.5 * region_.bw * region_.bh);
         }
-        dct(kFoldCols, f1, count, nxp_, 0, nxp_, 1);
-        dct(kFoldRows, f2, count, 0, nxp_, nxp_, 1);
-        for (int e = 0; e < count; ++e)
-            addReduction(kRedEnergy, specs[e].bank,
-                         dctWorkgroups(dctTileFor(kFoldRows, dct_precision_, ny_, nx_), kFoldRows, ny_, nx_),
-                         0.5 * region_.bw * region_.bh);
         if (!want_grad) return;
         // [T1 | T2] = mirror_x(phihat) ; Ex = mirror_y(T2, cosine) ; Ey = mirror_y(T1, sine)
         DctBatch i1{}, i2{};
@@ -609,19 +635,22 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
         dct(kMirrorRows, i2, 2 * count, 0, 2 * nxp_, nxp_, 2);
     } else {
         // forward: T = rho * Bx ; chat = Ay * T -> phihat (+ energy)
-        GemmBatch f1{}, f2{};
-        for (int e = 0; e < count; ++e) {
-            f1.A[e] = slots_[e].rho.get();
-            f1.B[e] = bxf_.get();
-            f1.C[e] = slots_[e].tmp.get();
-            f2.A[e] = ayf_.get();
-            f2.B[e] = slots_[e].tmp.get();
-            f2.C[e] = slots_[e].phi.get();
+        if (nf > 0) {
+            GemmBatch f1{}, f2{};
+            for (int k = 0; k < nf; ++k) {
+                const Slot& sl = slots_[fwd[k]];
+                f1.A[k] = sl.rho.get();
+                f1.B[k] = bxf_.get();
+                f1.C[k] = sl.tmp.get();
+                f2.A[k] = ayf_.get();
+                f2.B[k] = sl.tmp.get();
+                f2.C[k] = sl.phi.get();
+            }
+            gemm(f1, nf, nyp_, nxp_, nxp_, nxp_, nxp_, nxp_, kEpiStore, 0);
+            gemm(f2, nf, nyp_, nxp_, nyp_, nyp_, nxp_, nxp_, kEpiSpectral, 1);
+            for (int k = 0; k < nf; ++k)
+                addReduction(kRedEnergy, specs[fwd[k]].bank, energy_rows, 0.5 * region_.bw * region_.bh);
         }
-        gemm(f1, count, nyp_, nxp_, nxp_, nxp_, nxp_, nxp_, kEpiStore);
-        gemm(f2, count, nyp_, nxp_, nyp_, nyp_, nxp_, nxp_, kEpiSpectral);
-        for (int e = 0; e < count; ++e)
-            addReduction(kRedEnergy, specs[e].bank, (nxp_ / 32) * (nyp_ / 32), 0.5 * region_.bw * region_.bh);
         if (!want_grad) return;
         // [T1 | T2] = phihat * [Cx_inv | Sx_inv] ; Ex = Ay_inv * T2 ; Ey = Sy_inv * T1
         GemmBatch i1{}, i2{};
@@ -637,8 +666,8 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
             i2.B[2 * e + 1] = sl.t12.get();
             i2.C[2 * e + 1] = sl.field.get() + 1;
         }
-        gemm(i1, count, nyp_, 2 * nxp_, nxp_, nxp_, 2 * nxp_, 2 * nxp_, kEpiStore);
-        gemm(i2, 2 * count, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiInterleave);
+        gemm(i1, count, nyp_, 2 * nxp_, nxp_, nxp_, 2 * nxp_, 2 * nxp_, kEpiStore, 2);
+        gemm(i2, 2 * count, nyp_, nxp_, nyp_, nyp_, 2 * nxp_, nxp_, kEpiInterleave, 3);
     }
     have_fields_ = true;
 
@@ -683,7 +712,7 @@ void StreamLaunches::wire(const WireArgs& a, int count) { launchWire(a, count, s
 void StreamLaunches::zero(double* p, int n) { launchZero(p, static_cast<size_t>(n), s_); }
 void StreamLaunches::density(const DensityArgs& a, int count) { launchDensity(a, count, s_); }
 void StreamLaunches::gemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
-                          const SpectralArgs& sp, int precision) {
+                          const SpectralArgs& sp, int precision, int) {
     launchGemm(g, batch, M, N, K, lda, ldb, ldc, epilogue, sp, precision, s_);
 }
 void StreamLaunches::dct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc,
@@ -738,22 +767,21 @@ void RecordLaunches::density(const DensityArgs& a, int count) {
     const DensityEntry d = depositEntry(a, count), b = binsEntry(a, count);
     add(kStageDeposit, kBatchDeposit, d, d.gx * d.gy);
     add(kStageBins, kBatchBins, b, b.gx * b.gy);
-    spectral_ = 0;
 }
 void RecordLaunches::gemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
-                          const SpectralArgs& sp, int precision) {
+                          const SpectralArgs& sp, int precision, int product) {
     if (precision != 0) throw std::invalid_argument("only the FP32 DCT products are built");
-    if (spectral_ >= 4) throw std::logic_error("more than four spectral products in one evaluation");
+    if (product < 0 || product >= 4) throw std::logic_error("an evaluation chain has four spectral products");
     const GemmEntry e = gemmEntry(g, batch, M, N, K, lda, ldb, ldc, epilogue, sp);
     const int k = epilogue == kEpiSpectral ? kBatchGemmSpectral
                   : epilogue == kEpiInterleave ? kBatchGemmInterleave : kBatchGemmStore;
-    add(kStageSpectral + spectral_++, k, e, e.gx * e.gy * e.gz);
+    add(kStageSpectral + product, k, e, e.gx * e.gy * e.gz);
 }
 void RecordLaunches::dct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc,
                          int step, int precision, const SpectralArgs& sp) {
-    if (spectral_ >= 4) throw std::logic_error("more than four spectral products in one evaluation");
     const DctEntry e = dctEntry(mode, d, batch, rows, cols, lda, ldb, ldc, step, precision, sp);
-    add(kStageSpectral + spectral_++, kBatchDct + dctVariant(e), e, e.gx * e.gy * e.gz);
+    // the modes are numbered in chain order: fold-cols, fold-rows, mirror-cols, mirror-rows
+    add(kStageSpectral + mode, kBatchDct + dctVariant(e), e, e.gx * e.gy * e.gz);
 }
 void RecordLaunches::gather(const GatherArgs& a, int count) {
     if (count < 1 || count > kMaxEval) throw std::invalid_argument("bad evaluation batch");
@@ -776,10 +804,11 @@ void RecordLaunches::copy(double2* dst, const double2* src, int n) {
     add(kStageCopy, kBatchCopy, e, e.gx);
 }
 
-void Backend::addReduction(int kind, int bank, int blocks, double p0, double p1, double p2) {
+void Backend::addReduction(int kind, int bank, int blocks, double p0, double p1, double p2, int from) {
     if (pending_.n == kMaxRedJobs) throw std::logic_error("too many pending reductions");
-    pending_.job[pending_.n++] = RedJob{partials(kind, bank), blocks, kind, dev_scal_ + bank * kScalarCount,
-                                        scal_copy_.get() + bank * kScalarCount, p0, p1, p2};
+    pending_.job[pending_.n++] = RedJob{partials(kind, from >= 0 ? from : bank), blocks, kind,
+                                        dev_scal_ + bank * kScalarCount, scal_copy_.get() + bank * kScalarCount, p0,
+                                        p1, p2};
 }
 
 void Backend::finalizePending() {
@@ -1114,37 +1143,42 @@ struct DeviceScratchBuffer {
 }  // namespace
 }  // namespace mpgp
 
-extern "C" mp_gp_status mp_gp_dct_tile_time(int32_t nx, int32_t ny, int32_t precision, int32_t mode, int32_t tile,
-                                            int32_t evaluations, int32_t reps, double* microseconds) {
-    using namespace mpgp;
-    g_last_error.clear();
-    try {
-        if (!microseconds || reps < 1 || evaluations < 1 || evaluations > kMaxEval)
-            throw std::invalid_argument("bad tuning arguments");
+namespace mpgp {
+namespace {
+// One product of a given tile on operands of the evaluation chain's shapes (gp_backend.cpp,
+// enqueueEvaluate): data grids of ny x 2 nx floats per product (the widest, [T1 | T2] and the
+// interleaved field) and factors of 2 n^2 floats, on device 0 and a stream of its own.
+struct TileProduct {
+    int nx, ny, precision, mode, tile, batch;
+    std::size_t grid, factor;
+    DeviceScratchBuffer data, fac, out, part;
+    hipStream_t s = nullptr;
+    DctBatch d{};
+    SpectralArgs sp{};
+    int lda = 0, ldb = 0, ldc = 0, step = 1;
+
+    static std::size_t gridOf(int nx, int ny) { return static_cast<std::size_t>(ny) * 2 * nx; }
+    static std::size_t factorOf(int nx, int ny) {
+        return 2 * static_cast<std::size_t>(std::max(nx, ny)) * std::max(nx, ny);
+    }
+    static int batchOf(int mode, int evaluations) { return mode == kMirrorRows ? 2 * evaluations : evaluations; }
+    static void check(int nx, int ny, int precision, int mode, int tile) {
         if (!foldedDct(nx, ny)) throw std::invalid_argument("half-length products need both bin counts multiples of 64");
         if (mode < kFoldCols || mode > kMirrorRows) throw std::invalid_argument("unknown DCT product mode");
         if (precision != MP_DCT_FP32 && precision != MP_DCT_BF16X3) throw std::invalid_argument("unknown DCT precision");
         if (!dctTileValid(tile, mode, ny, nx)) throw std::invalid_argument("DCT tile does not divide the product");
         std::string reason;
         if (allocationLimit(0, ~0ull, reason) == 0) throw std::runtime_error(reason);
-        // operands shaped as the evaluation's products (gp_backend.cpp, enqueueEvaluate): data grids of
-        // ny x 2 nx floats (the widest, [T1 | T2] and the interleaved field), factors of n x 2n
-        const std::size_t grid = static_cast<std::size_t>(ny) * 2 * nx, factor = 2 * static_cast<std::size_t>(std::max(nx, ny)) * std::max(nx, ny);
-        const int batch = mode == kMirrorRows ? 2 * evaluations : evaluations;
-        DeviceScratchBuffer data(sizeof(float) * grid * batch), fac(sizeof(float) * factor), out(sizeof(float) * grid * batch),
-            part(sizeof(double) * static_cast<std::size_t>(dctWorkgroups(tile, mode, ny, nx)) * batch);
-        hipStream_t s = nullptr;
+    }
+    TileProduct(int nx_, int ny_, int precision_, int mode_, int tile_, int evaluations)
+        : nx(nx_), ny(ny_), precision(precision_), mode(mode_), tile(tile_), batch(batchOf(mode_, evaluations)),
+          grid(gridOf(nx_, ny_)), factor(factorOf(nx_, ny_)), data(sizeof(float) * grid * batch),
+          fac(sizeof(float) * factor), out(sizeof(float) * grid * batch),
+          part(sizeof(double) * static_cast<std::size_t>(dctWorkgroups(tile_, mode_, ny_, nx_)) * batch) {
         hipCheck(hipStreamCreate(&s), "create tuning stream");
-        std::unique_ptr<std::remove_pointer_t<hipStream_t>, decltype(&hipStreamDestroy)> own(s, hipStreamDestroy);
-        {
-            std::vector<float> h(std::max(grid * batch, factor));
-            for (std::size_t i = 0; i < h.size(); ++i) h[i] = static_cast<float>((i * 2654435761u % 1000) * 1e-3 - 0.5);
-            hipCheck(hipMemcpy(data.p, h.data(), sizeof(float) * grid * batch, hipMemcpyHostToDevice), "fill tuning data");
-            hipCheck(hipMemcpy(fac.p, h.data(), sizeof(float) * factor, hipMemcpyHostToDevice), "fill tuning factors");
-        }
+        hipCheck(hipMemsetAsync(out.p, 0, sizeof(float) * grid * batch, s), "clear tuning output");
         const int nxp = nx;  // multiples of 64 need no padding
-        DctBatch d{};
-        SpectralArgs sp{nx, ny, static_cast<double>(nx), static_cast<double>(ny), 1.0 / (4.0 * nx * ny), {}};
+        sp = SpectralArgs{nx, ny, static_cast<double>(nx), static_cast<double>(ny), 1.0 / (4.0 * nx * ny), {}};
         for (int e = 0; e < batch; ++e) {
             const bool data_left = mode == kFoldCols || mode == kMirrorCols;
             float* dg = static_cast<float*>(data.p) + grid * e;
@@ -1154,27 +1188,80 @@ extern "C" mp_gp_status mp_gp_dct_tile_time(int32_t nx, int32_t ny, int32_t prec
             d.sigma[e] = e % 2 ? -1.f : 1.f;
             sp.partials[e] = static_cast<double*>(part.p) + static_cast<std::size_t>(dctWorkgroups(tile, mode, ny, nx)) * e;
         }
-        int lda = 0, ldb = 0, ldc = nxp, step = 1;
+        ldc = nxp;
         switch (mode) {
             case kFoldCols: lda = nxp; break;
             case kFoldRows: ldb = nxp; break;
             case kMirrorCols: lda = nxp; ldc = 2 * nxp; break;
             default: ldb = 2 * nxp; step = 2; break;
         }
-        for (int r = 0; r < 3; ++r)
-            launchDctTile(tile, mode, d, batch, ny, nx, lda, ldb, ldc, step, precision, sp, s);
+    }
+    ~TileProduct() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+    void fill(const float* host_data, const float* host_factors) {
+        hipCheck(hipMemcpyAsync(data.p, host_data, sizeof(float) * grid * batch, hipMemcpyHostToDevice, s),
+                 "upload tile operands");
+        hipCheck(hipMemcpyAsync(fac.p, host_factors, sizeof(float) * factor, hipMemcpyHostToDevice, s),
+                 "upload tile factors");
+        hipCheck(hipStreamSynchronize(s), "upload tile operands");
+    }
+    void launch() { launchDctTile(tile, mode, d, batch, ny, nx, lda, ldb, ldc, step, precision, sp, s); }
+};
+}  // namespace
+}  // namespace mpgp
+
+extern "C" mp_gp_status mp_gp_dct_tile_time(int32_t nx, int32_t ny, int32_t precision, int32_t mode, int32_t tile,
+                                            int32_t evaluations, int32_t reps, double* microseconds) {
+    using namespace mpgp;
+    g_last_error.clear();
+    try {
+        if (!microseconds || reps < 1 || evaluations < 1 || evaluations > kMaxEval)
+            throw std::invalid_argument("bad tuning arguments");
+        TileProduct::check(nx, ny, precision, mode, tile);
+        TileProduct t(nx, ny, precision, mode, tile, evaluations);
+        {
+            std::vector<float> h(std::max(t.grid * t.batch, t.factor));
+            for (std::size_t i = 0; i < h.size(); ++i) h[i] = static_cast<float>((i * 2654435761u % 1000) * 1e-3 - 0.5);
+            t.fill(h.data(), h.data());
+        }
+        for (int r = 0; r < 3; ++r) t.launch();
         hipEvent_t a, b;
         hipCheck(hipEventCreate(&a), "create tuning event");
         hipCheck(hipEventCreate(&b), "create tuning event");
-        hipCheck(hipEventRecord(a, s), "record tuning event");
-        for (int r = 0; r < reps; ++r) launchDctTile(tile, mode, d, batch, ny, nx, lda, ldb, ldc, step, precision, sp, s);
-        hipCheck(hipEventRecord(b, s), "record tuning event");
+        hipCheck(hipEventRecord(a, t.s), "record tuning event");
+        for (int r = 0; r < reps; ++r) t.launch();
+        hipCheck(hipEventRecord(b, t.s), "record tuning event");
         hipCheck(hipEventSynchronize(b), "time tile");
         float ms = 0.f;
         hipCheck(hipEventElapsedTime(&ms, a, b), "read tuning events");
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         *microseconds = 1000.0 * ms / reps;
+        return MP_GP_OK;
+    } catch (...) {
+        return translate(std::current_exception());
+    }
+}
+
+extern "C" mp_gp_status mp_gp_dct_tile_run(int32_t nx, int32_t ny, int32_t precision, int32_t mode, int32_t tile,
+                                           const float* data, int64_t data_count, const float* factors,
+                                           int64_t factor_count, float* out) {
+    using namespace mpgp;
+    g_last_error.clear();
+    try {
+        if (!data || !factors || !out) throw std::invalid_argument("null tile operand");
+        TileProduct::check(nx, ny, precision, mode, tile);
+        const int batch = TileProduct::batchOf(mode, 1);
+        if (data_count != static_cast<int64_t>(TileProduct::gridOf(nx, ny) * batch) ||
+            factor_count != static_cast<int64_t>(TileProduct::factorOf(nx, ny)))
+            throw std::invalid_argument("tile operand sizes do not match the product");
+        TileProduct t(nx, ny, precision, mode, tile, 1);
+        t.fill(data, factors);
+        t.launch();
+        hipCheck(hipMemcpyAsync(out, t.out.p, sizeof(float) * t.grid * t.batch, hipMemcpyDeviceToHost, t.s),
+                 "download tile output");
+        hipCheck(hipStreamSynchronize(t.s), "run tile");
         return MP_GP_OK;
     } catch (...) {
         return translate(std::current_exception());

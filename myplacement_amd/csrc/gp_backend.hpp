@@ -117,8 +117,9 @@ public:
     virtual void wire(const WireArgs& a, int count) = 0;
     virtual void zero(double* p, int n) = 0;
     virtual void density(const DensityArgs& a, int count) = 0;
+    // `product`: the product's place 0..3 in the evaluation chain (two forward, two inverse)
     virtual void gemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
-                      const SpectralArgs& sp, int precision) = 0;
+                      const SpectralArgs& sp, int precision, int product) = 0;
     virtual void dct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
                      int precision, const SpectralArgs& sp) = 0;
     virtual void gather(const GatherArgs& a, int count) = 0;
@@ -137,7 +138,7 @@ public:
     void zero(double* p, int n) override;
     void density(const DensityArgs& a, int count) override;
     void gemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
-              const SpectralArgs& sp, int precision) override;
+              const SpectralArgs& sp, int precision, int product) override;
     void dct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
              int precision, const SpectralArgs& sp) override;
     void gather(const GatherArgs& a, int count) override;
@@ -184,7 +185,7 @@ public:
     void zero(double* p, int n) override;
     void density(const DensityArgs& a, int count) override;
     void gemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
-              const SpectralArgs& sp, int precision) override;
+              const SpectralArgs& sp, int precision, int product) override;
     void dct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
              int precision, const SpectralArgs& sp) override;
     void gather(const GatherArgs& a, int count) override;
@@ -198,7 +199,6 @@ public:
 private:
     template <typename E>
     void add(int stage, int kernel, const E& e, int blocks, std::size_t lds = 0);
-    int spectral_ = 0;  // product index within the current evaluation chain
 };
 
 struct HostScalars {
@@ -242,6 +242,11 @@ public:
         const SessionOut* session;   // null: write the separate wire / density gradients
         int bank;                    // scalar bank of the results
         bool reorder;                // first rebuild the particle row order at these positions (first spec only)
+        // >= 0 (first spec only): the last evaluation enqueued, with scalar bank `density_from`, was
+        // at these same positions, so slot 0 still holds their density and potential and that
+        // bank's metric and energy partial rows are still current: the deposit, the bin metrics and
+        // the forward products are not run again (their results would be the same bits)
+        int density_from = -1;
     };
     // Enqueues `count` (1 or 2) evaluations with smoothing `gamma` as one chain of batched
     // launches.  The density deposit visits particles in the shared row order, which only has to
@@ -279,8 +284,9 @@ public:
     const double2* sizes() const { return size_.get(); }
     // Partial rows of reduction `kind` destined for scalar bank `bank`.
     double* partials(int kind, int bank) const { return partials_[kind * kScalarBanks + bank].get(); }
-    // Queues the final reduction of `blocks` partial rows; done by the next fetchScalars().
-    void addReduction(int kind, int bank, int blocks, double p0 = 0, double p1 = 0, double p2 = 0);
+    // Queues the final reduction of `blocks` partial rows; done by the next fetchScalars().  The
+    // rows are those of bank `from` when it is given (>= 0), else those of `bank`.
+    void addReduction(int kind, int bank, int blocks, double p0 = 0, double p1 = 0, double p2 = 0, int from = -1);
 
     // The kernels' destination: the backend's stream by default; a batch driver substitutes a
     // recorder (and then finalizes the pending reductions itself, see takePending).

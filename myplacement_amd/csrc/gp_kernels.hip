@@ -982,8 +982,9 @@ __device__ inline void dctStore(const DctBatch& d, int z, float* __restrict__ C,
 // parities.  Tile t's contraction is split over its S wavefronts: wavefront s takes chunks s, s + S,
 // s + 2S, ... in order (two in flight), and with S > 1 the partial tiles meet in LDS, summed in
 // wavefront order, before the epilogue; with S = 1 the epilogue runs from the accumulators.  The
-// per-element summation order depends on S only, never on TR, TC or W.
-template <int MODE, int KPL, int TR, int TC, int W, int S>
+// per-element summation order depends on S only, never on TR, TC, W or F (F: the rotating operand
+// prefetch).
+template <int MODE, int KPL, int TR, int TC, int W, int S, int F>
 __device__ __forceinline__ void dctBody(const DctBatch& d, int n, int lda, int ldb, int ldc, int step,
                                         const SpectralArgs& sp, const Blk& bk) {
     constexpr int RH = TR / 16, CH = TC / 16;
@@ -1007,13 +1008,35 @@ __device__ __forceinline__ void dctBody(const DctBatch& d, int n, int lda, int l
             for (int ch = 0; ch < CH; ++ch) acc[p][rh][ch] = floatx4{0.f, 0.f, 0.f, 0.f};
     constexpr int chunk = 4 * KPL;
     constexpr int stride = chunk * S;  // between this wavefront's chunks
-    for (int k0 = part * chunk; k0 < h; k0 += 2 * stride) {
+    // Chunks enter the accumulators in order k0, k0 + stride, k0 + 2 stride, ... either way.
+    if constexpr (F) {
+        // two operand buffers rotate: each one's next load is issued as soon as its products have
+        // been issued, so a chunk's loads fly while the previous chunk's products run
         float a0[2][RH][KPL], b0[2][CH][KPL], a1[2][RH][KPL], b1[2][CH][KPL];
-        const bool second = k0 + stride < h;
-        dctLoad<MODE, KPL>(A, B, n, h, lda, ldb, row0, col0, k0 + KPL * grp, r16, a0, b0);
-        if (second) dctLoad<MODE, KPL>(A, B, n, h, lda, ldb, row0, col0, k0 + stride + KPL * grp, r16, a1, b1);
-        dctMma(a0, b0, acc);
-        if (second) dctMma(a1, b1, acc);
+        int k0 = part * chunk;
+        if (k0 < h) dctLoad<MODE, KPL>(A, B, n, h, lda, ldb, row0, col0, k0 + KPL * grp, r16, a0, b0);
+        if (k0 + stride < h)
+            dctLoad<MODE, KPL>(A, B, n, h, lda, ldb, row0, col0, k0 + stride + KPL * grp, r16, a1, b1);
+        for (; k0 < h; k0 += 2 * stride) {
+            dctMma(a0, b0, acc);
+            if (k0 + 2 * stride < h)
+                dctLoad<MODE, KPL>(A, B, n, h, lda, ldb, row0, col0, k0 + 2 * stride + KPL * grp, r16, a0, b0);
+            if (k0 + stride < h) {
+                dctMma(a1, b1, acc);
+                if (k0 + 3 * stride < h)
+                    dctLoad<MODE, KPL>(A, B, n, h, lda, ldb, row0, col0, k0 + 3 * stride + KPL * grp, r16, a1, b1);
+            }
+        }
+    } else {
+        // two chunks loaded together, then their products
+        for (int k0 = part * chunk; k0 < h; k0 += 2 * stride) {
+            float a0[2][RH][KPL], b0[2][CH][KPL], a1[2][RH][KPL], b1[2][CH][KPL];
+            const bool second = k0 + stride < h;
+            dctLoad<MODE, KPL>(A, B, n, h, lda, ldb, row0, col0, k0 + KPL * grp, r16, a0, b0);
+            if (second) dctLoad<MODE, KPL>(A, B, n, h, lda, ldb, row0, col0, k0 + stride + KPL * grp, r16, a1, b1);
+            dctMma(a0, b0, acc);
+            if (second) dctMma(a1, b1, acc);
+        }
     }
     double energy = 0.0;
     if constexpr (S == 1) {
@@ -1059,10 +1082,10 @@ __device__ __forceinline__ void dctBody(const DctBatch& d, int n, int lda, int l
         publishPartials<1, kWaves>(v, 0U, bk.y * bk.gx + bk.x, sp.partials[z], lds);
     }
 }
-template <int MODE, int KPL, int TR, int TC, int W, int S>
+template <int MODE, int KPL, int TR, int TC, int W, int S, int F>
 __global__ __launch_bounds__(64 * W * S) void k_dct(DctBatch d, int n, int lda, int ldb, int ldc, int step,
                                                      SpectralArgs sp) {
-    dctBody<MODE, KPL, TR, TC, W, S>(d, n, lda, ldb, ldc, step, sp, nativeBlk());
+    dctBody<MODE, KPL, TR, TC, W, S, F>(d, n, lda, ldb, ldc, step, sp, nativeBlk());
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1406,12 +1429,12 @@ __global__ __launch_bounds__(kThreads) void k_gemm_batch(const GemmEntry* e, con
     const GemmEntry& E = e[j];
     gemmBody<EPI>(E.g, E.K, E.lda, E.ldb, E.ldc, E.sp, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, E.gz));
 }
-template <int MODE, int KPL, int TR, int TC, int W, int S>
+template <int MODE, int KPL, int TR, int TC, int W, int S, int F>
 __global__ __launch_bounds__(64 * W * S) void k_dct_batch(const DctEntry* e, const int* first, int n) {
     const int j = batchEntry<64 * W * S>(first, n);
     const DctEntry& E = e[j];
-    dctBody<MODE, KPL, TR, TC, W, S>(E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp,
-                                     entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, E.gz));
+    dctBody<MODE, KPL, TR, TC, W, S, F>(E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp,
+                                        entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, E.gz));
 }
 __global__ __launch_bounds__(kThreads) void k_gather_batch(const GatherEntry* e, const int* first, int n) {
     const int j = batchEntry<kThreads>(first, n);
@@ -1536,13 +1559,13 @@ template <int V>
 void launchDctV(dim3 grid, const DctBatch& d, int n, int lda, int ldb, int ldc, int step, const SpectralArgs& sp,
                 hipStream_t s) {
     using X = DctVariant<V>;
-    hipLaunchKernelGGL((k_dct<X::mode, X::kpl, X::t.tr, X::t.tc, X::t.w, X::t.s>), grid, dim3(64 * X::t.w * X::t.s), 0,
+    hipLaunchKernelGGL((k_dct<X::mode, X::kpl, X::t.tr, X::t.tc, X::t.w, X::t.s, X::t.f>), grid, dim3(64 * X::t.w * X::t.s), 0,
                        s, d, n, lda, ldb, ldc, step, sp);
 }
 template <int V>
 void launchDctBatchV(dim3 grid, const DctEntry* e, const int* first, int n, hipStream_t s) {
     using X = DctVariant<V>;
-    hipLaunchKernelGGL((k_dct_batch<X::mode, X::kpl, X::t.tr, X::t.tc, X::t.w, X::t.s>), grid,
+    hipLaunchKernelGGL((k_dct_batch<X::mode, X::kpl, X::t.tr, X::t.tc, X::t.w, X::t.s, X::t.f>), grid,
                        dim3(64 * X::t.w * X::t.s), 0, s, e, first, n);
 }
 template <int... V>

@@ -161,16 +161,19 @@ void launchGemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int
 //               A: factor halves [2][h][h], B: n x N
 enum DctMode : int { kFoldCols = 0, kFoldRows = 1, kMirrorCols = 2, kMirrorRows = 3 };
 // Tile shapes of the half-length products: a workgroup holds w tiles of tr x tc half-size outputs
-// (both parities) side by side, each tile's contraction split over s wavefronts.  Results depend on
-// s only (the order of each element's sum), never on tr, tc or w.  Tile 0 is valid for every folded
-// grid and is the default where the tuned table (dct_tiles.inc, written by
-// myplacement_amd/autotune.py on an MI355X) names none.
+// (both parities) side by side, each tile's contraction split over s wavefronts; f = 1 rotates two
+// operand buffers so that a chunk's loads fly while the previous chunk's products run (f = 0 loads
+// two chunks, then runs their products).  Results depend on s only (the order of each element's
+// sum), never on tr, tc, w or f.  Tile 0 is valid for every folded grid and is the default where the
+// tuned table (dct_tiles.inc, written by myplacement_amd/autotune.py on an MI355X) names none.
 struct DctTile {
-    int tr, tc, w, s;
+    int tr, tc, w, s, f;
 };
-constexpr int kDctTileCount = 8;
-constexpr DctTile kDctTiles[kDctTileCount] = {{32, 32, 1, 8}, {32, 32, 1, 4}, {16, 32, 1, 4}, {32, 32, 1, 2},
-                                              {32, 32, 1, 1}, {32, 32, 2, 1}, {16, 32, 2, 1}, {16, 16, 4, 1}};
+constexpr int kDctTileCount = 16;
+constexpr DctTile kDctTiles[kDctTileCount] = {
+    {32, 32, 1, 8, 0}, {32, 32, 1, 4, 0}, {16, 32, 1, 4, 0}, {32, 32, 1, 2, 0}, {32, 32, 1, 1, 0}, {32, 32, 2, 1, 0},
+    {16, 32, 2, 1, 0}, {16, 16, 4, 1, 0}, {32, 32, 1, 8, 1}, {32, 32, 1, 4, 1}, {16, 32, 1, 4, 1}, {32, 32, 1, 2, 1},
+    {32, 32, 1, 1, 1}, {32, 32, 2, 1, 1}, {16, 32, 2, 1, 1}, {16, 16, 4, 1, 1}};
 bool dctTileValid(int tile, int mode, int rows, int cols);
 int dctTileFor(int mode, int precision, int rows, int cols);  // the tuned tile of a product
 int dctWorkgroups(int tile, int mode, int rows, int cols);    // per product (energy partial rows)

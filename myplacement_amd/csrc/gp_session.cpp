@@ -36,7 +36,8 @@ void validateOptions(const mp_gp_options& o) {  // AdaptiveGlobalPlacer.cpp:413-
         !std::isfinite(o.target_density) || !std::isfinite(o.maximum_bin_overflow) ||
         !std::isfinite(o.initial_smoothing) || !std::isfinite(o.final_smoothing) ||
         !std::isfinite(o.maximum_density_penalty) || !(o.density_order_reuse_rows >= 0.0) ||
-        !std::isfinite(o.density_order_reuse_rows) || (o.capture_graphs != 0 && o.capture_graphs != 1))
+        !std::isfinite(o.density_order_reuse_rows) || (o.capture_graphs != 0 && o.capture_graphs != 1) ||
+        (o.recompute_density != 0 && o.recompute_density != 1))
         throw std::invalid_argument("Adaptive global placement options are outside their valid range.");
     if (static_cast<std::size_t>(o.bins_x) > (std::size_t{1} << 20) / static_cast<std::size_t>(o.bins_y))
         throw std::invalid_argument("Density grid exceeds the safe one-million-bin workspace limit.");

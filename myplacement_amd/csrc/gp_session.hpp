@@ -484,11 +484,17 @@ private:
         const Backend::SessionOut so_cur =
             sessionOut(Rc_.get(), Gc_.get(), try_momentum ? XP_.get() : nullptr, pending_ck_ ? pending_ck_->get() : nullptr);
         const Backend::SessionOut so_look = sessionOut(Rl_.get(), Gl_.get(), nullptr, nullptr);
-        Backend::EvalSpec specs[2] = {{X_.get(), &so_cur, kBankCurrent, reorder},
+        // X is the candidate accepted last iteration, and that candidate's evaluation was the last one
+        // enqueued: its density, potential and energy are X's (smoothing and penalty, which changed
+        // since, enter neither), so X's evaluation takes them instead of computing them again
+        const bool reuse = it > 0 && density_at_x_ && o_.recompute_density == 0;
+        density_at_x_ = false;
+        Backend::EvalSpec specs[2] = {{X_.get(), &so_cur, kBankCurrent, reorder, reuse ? int{kBankCandidate} : -1},
                                       {B_.get(), &so_look, kBankLook, fresh}};
         // the chain up to the first read-back, as one graph when asked (its shape: the momentum
-        // look-ahead, the row-order rebuild, the evaluations one after another)
-        if (it > 0 && o_.capture_graphs) be_->beginCapture((try_momentum ? 1 : 0) | (reorder ? 2 : 0) | (fresh ? 4 : 0));
+        // look-ahead, the row-order rebuild, the evaluations one after another, the density taken over)
+        if (it > 0 && o_.capture_graphs)
+            be_->beginCapture((try_momentum ? 1 : 0) | (reorder ? 2 : 0) | (fresh ? 4 : 0) | (reuse ? 8 : 0));
         if (try_momentum) {
             be_->timedBegin(kTController);
             be_->launches().lookahead(LookaheadArgs{X_.get(), XP_.get(), beta, B_.get(), clamp_, P_});
@@ -659,6 +665,7 @@ private:
             // X_prev <- X, X <- Y (velocity X - X_prev, :861-863)
             X_.swapWith(Y_);
             XP_.swapWith(Y_);
+            density_at_x_ = true;  // the accepted candidate's evaluation was the last one enqueued
             mX_ = used_disp + (momentum_used ? look_drift : 0.0);
             dX_ = base_drift + used_disp;
             last_overflow_ = acc.s.optimizer_density.normalized_overflow;
@@ -699,6 +706,7 @@ private:
     double dX_ = 0.0, mX_ = 0.0;  // movement bounds: X from the row order's positions, X from X_prev
     int momentum_age_ = 0, worse_streak_ = 0, feasible_refinement_ = 0;
     bool has_previous_ = false, stopped_ = false;
+    bool density_at_x_ = false;  // slot 0 and the candidate bank's partial rows hold X's density
     EvalRec current_;
     Checkpoint feasible_, fallback_;
     mp_gp_result result_{};

@@ -365,3 +365,35 @@ def test_graph_captured_iterations_equal_direct_launches(case):
     assert np.array_equal(runs[0][0], runs[1][0])
     assert np.array_equal(runs[0][2], runs[1][2])
     assert runs[0][1]["gradient_evaluations"] == runs[1][1]["gradient_evaluations"]
+
+
+@pytest.mark.parametrize("case", ["plain-64", "plain-50x30-fresh-order", "mid-128-quadratic"])
+def test_density_taken_over_equals_recomputed(case):
+    """An iteration's gradient evaluation at the candidate accepted the iteration before takes that
+    candidate's density, potential and energy (recompute_density = 0, the default) instead of
+    computing them again (recompute_density = 1): smoothing and penalty, the only inputs that changed,
+    enter neither, so the runs must be bit-identical.  Cases: the half-length DCT products (64), the
+    dense products of a grid that is not a multiple of 64 with the row order rebuilt for every
+    evaluation (the look-ahead then evaluated on its own), and a quadratic start at 128 bins."""
+    if case == "plain-64":
+        d, o = _setup(design.plain_design(), 0.9, 64)
+    elif case == "plain-50x30-fresh-order":
+        d, o = _setup(design.plain_design(), 0.9, (50, 30))
+        o.density_order_reuse_rows = 0.0
+    else:
+        d = design.large_design(seed=3, cells=20000, nets=21000, row_count=200, sites=400, macros=6)
+        d.center, _ = hb.quadratic_placement(d)
+        o = abi.default_options(target_density=0.8, bins_x=128, bins_y=128)
+    o.iterations = 150
+    o.feasible_refinement_iterations = 2 ** 31 - 1
+    runs = []
+    for rd in (1, 0):
+        o.recompute_density = rd
+        say(f"[gpu] {case}: recompute_density={rd}")
+        c, r, hist = hb.global_place(d, o)
+        runs.append((c, r, np.array([[row[k] for k in abi.HISTORY_COLUMNS] for row in hist])))
+    assert runs[0][1]["accepted_iterations"] > 20
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][2], runs[1][2])
+    for k in ("gradient_evaluations", "value_evaluations", "momentum_restarts", "rejected_candidates"):
+        assert runs[0][1][k] == runs[1][1][k], k
