@@ -300,8 +300,8 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
     const int max_blocks = std::max({wireBlocks(n_small_, n_medium_, n_large_), densityBinBlocks(region_),
                                      particleBlocks(P_), (nxp_ / 32) * (nyp_ / 32), 1});
     for (int k = 0; k < kRedKinds * kScalarBanks; ++k)
-        partials_[k].allocate(static_cast<std::size_t>(max_blocks) * redWidth(k / kScalarBanks), budget_,
-                              "reduction partials");
+        partials_[k].allocate(static_cast<std::size_t>(max_blocks + 1) * redWidth(k / kScalarBanks), budget_,
+                              "reduction partials");  // + the row a wirelength evaluated ahead writes
     scal_copy_.allocate(static_cast<std::size_t>(kScalarCount) * kScalarBanks, budget_, "scalar block copy");
     wire_out_.allocate(P_, budget_, "wire gradient");
     density_out_.allocate(P_, budget_, "density gradient");
@@ -506,15 +506,7 @@ void Backend::sortParticles(const double2* pos) {
     timedEnd(kTSort, 64.0 * P_);
 }
 
-void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bool want_grad) {
-    if (count < 1 || count > kSlots) throw std::logic_error("one or two evaluations per batch");
-    for (int e = 0; e < count; ++e) {
-        if (!specs[e].session && count != 1) throw std::logic_error("backend-mode evaluations are not batched");
-        if (specs[e].density_from >= 0 && (e != 0 || specs[e].density_from >= kScalarBanks))
-            throw std::logic_error("only the first evaluation may take the density of slot 0");
-    }
-    if (specs[0].reorder) sortParticles(specs[0].pos);
-
+WireArgs Backend::wireArgs(double gamma, bool want_grad) const {
     WireArgs wa{};
     wa.net_off = net_off_.get();
     wa.net_w = net_w_.get();
@@ -529,22 +521,47 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
     }
     wa.gamma = gamma;
     wa.want_grad = want_grad ? 1 : 0;
+    return wa;
+}
+
+void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bool want_grad) {
+    if (count < 1 || count > kSlots) throw std::logic_error("one or two evaluations per batch");
     for (int e = 0; e < count; ++e) {
-        wa.pos[e] = specs[e].pos;
-        wa.pin_grad[e] = slots_[e].pin_grad.get();
-        wa.partials[e] = partials(kRedWire, specs[e].bank);
+        if (!specs[e].session && count != 1) throw std::logic_error("backend-mode evaluations are not batched");
+        if (specs[e].density_from >= 0 && (e != 0 || specs[e].density_from >= kScalarBanks))
+            throw std::logic_error("only the first evaluation may take the density of slot 0");
     }
-    timedBegin(kTWire);
-    if (n_small_ + n_medium_ + n_large_ > 0) {
-        launches_->wire(wa, count);
-        for (int e = 0; e < count; ++e) addReduction(kRedWire, specs[e].bank, wireBlocks(n_small_, n_medium_, n_large_));
-    } else {
-        for (int e = 0; e < count; ++e) launches_->zero(dev_scal_ + specs[e].bank * kScalarCount + kHpwl, 2);
+    if (specs[0].reorder) sortParticles(specs[0].pos);
+
+    WireArgs wa = wireArgs(gamma, want_grad);
+    const bool nets = n_small_ + n_medium_ + n_large_ > 0;
+    const int wire_rows = wireBlocks(n_small_, n_medium_, n_large_);
+    int nw = 0;  // evaluations whose wirelength runs here
+    for (int e = 0; e < count; ++e) {
+        if (specs[e].wire_ready) {
+            if (e != 0 || !want_grad || !nets) throw std::logic_error("only a first gradient evaluation is evaluated ahead");
+            addReduction(kRedWire, specs[e].bank, wire_rows, gamma, 1.0);
+            continue;
+        }
+        wa.pos[nw] = specs[e].pos;
+        wa.pin_grad[nw] = slots_[e].pin_grad.get();
+        wa.partials[nw] = partials(kRedWire, specs[e].bank);
+        ++nw;
     }
-    // algorithmic bytes per evaluation, by the convention of one pass over each array at the path's
-    // nominal widths (4-byte indices and scalars, 8-byte positions): 12 per pin, 8 per net, 16 per
-    // particle with the gradient and 8 without
-    timedEnd(kTWire, count * (12.0 * I_ + 8.0 * N_ + (want_grad ? 16.0 : 8.0) * P_));
+    if (nw > 0) {
+        timedBegin(kTWire);
+        if (nets) {
+            launches_->wire(wa, nw);
+            for (int e = 0; e < count; ++e)
+                if (!specs[e].wire_ready) addReduction(kRedWire, specs[e].bank, wire_rows);
+        } else {
+            for (int e = 0; e < count; ++e) launches_->zero(dev_scal_ + specs[e].bank * kScalarCount + kHpwl, 2);
+        }
+        // algorithmic bytes per evaluation, by the convention of one pass over each array at the
+        // path's nominal widths (4-byte indices and scalars, 8-byte positions): 12 per pin, 8 per
+        // net, 16 per particle with the gradient and 8 without
+        timedEnd(kTWire, nw * (12.0 * I_ + 8.0 * N_ + (want_grad ? 16.0 : 8.0) * P_));
+    }
 
     // The evaluations whose density is computed here (fwd[0..nf)); an evaluation at the positions
     // of the one before it takes that one's density, potential and partial rows instead.
@@ -829,6 +846,7 @@ void Backend::beginCapture(int shape) {
 const HostScalars& Backend::fetchScalars() {
     const int jobs = pending_.n;
     if (jobs == 0 && capture_shape_ < 0) {
+        ahead_pending_ = false;  // nothing was evaluated to go ahead from
         hipCheck(hipStreamSynchronize(stream_), "synchronize placement evaluation");
         return *host_scal_;
     }
@@ -836,6 +854,24 @@ const HostScalars& Backend::fetchScalars() {
     pending_.flag = dev_done_;
     if (jobs > 0) launchFinalize(pending_, stream_);
     pending_.n = 0;
+    auto launchAhead = [&] {
+        if (!ahead_pending_) return;
+        ahead_pending_ = false;
+        if (n_small_ + n_medium_ + n_large_ == 0) return;
+        WireArgs wa = wireArgs(0.0, true);
+        wa.pos[0] = ahead_.pos;
+        wa.pin_grad[0] = slots_[0].pin_grad.get();
+        wa.partials[0] = partials(kRedWire, ahead_.bank);
+        wa.ahead_overflow = ahead_.overflow;
+        wa.ahead_denominator = ahead_.denominator;
+        wa.ahead_final = ahead_.final_smoothing;
+        wa.ahead_span = ahead_.span;
+        wa.ahead_scale = ahead_.scale;
+        wa.ahead_rows = wireBlocks(n_small_, n_medium_, n_large_);
+        timedBegin(kTWire);
+        launchWire(wa, 1, stream_);
+        timedEnd(kTWire, 12.0 * I_ + 8.0 * N_ + 16.0 * P_);
+    };
     if (capture_shape_ >= 0) {
         const int shape = capture_shape_;
         capture_shape_ = -1;
@@ -861,10 +897,13 @@ const HostScalars& Backend::fetchScalars() {
             it = graphs_.end() - 1;
         }
         hipCheck(hipGraphLaunch(it->second, stream_), "launch graph");
+        launchAhead();
         if (jobs == 0) {
             hipCheck(hipStreamSynchronize(stream_), "synchronize placement evaluation");
             return *host_scal_;
         }
+    } else {
+        launchAhead();
     }
     // Poll the completion words: the results are on the host as soon as the last one lands, some
     // microseconds before the stream reports the kernel retired.  The stream is queried now and

@@ -247,6 +247,10 @@ public:
         // bank's metric and energy partial rows are still current: the deposit, the bin metrics and
         // the forward products are not run again (their results would be the same bits)
         int density_from = -1;
+        // (first spec only) the wirelength at these positions was evaluated ahead (wireAhead) with
+        // smoothing `gamma`, into slot 0 and this bank's partial rows: it is not run again, and the
+        // finalize checks that the smoothing it used is `gamma`
+        bool wire_ready = false;
     };
     // Enqueues `count` (1 or 2) evaluations with smoothing `gamma` as one chain of batched
     // launches.  The density deposit visits particles in the shared row order, which only has to
@@ -254,6 +258,17 @@ public:
     void enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bool want_grad);
     // Rebuilds the particle row order at `pos`.
     void sortParticles(const double2* pos);
+    // Asks the next fetchScalars() to enqueue, right after its finalize and before it waits, the
+    // wirelength gradient at `pos` into slot 0 and bank `bank`'s partial rows, with the smoothing
+    // law of WireArgs at the optimizer overflow the finalize leaves at `overflow` (device memory):
+    // the controller's next evaluation at `pos`, run while the host decides.
+    struct WireAhead {
+        const double2* pos;
+        int bank;
+        const double* overflow;
+        double denominator, final_smoothing, span, scale;
+    };
+    void wireAhead(const WireAhead& w) { ahead_ = w; ahead_pending_ = true; }
     // Reduces the pending partials into the scalar block and waits until k_finalize has written
     // it (by polling its completion words; the rest of the kernel's retirement is not waited for).
     const HostScalars& fetchScalars();
@@ -370,6 +385,9 @@ private:
     hipEvent_t poolEvent();
     unsigned timing_mask_ = 0;
     int capture_shape_ = -1;                              // the chain being captured, or -1
+    WireAhead ahead_{};
+    bool ahead_pending_ = false;
+    WireArgs wireArgs(double gamma, bool want_grad) const;
     std::vector<std::pair<int, hipGraphExec_t>> graphs_;  // executable graph per chain shape
     std::vector<hipEvent_t> ev_pool_;
     std::size_t ev_next_ = 0;

@@ -291,13 +291,25 @@ __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_gra
 // quarter-wavefront each), small nets (a thread each): the longest-running workgroups are
 // dispatched first.
 constexpr int kLargePins = 8;  // pins per lane held in registers for a large net (512 per net)
+// The controller's smoothing law with exponent 1, operation for operation as the host evaluates it
+// (clampd = max(lo, min(v, hi)); no contraction), so that both get the same bits.
+__device__ inline double aheadSmoothing(const WireArgs& a) {
+#pragma clang fp contract(off)
+    const double r = *a.ahead_overflow / a.ahead_denominator;
+    const double m = 1.0 < r ? 1.0 : r;
+    const double ratio = 0.0 < m ? m : 0.0;
+    return (a.ahead_final + a.ahead_span * ratio) * a.ahead_scale;
+}
+
 __device__ __forceinline__ void wireBody(const WireArgs& a, int large_blocks, int medium_blocks, const Blk& bk) {
     __shared__ double lds[(kWaves + 1) * 2];
     double wl = 0.0, hp = 0.0;
-    const double inv_gamma = 1.0 / a.gamma;
-    const float gamma_f = static_cast<float>(a.gamma);
+    const double gamma = a.ahead_overflow ? aheadSmoothing(a) : a.gamma;
+    const double inv_gamma = 1.0 / gamma;
+    const float gamma_f = static_cast<float>(gamma);
     const int2 eb = batchBlock(bk);
     const int block = eb.y;
+    if (a.ahead_overflow && block == 0 && threadIdx.x == 0) a.partials[eb.x][2 * a.ahead_rows] = gamma;
     const double2* pos = a.pos[eb.x];
     float2* pin_grad = a.pin_grad[eb.x];
     if (block < large_blocks) {
@@ -1346,6 +1358,8 @@ __device__ __forceinline__ void finalizeBody(const RedJob& j, volatile unsigned*
         case kRedWire:
             put(kHpwl, acc[0]);
             put(kSmooth, acc[1]);
+            // evaluated ahead: the smoothing it used (after the rows) against the host's (p0)
+            if (j.p1 != 0.0) put(kWireGammaMismatch, j.partials[2 * static_cast<size_t>(j.blocks)] == j.p0 ? 0.0 : 1.0);
             break;
         case kRedMetrics:
             for (int f = 0; f < 2; ++f) {

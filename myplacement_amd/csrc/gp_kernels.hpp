@@ -28,6 +28,7 @@ enum ScalarSlot : int {
     kDecidedLook = 28,  // the device's step decision (1: look-ahead base), step and displacement cap
     kDecidedStep = 29,
     kDecidedDisp = 30,
+    kWireGammaMismatch = 31,  // 1: a wirelength evaluated ahead used another smoothing than the host's
     kScalarCount = 32
 };
 // The scalar block holds kScalarBanks banks of kScalarCount slots, so several evaluations can be
@@ -85,6 +86,13 @@ struct WireArgs {
     int want_grad;
     float2* pin_grad[kMaxEval];  // per-pin d(WL)/d(pin), internal pin order
     double* partials[kMaxEval];
+    // Evaluated ahead (one evaluation): the smoothing is not `gamma` but the controller's law
+    // (AdaptiveGlobalPlacer.cpp:456-462, exponent 1) at the optimizer overflow the device holds at
+    // *ahead_overflow: (final + span * clamp(overflow / denominator, 0, 1)) * scale; workgroup 0
+    // writes it after the partial rows, at partials[0][2 * ahead_rows].
+    const double* ahead_overflow;
+    double ahead_denominator, ahead_final, ahead_span, ahead_scale;
+    int ahead_rows;
 };
 int wireBlocks(int n_small, int n_medium, int n_large);
 void launchWire(const WireArgs& a, int count, hipStream_t s);

@@ -373,6 +373,19 @@ private:
         be_->enqueueEvaluate(&spec, 1, smoothing_, true);
         dX_ = 0.0;
     }
+    // After a candidate's evaluation at Y: its wirelength gradient at the smoothing the next
+    // iteration will use if Y is accepted (the law at Y's optimizer overflow) is enqueued right after
+    // the read-back's finalize, so it runs while the host decides.  Only where the device evaluates
+    // the law as the host does (exponent 1) and the launches are the session's own (a batch driver
+    // runs its own finalize).
+    void requestWireAhead() {
+        wire_ahead_ = !rec_ && o_.recompute_density == 0 && o_.smoothing_overflow_exponent == 1.0;
+        if (wire_ahead_)
+            be_->wireAhead(Backend::WireAhead{Y_.get(), kBankCurrent,
+                                              be_->deviceBank(kBankCandidate) + kOptDensity + 1,
+                                              std::max(initial_overflow_, kEps), o_.final_smoothing,
+                                              o_.initial_smoothing - o_.final_smoothing, max_bin_});
+    }
     // A new best position is copied by the next gradient evaluation at X (its gather writes the
     // copy); a copy still pending when that evaluation will not come is made here.
     void flushCheckpoint() {
@@ -489,12 +502,19 @@ private:
         // since, enter neither), so X's evaluation takes them instead of computing them again
         const bool reuse = it > 0 && density_at_x_ && o_.recompute_density == 0;
         density_at_x_ = false;
-        Backend::EvalSpec specs[2] = {{X_.get(), &so_cur, kBankCurrent, reorder, reuse ? int{kBankCandidate} : -1},
-                                      {B_.get(), &so_look, kBankLook, fresh}};
+        // and its wirelength gradient at this iteration's smoothing went ahead after that candidate's
+        // read-back (requestWireAhead)
+        const bool wire_ready = it > 0 && wire_at_x_;
+        wire_at_x_ = false;
+        Backend::EvalSpec specs[2] = {
+            {X_.get(), &so_cur, kBankCurrent, reorder, reuse ? int{kBankCandidate} : -1, wire_ready},
+            {B_.get(), &so_look, kBankLook, fresh}};
         // the chain up to the first read-back, as one graph when asked (its shape: the momentum
-        // look-ahead, the row-order rebuild, the evaluations one after another, the density taken over)
+        // look-ahead, the row-order rebuild, the evaluations one after another, the density taken
+        // over, the wirelength gone ahead)
         if (it > 0 && o_.capture_graphs)
-            be_->beginCapture((try_momentum ? 1 : 0) | (reorder ? 2 : 0) | (fresh ? 4 : 0) | (reuse ? 8 : 0));
+            be_->beginCapture((try_momentum ? 1 : 0) | (reorder ? 2 : 0) | (fresh ? 4 : 0) | (reuse ? 8 : 0) |
+                              (wire_ready ? 16 : 0));
         if (try_momentum) {
             be_->timedBegin(kTController);
             be_->launches().lookahead(LookaheadArgs{X_.get(), XP_.get(), beta, B_.get(), clamp_, P_});
@@ -536,8 +556,13 @@ private:
         }
         const bool need_read = it > 0;
         HostScalars first{};
-        if (need_read) first = co_await ScalarWait{this};
+        if (need_read) {
+            requestWireAhead();
+            first = co_await ScalarWait{this};
+        }
         if (it > 0) current_ = record(first, true, kBankCurrent);
+        if (wire_ready && first.bank(kBankCurrent)[kWireGammaMismatch] != 0.0)
+            throw std::logic_error("the wirelength evaluated ahead used another smoothing than the host's");
         const EvalRec& cur = current_;
 
         bool restarted = worse_streak_ >= 2;
@@ -612,6 +637,7 @@ private:
                     const Backend::SessionOut so_cand = sessionOut(nullptr, nullptr, nullptr, nullptr);
                     const Backend::EvalSpec cand_spec{Y_.get(), &so_cand, kBankCandidate, fresh};
                     be_->enqueueEvaluate(&cand_spec, 1, smoothing_, false);
+                    requestWireAhead();
                     fetched = co_await ScalarWait{this};
                 }
                 const HostScalars& h = fetched;
@@ -666,6 +692,7 @@ private:
             X_.swapWith(Y_);
             XP_.swapWith(Y_);
             density_at_x_ = true;  // the accepted candidate's evaluation was the last one enqueued
+            wire_at_x_ = wire_ahead_;
             mX_ = used_disp + (momentum_used ? look_drift : 0.0);
             dX_ = base_drift + used_disp;
             last_overflow_ = acc.s.optimizer_density.normalized_overflow;
@@ -707,6 +734,8 @@ private:
     int momentum_age_ = 0, worse_streak_ = 0, feasible_refinement_ = 0;
     bool has_previous_ = false, stopped_ = false;
     bool density_at_x_ = false;  // slot 0 and the candidate bank's partial rows hold X's density
+    bool wire_ahead_ = false;    // the last read-back ran the wirelength ahead at Y
+    bool wire_at_x_ = false;     // slot 0 and the current bank's wire rows hold X's at this smoothing
     EvalRec current_;
     Checkpoint feasible_, fallback_;
     mp_gp_result result_{};
