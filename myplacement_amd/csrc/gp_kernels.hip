@@ -78,15 +78,23 @@ __device__ inline int2 batchBlockChunked(const Blk& b) {
     return make_int2(lin % b.gy, xcdChunk(lin / b.gy, b.gx, 8 / b.gy));
 }
 
+// Flattened block a workgroup of a launch batched over jobs runs.  The hardware hands workgroups
+// to the eight XCDs round robin (workgroup i to XCD i mod 8); this bijection gives each XCD one
+// contiguous range of the flattened blocks, i.e. whole jobs, so that a job's positions, pins and
+// grids are cached in one XCD's L2 instead of in all eight.
+__device__ inline int batchSlot() {
+    return xcdChunk(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), 8);
+}
+
 // Entry of a launch batched over jobs: entry e owns the flattened blocks [first[e], first[e + 1]),
-// first[0] = 0.  Every thread tests entries against this workgroup's index; the count of entries
-// starting at or before it is the entry's index + 1.
+// first[0] = 0.  Every thread tests entries against the workgroup's flattened block `slot`; the
+// count of entries starting at or before it is the entry's index + 1.
 template <int THREADS>
-__device__ inline int batchEntry(const int* __restrict__ first, int entries) {
+__device__ inline int batchEntry(const int* __restrict__ first, int entries, int slot) {
     int count = 0;
     for (int k0 = 0; k0 < entries; k0 += THREADS) {
         const int k = k0 + static_cast<int>(threadIdx.x);
-        count += __syncthreads_count(k < entries && first[k] <= static_cast<int>(blockIdx.x));
+        count += __syncthreads_count(k < entries && first[k] <= slot);
     }
     return count - 1;
 }
@@ -1408,73 +1416,86 @@ int blocksFor(long long n) { return static_cast<int>((n + kThreads - 1) / kThrea
 // The same stages batched over independent jobs: one launch runs the stage for every entry, each
 // workgroup finding its entry and its block coordinates within it.
 __global__ __launch_bounds__(kThreads) void k_wire_batch(const WireEntry* e, const int* first, int n) {
-    const int j = batchEntry<kThreads>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<kThreads>(first, n, v);
     const WireEntry& E = e[j];
-    wireBody(E.a, E.large_blocks, E.medium_blocks, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, 1));
+    wireBody(E.a, E.large_blocks, E.medium_blocks, entryBlk(v - first[j], E.gx, E.gy, 1));
 }
 __global__ __launch_bounds__(kSortThreads) void k_dhist_batch(const SortEntry* e, const int* first, int n) {
-    const int j = batchEntry<kSortThreads>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<kSortThreads>(first, n, v);
     const SortEntry& E = e[j];
-    dhistBody(E.a, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, 1, 1));
+    dhistBody(E.a, entryBlk(v - first[j], E.gx, 1, 1));
 }
 __global__ __launch_bounds__(kThreads) void k_dscan_batch(const SortEntry* e, const int* first, int n) {
-    const int j = batchEntry<kThreads>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<kThreads>(first, n, v);
     const SortEntry& E = e[j];
-    dscanBody(E.a, E.rows, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, 1, 1));
+    dscanBody(E.a, E.rows, entryBlk(v - first[j], E.gx, 1, 1));
 }
 __global__ __launch_bounds__(kSortThreads) void k_dplace_batch(const SortEntry* e, const int* first, int n) {
-    const int j = batchEntry<kSortThreads>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<kSortThreads>(first, n, v);
     const SortEntry& E = e[j];
-    dplaceBody(E.a, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, 1, 1));
+    dplaceBody(E.a, entryBlk(v - first[j], E.gx, 1, 1));
 }
 __global__ __launch_bounds__(kDepositThreads) void k_ddeposit_batch(const DensityEntry* e, const int* first, int n) {
-    const int j = batchEntry<kDepositThreads>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<kDepositThreads>(first, n, v);
     const DensityEntry& E = e[j];
-    ddepositBody(E.a, E.sorted_blocks, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, 1));
+    ddepositBody(E.a, E.sorted_blocks, entryBlk(v - first[j], E.gx, E.gy, 1));
 }
 __global__ __launch_bounds__(kThreads) void k_dbins_batch(const DensityEntry* e, const int* first, int n) {
-    const int j = batchEntry<kThreads>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<kThreads>(first, n, v);
     const DensityEntry& E = e[j];
-    dbinsBody(E.a, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, 1));
+    dbinsBody(E.a, entryBlk(v - first[j], E.gx, E.gy, 1));
 }
 template <int EPI>
 __global__ __launch_bounds__(kThreads) void k_gemm_batch(const GemmEntry* e, const int* first, int n) {
-    const int j = batchEntry<kThreads>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<kThreads>(first, n, v);
     const GemmEntry& E = e[j];
-    gemmBody<EPI>(E.g, E.K, E.lda, E.ldb, E.ldc, E.sp, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, E.gz));
+    gemmBody<EPI>(E.g, E.K, E.lda, E.ldb, E.ldc, E.sp, entryBlk(v - first[j], E.gx, E.gy, E.gz));
 }
 template <int MODE, int KPL, int TR, int TC, int W, int S, int F>
 __global__ __launch_bounds__(64 * W * S) void k_dct_batch(const DctEntry* e, const int* first, int n) {
-    const int j = batchEntry<64 * W * S>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<64 * W * S>(first, n, v);
     const DctEntry& E = e[j];
     dctBody<MODE, KPL, TR, TC, W, S, F>(E.d, E.n, E.lda, E.ldb, E.ldc, E.step, E.sp,
-                                        entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, E.gz));
+                                        entryBlk(v - first[j], E.gx, E.gy, E.gz));
 }
 __global__ __launch_bounds__(kThreads) void k_gather_batch(const GatherEntry* e, const int* first, int n) {
-    const int j = batchEntry<kThreads>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<kThreads>(first, n, v);
     const GatherEntry& E = e[j];
-    gatherBody(E.a, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, E.gy, 1));
+    gatherBody(E.a, entryBlk(v - first[j], E.gx, E.gy, 1));
 }
 __global__ __launch_bounds__(kThreads) void k_lookahead_batch(const LookaheadEntry* e, const int* first, int n) {
-    const int j = batchEntry<kThreads>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<kThreads>(first, n, v);
     const LookaheadEntry& E = e[j];
-    lookaheadBody(E.l, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, 1, 1));
+    lookaheadBody(E.l, entryBlk(v - first[j], E.gx, 1, 1));
 }
 __global__ __launch_bounds__(kThreads) void k_propose_batch(const ProposeEntry* e, const int* first, int n) {
-    const int j = batchEntry<kThreads>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<kThreads>(first, n, v);
     const ProposeEntry& E = e[j];
-    proposeBody(E.p, entryBlk(static_cast<int>(blockIdx.x) - first[j], E.gx, 1, 1));
+    proposeBody(E.p, entryBlk(v - first[j], E.gx, 1, 1));
 }
 __global__ __launch_bounds__(kThreads) void k_copy_batch(const CopyEntry* e, const int* first, int n) {
-    const int j = batchEntry<kThreads>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<kThreads>(first, n, v);
     const CopyEntry& E = e[j];
-    const int p = (static_cast<int>(blockIdx.x) - first[j]) * kThreads + static_cast<int>(threadIdx.x);
+    const int p = (v - first[j]) * kThreads + static_cast<int>(threadIdx.x);
     if (p < E.n) E.dst[p] = E.src[p];
 }
 __global__ __launch_bounds__(kThreads) void k_zero_batch(const ZeroEntry* e, const int* first, int n) {
-    const int j = batchEntry<kThreads>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<kThreads>(first, n, v);
     const ZeroEntry& E = e[j];
-    const int p = (static_cast<int>(blockIdx.x) - first[j]) * kThreads + static_cast<int>(threadIdx.x);
+    const int p = (v - first[j]) * kThreads + static_cast<int>(threadIdx.x);
     if (p < E.n) E.p[p] = 0.0;
 }
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize_batch(const RedJob* jobs, volatile unsigned* flags,
@@ -1483,7 +1504,8 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize_batch(const RedJo
 }
 // reductions finalized inside a round, read by a later kernel of the round (no completion words)
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize_entries(const RedJob* e, const int* first, int n) {
-    const int j = batchEntry<kFinalizeThreads>(first, n);
+    const int v = batchSlot();
+    const int j = batchEntry<kFinalizeThreads>(first, n, v);
     finalizeBody(e[j], nullptr, 0U);
 }
 
