@@ -24,7 +24,8 @@ import numpy as np
 from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmp_gp_hip.so")
+# MP_GP_LIBRARY names another build of the same library (the host-sanitized one the CPU tests use)
+LIB_PATH = os.environ.get("MP_GP_LIBRARY") or os.path.join(HERE, "libmp_gp_hip.so")
 
 _lib = None
 
