@@ -4,9 +4,9 @@ job's result must be bit-equal to its own mp_gp_global_place run (each batched w
 job's own workgroup body; every evaluation is deterministic).
 
 The mix covers the batched kernels' variants side by side in one launch: dense DCT products (32 x
-32 grids) next to half-length ones (64 x 64, 64 x 128), FP32 next to bf16x3 products, and a job
-that rebuilds its row order for every evaluation (its look-ahead evaluation then runs after the
-current one: a second pass of the round)."""
+32 and 96 x 80 grids) next to half-length ones (64 x 64, 128 x 128, 64 x 128), FP32 next to bf16x3
+products, and a job that rebuilds its row order for every evaluation (its look-ahead evaluation then
+runs after the current one: a second pass of the round)."""
 
 import sys
 import threading
@@ -45,6 +45,8 @@ def _mix():
         _job(302, 2500, bins_x=64, bins_y=128, dct_precision=abi.MP_DCT_BF16X3),
         _job(303, 9000, density_order_reuse_rows=0.0),
         _job(304, 1200, bins_x=64, bins_y=64, dct_precision=abi.MP_DCT_BF16X3, iterations=80),
+        _job(305, 20000, bins_x=128, bins_y=128),
+        _job(306, 6000, bins_x=96, bins_y=80),
     ]
 
 

@@ -80,6 +80,7 @@ def _cases():
         ("plain-48", lambda: (design.plain_design(seed=7), 0.9, 48)),
         ("plain-50x30", lambda: (design.plain_design(seed=8), 0.9, (50, 30))),
         ("plain-30x52", lambda: (design.plain_design(seed=9), 0.9, (30, 52))),
+        ("plain-98x70", lambda: (design.plain_design(seed=11), 0.9, (98, 70))),
         ("wide-nets-64", lambda: (with_wide_nets(design.plain_design(seed=10), WIDE_DEGREES), 0.9, 64)),
         ("mid-128", lambda: (design.large_design(seed=3, cells=20000, nets=21000, row_count=200, sites=400,
                                                  macros=6), 0.8, 128)),
@@ -124,11 +125,12 @@ def test_evaluate_matches_oracle(ob, name, make):
     be.close()
 
 
-@pytest.mark.parametrize("bx,by", [(64, 64), (128, 64), (48, 48), (50, 30), (30, 52)])
+@pytest.mark.parametrize("bx,by", [(64, 64), (128, 64), (48, 48), (50, 30), (30, 52), (98, 70), (128, 128)])
 def test_fields_match_oracle(ob, bx, by):
     """Source, potential and field grids: half-length products (both sides multiples of 64,
     square and not), the dense products (48 x 48) and the dense products on grids whose width is not
-    a multiple of four (the bin kernel's per-bin path, padded non-square products)."""
+    a multiple of four (the bin kernel's per-bin path, padded non-square products, with padding of
+    14 and 26 cells at 50 x 30 and 98 x 70)."""
     d = design.plain_design()
     d.center = hb.random_placement(d, 2026)
     o = abi.default_options(target_density=0.9, bins_x=bx, bins_y=by)
