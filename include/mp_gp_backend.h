@@ -351,7 +351,9 @@ mp_gp_status mp_gp_global_place(const mp_design* design, const mp_gp_options* op
  * for all, then every job's controller decides its next step.  Each job's result is bit-equal to
  * its own mp_gp_global_place run.  All jobs run on options[0].device and hold their device state
  * together.  centers[i]: [2 M_i] in/out; results, statuses: [count] (may be NULL).  Returns the
- * first failing job's status (MP_GP_OK when all succeed); statuses[i] has each job's own. */
+ * first failing job's status (MP_GP_OK when all succeed); statuses[i] has each job's own.  From
+ * the first batch or service on, the process keeps freed host memory in its heap (glibc trim
+ * threshold and top pad raised), so later jobs' set-up arrays reuse touched pages. */
 mp_gp_status mp_gp_global_place_batch(int32_t count, const mp_design* designs, const mp_gp_options* options,
                                        double* const* centers, mp_gp_result* results, mp_gp_status* statuses);
 
@@ -360,7 +362,8 @@ mp_gp_status mp_gp_global_place_batch(int32_t count, const mp_design* designs, c
  * MP_GP_SERVE_STREAMS: each runs mp_gp_global_place to completion on its own stream and worker
  * thread, so admitted jobs overlap on the device.  MP_GP_SERVE_BATCHED: one driver runs every
  * admitted job in the lockstep rounds of mp_gp_global_place_batch, and admits waiting jobs between
- * rounds (continuous batching).  A job's result does not depend on what runs beside it.         */
+ * rounds (continuous batching).  A job's result does not depend on what runs beside it.  Creating
+ * a service makes the process keep freed host memory in its heap (see mp_gp_global_place_batch). */
 typedef enum mp_gp_serve_mode { MP_GP_SERVE_STREAMS = 0, MP_GP_SERVE_BATCHED = 1 } mp_gp_serve_mode;
 
 typedef struct mp_gp_server_options {

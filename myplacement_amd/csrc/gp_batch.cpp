@@ -122,6 +122,7 @@ struct BatchDriver::Impl {
     std::map<HostScalars*, HostScalars*> scalar_device;  // chunk host address -> device address
 
     Impl(int d, std::uint64_t memory_cap) : device(d) {
+        keepFreedHostMemory();  // jobs' set-up arrays reuse the pages earlier jobs touched
         hipCheck(hipSetDevice(device), "select HIP device");
         hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "create batch stream");
         loadDeviceCode(stream);

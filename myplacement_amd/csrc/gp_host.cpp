@@ -16,9 +16,13 @@
 
 #include "gp_host.hpp"
 
+#include <malloc.h>
+
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <random>
 #include <stdexcept>
@@ -309,6 +313,14 @@ Prepared::Prepared(Netlist&& n, const mp_gp_options& o) : net(std::move(n)) {
 }
 
 thread_local int t_parallel_limit = 8;
+
+void keepFreedHostMemory() noexcept {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        (void)mallopt(M_TRIM_THRESHOLD, INT_MAX);
+        (void)mallopt(M_TOP_PAD, 1 << 30);
+    });
+}
 
 void stableSortByKey(std::uint32_t* idx, std::size_t n, const std::uint32_t* key, int key_bits) {
     constexpr int kBits = 11;

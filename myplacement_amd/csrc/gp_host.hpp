@@ -57,6 +57,13 @@ struct Prepared {
 
 void randomPlacement(const mp_design& d, std::uint64_t seed, double* centers);
 
+// From the first call on (once per process), freed host memory stays in the process heap: glibc's
+// trim threshold is raised past any set-up's size and its heap grows in 1 GiB steps.  A job's
+// set-up arrays then reuse pages an earlier job already touched instead of faulting in fresh ones
+// (a fifth of a 150k-cell job's set-up time).  The placement service and the batch driver call it;
+// the process keeps its peak heap for the rest of its life.
+void keepFreedHostMemory() noexcept;
+
 // Most host threads parallelFor may use from this thread (8; a caller that already runs many
 // preparations side by side lowers it for its small jobs).
 extern thread_local int t_parallel_limit;

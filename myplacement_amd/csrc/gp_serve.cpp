@@ -13,6 +13,7 @@
 
 #include "mp_gp_backend.h"
 #include "gp_batch.hpp"
+#include "gp_host.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -84,6 +85,7 @@ public:
             throw std::invalid_argument("the placement service needs at least one slot and a memory budget");
         if (o_.mode != MP_GP_SERVE_BATCHED && o_.mode != MP_GP_SERVE_STREAMS)
             throw std::invalid_argument("unknown placement service mode");
+        keepFreedHostMemory();  // jobs' set-up arrays reuse the pages earlier jobs touched
         if (o_.mode == MP_GP_SERVE_BATCHED) {
             workers_.emplace_back([this] { drive(); });  // its driver loads the device code
         } else {
