@@ -21,6 +21,7 @@
 #include "gp_host.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -215,11 +216,26 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
     for (std::size_t q = 0; q < Ii; ++q)
         if (ipin_part[q] >= 0) ++count[ipin_part[q] + 1];
     std::partial_sum(count.begin(), count.end(), count.begin());
+    // Slots of the per-pin gradient terms: k_gather's workgroup w (kGatherParticles particles) runs
+    // on XCD w mod 8, so the terms are stored in kPinRegions = 4 regions, region k holding, in pin
+    // order, the pins of the particles of workgroups w = k mod 4.  The gather on XCDs k and k + 4
+    // then reads one region, a quarter of the terms, which their L2s hold, and a wavefront's
+    // stores of one pin slot fall in four runs.
+    std::vector<int> pin_dest(Ii, -1);
+    {
+        auto region = [](int part) { return (part / kGatherParticles) % kPinRegions; };
+        std::array<int, kPinRegions + 1> base{};
+        for (std::size_t q = 0; q < Ii; ++q)
+            if (ipin_part[q] >= 0) ++base[region(ipin_part[q]) + 1];
+        std::partial_sum(base.begin(), base.end(), base.begin());
+        for (std::size_t q = 0; q < Ii; ++q)
+            if (ipin_part[q] >= 0) pin_dest[q] = base[region(ipin_part[q])]++;
+    }
     std::vector<int> ppin_idx(static_cast<std::size_t>(count[P_]));
     {
         std::vector<int> fill(count.begin(), count.end() - 1);
         for (std::size_t q = 0; q < Ii; ++q)
-            if (ipin_part[q] >= 0) ppin_idx[fill[ipin_part[q]]++] = static_cast<int>(q);
+            if (ipin_part[q] >= 0) ppin_idx[fill[ipin_part[q]]++] = pin_dest[q];
     }
     std::vector<double2> sizes(P_);
     for (int i = 0; i < P_; ++i) sizes[i] = make_double2(L.particle_width[i], L.particle_height[i]);
@@ -257,6 +273,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
     pin_part_.allocate(Ii, budget_, "pin-particle map");
     pin_off_.allocate(Ii, budget_, "pin offsets");
     ppin_idx_.allocate(ppin_idx.size(), budget_, "particle-pin index");
+    pin_dest_.allocate(Ii, budget_, "pin gradient slots");
     ppin_off_.allocate(static_cast<std::size_t>(P_) + 1, budget_, "particle-pin offsets");
     // density: particles taller than two bin rows are not row-sorted (their own deposit workgroups
     // add them to the global layers)
@@ -332,6 +349,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
     pin_off_.upload(ipin_off.data(), Ii, stream, "upload pin offsets");
     ppin_off_.upload(count.data(), count.size(), stream, "upload particle-pin offsets");
     ppin_idx_.upload(ppin_idx.data(), ppin_idx.size(), stream, "upload particle-pin index");
+    pin_dest_.upload(pin_dest.data(), Ii, stream, "upload pin gradient slots");
     static_charge_.upload(static_charge.data(), G, stream, "upload static density");
     const auto fx = basis(nxp_, nxp_, nx_, 0, W);
     const auto fy = basis(nyp_, nyp_, ny_, 1, H);
@@ -512,6 +530,7 @@ WireArgs Backend::wireArgs(double gamma, bool want_grad) const {
     wa.net_w = net_w_.get();
     wa.pin_part = pin_part_.get();
     wa.pin_off = pin_off_.get();
+    wa.pin_dest = pin_dest_.get();
     wa.n_small = n_small_;
     wa.n_medium = n_medium_;
     wa.n_large = n_large_;

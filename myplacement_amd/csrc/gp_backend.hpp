@@ -353,7 +353,7 @@ private:
     DeviceArray<double2> pos_, size_;
     DeviceArray<double> charge_;
     DeviceArray<int> layer_;
-    DeviceArray<int> net_off_, pin_part_, ppin_off_, ppin_idx_;
+    DeviceArray<int> net_off_, pin_part_, ppin_off_, ppin_idx_, pin_dest_;
     int n_small_ = 0, n_medium_ = 0, n_large_ = 0;
     int small_net_[kSmallDegree + 2] = {}, small_pin_[kSmallDegree + 2] = {};  // small-net degree groups
     DeviceArray<double> net_w_;

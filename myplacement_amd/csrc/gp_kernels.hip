@@ -220,7 +220,9 @@ __device__ inline float groupSum(float v) {
 // Lane `sub` owns pins b + sub + W i.  Nets of up to W * PER pins keep their pins in registers
 // from one load pass (all loads of a lane in flight together); larger nets stream them three times.
 // The gradient pass recomputes the pin exponentials rather than holding them (71 instead of 78
-// VGPRs: 7 waves per SIMD instead of 6 for this gather-latency-bound kernel).
+// VGPRs: 7 waves per SIMD instead of 6 for this gather-latency-bound kernel), and stores each pin's
+// term at the slot pin_dest gives (issuing those slot loads with the pin loads instead takes 78
+// VGPRs and is slower).
 template <int W, int PER>
 __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_grad, int net, int sub, double inv_gamma,
                           float gamma_f, double& hp, double& wl) {
@@ -258,8 +260,9 @@ __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_gra
 #pragma unroll
             for (int i = 0; i < PER; ++i) {
                 const int p = b + sub + W * i;
-                if (p < e)
-                    pin_grad[p] = make_float2(ex32(qx[i] - mx, inv_gamma) * ipx - ex32(nx - qx[i], inv_gamma) * inx,
+                const int d = p < e ? a.pin_dest[p] : -1;
+                if (d >= 0)
+                    pin_grad[d] = make_float2(ex32(qx[i] - mx, inv_gamma) * ipx - ex32(nx - qx[i], inv_gamma) * inx,
                                               ex32(qy[i] - my, inv_gamma) * ipy - ex32(ny - qy[i], inv_gamma) * iny);
             }
         }
@@ -282,8 +285,10 @@ __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_gra
             const float wf = static_cast<float>(w);
             const float ipx = wf / spx, inx = wf / snx, ipy = wf / spy, iny = wf / sny;
             for (int p = b + sub; p < e; p += W) {
+                const int d = a.pin_dest[p];
+                if (d < 0) continue;
                 const double2 q = pinPosition(pos, a.pin_part, a.pin_off, p);
-                pin_grad[p] = make_float2(ex32(q.x - mx, inv_gamma) * ipx - ex32(nx - q.x, inv_gamma) * inx,
+                pin_grad[d] = make_float2(ex32(q.x - mx, inv_gamma) * ipx - ex32(nx - q.x, inv_gamma) * inx,
                                           ex32(q.y - my, inv_gamma) * ipy - ex32(ny - q.y, inv_gamma) * iny);
             }
         }
@@ -367,11 +372,12 @@ __device__ __forceinline__ void wireBody(const WireArgs& a, int large_blocks, in
                 const float wf = static_cast<float>(w);
                 const float ipx = wf / spx, inx = wf / snx, ipy = wf / spy, iny = wf / sny;
 #pragma unroll
-                for (int i = 0; i < kSmallDegree; ++i)
-                    if (i < deg)
-                        pin_grad[b + i * size] =
-                            make_float2(ex32(qx[i] - mx, inv_gamma) * ipx - ex32(nx - qx[i], inv_gamma) * inx,
-                                        ex32(qy[i] - my, inv_gamma) * ipy - ex32(ny - qy[i], inv_gamma) * iny);
+                for (int i = 0; i < kSmallDegree; ++i) {
+                    const int d = i < deg ? a.pin_dest[b + i * size] : -1;
+                    if (d >= 0)
+                        pin_grad[d] = make_float2(ex32(qx[i] - mx, inv_gamma) * ipx - ex32(nx - qx[i], inv_gamma) * inx,
+                                                  ex32(qy[i] - my, inv_gamma) * ipy - ex32(ny - qy[i], inv_gamma) * iny);
+                }
             }
         }
     }
@@ -1204,6 +1210,7 @@ __device__ __forceinline__ void gatherBody(const GatherArgs& a, const Blk& bk) {
     }
     publishPartials<7>(v, 0U, eb.y, a.partials[e], lds);
 }
+static_assert(kThreads == kGatherParticles, "the pin-term regions follow the gather's workgroups");
 __global__ __launch_bounds__(kThreads) void k_gather(GatherArgs a) { gatherBody(a, nativeBlk()); }
 
 // ---------------------------------------------------------------------------------------------

@@ -82,9 +82,10 @@ struct WireArgs {
     int n_small, n_medium, n_large;
     int small_net[kSmallDegree + 2];  // [d]: first net of degree d (d = 2..kSmallDegree), [kSmallDegree + 1] = n_small
     int small_pin[kSmallDegree + 2];  // [d]: first pin of that group
+    const int* pin_dest;  // [pin] its slot in pin_grad, -1 for pins of fixed particles
     double gamma;
     int want_grad;
-    float2* pin_grad[kMaxEval];  // per-pin d(WL)/d(pin), internal pin order
+    float2* pin_grad[kMaxEval];  // per-pin d(WL)/d(pin), at the slots pin_dest gives
     double* partials[kMaxEval];
     // Evaluated ahead (one evaluation): the smoothing is not `gamma` but the controller's law
     // (AdaptiveGlobalPlacer.cpp:456-462, exponent 1) at the optimizer overflow the device holds at
@@ -209,7 +210,7 @@ struct GatherArgs {
     int ldf;
     Region r;
     const int* ppin_off;   // [P+1] particle -> pin CSR
-    const int* ppin_idx;   // internal pin index of each CSR entry
+    const int* ppin_idx;   // pin_grad slot of each CSR entry
     const float2* pin_grad[kMaxEval];
     int n;
     // backend mode (one evaluation): write the two gradients
@@ -228,6 +229,10 @@ struct GatherArgs {
     double* partials[kMaxEval];
 };
 int particleBlocks(int n);
+constexpr int kGatherParticles = 256;  // particles per k_gather workgroup (one per thread)
+// Regions of the per-pin gradient terms: region k holds the terms of the particles of the gather
+// workgroups w = k mod kPinRegions (the backend's set-up assigns the slots, pin_dest).
+constexpr int kPinRegions = 4;
 void launchGather(const GatherArgs& a, int count, hipStream_t s);
 
 // ---- controller vector passes (AdaptiveGlobalPlacer.cpp O(P) loops) ---------------------------
