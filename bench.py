@@ -17,12 +17,13 @@ BF16 (bf16x3).  --workload plain: 2,000 cells, 64 x 64 bins, random initial plac
 --workload batch: 256 independent blocks (see batch_sample) through the placement service.
 `roofline_dct` reports the DCT products against the MFMA peak of their arithmetic.
 
-Prints one JSON line.  `roofline` reports k_wire, the kernel with the largest share of GPU time,
-against MI355X's HBM peak: its algorithmic bytes (per evaluation 12 per pin, 8 per net, 16 per
-particle with the gradient and 8 without -- one pass over each array at 4-byte indices and 8-byte
-positions) over its launch durations, timed with HIP events on the session's stream inside the
-timed region; `traffic` is its measured HBM bytes per launch from the counter summary under
-profiles/.  `cpu_baseline` is the FP64 CPU restatement of the
+Prints one JSON line.  `roofline` reports k_wire_dbins, the kernel with the largest share of GPU
+time (the wirelength of the candidate and look-ahead evaluations with the density's bin pass in the
+same launch), against MI355X's HBM peak: its algorithmic bytes (per evaluation of the wirelength 12
+per pin, 8 per net, 16 per particle with the gradient and 8 without, and of the bin pass 24 per bin
+-- one pass over each array at 4-byte indices and 8-byte values) over its launch durations, timed
+with HIP events on the session's stream inside the timed region; `traffic` is its measured HBM
+bytes per launch from the counter summary under profiles/.  `cpu_baseline` is the FP64 CPU restatement of the
 reference path (oracle/) timed on one host core on a bounded number of iterations of the same
 workload.  `kernel_timing_us` (average per launch of every kernel class) comes from a separate
 stretch of iterations after the timed region, so that its events do not slow the timed steps.
@@ -31,7 +32,7 @@ stretch of iterations after the timed region, so that its events do not slow the
 stream (p50 / p99 submit-to-result latency, jobs per hour; see serving_sample).
 
 --copy-calibration MB runs one plain device-to-device copy of MB MiB (torch) and exits: a kernel
-of known traffic whose FETCH_SIZE / WRITE_SIZE counters scale those of k_wire in a profiling run.
+of known traffic whose FETCH_SIZE / WRITE_SIZE counters scale those of k_wire_dbins in a profiling run.
 """
 
 from __future__ import annotations
@@ -365,6 +366,10 @@ def batch_sample(jobs: int, runs_spec, iterations: int, seed: int = 2026) -> dic
                       f"{iterations} iterations each, all submitted at once"}
 
 
+# the roofline's kernel: its timing class and its name in the profiler's traces
+ROOF_CLASS, ROOF_KERNEL = "k_wire_bins", "k_wire_dbins"
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed counter summary (profiles/), or None."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_hbm_traffic.md")
@@ -469,7 +474,7 @@ def main() -> int:
     h0 = len(sess.history())
     timed = not args.no_kernel_timing
     if timed:
-        sess.enable_kernel_timing(True, classes=["k_wire"])
+        sess.enable_kernel_timing(True, classes=[ROOF_CLASS])
     t1 = time.perf_counter()
     done = sess.step(args.steps)
     sess.positions()  # device work of the last step is complete
@@ -478,18 +483,19 @@ def main() -> int:
     hist = sess.history()[h0:]
     roof, timing = None, {}
     if timed:
-        live = sess.kernel_timing()["k_wire"]
+        live = sess.kernel_timing()[ROOF_CLASS]
         sess.enable_kernel_timing(False)
         # every kernel class, timed over a separate stretch of the same run
         sess.enable_kernel_timing(True)
         sess.step(args.timing_steps)
         timing = sess.kernel_timing()
         sess.enable_kernel_timing(False)
-        timing["k_wire"] = {kk: timing["k_wire"][kk] - live[kk] for kk in ("total_us", "launches", "work")}
+        timing[ROOF_CLASS] = {kk: timing[ROOF_CLASS][kk] - live[kk] for kk in ("total_us", "launches", "work")}
         avg_us = live["total_us"] / live["launches"]
         achieved = live["work"] / (live["total_us"] * 1e-6) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_wire") if args.workload == "large" else None, "kernel": "k_wire",
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": measured_traffic(ROOF_KERNEL) if args.workload == "large" else None, "kernel": ROOF_KERNEL,
                 "avg_launch_us": avg_us, "launches": live["launches"],
                 "algorithmic_bytes_per_launch": live["work"] / live["launches"]}
     _, res = sess.finish()

@@ -344,7 +344,7 @@ class PlacementSession:
         _raise(library().mp_gp_session_positions(self._h, _p(xy)))
         return xy
 
-    TIMING_CLASSES = ("k_wire", "k_density", "k_density_sort", "k_gemm", "k_gather", "controller")
+    TIMING_CLASSES = ("k_wire", "k_density", "k_density_sort", "k_gemm", "k_gather", "controller", "k_wire_bins")
 
     def enable_kernel_timing(self, on: bool = True, classes=None) -> None:
         """Times the named kernel classes (all when `classes` is None) with HIP events."""

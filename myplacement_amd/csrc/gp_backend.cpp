@@ -463,7 +463,7 @@ void Backend::enableTiming(unsigned class_mask) {
 }
 
 std::vector<std::string> Backend::timingNames() const {
-    return {"k_wire", "k_density", "k_density_sort", "k_gemm", "k_gather", "controller"};
+    return {"k_wire", "k_density", "k_density_sort", "k_gemm", "k_gather", "controller", "k_wire_bins"};
 }
 
 void Backend::timing(std::vector<double>& total_us, std::vector<long long>& launches, std::vector<double>& work) {
@@ -567,7 +567,14 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
         wa.partials[nw] = partials(kRedWire, specs[e].bank);
         ++nw;
     }
-    if (nw > 0) {
+    // The wirelength goes with the density's bin pass (one launch, k_wire_dbins) when both run
+    // here and the launcher fuses them; otherwise it is launched on its own, first.
+    const bool wire_with_bins = nw > 0 && nets && launches_->fusesWireBins() &&
+                                std::any_of(specs, specs + count, [](const EvalSpec& s) { return s.density_from < 0; });
+    if (wire_with_bins) {
+        for (int e = 0; e < count; ++e)
+            if (!specs[e].wire_ready) addReduction(kRedWire, specs[e].bank, wire_rows);
+    } else if (nw > 0) {
         timedBegin(kTWire);
         if (nets) {
             launches_->wire(wa, nw);
@@ -610,11 +617,19 @@ void Backend::enqueueEvaluate(const EvalSpec* specs, int count, double gamma, bo
             da.partials[k] = partials(kRedMetrics, specs[e].bank);
             sp.partials[k] = partials(kRedEnergy, specs[e].bank);
         }
-        timedBegin(kTBand);
-        launches_->density(da, nf);
         // algorithmic bytes per evaluation (same convention): scatter 20 per particle and 12 per bin,
         // metrics and deviation 24 per bin
-        timedEnd(kTBand, nf * (20.0 * P_ + 36.0 * bins));
+        timedBegin(kTBand);
+        if (wire_with_bins) {
+            launches_->deposit(da, nf);
+            timedEnd(kTBand, nf * (20.0 * P_ + 12.0 * bins));
+            timedBegin(kTWireBins);
+            launches_->wireBins(wa, nw, da, nf);
+            timedEnd(kTWireBins, nw * (12.0 * I_ + 8.0 * N_ + (want_grad ? 16.0 : 8.0) * P_) + nf * 24.0 * bins);
+        } else {
+            launches_->density(da, nf);
+            timedEnd(kTBand, nf * (20.0 * P_ + 36.0 * bins));
+        }
         for (int k = 0; k < nf; ++k)
             addReduction(kRedMetrics, specs[fwd[k]].bank, densityBinBlocks(region_), bins, dark_total_,
                          placeable_total_);
@@ -747,6 +762,10 @@ void StreamLaunches::sort(const DensityArgs& a) { launchDensitySort(a, s_); }
 void StreamLaunches::wire(const WireArgs& a, int count) { launchWire(a, count, s_); }
 void StreamLaunches::zero(double* p, int n) { launchZero(p, static_cast<size_t>(n), s_); }
 void StreamLaunches::density(const DensityArgs& a, int count) { launchDensity(a, count, s_); }
+void StreamLaunches::deposit(const DensityArgs& a, int count) { launchDeposit(a, count, s_); }
+void StreamLaunches::wireBins(const WireArgs& w, int nw, const DensityArgs& d, int nd) {
+    launchWireBins(w, nw, d, nd, s_);
+}
 void StreamLaunches::gemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
                           const SpectralArgs& sp, int precision, int) {
     launchGemm(g, batch, M, N, K, lda, ldb, ldc, epilogue, sp, precision, s_);

@@ -117,6 +117,13 @@ public:
     virtual void wire(const WireArgs& a, int count) = 0;
     virtual void zero(double* p, int n) = 0;
     virtual void density(const DensityArgs& a, int count) = 0;
+    // Launchers that run the density's bin pass and the wirelength as one launch: density(d, nd)
+    // and wire(w, nw) become deposit(d, nd) followed by wireBins(w, nw, d, nd).
+    virtual bool fusesWireBins() const { return false; }
+    virtual void deposit(const DensityArgs&, int) { throw std::logic_error("this launcher does not split the density"); }
+    virtual void wireBins(const WireArgs&, int, const DensityArgs&, int) {
+        throw std::logic_error("this launcher does not fuse the wirelength and the bin pass");
+    }
     // `product`: the product's place 0..3 in the evaluation chain (two forward, two inverse)
     virtual void gemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
                       const SpectralArgs& sp, int precision, int product) = 0;
@@ -137,6 +144,9 @@ public:
     void wire(const WireArgs& a, int count) override;
     void zero(double* p, int n) override;
     void density(const DensityArgs& a, int count) override;
+    bool fusesWireBins() const override { return true; }
+    void deposit(const DensityArgs& a, int count) override;
+    void wireBins(const WireArgs& w, int nw, const DensityArgs& d, int nd) override;
     void gemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,
               const SpectralArgs& sp, int precision, int product) override;
     void dct(int mode, const DctBatch& d, int batch, int rows, int cols, int lda, int ldb, int ldc, int step,
@@ -381,7 +391,7 @@ private:
     unsigned seq_ = 0;                  // completion value of the last finalize
 
     // timing (events come from a pool that is reused once they have been read)
-    static constexpr int kTimingClasses = 6;
+    static constexpr int kTimingClasses = 7;
     hipEvent_t poolEvent();
     unsigned timing_mask_ = 0;
     int capture_shape_ = -1;                              // the chain being captured, or -1
@@ -397,7 +407,7 @@ private:
     std::vector<long long> t_count_;
 };
 
-enum TimingClass : int { kTWire = 0, kTBand = 1, kTSort = 2, kTDct = 3, kTGather = 4, kTController = 5 };
+enum TimingClass : int { kTWire = 0, kTBand = 1, kTSort = 2, kTDct = 3, kTGather = 4, kTController = 5, kTWireBins = 6 };
 
 }  // namespace mpgp
 

@@ -6,6 +6,8 @@
 //   k_dhist / k_dscan / k_dplace / k_ddeposit / k_dbins
 //               depositParticlesKernel + buildDensityMetricsKernel + 10 reductions (:339-398,
 //               :1022-1038); CPU DensityMap::addRectangle / metrics, Metrics.cpp:74-119
+//   k_wire_dbins
+//               k_dbins and k_wire of the same evaluation chain in one launch
 //   k_gemm      forwardDct / solveSpectralKernel / inverse2d (:547-567, :679-742); CPU
 //               ElectrostaticField.cpp:205-237 (FFTW REDFT10 / REDFT01 / RODFT01 as dense products)
 //   k_gather    sampleDensityGradientKernel (:575-591) + the gradient combine of
@@ -699,6 +701,24 @@ __device__ __forceinline__ void dbinsBody(const DensityArgs& a, const Blk& bk) {
     publishPartials<9>(v, (1U << 7) | (1U << 8), eb.y, a.partials[eb.x], lds);
 }
 __global__ __launch_bounds__(kThreads) void k_dbins(DensityArgs a) { dbinsBody(a, nativeBlk()); }
+
+// The bin pass and the wirelength of the same evaluations in one launch: the bin workgroups first,
+// then the wirelength ones, each finding its block coordinates in its own launch shape.  The two
+// are independent (the bins read the deposit's layers, the wirelength the positions and pins) and
+// both write their own partial rows, so the results are those of the two launches; the short,
+// latency-bound bin pass (one wavefront per SIMD) runs inside the wirelength's time instead of
+// after it, and the launch between them is gone.  The bin body takes fewer VGPRs than the
+// wirelength body, so the wirelength keeps its occupancy.
+__global__ __launch_bounds__(kThreads) void k_wire_dbins(WireArgs w, int large_blocks, int medium_blocks, int wire_gx,
+                                                          int wire_gy, DensityArgs d, int bins_gx, int bins_gy) {
+    const int b = static_cast<int>(blockIdx.x), nb = bins_gx * bins_gy;
+    if (b < nb) {
+        dbinsBody(d, Blk{b % bins_gx, b / bins_gx, 0, bins_gx, bins_gy, 1});
+    } else {
+        const int v = b - nb;
+        wireBody(w, large_blocks, medium_blocks, Blk{v % wire_gx, v / wire_gx, 0, wire_gx, wire_gy, 1});
+    }
+}
 
 // ---------------------------------------------------------------------------------------------
 // C = A * B on v_mfma_f32_16x16x4_f32.  Workgroup tile 32x32, four wavefronts split K in
@@ -1552,7 +1572,7 @@ int densityBinBlocks(const Region& r) {
     return static_cast<int>((static_cast<long long>(r.nx) * r.ny + per_block - 1) / per_block);
 }
 
-void launchDensity(const DensityArgs& a, int count, hipStream_t s) {
+void launchDeposit(const DensityArgs& a, int count, hipStream_t s) {
     if (count < 1 || count > kMaxEval) throw std::invalid_argument("bad evaluation batch");
     const int sorted_blocks = (a.n + kDepositThreads - 1) / kDepositThreads;
     const int tall_blocks = (a.n_tall + kDepositThreads - 1) / kDepositThreads;
@@ -1561,8 +1581,26 @@ void launchDensity(const DensityArgs& a, int count, hipStream_t s) {
                            sorted_blocks);
         check(hipGetLastError(), "launch density deposit kernel");
     }
+}
+
+void launchDensity(const DensityArgs& a, int count, hipStream_t s) {
+    launchDeposit(a, count, s);
     hipLaunchKernelGGL(k_dbins, dim3(densityBinBlocks(a.r), count), dim3(kThreads), 0, s, a);
     check(hipGetLastError(), "launch density bin kernel");
+}
+
+void launchWireBins(const WireArgs& w, int nw, const DensityArgs& d, int nd, hipStream_t s) {
+    if (nw < 1 || nw > kMaxEval || nd < 1 || nd > kMaxEval) throw std::invalid_argument("bad evaluation batch");
+    const int wire_gx = wireBlocks(w.n_small, w.n_medium, w.n_large);
+    if (wire_gx == 0) throw std::invalid_argument("no nets for the wirelength");
+    if (w.small_net[2] != 0 || w.small_net[kSmallDegree + 1] != w.n_small)
+        throw std::invalid_argument("small-net degree groups do not cover the small nets");
+    const int large_blocks = (w.n_large + kWaves - 1) / kWaves;
+    const int medium_blocks = (w.n_medium + 4 * kWaves - 1) / (4 * kWaves);
+    const int bins_gx = densityBinBlocks(d.r);
+    hipLaunchKernelGGL(k_wire_dbins, dim3(bins_gx * nd + wire_gx * nw), dim3(kThreads), 0, s, w, large_blocks,
+                       medium_blocks, wire_gx, nw, d, bins_gx, nd);
+    check(hipGetLastError(), "launch density bin and wirelength kernel");
 }
 
 void launchGemm(const GemmBatch& g, int batch, int M, int N, int K, int lda, int ldb, int ldc, int epilogue,

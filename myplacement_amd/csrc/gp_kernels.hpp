@@ -132,7 +132,11 @@ int densityHistBlocks(int n);
 int densityBinBlocks(const Region& r);
 void launchDensitySort(const DensityArgs& a, hipStream_t s);
 // Deposit and bin metrics of `count` evaluations (partials: densityBinBlocks rows each).
-void launchDensity(const DensityArgs& a, int count, hipStream_t s);
+void launchDensity(const DensityArgs& a, int count, hipStream_t s);  // launchDeposit, then the bin pass
+void launchDeposit(const DensityArgs& a, int count, hipStream_t s);
+// the bin pass of launchDensity(d, nd), after launchDeposit, and launchWire(w, nw) as one launch
+// (k_wire_dbins); the same results
+void launchWireBins(const WireArgs& w, int nw, const DensityArgs& d, int nd, hipStream_t s);
 
 // ---- DCT products on MFMA (FP32) ---------------------------------------------------------------
 // kEpiInterleave stores C[i][j] at C + 2 * (i * ldc + j) (two products into one interleaved grid)
