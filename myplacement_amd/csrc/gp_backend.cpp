@@ -890,12 +890,8 @@ const HostScalars& Backend::fetchScalars() {
     }
     pending_.seq = ++seq_;
     pending_.flag = dev_done_;
-    if (jobs > 0) launchFinalize(pending_, stream_);
-    pending_.n = 0;
-    auto launchAhead = [&] {
-        if (!ahead_pending_) return;
-        ahead_pending_ = false;
-        if (n_small_ + n_medium_ + n_large_ == 0) return;
+    const bool nets = n_small_ + n_medium_ + n_large_ > 0;
+    auto aheadArgs = [&] {
         WireArgs wa = wireArgs(0.0, true);
         wa.pos[0] = ahead_.pos;
         wa.pin_grad[0] = slots_[0].pin_grad.get();
@@ -906,8 +902,35 @@ const HostScalars& Backend::fetchScalars() {
         wa.ahead_span = ahead_.span;
         wa.ahead_scale = ahead_.scale;
         wa.ahead_rows = wireBlocks(n_small_, n_medium_, n_large_);
+        return wa;
+    };
+    // The wirelength evaluated ahead goes into the finalize's launch (k_finalize_wire) when this
+    // finalize forms the overflow its smoothing law reads (a pending metrics reduction whose
+    // device-memory copy holds it) and no graph is being captured.
+    const RedJob* metrics = nullptr;
+    if (jobs > 0 && ahead_pending_ && nets && capture_shape_ < 0)
+        for (int k = 0; k < jobs; ++k) {
+            const RedJob& j = pending_.job[k];
+            if (j.kind == kRedMetrics && j.out_dev && j.out_dev + kOptDensity + 1 == ahead_.overflow) metrics = &j;
+        }
+    if (metrics) {
+        ahead_pending_ = false;
+        WireArgs wa = aheadArgs();
+        wa.ahead_metrics = metrics->partials;
+        wa.ahead_metric_rows = metrics->blocks;
         timedBegin(kTWire);
-        launchWire(wa, 1, stream_);
+        launchFinalizeWire(pending_, wa, stream_);
+        timedEnd(kTWire, 12.0 * I_ + 8.0 * N_ + 16.0 * P_);
+    } else if (jobs > 0) {
+        launchFinalize(pending_, stream_);
+    }
+    pending_.n = 0;
+    auto launchAhead = [&] {
+        if (!ahead_pending_) return;
+        ahead_pending_ = false;
+        if (!nets) return;
+        timedBegin(kTWire);
+        launchWire(aheadArgs(), 1, stream_);
         timedEnd(kTWire, 12.0 * I_ + 8.0 * N_ + 16.0 * P_);
     };
     if (capture_shape_ >= 0) {

@@ -8,6 +8,8 @@
 //               :1022-1038); CPU DensityMap::addRectangle / metrics, Metrics.cpp:74-119
 //   k_wire_dbins
 //               k_dbins and k_wire of the same evaluation chain in one launch
+//   k_finalize_wire
+//               k_finalize of a read-back and k_wire evaluated ahead in one launch
 //   k_gemm      forwardDct / solveSpectralKernel / inverse2d (:547-567, :679-742); CPU
 //               ElectrostaticField.cpp:205-237 (FFTW REDFT10 / REDFT01 / RODFT01 as dense products)
 //   k_gather    sampleDensityGradientKernel (:575-591) + the gradient combine of
@@ -308,18 +310,43 @@ __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_gra
 constexpr int kLargePins = 8;  // pins per lane held in registers for a large net (512 per net)
 // The controller's smoothing law with exponent 1, operation for operation as the host evaluates it
 // (clampd = max(lo, min(v, hi)); no contraction), so that both get the same bits.
-__device__ inline double aheadSmoothing(const WireArgs& a) {
+__device__ inline double aheadSmoothing(const WireArgs& a, double overflow) {
 #pragma clang fp contract(off)
-    const double r = *a.ahead_overflow / a.ahead_denominator;
+    const double r = overflow / a.ahead_denominator;
     const double m = 1.0 < r ? 1.0 : r;
     const double ratio = 0.0 < m ? m : 0.0;
     return (a.ahead_final + a.ahead_span * ratio) * a.ahead_scale;
 }
 
+// The optimizer's normalized overflow (DensityMap::metrics, Metrics.cpp:101-119) from a metrics
+// reduction's partial rows, formed as k_finalize forms it: thread t adds rows t, t + 256, ... in
+// that order, blockReduce combines the threads, thread 0 divides; the same bits as the finalized
+// value.  Every thread of the block calls it and gets the value.
+constexpr int kMetricsWidth = 9;
+__device__ double metricOverflow(const double* __restrict__ p, int rows, double* lds) {
+    static_assert(kThreads == kFinalizeThreads, "the overflow's sums follow k_finalize's thread layout");
+    double v[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < rows; b += kThreads) {
+        v[0] += p[static_cast<size_t>(b) * kMetricsWidth + 0];  // total overflow area
+        v[1] += p[static_cast<size_t>(b) * kMetricsWidth + 4];  // normalization area
+    }
+    blockReduce<2>(v, 0U, lds);
+    if (threadIdx.x == 0) lds[0] = v[0] / fmax(v[1], kEps);
+    __syncthreads();
+    const double r = lds[0];
+    __syncthreads();
+    return r;
+}
+
 __device__ __forceinline__ void wireBody(const WireArgs& a, int large_blocks, int medium_blocks, const Blk& bk) {
     __shared__ double lds[(kWaves + 1) * 2];
     double wl = 0.0, hp = 0.0;
-    const double gamma = a.ahead_overflow ? aheadSmoothing(a) : a.gamma;
+    double overflow = 0.0;
+    if (a.ahead_metrics)  // uniform: every thread of the block takes the reduction
+        overflow = metricOverflow(a.ahead_metrics, a.ahead_metric_rows, lds);
+    else if (a.ahead_overflow)
+        overflow = *a.ahead_overflow;
+    const double gamma = a.ahead_overflow ? aheadSmoothing(a, overflow) : a.gamma;
     const double inv_gamma = 1.0 / gamma;
     const float gamma_f = static_cast<float>(gamma);
     const int2 eb = batchBlock(bk);
@@ -1349,34 +1376,39 @@ __global__ __launch_bounds__(kThreads) void k_zero(double* p, size_t count) {
 // Accumulates the partial rows tid, tid + T, tid + 2T, ... (T threads) of a K-wide reduction in
 // that order.  Rows are read kFinalizeBatch at a time with every load issued before the first
 // add: the partials come from another XCD's kernel, so each load waits on the fabric.
+// (B rows in flight; the rows a thread adds, and their order, do not depend on B.)
 constexpr int kFinalizeBatch = 4;
-template <int K>
+template <int K, int B = kFinalizeBatch>
 __device__ void accumulateRows(const double* __restrict__ p, int rows, unsigned max_mask, double (&acc)[9]) {
-    for (int b0 = threadIdx.x; b0 < rows; b0 += kFinalizeBatch * kFinalizeThreads) {
-        double x[kFinalizeBatch][K];
+    for (int b0 = threadIdx.x; b0 < rows; b0 += B * kFinalizeThreads) {
+        double x[B][K];
 #pragma unroll
-        for (int u = 0; u < kFinalizeBatch; ++u) {
+        for (int u = 0; u < B; ++u) {
             const int b = b0 + u * kFinalizeThreads;
 #pragma unroll
             for (int k = 0; k < K; ++k) x[u][k] = b < rows ? p[static_cast<size_t>(b) * K + k] : 0.0;
         }
         // rows past the end add 0 and take max with 0: every accumulator starts at 0
 #pragma unroll
-        for (int u = 0; u < kFinalizeBatch; ++u)
+        for (int u = 0; u < B; ++u)
 #pragma unroll
             for (int k = 0; k < K; ++k) acc[k] = ((max_mask >> k) & 1U) ? fmax(acc[k], x[u][k]) : acc[k] + x[u][k];
     }
 }
 
+// LEAN: one row in flight for the wide reductions (metrics, gather), so that the body takes no more
+// VGPRs than the wirelength's beside which k_finalize_wire runs it; the same sums.
+template <bool LEAN = false>
 __device__ __forceinline__ void finalizeBody(const RedJob& j, volatile unsigned* flag, unsigned seq) {
     __shared__ double lds[(kFinalizeThreads / 64 + 1) * 9];
+    constexpr int BW = LEAN ? 1 : kFinalizeBatch;
     const unsigned max_mask = j.kind == kRedMetrics ? ((1U << 7) | (1U << 8)) : (j.kind == kRedPropose ? 2U : 0U);
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     switch (j.kind) {
         case kRedWire: accumulateRows<2>(j.partials, j.blocks, max_mask, acc); break;
-        case kRedMetrics: accumulateRows<9>(j.partials, j.blocks, max_mask, acc); break;
+        case kRedMetrics: accumulateRows<9, BW>(j.partials, j.blocks, max_mask, acc); break;
         case kRedEnergy: accumulateRows<1>(j.partials, j.blocks, max_mask, acc); break;
-        case kRedGather: accumulateRows<7>(j.partials, j.blocks, max_mask, acc); break;
+        case kRedGather: accumulateRows<7, BW>(j.partials, j.blocks, max_mask, acc); break;
         default: accumulateRows<2>(j.partials, j.blocks, max_mask, acc); break;
     }
     blockReduce<9, kFinalizeThreads / 64>(acc, max_mask, lds);
@@ -1435,6 +1467,21 @@ __device__ __forceinline__ void finalizeBody(const RedJob& j, volatile unsigned*
 }
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(RedList l) {
     finalizeBody(l.job[blockIdx.x], l.flag ? l.flag + blockIdx.x : nullptr, l.seq);
+}
+
+// The read-back's finalize and the wirelength evaluated ahead in one launch: workgroups [0, l.n)
+// finalize the pending reductions and store their completion words, the rest run the wirelength
+// gradient (one evaluation) at the smoothing law of the optimizer overflow that the finalize is
+// forming in the same launch, which each of them forms itself from the same partial rows
+// (w.ahead_metrics, metricOverflow).  The host sees the scalars as before; the finalize's launch
+// and its gap before the wirelength are gone.
+__global__ __launch_bounds__(kThreads) void k_finalize_wire(RedList l, WireArgs w, int large_blocks,
+                                                             int medium_blocks, int wire_gx) {
+    const int b = static_cast<int>(blockIdx.x);
+    if (b < l.n)
+        finalizeBody<true>(l.job[b], l.flag ? l.flag + b : nullptr, l.seq);
+    else
+        wireBody(w, large_blocks, medium_blocks, Blk{b - l.n, 0, 0, wire_gx, 1, 1});
 }
 
 int blocksFor(long long n) { return static_cast<int>((n + kThreads - 1) / kThreads); }
@@ -1759,6 +1806,19 @@ void launchFinalize(const RedList& l, hipStream_t s) {
     if (l.n == 0) return;
     hipLaunchKernelGGL(k_finalize, dim3(l.n), dim3(kFinalizeThreads), 0, s, l);
     check(hipGetLastError(), "launch reduction finalize kernel");
+}
+
+void launchFinalizeWire(const RedList& l, const WireArgs& w, hipStream_t s) {
+    const int wire_gx = wireBlocks(w.n_small, w.n_medium, w.n_large);
+    if (l.n < 1 || wire_gx == 0 || !w.ahead_metrics || !w.ahead_overflow)
+        throw std::invalid_argument("the fused finalize needs pending reductions and a wirelength evaluated ahead");
+    if (w.small_net[2] != 0 || w.small_net[kSmallDegree + 1] != w.n_small)
+        throw std::invalid_argument("small-net degree groups do not cover the small nets");
+    const int large_blocks = (w.n_large + kWaves - 1) / kWaves;
+    const int medium_blocks = (w.n_medium + 4 * kWaves - 1) / (4 * kWaves);
+    hipLaunchKernelGGL(k_finalize_wire, dim3(l.n + wire_gx), dim3(kThreads), 0, s, l, w, large_blocks, medium_blocks,
+                       wire_gx);
+    check(hipGetLastError(), "launch reduction finalize and wirelength kernel");
 }
 
 void loadDeviceCode(hipStream_t s) {

@@ -94,9 +94,17 @@ struct WireArgs {
     const double* ahead_overflow;
     double ahead_denominator, ahead_final, ahead_span, ahead_scale;
     int ahead_rows;
+    // With ahead_metrics set, every workgroup forms that overflow itself from the metrics
+    // reduction's partial rows (ahead_metric_rows of them), with k_finalize's sums in k_finalize's
+    // order, instead of reading *ahead_overflow: the finalize then runs in the same launch.
+    const double* ahead_metrics;
+    int ahead_metric_rows;
 };
 int wireBlocks(int n_small, int n_medium, int n_large);
 void launchWire(const WireArgs& a, int count, hipStream_t s);
+// launchFinalize(l) and launchWire(w, 1) as one launch (k_finalize_wire): the finalize's
+// workgroups first, then the wirelength evaluated ahead (w.ahead_metrics set)
+void launchFinalizeWire(const RedList& l, const WireArgs& w, hipStream_t s);
 
 // ---- density: row order, fixed-point deposit, bin metrics ---------------------------------------
 constexpr int kSortThreads = 1024;    // particles per histogram / placement workgroup
