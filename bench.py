@@ -22,7 +22,8 @@ time (the wirelength of the candidate and look-ahead evaluations with the densit
 same launch), against MI355X's HBM peak: its algorithmic bytes (per evaluation of the wirelength 12
 per pin, 8 per net, 16 per particle with the gradient and 8 without, and of the bin pass 24 per bin
 -- one pass over each array at 4-byte indices and 8-byte values) over its launch durations, timed
-with HIP events on the session's stream inside the timed region; `traffic` is its measured HBM
+with HIP events on the session's stream inside the timed region (every fourth launch, so that the
+events slow the timed steps less); `traffic` is its measured HBM
 bytes per launch from the counter summary under profiles/.  `cpu_baseline` is the FP64 CPU restatement of the
 reference path (oracle/) timed on one host core on a bounded number of iterations of the same
 workload.  `kernel_timing_us` (average per launch of every kernel class) comes from a separate
@@ -366,8 +367,9 @@ def batch_sample(jobs: int, runs_spec, iterations: int, seed: int = 2026) -> dic
                       f"{iterations} iterations each, all submitted at once"}
 
 
-# the roofline's kernel: its timing class and its name in the profiler's traces
-ROOF_CLASS, ROOF_KERNEL = "k_wire_bins", "k_wire_dbins"
+# the roofline's kernel: its timing class and its name in the profiler's traces; every ROOF_EVERY-th
+# launch of it in the timed region is bracketed by events (each event pair costs the stream ~1.5 us)
+ROOF_CLASS, ROOF_KERNEL, ROOF_EVERY = "k_wire_bins", "k_wire_dbins", 4
 
 
 def measured_traffic(kernel: str):
@@ -474,7 +476,7 @@ def main() -> int:
     h0 = len(sess.history())
     timed = not args.no_kernel_timing
     if timed:
-        sess.enable_kernel_timing(True, classes=[ROOF_CLASS])
+        sess.enable_kernel_timing(True, classes=[ROOF_CLASS], every=ROOF_EVERY)
     t1 = time.perf_counter()
     done = sess.step(args.steps)
     sess.positions()  # device work of the last step is complete

@@ -330,6 +330,10 @@ mp_gp_status mp_gp_session_particle_order(const mp_gp_session* session, int64_t*
  * the launch count and the summed algorithmic work (bytes, or flops for the DCT products) of the
  * timed launches. */
 mp_gp_status mp_gp_session_enable_kernel_timing(mp_gp_session* session, int enable);
+/* The same, timing only every `every`-th launch of each enabled class (every >= 1), so that fewer
+ * events enter the stream; the read then covers the timed launches only.  No counterpart in the
+ * reference (measurement support). */
+mp_gp_status mp_gp_session_enable_kernel_timing_every(mp_gp_session* session, int enable, int every);
 int32_t mp_gp_session_kernel_timing(mp_gp_session* session, char* names, size_t names_len, double* total_us,
                                     int64_t* launches, double* work, int32_t capacity);
 

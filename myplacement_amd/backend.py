@@ -81,6 +81,7 @@ def library():
         lib.mp_gp_session_history.restype = C.c_int64
         lib.mp_gp_session_positions.argtypes = [V, V]
         lib.mp_gp_session_enable_kernel_timing.argtypes = [V, C.c_int]
+        lib.mp_gp_session_enable_kernel_timing_every.argtypes = [V, C.c_int, C.c_int]
         lib.mp_gp_session_kernel_timing.argtypes = [V, C.c_char_p, C.c_size_t, V, V, V, C.c_int32]
         lib.mp_gp_session_kernel_timing.restype = C.c_int32
         lib.mp_gp_session_particle_order.argtypes = [V, V]
@@ -346,12 +347,13 @@ class PlacementSession:
 
     TIMING_CLASSES = ("k_wire", "k_density", "k_density_sort", "k_gemm", "k_gather", "controller", "k_wire_bins")
 
-    def enable_kernel_timing(self, on: bool = True, classes=None) -> None:
-        """Times the named kernel classes (all when `classes` is None) with HIP events."""
+    def enable_kernel_timing(self, on: bool = True, classes=None, every: int = 1) -> None:
+        """Times the named kernel classes (all when `classes` is None) with HIP events; with
+        `every` > 1 only every every-th launch of each class."""
         mask = 0
         if on:
             mask = -1 if classes is None else sum(1 << self.TIMING_CLASSES.index(c) for c in classes)
-        _raise(library().mp_gp_session_enable_kernel_timing(self._h, mask))
+        _raise(library().mp_gp_session_enable_kernel_timing_every(self._h, mask, int(every)))
 
     def kernel_timing(self) -> dict:
         names = C.create_string_buffer(256)

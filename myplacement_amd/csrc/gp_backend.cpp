@@ -440,21 +440,30 @@ hipEvent_t Backend::poolEvent() {
 
 void Backend::timedBegin(int cls) {
     if (!((timing_mask_ >> cls) & 1U)) return;
+    t_open_[cls] = t_calls_[cls]++ % static_cast<unsigned long long>(timing_every_) == 0;
+    if (!t_open_[cls]) return;
     const hipEvent_t e = poolEvent();
     hipCheck(hipEventRecord(e, stream_), "record timing event");
     ev_begin_[cls].push_back(e);
 }
 
 void Backend::timedEnd(int cls, double work) {
-    if (!((timing_mask_ >> cls) & 1U)) return;
+    if (!((timing_mask_ >> cls) & 1U) || !t_open_[cls]) return;
+    t_open_[cls] = false;
     pending_work_[cls].push_back(work);
     const hipEvent_t e = poolEvent();
     hipCheck(hipEventRecord(e, stream_), "record timing event");
     ev_end_[cls].push_back(e);
 }
 
-void Backend::enableTiming(unsigned class_mask) {
+void Backend::enableTiming(unsigned class_mask, int every) {
+    if (every < 1) throw InvalidArgument("the timing period must be at least 1");
     timing_mask_ = class_mask;
+    timing_every_ = every;
+    for (int c = 0; c < kTimingClasses; ++c) {
+        t_calls_[c] = 0;
+        t_open_[c] = false;
+    }
     if (t_total_.empty()) {
         t_total_.assign(kTimingClasses, 0.0);
         t_work_.assign(kTimingClasses, 0.0);

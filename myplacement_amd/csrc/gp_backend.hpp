@@ -327,7 +327,8 @@ public:
 
     // Optional per-kernel-class timing with HIP events on the backend's stream; bit i of the mask
     // times class i of timingNames() (0: off).
-    void enableTiming(unsigned class_mask);
+    // every: time only every every-th launch of each enabled class (fewer events in the stream)
+    void enableTiming(unsigned class_mask, int every = 1);
     std::vector<std::string> timingNames() const;
     void timing(std::vector<double>& total_us, std::vector<long long>& launches, std::vector<double>& work);
     void timedBegin(int cls);
@@ -394,6 +395,9 @@ private:
     static constexpr int kTimingClasses = 7;
     hipEvent_t poolEvent();
     unsigned timing_mask_ = 0;
+    int timing_every_ = 1;
+    unsigned long long t_calls_[kTimingClasses] = {};  // launches of each class since timing was enabled
+    bool t_open_[kTimingClasses] = {};                 // the class's current launch is being timed
     int capture_shape_ = -1;                              // the chain being captured, or -1
     WireAhead ahead_{};
     bool ahead_pending_ = false;

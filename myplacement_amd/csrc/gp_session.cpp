@@ -233,6 +233,13 @@ mp_gp_status mp_gp_session_enable_kernel_timing(mp_gp_session* h, int enable) {
     });
 }
 
+mp_gp_status mp_gp_session_enable_kernel_timing_every(mp_gp_session* h, int enable, int every) {
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("null session");
+        h->s->backend().enableTiming(static_cast<unsigned>(enable), every);
+    });
+}
+
 int32_t mp_gp_session_kernel_timing(mp_gp_session* h, char* names, size_t names_len, double* total_us,
                                     int64_t* launches, double* work, int32_t capacity) {
     if (!h) return 0;
