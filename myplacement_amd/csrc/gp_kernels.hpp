@@ -108,8 +108,8 @@ void launchFinalizeWire(const RedList& l, const WireArgs& w, hipStream_t s);
 
 // ---- density: row order, fixed-point deposit, bin metrics ---------------------------------------
 constexpr int kSortThreads = 1024;    // particles per histogram / placement workgroup
-constexpr int kDepositThreads = 512;  // particles per deposit workgroup
-constexpr int kWindowCells = 2048;    // LDS window of a deposit workgroup, cells per layer
+constexpr int kDepositThreads = 1024;  // particles per deposit workgroup
+constexpr int kWindowCells = 4096;     // LDS window of a deposit workgroup, cells per layer (64 KB)
 constexpr int kPermLayerShift = 30;   // row-order entries pack the particle layer above the id
 constexpr unsigned kPermIdMask = (1U << kPermLayerShift) - 1U;
 struct DensityArgs {
