@@ -22,7 +22,7 @@ time (the wirelength of the candidate and look-ahead evaluations with the densit
 same launch), against MI355X's HBM peak: its algorithmic bytes (per evaluation of the wirelength 12
 per pin, 8 per net, 16 per particle with the gradient and 8 without, and of the bin pass 24 per bin
 -- one pass over each array at 4-byte indices and 8-byte values) over its launch durations, timed
-with HIP events on the session's stream inside the timed region (every fourth launch, so that the
+with HIP events on the session's stream inside the timed region (every fifth launch, so that the
 events slow the timed steps less); `traffic` is its measured HBM
 bytes per launch from the counter summary under profiles/.  `cpu_baseline` is the FP64 CPU restatement of the
 reference path (oracle/) timed on one host core on a bounded number of iterations of the same
@@ -368,8 +368,10 @@ def batch_sample(jobs: int, runs_spec, iterations: int, seed: int = 2026) -> dic
 
 
 # the roofline's kernel: its timing class and its name in the profiler's traces; every ROOF_EVERY-th
-# launch of it in the timed region is bracketed by events (each event pair costs the stream ~1.5 us)
-ROOF_CLASS, ROOF_KERNEL, ROOF_EVERY = "k_wire_bins", "k_wire_dbins", 4
+# launch of it in the timed region is bracketed by events (each event pair costs the stream ~1.5 us).
+# The class's launches alternate look-ahead (gradient) and candidate (value-only) evaluations in
+# momentum iterations, so the period is odd: the sample then holds both kinds.
+ROOF_CLASS, ROOF_KERNEL, ROOF_EVERY = "k_wire_bins", "k_wire_dbins", 5
 
 
 def measured_traffic(kernel: str):
