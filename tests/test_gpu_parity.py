@@ -369,16 +369,21 @@ def test_graph_captured_iterations_equal_direct_launches(case):
     assert runs[0][1]["gradient_evaluations"] == runs[1][1]["gradient_evaluations"]
 
 
-@pytest.mark.parametrize("case", ["plain-64", "plain-50x30-fresh-order", "mid-128-quadratic"])
+@pytest.mark.parametrize("case", ["plain-64", "plain-64-exponent-2", "plain-50x30-fresh-order", "mid-128-quadratic"])
 def test_density_taken_over_equals_recomputed(case):
     """An iteration's gradient evaluation at the candidate accepted the iteration before takes that
     candidate's density, potential and energy (recompute_density = 0, the default) instead of
     computing them again (recompute_density = 1): smoothing and penalty, the only inputs that changed,
-    enter neither, so the runs must be bit-identical.  Cases: the half-length DCT products (64), the
+    enter neither, so the runs must be bit-identical.  Cases: the half-length DCT products (64); the
+    same with smoothing exponent 2, where no wirelength goes ahead, so a momentum iteration's launch
+    of the bin pass and the wirelength carries two wirelength evaluations and one bin pass; the
     dense products of a grid that is not a multiple of 64 with the row order rebuilt for every
-    evaluation (the look-ahead then evaluated on its own), and a quadratic start at 128 bins."""
+    evaluation (the look-ahead then evaluated on its own); and a quadratic start at 128 bins."""
     if case == "plain-64":
         d, o = _setup(design.plain_design(), 0.9, 64)
+    elif case == "plain-64-exponent-2":
+        d, o = _setup(design.plain_design(), 0.9, 64)
+        o.smoothing_overflow_exponent = 2.0
     elif case == "plain-50x30-fresh-order":
         d, o = _setup(design.plain_design(), 0.9, (50, 30))
         o.density_order_reuse_rows = 0.0
