@@ -495,13 +495,16 @@ def main() -> int:
         timing = sess.kernel_timing()
         sess.enable_kernel_timing(False)
         timing[ROOF_CLASS] = {kk: timing[ROOF_CLASS][kk] - live[kk] for kk in ("total_us", "launches", "work")}
-        avg_us = live["total_us"] / live["launches"]
-        achieved = live["work"] / (live["total_us"] * 1e-6) / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS,
-                "traffic": measured_traffic(ROOF_KERNEL) if args.workload == "large" else None, "kernel": ROOF_KERNEL,
-                "avg_launch_us": avg_us, "launches": live["launches"],
-                "algorithmic_bytes_per_launch": live["work"] / live["launches"]}
+        if live["launches"] == 0:  # a timed region too short to reach a sampled launch
+            log(f"no {ROOF_KERNEL} launch was timed in {done} steps: no roofline")
+        else:
+            avg_us = live["total_us"] / live["launches"]
+            achieved = live["work"] / (live["total_us"] * 1e-6) / 1e9
+            roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": measured_traffic(ROOF_KERNEL) if args.workload == "large" else None,
+                    "kernel": ROOF_KERNEL, "avg_launch_us": avg_us, "launches": live["launches"],
+                    "algorithmic_bytes_per_launch": live["work"] / live["launches"]}
     _, res = sess.finish()
     dropin = dropin_sample(prep, args.dropin_evaluations) if args.dropin_evaluations > 0 else None
 
