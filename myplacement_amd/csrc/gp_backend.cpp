@@ -220,9 +220,11 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
     // on XCD w mod 8, so the terms are stored in kPinRegions = 4 regions, region k holding, in pin
     // order, the pins of the particles of workgroups w = k mod 4.  The gather on XCDs k and k + 4
     // then reads one region, a quarter of the terms, which their L2s hold, and a wavefront's
-    // stores of one pin slot fall in four runs.
+    // stores of one pin slot fall in four runs.  A batch driver's jobs keep the terms in pin order
+    // (no slot array): their launches give each job's workgroups to one XCD already.
+    const bool regions = borrowed == nullptr;
     std::vector<int> pin_dest(Ii, -1);
-    {
+    if (regions) {
         auto region = [](int part) { return (part / kGatherParticles) % kPinRegions; };
         std::array<int, kPinRegions + 1> base{};
         for (std::size_t q = 0; q < Ii; ++q)
@@ -235,7 +237,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
     {
         std::vector<int> fill(count.begin(), count.end() - 1);
         for (std::size_t q = 0; q < Ii; ++q)
-            if (ipin_part[q] >= 0) ppin_idx[fill[ipin_part[q]]++] = pin_dest[q];
+            if (ipin_part[q] >= 0) ppin_idx[fill[ipin_part[q]]++] = regions ? pin_dest[q] : static_cast<int>(q);
     }
     std::vector<double2> sizes(P_);
     for (int i = 0; i < P_; ++i) sizes[i] = make_double2(L.particle_width[i], L.particle_height[i]);
@@ -273,7 +275,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
     pin_part_.allocate(Ii, budget_, "pin-particle map");
     pin_off_.allocate(Ii, budget_, "pin offsets");
     ppin_idx_.allocate(ppin_idx.size(), budget_, "particle-pin index");
-    pin_dest_.allocate(Ii, budget_, "pin gradient slots");
+    if (regions) pin_dest_.allocate(Ii, budget_, "pin gradient slots");
     ppin_off_.allocate(static_cast<std::size_t>(P_) + 1, budget_, "particle-pin offsets");
     // density: particles taller than two bin rows are not row-sorted (their own deposit workgroups
     // add them to the global layers)
@@ -349,7 +351,7 @@ Backend::Backend(const mp_gp_static& L, std::size_t allocation_limit, const Borr
     pin_off_.upload(ipin_off.data(), Ii, stream, "upload pin offsets");
     ppin_off_.upload(count.data(), count.size(), stream, "upload particle-pin offsets");
     ppin_idx_.upload(ppin_idx.data(), ppin_idx.size(), stream, "upload particle-pin index");
-    pin_dest_.upload(pin_dest.data(), Ii, stream, "upload pin gradient slots");
+    if (regions) pin_dest_.upload(pin_dest.data(), Ii, stream, "upload pin gradient slots");
     static_charge_.upload(static_charge.data(), G, stream, "upload static density");
     const auto fx = basis(nxp_, nxp_, nx_, 0, W);
     const auto fy = basis(nyp_, nyp_, ny_, 1, H);

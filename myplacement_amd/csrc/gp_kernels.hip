@@ -185,6 +185,9 @@ __device__ int blockScan(int v, int* lds, int* total) {
     return v + before;
 }
 
+// Slot of pin p's gradient term: the region-major slot the set-up assigned, or pin order.
+__device__ inline int pinSlot(const WireArgs& a, int p) { return a.pin_dest ? a.pin_dest[p] : p; }
+
 __device__ inline double2 pinPosition(const double2* pos, const int* pin_part, const double2* pin_off, int p) {
     const int part = pin_part[p];
     const double2 o = pin_off[p];
@@ -264,7 +267,7 @@ __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_gra
 #pragma unroll
             for (int i = 0; i < PER; ++i) {
                 const int p = b + sub + W * i;
-                const int d = p < e ? a.pin_dest[p] : -1;
+                const int d = p < e ? pinSlot(a, p) : -1;
                 if (d >= 0)
                     pin_grad[d] = make_float2(ex32(qx[i] - mx, inv_gamma) * ipx - ex32(nx - qx[i], inv_gamma) * inx,
                                               ex32(qy[i] - my, inv_gamma) * ipy - ex32(ny - qy[i], inv_gamma) * iny);
@@ -289,7 +292,7 @@ __device__ void wireGroup(const WireArgs& a, const double2* pos, float2* pin_gra
             const float wf = static_cast<float>(w);
             const float ipx = wf / spx, inx = wf / snx, ipy = wf / spy, iny = wf / sny;
             for (int p = b + sub; p < e; p += W) {
-                const int d = a.pin_dest[p];
+                const int d = pinSlot(a, p);
                 if (d < 0) continue;
                 const double2 q = pinPosition(pos, a.pin_part, a.pin_off, p);
                 pin_grad[d] = make_float2(ex32(q.x - mx, inv_gamma) * ipx - ex32(nx - q.x, inv_gamma) * inx,
@@ -402,7 +405,7 @@ __device__ __forceinline__ void wireBody(const WireArgs& a, int large_blocks, in
                 const float ipx = wf / spx, inx = wf / snx, ipy = wf / spy, iny = wf / sny;
 #pragma unroll
                 for (int i = 0; i < kSmallDegree; ++i) {
-                    const int d = i < deg ? a.pin_dest[b + i * size] : -1;
+                    const int d = i < deg ? pinSlot(a, b + i * size) : -1;
                     if (d >= 0)
                         pin_grad[d] = make_float2(ex32(qx[i] - mx, inv_gamma) * ipx - ex32(nx - qx[i], inv_gamma) * inx,
                                                   ex32(qy[i] - my, inv_gamma) * ipy - ex32(ny - qy[i], inv_gamma) * iny);
@@ -522,9 +525,10 @@ __global__ __launch_bounds__(kSortThreads) void k_dplace(DensityArgs a) { dplace
 // walks them in slabs that fit its LDS window: accumulate the slab in LDS, then add its non-zero
 // cells to the global fixed-point layers.  Integer addition keeps the result independent of the
 // order in which workgroups and lanes arrive.
+template <int T, int WIN>
 __device__ __forceinline__ void ddepositBody(const DensityArgs& a, int sorted_blocks, const Blk& bk) {
-    __shared__ unsigned long long win[2 * kWindowCells];
-    __shared__ int lds[2 * (kDepositThreads / 64)];
+    __shared__ unsigned long long win[2 * WIN];
+    __shared__ int lds[2 * (T / 64)];
     const int2 eb = batchBlockChunked(bk);
     const Region& r = a.r;
     const int nx = r.nx;
@@ -534,7 +538,7 @@ __device__ __forceinline__ void ddepositBody(const DensityArgs& a, int sorted_bl
     int p = -1, layer = 0;
     double2 sz = make_double2(0.0, 0.0);
     if (eb.y < sorted_blocks) {
-        const int q = eb.y * kDepositThreads + threadIdx.x;
+        const int q = eb.y * T + threadIdx.x;
         if (q < a.keystart[r.ny]) {
             const unsigned e = a.perm[q];
             p = static_cast<int>(e & kPermIdMask);
@@ -542,7 +546,7 @@ __device__ __forceinline__ void ddepositBody(const DensityArgs& a, int sorted_bl
             sz = a.sorted_size[q];
         }
     } else {
-        const int k = (eb.y - sorted_blocks) * kDepositThreads + threadIdx.x;
+        const int k = (eb.y - sorted_blocks) * T + threadIdx.x;
         if (k < a.n_tall) {
             p = a.tall_ids[k];
             sz = a.size[p];
@@ -567,14 +571,14 @@ __device__ __forceinline__ void ddepositBody(const DensityArgs& a, int sorted_bl
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         lds[wave] = lo;
-        lds[kDepositThreads / 64 + wave] = hi;
+        lds[T / 64 + wave] = hi;
     }
     __syncthreads();
     lo = INT_MAX;
     hi = INT_MIN;
-    for (int w = 0; w < kDepositThreads / 64; ++w) {
+    for (int w = 0; w < T / 64; ++w) {
         lo = min(lo, lds[w]);
-        hi = max(hi, lds[kDepositThreads / 64 + w]);
+        hi = max(hi, lds[T / 64 + w]);
     }
     if (lo > hi) return;  // nothing to deposit (uniform across the workgroup)
     const double scale = layer == 1 ? a.t : 1.0;
@@ -593,7 +597,7 @@ __device__ __forceinline__ void ddepositBody(const DensityArgs& a, int sorted_bl
             }
         }
     };
-    const int slab_rows = kWindowCells / nx;
+    const int slab_rows = WIN / nx;
     if (slab_rows == 0) {  // grids wider than the window: straight to the global layers
         if (live) deposit(r0, r1, grid, 0, G);
         return;
@@ -602,12 +606,12 @@ __device__ __forceinline__ void ddepositBody(const DensityArgs& a, int sorted_bl
     for (int s0 = lo; s0 <= hi; s0 += slab_rows) {
         const int s1 = min(hi, s0 + slab_rows - 1);
         const int cells = (s1 - s0 + 1) * nx;
-        for (int i = threadIdx.x; i < 2 * cells; i += kDepositThreads) win[i] = 0ULL;
+        for (int i = threadIdx.x; i < 2 * cells; i += T) win[i] = 0ULL;
         __syncthreads();
         if (live && r0 <= s1 && r1 >= s0) deposit(max(r0, s0), min(r1, s1), win, s0, static_cast<size_t>(cells));
         __syncthreads();
         const size_t base = static_cast<size_t>(s0) * nx;
-        for (int i = threadIdx.x; i < cells; i += kDepositThreads) {
+        for (int i = threadIdx.x; i < cells; i += T) {
             const unsigned long long m = win[i], f = win[cells + i];
             if (m) atomicAdd(grid + base + i, m);
             if (f) atomicAdd(grid + G + base + i, f);
@@ -616,7 +620,11 @@ __device__ __forceinline__ void ddepositBody(const DensityArgs& a, int sorted_bl
     }
 }
 __global__ __launch_bounds__(kDepositThreads) void k_ddeposit(DensityArgs a, int sorted_blocks) {
-    ddepositBody(a, sorted_blocks, nativeBlk());
+    ddepositBody<kDepositThreads, kWindowCells>(a, sorted_blocks, nativeBlk());
+}
+// the batch driver's shape (its jobs' grids are small; the sums are the same integers)
+__global__ __launch_bounds__(kBatchDepositThreads) void k_ddeposit_small(DensityArgs a, int sorted_blocks) {
+    ddepositBody<kBatchDepositThreads, kBatchWindowCells>(a, sorted_blocks, nativeBlk());
 }
 
 // Bin metrics (Metrics.cpp:101-119 per bin) and the FP32 deviation grid for the DCT from the
@@ -1513,11 +1521,11 @@ __global__ __launch_bounds__(kSortThreads) void k_dplace_batch(const SortEntry* 
     const SortEntry& E = e[j];
     dplaceBody(E.a, entryBlk(v - first[j], E.gx, 1, 1));
 }
-__global__ __launch_bounds__(kDepositThreads) void k_ddeposit_batch(const DensityEntry* e, const int* first, int n) {
+__global__ __launch_bounds__(kBatchDepositThreads) void k_ddeposit_batch(const DensityEntry* e, const int* first, int n) {
     const int v = batchSlot();
-    const int j = batchEntry<kDepositThreads>(first, n, v);
+    const int j = batchEntry<kBatchDepositThreads>(first, n, v);
     const DensityEntry& E = e[j];
-    ddepositBody(E.a, E.sorted_blocks, entryBlk(v - first[j], E.gx, E.gy, 1));
+    ddepositBody<kBatchDepositThreads, kBatchWindowCells>(E.a, E.sorted_blocks, entryBlk(v - first[j], E.gx, E.gy, 1));
 }
 __global__ __launch_bounds__(kThreads) void k_dbins_batch(const DensityEntry* e, const int* first, int n) {
     const int v = batchSlot();
@@ -1849,8 +1857,8 @@ SortEntry sortEntry(const DensityArgs& a, int stage) {
     return SortEntry{a, hb, stage == 1 ? a.r.ny : hb};
 }
 DensityEntry depositEntry(const DensityArgs& a, int count) {
-    const int sorted_blocks = (a.n + kDepositThreads - 1) / kDepositThreads;
-    const int tall_blocks = (a.n_tall + kDepositThreads - 1) / kDepositThreads;
+    const int sorted_blocks = (a.n + kBatchDepositThreads - 1) / kBatchDepositThreads;
+    const int tall_blocks = (a.n_tall + kBatchDepositThreads - 1) / kBatchDepositThreads;
     return DensityEntry{a, sorted_blocks, sorted_blocks + tall_blocks, count};
 }
 DensityEntry binsEntry(const DensityArgs& a, int count) { return DensityEntry{a, 0, densityBinBlocks(a.r), count}; }
@@ -1902,7 +1910,7 @@ void launchBatched(BatchKernel k, const void* entries, const int* first, int n, 
                                static_cast<const SortEntry*>(entries), first, n);
             break;
         case kBatchDeposit:
-            hipLaunchKernelGGL(k_ddeposit_batch, grid, dim3(kDepositThreads), 0, s,
+            hipLaunchKernelGGL(k_ddeposit_batch, grid, dim3(kBatchDepositThreads), 0, s,
                                static_cast<const DensityEntry*>(entries), first, n);
             break;
         case kBatchBins:
@@ -1975,7 +1983,7 @@ void launchEntry(BatchKernel k, const void* entry, std::size_t dyn_lds, hipStrea
         }
         case kBatchDeposit: {
             const DensityEntry& E = *static_cast<const DensityEntry*>(entry);
-            hipLaunchKernelGGL(k_ddeposit, dim3(E.gx, E.gy), dim3(kDepositThreads), 0, s, E.a, E.sorted_blocks);
+            hipLaunchKernelGGL(k_ddeposit_small, dim3(E.gx, E.gy), dim3(kBatchDepositThreads), 0, s, E.a, E.sorted_blocks);
             break;
         }
         case kBatchBins: {

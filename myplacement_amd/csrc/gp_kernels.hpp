@@ -82,7 +82,7 @@ struct WireArgs {
     int n_small, n_medium, n_large;
     int small_net[kSmallDegree + 2];  // [d]: first net of degree d (d = 2..kSmallDegree), [kSmallDegree + 1] = n_small
     int small_pin[kSmallDegree + 2];  // [d]: first pin of that group
-    const int* pin_dest;  // [pin] its slot in pin_grad, -1 for pins of fixed particles
+    const int* pin_dest;  // [pin] its slot in pin_grad, -1 for pins of fixed particles (null: pin order)
     double gamma;
     int want_grad;
     float2* pin_grad[kMaxEval];  // per-pin d(WL)/d(pin), at the slots pin_dest gives
@@ -110,6 +110,9 @@ void launchFinalizeWire(const RedList& l, const WireArgs& w, hipStream_t s);
 constexpr int kSortThreads = 1024;    // particles per histogram / placement workgroup
 constexpr int kDepositThreads = 1024;  // particles per deposit workgroup
 constexpr int kWindowCells = 4096;     // LDS window of a deposit workgroup, cells per layer (64 KB)
+// the same for the batch driver's jobs (small grids: more, smaller workgroups per job)
+constexpr int kBatchDepositThreads = 512;
+constexpr int kBatchWindowCells = 2048;
 constexpr int kPermLayerShift = 30;   // row-order entries pack the particle layer above the id
 constexpr unsigned kPermIdMask = (1U << kPermLayerShift) - 1U;
 struct DensityArgs {
