@@ -22,8 +22,8 @@ time (the wirelength of the candidate and look-ahead evaluations with the densit
 same launch), against MI355X's HBM peak: its algorithmic bytes (per evaluation of the wirelength 12
 per pin, 8 per net, 16 per particle with the gradient and 8 without, and of the bin pass 24 per bin
 -- one pass over each array at 4-byte indices and 8-byte values) over its launch durations, timed
-with HIP events on the session's stream inside the timed region (every fifth launch, so that the
-events slow the timed steps less); `traffic` is its measured HBM
+with HIP events on the session's stream inside the timed region (every fifth launch, every third
+in runs under 100 steps, so that the events slow the timed steps less); `traffic` is its measured HBM
 bytes per launch from the counter summary under profiles/.  `cpu_baseline` is the FP64 CPU restatement of the
 reference path (oracle/) timed on one host core on a bounded number of iterations of the same
 workload.  `kernel_timing_us` (average per launch of every kernel class) comes from a separate
@@ -370,8 +370,13 @@ def batch_sample(jobs: int, runs_spec, iterations: int, seed: int = 2026) -> dic
 # the roofline's kernel: its timing class and its name in the profiler's traces; every ROOF_EVERY-th
 # launch of it in the timed region is bracketed by events (each event pair costs the stream ~1.5 us).
 # The class's launches alternate look-ahead (gradient) and candidate (value-only) evaluations in
-# momentum iterations, so the period is odd: the sample then holds both kinds.
-ROOF_CLASS, ROOF_KERNEL, ROOF_EVERY = "k_wire_bins", "k_wire_dbins", 5
+# momentum iterations, so the period is odd: the sample then holds both kinds.  Short timed regions
+# (under 100 steps) time every third launch, so that they still hold several.
+ROOF_CLASS, ROOF_KERNEL = "k_wire_bins", "k_wire_dbins"
+
+
+def roof_every(steps: int) -> int:
+    return 5 if steps >= 100 else 3
 
 
 def measured_traffic(kernel: str):
@@ -478,7 +483,7 @@ def main() -> int:
     h0 = len(sess.history())
     timed = not args.no_kernel_timing
     if timed:
-        sess.enable_kernel_timing(True, classes=[ROOF_CLASS], every=ROOF_EVERY)
+        sess.enable_kernel_timing(True, classes=[ROOF_CLASS], every=roof_every(args.steps))
     t1 = time.perf_counter()
     done = sess.step(args.steps)
     sess.positions()  # device work of the last step is complete
