@@ -367,7 +367,7 @@ def batch_sample(jobs: int, runs_spec, iterations: int, seed: int = 2026) -> dic
                       f"{iterations} iterations each, all submitted at once"}
 
 
-# the roofline's kernel: its timing class and its name in the profiler's traces; every ROOF_EVERY-th
+# the roofline's kernel: its timing class and its name in the profiler's traces; every roof_every(steps)-th
 # launch of it in the timed region is bracketed by events (each event pair costs the stream ~1.5 us).
 # The class's launches alternate look-ahead (gradient) and candidate (value-only) evaluations in
 # momentum iterations, so the period is odd: the sample then holds both kinds.  Short timed regions
